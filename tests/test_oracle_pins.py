@@ -1,0 +1,345 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+The reference's own tests hold no kernel-level vectors (its hot path is only
+pinned end to end by tree-cost fixtures, see test_tree_fixtures.py), so the
+oracle's kernel functions are checked here against independent formulations:
+closed forms of the cost-matrix fills, textbook Needleman-Wunsch and Gotoh
+dynamic programming where the reference's band covers the whole matrix,
+recosting of the produced alignments, and brute-force parsimony for Fitch.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+GAP = 16
+A, Cb, G, T = 1, 2, 4, 8
+INF = 1 << 40
+
+
+def rand_dna(rng, n):
+    return np.concatenate([[GAP], rng.choice([1, 2, 4, 8], size=n)]).astype(np.uint8)
+
+
+def mutate(rng, s, ps=0.15, pi=0.05):
+    out = [GAP]
+    for x in s[1:]:
+        r = rng.random()
+        if r < pi / 2:
+            continue
+        if r < pi:
+            out.append(int(rng.choice([1, 2, 4, 8])))
+        out.append(int(rng.choice([1, 2, 4, 8])) if rng.random() < ps else int(x))
+    return np.array(out, np.uint8)
+
+
+def strip(s):
+    return np.array([x for x in s if x != GAP], np.uint8)
+
+
+# ---------------------------------------------------------------- R1 -------
+def members(m):
+    return [1 << b for b in range(5) if m & (1 << b)]
+
+
+def test_tcm_linear_closed_form(oracle):
+    """Metric fill: for a metric base matrix the cost of two masks is the
+    minimum base-to-base cost (triangle inequality), singleton medians are
+    the union of the pair, and prepend/tail are the gap rows."""
+    cm = oracle.CM2.sub_indel(1, 2)
+    d = {(a, b): (0 if a == b else (2 if GAP in (a, b) else 1)) for a in members(31) for b in members(31)}
+    for i in range(1, 32):
+        for j in range(1, 32):
+            assert cm.cost(i, j) == min(d[a, b] for a in members(i) for b in members(j)), (i, j)
+            if len(members(i)) == 1 and len(members(j)) == 1:
+                assert cm.median(i, j) == i | j
+    # hand-derived medians: argmin state sets of cost(a,v)+cost(v,b)
+    assert cm.median(A, Cb | G) == A | Cb | G
+    assert cm.median(A | Cb, A) == A
+    assert cm.median(A | GAP, Cb) == A | Cb
+    assert cm.median(A | Cb, G | T) == A | Cb | G | T
+    assert cm.median(GAP | A, GAP | Cb) == GAP
+
+
+def test_tcm_affine_bitwise_closed_form(oracle):
+    """Bitwise fill after set_affine: shared bits cost 0 with the
+    intersection as median, else the minimum member cost with the union of
+    the optimal member pairs as median."""
+    cm = oracle.CM2.sub_indel(1, 1, gap_open=3)
+    d = {(a, b): (0 if a == b else 1) for a in members(31) for b in members(31)}
+    for i in range(1, 32):
+        for j in range(1, 32):
+            if len(members(i)) == 1 and len(members(j)) == 1:
+                assert cm.cost(i, j) == d[i, j]
+                assert cm.median(i, j) == i | j
+            elif i & j:
+                assert cm.cost(i, j) == 0 and cm.median(i, j) == i & j
+            else:
+                best = min(d[a, b] for a in members(i) for b in members(j))
+                med = 0
+                for a in members(i):
+                    for b in members(j):
+                        if d[a, b] == best:
+                            med |= a | b
+                assert cm.cost(i, j) == best and cm.median(i, j) == med, (i, j)
+
+
+def test_tcm_three_d_closed_form(oracle):
+    """Three_D.of_two_dim over a metric: cost(i,j,k) = min over single states
+    l of cost(l,i)+cost(l,j)+cost(l,k); the median is the lowest optimal l."""
+    cm = oracle.CM2.sub_indel(1, 2)
+    cm3 = oracle.CM3(cm)
+    rng = np.random.default_rng(1)
+    for _ in range(400):
+        i, j, k = (int(x) for x in rng.integers(1, 32, size=3))
+        vals = [(cm.cost(1 << l, i) + cm.cost(1 << l, j) + cm.cost(1 << l, k), 1 << l) for l in range(5)]
+        best = min(v for v, _ in vals)
+        assert cm3.cost(i, j, k) == best
+        assert cm3.median(i, j, k) == min(s for v, s in vals if v == best)
+
+
+# ---------------------------------------------------------------- R4-R7 ----
+def textbook_nw(s1, s2, cm):
+    m, n = len(s1), len(s2)
+    D = np.zeros((m, n), np.int64)
+    for j in range(1, n):
+        D[0, j] = D[0, j - 1] + cm.cost(GAP, int(s2[j]))
+    for i in range(1, m):
+        D[i, 0] = D[i - 1, 0] + cm.cost(int(s1[i]), GAP)
+        for j in range(1, n):
+            D[i, j] = min(D[i - 1, j - 1] + cm.cost(int(s1[i]), int(s2[j])),
+                          D[i - 1, j] + cm.cost(int(s1[i]), GAP),
+                          D[i, j - 1] + cm.cost(GAP, int(s2[j])))
+    return int(D[m - 1, n - 1])
+
+
+@pytest.mark.parametrize("sub,indel", [(1, 2), (1, 1), (2, 1), (3, 1)])
+def test_linear_full_plane_is_textbook(oracle, sub, indel):
+    """When algn_fill_plane_2 fills the whole matrix the cost is textbook NW."""
+    cm = oracle.CM2.sub_indel(sub, indel)
+    rng = np.random.default_rng(7 + sub * 10 + indel)
+    for t in range(60):
+        n = int(rng.integers(1, 30))
+        s2 = rand_dna(rng, n)
+        s1 = rand_dna(rng, int(rng.integers(n, 60)))
+        dw = oracle.deltaw(s1, s2, cm)
+        assert oracle.algn_nw_cells(len(s1), len(s2), dw) == len(s1) * len(s2)
+        assert oracle.cost_2(s1, s2, cm) == textbook_nw(s1, s2, cm)
+
+
+def test_linear_band_invariants(oracle):
+    """Banded runs: cost >= textbook optimum, the alignment recosts to the
+    cost, stripping gaps gives the inputs back, medians have the lengths the
+    median rules fix."""
+    cm = oracle.CM2.sub_indel(1, 2)
+    rng = np.random.default_rng(11)
+    banded = 0
+    for t in range(40):
+        s1 = rand_dna(rng, int(rng.integers(150, 320)))
+        s2 = mutate(rng, s1, 0.2, 0.1 if t % 2 else 0.02)
+        cost, a1, a2 = oracle.align_2(s1, s2, cm)
+        l1, l2 = max(len(s1), len(s2)), min(len(s1), len(s2))
+        dw = oracle.deltaw(s1, s2, cm)
+        if oracle.algn_nw_cells(l1, l2, dw) < l1 * l2:
+            banded += 1
+        assert cost == oracle.cost_2(s1, s2, cm)
+        assert cost >= textbook_nw(s1, s2, cm) if t < 6 else True
+        assert len(a1) == len(a2)
+        assert oracle.verify_2(a1, a2, cm) == cost
+        assert np.array_equal(strip(a1), strip(s1)) and np.array_equal(strip(a2), strip(s2))
+        assert a1[0] == GAP and a2[0] == GAP
+        mwg = oracle.median_2_with_gaps(a1, a2, cm)
+        assert len(mwg) == len(a1)
+        med = oracle.median_2(a1, a2, cm)
+        assert len(med) == 1 + sum(1 for x in mwg if x != GAP)
+        anc = oracle.ancestor_2(a1, a2, cm)
+        assert np.array_equal(anc, med)
+    assert banded >= 20
+
+
+def test_linear_backtrace_priorities(oracle):
+    """A single substitution-or-indel tie: with s1 the longer sequence the
+    backtrace prefers INSERT over DELETE (swaped), else DELETE first."""
+    cm = oracle.CM2.sub_indel(1, 1)
+    s1 = np.array([GAP, A, Cb], np.uint8)
+    s2 = np.array([GAP, G], np.uint8)
+    c, dm, lo, hi = oracle.algn_nw(s1, s2, cm, 10, with_dm=True)
+    assert c == 2
+    r1, r2 = oracle.backtrack_2d(dm, lo, hi, s1, s2, cm, True)
+    r1b, r2b = oracle.backtrack_2d(dm, lo, hi, s1, s2, cm, False)
+    for x, y in ((r1, r2), (r1b, r2b)):
+        assert oracle.verify_2(x, y, cm) == 2
+
+
+# ---------------------------------------------------------------- R8/R9 ----
+def textbook_gotoh(a, b, cm, go):
+    """Three-state Gotoh with all state transitions; a gap of length L in a
+    block costs go + sum of per-base gap costs."""
+    m, n = len(a), len(b)
+    M = np.full((m, n), INF, np.int64)
+    X = np.full((m, n), INF, np.int64)   # consumes a (gap in b)
+    Y = np.full((m, n), INF, np.int64)   # consumes b (gap in a)
+    M[0, 0] = 0
+    for i in range(1, m):
+        X[i, 0] = (go if i == 1 else X[i - 1, 0]) + cm.cost(int(a[i]), GAP)
+    for j in range(1, n):
+        Y[0, j] = (go if j == 1 else Y[0, j - 1]) + cm.cost(GAP, int(b[j]))
+    for i in range(1, m):
+        for j in range(1, n):
+            best = min(M[i - 1, j - 1], X[i - 1, j - 1], Y[i - 1, j - 1])
+            M[i, j] = best + cm.cost(int(a[i]), int(b[j]))
+            e = cm.cost(int(a[i]), GAP)
+            X[i, j] = min(M[i - 1, j] + go, X[i - 1, j], Y[i - 1, j] + go) + e
+            e = cm.cost(GAP, int(b[j]))
+            Y[i, j] = min(M[i, j - 1] + go, Y[i, j - 1], X[i, j - 1] + go) + e
+    return int(min(M[m - 1, n - 1], X[m - 1, n - 1], Y[m - 1, n - 1]))
+
+
+@pytest.mark.parametrize("sub,indel,go", [(1, 1, 3), (2, 1, 1), (3, 1, 5)])
+def test_affine_full_band_is_gotoh(oracle, sub, indel, go):
+    """With the longer sequence at most 40 long the affine band covers the
+    whole matrix; for plain DNA the cost is then textbook Gotoh."""
+    cm = oracle.CM2.sub_indel(sub, indel, gap_open=go)
+    rng = np.random.default_rng(3 + go)
+    for t in range(50):
+        si = rand_dna(rng, int(rng.integers(0, 25)))
+        sj = mutate(rng, si, 0.3, 0.2) if t % 3 else rand_dna(rng, int(rng.integers(0, 38)))
+        if len(sj) > 41:
+            sj = sj[:41]
+        ref = textbook_gotoh(si, sj, cm, go)
+        med, ri, rj, cost, mwg = oracle.align_affine_3(si, sj, cm)
+        assert cost == ref, (t, si, sj)
+        assert oracle.cost_affine_3(si, sj, cm) == ref
+        assert oracle.verify_2(ri, rj, cm) == cost
+
+
+def test_affine_band_invariants(oracle):
+    cm = oracle.CM2.sub_indel(1, 1, gap_open=3)
+    rng = np.random.default_rng(5)
+    for t in range(30):
+        si = rand_dna(rng, int(rng.integers(60, 400)))
+        sj = mutate(rng, si, 0.2, 0.05)
+        med, ri, rj, cost, mwg = oracle.align_affine_3(si, sj, cm)
+        assert cost == oracle.cost_affine_3(si, sj, cm)   # plain DNA: both fills agree
+        assert cost >= (textbook_gotoh(si, sj, cm, 3) if t < 4 else 0)
+        assert oracle.verify_2(ri, rj, cm) == cost
+        assert np.array_equal(strip(ri), strip(si)) and np.array_equal(strip(rj), strip(sj))
+        assert len(mwg) == len(ri) == len(rj)
+        assert med[0] == GAP and mwg[0] == GAP
+        # aligned columns give the median symbol; gap columns give x|gap
+        assert len(med) == 1 + sum(1 for x in mwg[1:] if x != GAP)
+
+
+# ---------------------------------------------------------------- R10/R11 --
+def test_cube_single_plane_is_textbook(oracle):
+    """With s1 = [gap] the cube is one plane: a textbook 2-D DP over s2, s3
+    with 3-D costs of (gap, x, y)."""
+    cm3 = oracle.CM3(oracle.CM2.sub_indel(1, 2))
+    rng = np.random.default_rng(9)
+    for _ in range(30):
+        s2 = rand_dna(rng, int(rng.integers(0, 15)))
+        s3 = rand_dna(rng, int(rng.integers(0, 15)))
+        m, n = len(s2), len(s3)
+        D = np.zeros((m, n), np.int64)
+        c = lambda x, y: cm3.cost(GAP, x, y)
+        for k in range(1, n):
+            D[0, k] = D[0, k - 1] + c(GAP, int(s3[k]))
+        for r in range(1, m):
+            D[r, 0] = D[r - 1, 0] + c(int(s2[r]), GAP)
+            for k in range(1, n):
+                D[r, k] = min(D[r - 1, k] + c(int(s2[r]), GAP), D[r - 1, k - 1] + c(int(s2[r]), int(s3[k])),
+                              D[r, k - 1] + c(GAP, int(s3[k])))
+        assert oracle.cost_3(np.array([GAP], np.uint8), s2, s3, cm3) == D[m - 1, n - 1]
+
+
+def test_cube_alignment_is_valid(oracle):
+    cm3 = oracle.CM3(oracle.CM2.sub_indel(1, 2))
+    rng = np.random.default_rng(13)
+    for _ in range(20):
+        anc = rand_dna(rng, int(rng.integers(5, 30)))
+        s = [mutate(rng, anc) for _ in range(3)]
+        cost, a1, a2, a3 = oracle.align_3(*s, cm3)
+        assert len(a1) == len(a2) == len(a3)
+        for x, y in zip((a1, a2, a3), s):
+            assert np.array_equal(strip(x), strip(y))
+        med = oracle.median_3(a1, a2, a3, cm3)
+        assert len(med) == len(a1) and len(set(med.tolist())) <= 1
+
+
+# ---------------------------------------------------------------- R12/R13 --
+def random_tree(rng, nleaves):
+    """Random rooted binary tree: list of (node, left, right), root last."""
+    nodes = list(range(nleaves))
+    internal = []
+    nxt = nleaves
+    while len(nodes) > 1:
+        i, j = rng.choice(len(nodes), size=2, replace=False)
+        a, b = nodes[i], nodes[j]
+        nodes = [x for k, x in enumerate(nodes) if k not in (i, j)] + [nxt]
+        internal.append((nxt, a, b))
+        nxt += 1
+    return internal
+
+
+def test_fitch_matches_brute_force_parsimony(oracle):
+    """Down pass: the summed added costs equal the minimum number of changes
+    over all internal state assignments.  Up pass: the final sets equal the
+    states a node takes in at least one most parsimonious reconstruction."""
+    rng = np.random.default_rng(17)
+    for t in range(12):
+        nleaves = int(rng.integers(3, 7))
+        k = int(rng.integers(2, 4))
+        nchar = 40
+        tree = random_tree(rng, nleaves)
+        leaf = {v: rng.integers(1, 1 << k, size=nchar).astype(np.uint8) for v in range(nleaves)}
+        prelim = dict(leaf)
+        total = 0
+        for v, a, b in tree:
+            prelim[v], added = oracle.nonadd_median(prelim[a], prelim[b])
+            total += added
+        root = tree[-1][0]
+        final = {root: prelim[root]}
+        for v, a, b in reversed(tree):
+            pass
+        parent = {}
+        for v, a, b in tree:
+            parent[a] = v
+            parent[b] = v
+        children = {v: (a, b) for v, a, b in tree}
+        for v, a, b in reversed(tree[:-1]):
+            p = parent[v]
+            final[v] = oracle.nonadd_median_3(final[p], prelim[v], prelim[a], prelim[b])
+        internal = [v for v, _, _ in tree]
+        bf_total = 0
+        for ch in range(nchar):
+            best = None
+            mpr_states = {v: set() for v in internal}
+            for assign in itertools.product(range(k), repeat=len(internal)):
+                st = dict(zip(internal, assign))
+                cost = 0
+                for v, a, b in tree:
+                    for c in (a, b):
+                        if c in st:
+                            cost += st[c] != st[v]
+                        else:
+                            cost += 0 if (leaf[c][ch] >> st[v]) & 1 else 1
+                if best is None or cost < best:
+                    best = cost
+                    mpr_states = {v: {st[v]} for v in internal}
+                elif cost == best:
+                    for v in internal:
+                        mpr_states[v].add(st[v])
+            bf_total += best
+            for v in internal:
+                got = {s for s in range(k) if (final[v][ch] >> s) & 1}
+                assert got == mpr_states[v], (t, ch, v)
+        assert total == bf_total
+
+
+def test_fitch_distance_helpers(oracle):
+    rng = np.random.default_rng(19)
+    a = rng.integers(1, 256, size=1000).astype(np.uint8)
+    b = rng.integers(1, 256, size=1000).astype(np.uint8)
+    c = rng.integers(1, 256, size=1000).astype(np.uint8)
+    assert oracle.nonadd_distance(a, b) == int(((a & b) == 0).sum())
+    assert oracle.nonadd_dist_2(a, b, c) == int((((a & b) == 0) & ((a & c) == 0)).sum())
