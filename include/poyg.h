@@ -1,0 +1,208 @@
+/*
+ * poyg.h -- C ABI of the MI355X (gfx950) direct-optimization engine.
+ *
+ * The engine replaces the C stubs behind POY's OCaml `external` declarations
+ * for the alignment hot path.  Each per-call entry point below names the
+ * reference stub it replaces (paths relative to the lealanko/poyd tree); the
+ * OCaml-side stub that forwards a `value` to these functions is shown in
+ * INTEGRATION.md.  Sequences are arrays of 5-bit masks (A=1, C=2, G=4, T=8,
+ * gap=16) that include POY's leading gap at index 0.  Output sequences are
+ * written the way POY's seq_prepend fills a caller-allocated sequence: the
+ * produced symbols occupy out[cap-len .. cap-1] of a buffer of capacity cap.
+ *
+ * All functions return POYG_OK (0) or a negative POYG_E* code; the message of
+ * the last error on the calling thread is available from poyg_last_error().
+ * The library never longjmps; a stub raises OCaml's Failure on a non-zero
+ * code.  A context is used by one host thread at a time (POY enters C from a
+ * single thread, src/poyd/PoydThread.ml:40-77).
+ *
+ * Batched entry points take device pointers and a hipStream_t (passed as
+ * void*) so that inputs stay resident in HBM; per-call entry points take host
+ * pointers and synchronise.
+ */
+#ifndef POYG_H
+#define POYG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POYG_OK 0
+#define POYG_EINVAL (-1)     /* bad argument (length, alphabet, NULL) */
+#define POYG_ENOMEM (-2)     /* host or device allocation failed */
+#define POYG_EDEVICE (-3)    /* HIP runtime error */
+#define POYG_EUNSUPPORTED (-4)
+#define POYG_EBAND (-5)      /* a backtrace left the filled band */
+
+typedef struct poyg_ctx poyg_ctx;
+typedef struct poyg_cm poyg_cm;
+typedef struct poyg_cm3 poyg_cm3;
+typedef struct poyg_plan poyg_plan;
+
+const char *poyg_last_error(void);
+const char *poyg_version(void);
+
+/* ---- context -------------------------------------------------------- */
+int poyg_ctx_create(int device, poyg_ctx **out);
+void poyg_ctx_destroy(poyg_ctx *ctx);
+int poyg_ctx_sync(poyg_ctx *ctx);
+
+/* ---- device buffers (plumbing for callers without their own allocator;
+ * no reference counterpart) ------------------------------------------- */
+int poyg_dev_alloc(poyg_ctx *ctx, int64_t bytes, void **out);
+int poyg_dev_free(poyg_ctx *ctx, void *p);
+int poyg_dev_upload(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int poyg_dev_download(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes);
+/* The context's stream as a hipStream_t. */
+void *poyg_ctx_stream(poyg_ctx *ctx);
+
+/* ---- two-dimensional cost matrix (src/cm.c, src/cost_matrix.ml) ------- */
+/* cm_CAML_create (src/cm.c:1389-1398); only 5-bit combination alphabets. */
+int poyg_cm_create(int a_sz, int combinations, int cost_model, int gap_open, int all_elements,
+                   poyg_cm **out);
+void poyg_cm_destroy(poyg_cm *cm);
+/* cm_CAML_set_cost / set_worst / set_median / set_prepend / set_tail
+ * (src/cm.c:1295-1345). */
+int poyg_cm_set_cost(poyg_cm *cm, int a, int b, int v);
+int poyg_cm_set_worst(poyg_cm *cm, int a, int b, int v);
+int poyg_cm_set_median(poyg_cm *cm, int a, int b, int v);
+int poyg_cm_set_prepend(poyg_cm *cm, int a, int v);
+int poyg_cm_set_tail(poyg_cm *cm, int a, int v);
+/* cm_CAML_set_affine (src/cm.c:123-128 via Cost_matrix.Two_D.c_set_aff). */
+int poyg_cm_set_affine_flag(poyg_cm *cm, int cost_model, int gap_open);
+/* cm_CAML_get_cost / get_median / get_worst / get_prepend / get_tail /
+ * get_gap / get_lcm / get_affine / get_gap_opening. */
+int poyg_cm_get_cost(const poyg_cm *cm, int a, int b);
+int poyg_cm_get_median(const poyg_cm *cm, int a, int b);
+int poyg_cm_get_worst(const poyg_cm *cm, int a, int b);
+int poyg_cm_get_prepend(const poyg_cm *cm, int a);
+int poyg_cm_get_tail(const poyg_cm *cm, int a);
+int poyg_cm_get_gap(const poyg_cm *cm);
+int poyg_cm_get_affine(const poyg_cm *cm);
+int poyg_cm_get_gap_opening(const poyg_cm *cm);
+/* Cost_matrix.Two_D.of_list ~use_comb:true (src/cost_matrix.ml:520-538,
+ * 596-602): fills every ambiguity combination from a w x w base matrix. */
+int poyg_cm_of_list(const int *matrix, int w, poyg_cm **out);
+/* Cost_matrix.Two_D.set_affine (src/cost_matrix.ml:405-416): model 0 =
+ * Linnear, 1 = Affine gap_open, 2 = No_Alignment; refills the table. */
+int poyg_cm_set_affine(poyg_cm *cm, int model, int gap_open);
+/* Scripting CM.of_sub_indel / of_sub_indel_affine
+ * (src/scripting.ml:4365-4371); gap_open < 0 keeps the linear model. */
+int poyg_cm_of_sub_indel(int sub, int indel, int gap_open, poyg_cm **out);
+/* Cost_matrix.Three_D.of_two_dim (src/cost_matrix.ml:853-861). */
+int poyg_cm3_of_two_dim(const poyg_cm *cm, poyg_cm3 **out);
+void poyg_cm3_destroy(poyg_cm3 *cm3);
+int poyg_cm3_get_cost(const poyg_cm3 *cm3, int a, int b, int c);
+int poyg_cm3_get_median(const poyg_cm3 *cm3, int a, int b, int c);
+
+/* ---- per-call alignment entry points (host pointers) ------------------ */
+/* algn_CAML_simple_2 (src/algn.c:3408-3427): s1 must be at least as long
+ * as s2; deltawh is the band parameter Sequence.Align.cost_2 computes. */
+int poyg_algn_simple_2(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int l1,
+                       const uint8_t *s2, int l2, int deltawh, int *cost);
+/* algn_CAML_align_2d (src/algn.c:3986-3997): cost + backtrack_2d. */
+int poyg_algn_align_2d(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int l1,
+                       const uint8_t *s2, int l2, int deltawh, int swaped,
+                       uint8_t *r1, uint8_t *r2, int cap, int *rlen, int *cost);
+/* algn_CAML_align_affine_3 (src/algn.c:2550-2614): lens = median, medianwg,
+ * resi, resj lengths; resi follows si whichever sequence is shorter. */
+int poyg_algn_align_affine_3(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *si, int li,
+                             const uint8_t *sj, int lj, uint8_t *median, uint8_t *medianwg,
+                             uint8_t *resi, uint8_t *resj, int cap, int *lens, int *cost);
+/* algn_CAML_cost_affine_3 (src/algn.c:2627-2680). */
+int poyg_algn_cost_affine_3(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *si, int li,
+                            const uint8_t *sj, int lj, int *cost);
+/* algn_CAML_align_3d (src/algn.c:4004-4012) with the cost of
+ * algn_CAML_simple_3 (:3457-3471); r1..r3 receive the aligned triple. */
+int poyg_algn_align_3d(poyg_ctx *ctx, const poyg_cm3 *cm3, const uint8_t *s1, int l1,
+                       const uint8_t *s2, int l2, const uint8_t *s3, int l3,
+                       uint8_t *r1, uint8_t *r2, uint8_t *r3, int cap, int *rlen, int *cost);
+/* algn_CAML_median_2_no_gaps / _with_gaps / ancestor_2 / union / median_3
+ * (src/algn.c:4186-4235, 4287-4299) on already aligned sequences. */
+int poyg_algn_median_2_no_gaps(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len,
+                               uint8_t *out, int cap, int *olen);
+int poyg_algn_median_2_with_gaps(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len,
+                                 uint8_t *out, int cap, int *olen);
+int poyg_algn_ancestor_2(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len,
+                         uint8_t *out, int cap, int *olen);
+int poyg_algn_union(const uint8_t *a, const uint8_t *b, int len, uint8_t *out, int cap, int *olen);
+int poyg_algn_median_3(const poyg_cm3 *cm3, const uint8_t *a, const uint8_t *b, const uint8_t *c,
+                       int len, uint8_t *out, int cap, int *olen);
+/* algn_CAML_worst_2 / algn_CAML_verify_2 (src/algn.c:3372-3406). */
+int poyg_algn_worst_2(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len, int *cost);
+int poyg_algn_verify_2(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len, int *cost);
+
+/* ---- batched pairwise alignment (device pointers) --------------------- */
+/* Modes of a pair plan. */
+#define POYG_MODE_LINEAR_COST 0   /* Sequence.Align.cost_2, linear matrices */
+#define POYG_MODE_LINEAR_ALIGN 1  /* align_2 + median_2 (Scripting.DNA.Align.algn) */
+#define POYG_MODE_AFFINE_COST 2   /* cost_2 on affine matrices (cost_affine_3) */
+#define POYG_MODE_AFFINE_ALIGN 3  /* align_affine_3 */
+#define POYG_MODE_TRIPLE 4        /* align_3 + median_3 (poyg_plan_triples) */
+
+/* Plans a batch of n pairs (a_k, b_k).  off/len arrays are HOST arrays that
+ * describe where each sequence lives inside the device buffer passed to
+ * poyg_plan_run; lengths include the leading gap.  deltaw (host, may be
+ * NULL) gives the ?deltaw argument of Sequence.Align.cost_2 per pair for the
+ * linear modes (value < 0 = absent); it is ignored for affine modes. */
+int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n,
+                    const int64_t *off_a, const int32_t *len_a, const int64_t *off_b,
+                    const int32_t *len_b, const int32_t *deltaw, poyg_plan **out);
+void poyg_plan_destroy(poyg_plan *plan);
+/* Output layout: pair k owns out[out_off[k] .. out_off[k] + slots*cap_k),
+ * cap_k = round16(len_a[k] + len_b[k] + 2), in `slots` consecutive sequences each
+ * right-aligned in its cap_k bytes; out_len[slots*k + s] is the length of
+ * sequence s.  LINEAR_ALIGN slots: aligned a, aligned b, median_2 (3);
+ * AFFINE_ALIGN: median, medianwg, resa, resb (4); COST modes: none. */
+int poyg_plan_slots(const poyg_plan *plan);
+int64_t poyg_plan_out_bytes(const poyg_plan *plan);
+int poyg_plan_out_offsets(const poyg_plan *plan, int64_t *out_off);
+/* DP cells the plan evaluates (band cells of the reference algorithm). */
+int64_t poyg_plan_cells(const poyg_plan *plan);
+/* Runs the plan on `stream` (hipStream_t, or NULL for the context stream).
+ * d_seq: device sequences; d_cost: n int32; d_out / d_out_len as above
+ * (ignored by cost modes). */
+int poyg_plan_run(poyg_plan *plan, const uint8_t *d_seq, int32_t *d_cost, uint8_t *d_out,
+                  int32_t *d_out_len, void *stream);
+/* Kernel timings of the last run (ms, measured with HIP events on the run's
+ * stream): fill kernel and backtrace kernel. */
+int poyg_plan_last_times(const poyg_plan *plan, float *fill_ms, float *trace_ms);
+int poyg_plan_set_timing(poyg_plan *plan, int enable);
+
+/* Number of backtraces of the last run that left the filled band (0 when
+ * every walk stayed inside it). */
+int poyg_plan_status(poyg_plan *plan, int32_t *bad);
+
+/* ---- batched three-sequence alignment --------------------------------- */
+/* Plans n triples for algn_CAML_align_3d + algn_CAML_median_3
+ * (src/algn.c:3457-3478, 4004-4017, 4223-4235).  Runs through
+ * poyg_plan_run; slots: aligned s1, s2, s3, median_3, each of capacity
+ * round16(l1+l2+l3+1).  Direction cubes are processed in batches whose size
+ * is bounded by POYG_CUBE_DM_BUDGET_MB (default 16384). */
+int poyg_plan_triples(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64_t *off_1,
+                      const int32_t *len_1, const int64_t *off_2, const int32_t *len_2,
+                      const int64_t *off_3, const int32_t *len_3, poyg_plan **out);
+
+/* ---- Fitch over packed nonadditive characters ------------------------- */
+/* nonadd_median (src/nonaddCSc.c:422-462) for n node pairs over `len` u8
+ * characters: res[k] = fitch(a[ia[k]], b[ib[k]]), added[k] = changes.
+ * Rows of `chars` are len bytes long; device pointers. */
+int poyg_fitch_down(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n,
+                    const int32_t *d_ia, const int32_t *d_ib, const int32_t *d_ires,
+                    int64_t *d_added, void *stream);
+/* nonadd_median_3 (src/nonaddCSc.c:472-544): F = uppass(A, N, D1, D2). */
+int poyg_fitch_up(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_iA,
+                  const int32_t *d_iN, const int32_t *d_iD1, const int32_t *d_iD2,
+                  const int32_t *d_iF, void *stream);
+/* char_nonadd_CAML_median / median_3 / distance on host arrays. */
+int poyg_nonadd_median(poyg_ctx *ctx, const uint8_t *a, const uint8_t *b, uint8_t *res, int len,
+                       int64_t *added);
+int poyg_nonadd_median_3(poyg_ctx *ctx, const uint8_t *A, const uint8_t *N, const uint8_t *D1,
+                         const uint8_t *D2, uint8_t *F, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,844 @@
+// C ABI of the engine: contexts, plans and the per-call entry points.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "poyg_internal.h"
+
+namespace poyg {
+hipError_t launch_reset(int *counters, int n, hipStream_t s);
+hipError_t launch_prep(bool affine, int grid, const PairMeta *meta, int n, const uint8_t *seq, const DevTcm *tcm,
+                       int *aux, hipStream_t s);
+hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
+                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
+                           int32_t *cost, int *rowscratch, int row_width, hipStream_t s);
+hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
+                            const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
+                            hipStream_t s);
+hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
+                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
+                           int32_t *cost, int *rowscratch, int row_width, hipStream_t s);
+hipError_t launch_lin_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
+                            const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
+                            hipStream_t s);
+hipError_t launch_fitch_down(uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
+                             const int32_t *ires, int64_t *added, hipStream_t s);
+hipError_t launch_fitch_up(uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
+                           const int32_t *iD1, const int32_t *iD2, const int32_t *iF, hipStream_t s);
+}  // namespace poyg
+
+extern "C" {
+int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64_t *off_1,
+                            const int32_t *len_1, const int64_t *off_2, const int32_t *len_2,
+                            const int64_t *off_3, const int32_t *len_3, int device, int cus, void *stream,
+                            int64_t dm_budget, void **out);
+void poyg_triple_plan_destroy(void *p);
+int64_t poyg_triple_plan_out_bytes(void *p);
+int64_t poyg_triple_plan_cells(void *p);
+int poyg_triple_plan_out_offsets(void *p, int64_t *o);
+int poyg_triple_plan_run(void *pp, const uint8_t *d_seq, int32_t *d_cost, uint8_t *d_out, int32_t *d_out_len,
+                         void *stream, float *fill_ms, float *trace_ms);
+int poyg_triple_plan_status(void *pp, int32_t *bad);
+}
+
+using namespace poyg;
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return POYG_EDEVICE;                                                         \
+    }                                                                              \
+  } while (0)
+
+struct poyg_ctx {
+  int device = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+};
+
+// A device buffer owned by a plan.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int alloc(size_t n) {
+    if (n == 0) n = 16;
+    if (hipMalloc(&p, n) != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc of %zu bytes failed", n);
+      return POYG_ENOMEM;
+    }
+    bytes = n;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+struct poyg_plan {
+  poyg_ctx *ctx = nullptr;
+  void *triple = nullptr;   // three-sequence plan (mode POYG_MODE_TRIPLE)
+  int mode = 0;
+  int n = 0;
+  int slots = 0;
+  int64_t cells = 0;
+  int64_t out_bytes = 0;
+  std::vector<int64_t> out_off;
+  // device
+  DevBuf tcm, meta, order_all, order_lds, order_glb, dm, aux, rows, counters, status;
+  int n_lds = 0, n_glb = 0;
+  int grid_lds = 0, grid_glb = 0, row_width = 0;
+  bool timing = false;
+  float fill_ms = 0.f, trace_ms = 0.f;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  ~poyg_plan() {
+    if (triple) poyg_triple_plan_destroy(triple);
+    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &dm, &aux, &rows, &counters, &status})
+      b->release();
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+static int64_t round16(int64_t v) { return (v + 15) & ~(int64_t)15; }
+
+extern "C" {
+
+int poyg_ctx_create(int device, poyg_ctx **out) {
+  if (!out) return POYG_EINVAL;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) {
+    set_error("poyg_ctx_create: device %d not present (%d devices)", device, ndev);
+    return POYG_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(device));
+  poyg_ctx *c = new poyg_ctx;
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    set_error("poyg_ctx_create: stream creation failed");
+    return POYG_EDEVICE;
+  }
+  *out = c;
+  return POYG_OK;
+}
+
+void poyg_ctx_destroy(poyg_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int poyg_ctx_sync(poyg_ctx *ctx) {
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return POYG_OK;
+}
+
+int poyg_dev_alloc(poyg_ctx *ctx, int64_t bytes, void **out) {
+  if (!ctx || !out || bytes < 0) return POYG_EINVAL;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMalloc(out, (size_t)(bytes ? bytes : 16)));
+  return POYG_OK;
+}
+int poyg_dev_free(poyg_ctx *ctx, void *p) {
+  if (!ctx) return POYG_EINVAL;
+  if (p) HIP_TRY(hipFree(p));
+  return POYG_OK;
+}
+int poyg_dev_upload(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes) {
+  if (!ctx || bytes < 0) return POYG_EINVAL;
+  if (bytes == 0) return POYG_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return POYG_OK;
+}
+int poyg_dev_download(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes) {
+  if (!ctx || bytes < 0) return POYG_EINVAL;
+  if (bytes == 0) return POYG_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return POYG_OK;
+}
+void *poyg_ctx_stream(poyg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+void poyg_plan_destroy(poyg_plan *plan) { delete plan; }
+int poyg_plan_slots(const poyg_plan *p) { return p->slots; }
+int64_t poyg_plan_out_bytes(const poyg_plan *p) {
+  return p->triple ? poyg_triple_plan_out_bytes(p->triple) : p->out_bytes;
+}
+int64_t poyg_plan_cells(const poyg_plan *p) { return p->triple ? poyg_triple_plan_cells(p->triple) : p->cells; }
+int poyg_plan_out_offsets(const poyg_plan *p, int64_t *o) {
+  if (!o) return POYG_EINVAL;
+  if (p->triple) return poyg_triple_plan_out_offsets(p->triple, o);
+  memcpy(o, p->out_off.data(), sizeof(int64_t) * p->out_off.size());
+  return POYG_OK;
+}
+
+int poyg_plan_triples(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64_t *off_1, const int32_t *len_1,
+                      const int64_t *off_2, const int32_t *len_2, const int64_t *off_3, const int32_t *len_3,
+                      poyg_plan **out) {
+  if (!ctx || !out) return POYG_EINVAL;
+  const char *b = getenv("POYG_CUBE_DM_BUDGET_MB");
+  const int64_t budget = (b ? atoll(b) : 16384) << 20;
+  void *tp = nullptr;
+  int rc = poyg_triple_plan_create(ctx, cm3, n, off_1, len_1, off_2, len_2, off_3, len_3, ctx->device, ctx->cus,
+                                   ctx->stream, budget, &tp);
+  if (rc) return rc;
+  poyg_plan *P = new poyg_plan;
+  P->ctx = ctx;
+  P->triple = tp;
+  P->mode = POYG_MODE_TRIPLE;
+  P->n = n;
+  P->slots = 4;
+  *out = P;
+  return POYG_OK;
+}
+int poyg_plan_set_timing(poyg_plan *p, int enable) {
+  p->timing = enable != 0;
+  if (p->timing && !p->ev[0])
+    for (auto &e : p->ev) HIP_TRY(hipEventCreate(&e));
+  return POYG_OK;
+}
+int poyg_plan_last_times(const poyg_plan *p, float *fill_ms, float *trace_ms) {
+  if (fill_ms) *fill_ms = p->fill_ms;
+  if (trace_ms) *trace_ms = p->trace_ms;
+  return POYG_OK;
+}
+
+}  // extern "C"
+
+// Linear band geometry of algn_fill_plane_2 (src/algn.c:872-968).
+struct LinGeom {
+  int cs, w, h;
+};
+static LinGeom lin_geom(int m, int n, int dw) {
+  LinGeom g;
+  g.w = std::min(n, 50 + dw);
+  g.h = std::min(m, (m - n) + 50 + dw);
+  if ((float)m >= (float)((float)((float)3 / (float)2) * (float)n)) g.cs = kLinFull;
+  else if (2 * g.h < m) g.cs = kLinCase2;
+  else if (8 >= m - g.h) g.cs = kLinFull;
+  else g.cs = kLinCase3;
+  return g;
+}
+static void lin_row_host(int i, int m, int n, const LinGeom &g, int &lo, int &hi) {
+  if (g.cs == kLinFull) { lo = 0; hi = n - 1; return; }
+  if (i == 0) { lo = 0; hi = g.w - 1; return; }
+  const int w = g.w, h = g.h;
+  if (g.cs == kLinCase2) {
+    if (i < h) { lo = 0; hi = w + i - 1; }
+    else if (i < m - (h - 1)) { lo = 1 + (i - h); hi = lo + w + h - 2; }
+    else { lo = n - (w + h - 2) + (i - (m - (h - 1))); hi = n - 1; }
+    return;
+  }
+  if (i < n - w + 1) { lo = 0; hi = w + i - 1; }
+  else if (i < m - (n - w) + 1) { lo = 0; hi = n - 1; }
+  else { lo = 1 + (i - (m - (n - w) + 1)); hi = n - 1; }
+}
+
+static constexpr int kAffRBHost = 256;   // must match kAffRB in the kernels
+static constexpr int kLinRBHost = 512;   // must match kLinRB
+
+extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, const int64_t *off_a,
+                               const int32_t *len_a, const int64_t *off_b, const int32_t *len_b,
+                               const int32_t *deltaw, poyg_plan **out) {
+  if (!ctx || !cm || !out || n < 0 || (n > 0 && (!off_a || !len_a || !off_b || !len_b))) {
+    set_error("poyg_plan_pairs: bad arguments");
+    return POYG_EINVAL;
+  }
+  const bool affine = mode == POYG_MODE_AFFINE_COST || mode == POYG_MODE_AFFINE_ALIGN;
+  const bool align = mode == POYG_MODE_AFFINE_ALIGN || mode == POYG_MODE_LINEAR_ALIGN;
+  if (mode < 0 || mode > 3) return POYG_EINVAL;
+  if (affine != (cm->cost_model == 1)) {
+    set_error("poyg_plan_pairs: mode %d does not match the cost model %d of the matrix", mode, cm->cost_model);
+    return POYG_EINVAL;
+  }
+  if (!cm->combinations || cm->lcm != 5 || cm->gap != kGap) {
+    set_error("poyg_plan_pairs: only 5-bit DNA combination matrices are supported");
+    return POYG_EUNSUPPORTED;
+  }
+  if (!affine && !deltaw) {
+    set_error("poyg_plan_pairs: linear modes need the deltawh of every pair");
+    return POYG_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  poyg_plan *P = new poyg_plan;
+  P->ctx = ctx;
+  P->mode = mode;
+  P->n = n;
+  P->slots = align ? (affine ? 4 : 3) : 0;
+  std::vector<PairMeta> meta(n);
+  std::vector<int64_t> cells(n);
+  std::vector<char> glb(n, 0);
+  int64_t dm_bytes = 0, aux_ints = 0, out_bytes = 0;
+  int row_width = 0;
+  P->out_off.resize(n);
+  for (int k = 0; k < n; ++k) {
+    PairMeta &m = meta[k];
+    const int la = len_a[k], lb = len_b[k];
+    if (la < 1 || lb < 1) {
+      delete P;
+      set_error("poyg_plan_pairs: pair %d has an empty sequence (the leading gap is required)", k);
+      return POYG_EINVAL;
+    }
+    memset(&m, 0, sizeof m);
+    m.cap = (int32_t)round16((int64_t)la + lb + 2);
+    m.out_off = out_bytes;
+    P->out_off[k] = out_bytes;
+    out_bytes += (int64_t)P->slots * m.cap;
+    if (affine) {
+      // algn_CAML_align_affine_3: the shorter sequence indexes the rows
+      m.swap = la > lb;
+      m.off_r = m.swap ? off_b[k] : off_a[k];
+      m.off_c = m.swap ? off_a[k] : off_b[k];
+      m.lr = m.swap ? lb : la;
+      m.lc = m.swap ? la : lb;
+      const int ra = m.lr - 1, cb = m.lc - 1;
+      const int e0 = std::min(cb, std::max(40, cb - ra + 8));
+      m.p0 = e0;
+      m.stride = (int32_t)((std::min(cb + 1, e0 + 41) + 3) & ~3);
+      m.aux_off = aux_ints;
+      aux_ints += (2 * (int64_t)(cb + 1) + 63) & ~(int64_t)63;   // 256-byte aligned regions
+      if (align) {
+        m.dm_off = dm_bytes;
+        dm_bytes += (int64_t)ra * m.stride;
+      }
+      int64_t c = 0;
+      for (int i = 1; i <= ra; ++i) {
+        const int sp = 1 + (i > 40 ? i - 40 : 0), ep = std::min(cb, e0 + i - 1);
+        c += ep - sp + 1;
+      }
+      cells[k] = c;
+      if (m.stride > kAffRBHost) {
+        glb[k] = 1;
+        row_width = std::max(row_width, (cb + 2 + 63) & ~63);
+      }
+    } else {
+      // Sequence.Align.cost_2 / create_edited_2: the longer sequence goes first
+      m.swap = la < lb;
+      m.off_r = m.swap ? off_b[k] : off_a[k];
+      m.off_c = m.swap ? off_a[k] : off_b[k];
+      m.lr = m.swap ? lb : la;
+      m.lc = m.swap ? la : lb;
+      const LinGeom g = lin_geom(m.lr, m.lc, deltaw[k]);
+      m.p0 = g.cs;
+      m.p1 = g.w;
+      m.p2 = g.h;
+      int maxw = 0;
+      int64_t c = 0;
+      for (int i = 0; i < m.lr; ++i) {
+        int lo, hi;
+        lin_row_host(i, m.lr, m.lc, g, lo, hi);
+        maxw = std::max(maxw, hi - lo + 1);
+        c += hi - lo + 1;
+      }
+      cells[k] = c;
+      m.stride = (maxw + 3) & ~3;
+      m.aux_off = aux_ints;
+      aux_ints += (2 * (int64_t)m.lc + 63) & ~(int64_t)63;
+      if (align) {
+        m.dm_off = dm_bytes;
+        dm_bytes += (int64_t)m.lr * m.stride;
+      }
+      if (maxw > kLinRBHost) {
+        glb[k] = 1;
+        row_width = std::max(row_width, (m.lc + 1 + 63) & ~63);
+      }
+    }
+    P->cells += cells[k];
+  }
+  P->out_bytes = out_bytes;
+  // longest-first work lists
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a] > cells[b]; });
+  std::vector<int32_t> o_lds, o_glb;
+  for (int k : order) (glb[k] ? o_glb : o_lds).push_back(k);
+  P->n_lds = (int)o_lds.size();
+  P->n_glb = (int)o_glb.size();
+  P->row_width = row_width;
+  const int per_cu = affine ? 4 : 8;
+  P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
+  P->grid_glb = std::max(1, std::min(P->n_glb, 256));
+  DevTcm dt;
+  to_dev_tcm(cm, &dt);
+  int rc = 0;
+  rc |= P->tcm.alloc(sizeof(DevTcm));
+  rc |= P->meta.alloc(sizeof(PairMeta) * (size_t)n);
+  rc |= P->order_all.alloc(sizeof(int32_t) * (size_t)n);
+  rc |= P->order_lds.alloc(sizeof(int32_t) * (size_t)P->n_lds);
+  rc |= P->order_glb.alloc(sizeof(int32_t) * (size_t)P->n_glb);
+  rc |= P->dm.alloc((size_t)dm_bytes);
+  rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
+  rc |= P->rows.alloc(P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16);
+  rc |= P->counters.alloc(sizeof(int) * 4);
+  rc |= P->status.alloc(sizeof(int32_t) * 4);
+  if (rc) {
+    delete P;
+    return POYG_ENOMEM;
+  }
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipSuccess;
+  e = hipMemcpyAsync(P->tcm.p, &dt, sizeof dt, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(P->meta.p, meta.data(), sizeof(PairMeta) * n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(P->order_all.p, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && P->n_lds)
+    e = hipMemcpyAsync(P->order_lds.p, o_lds.data(), sizeof(int32_t) * P->n_lds, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && P->n_glb)
+    e = hipMemcpyAsync(P->order_glb.p, o_glb.data(), sizeof(int32_t) * P->n_glb, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete P;
+    set_error("poyg_plan_pairs: upload failed: %s", hipGetErrorString(e));
+    return POYG_EDEVICE;
+  }
+  *out = P;
+  return POYG_OK;
+}
+
+extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost, uint8_t *d_out,
+                             int32_t *d_out_len, void *stream) {
+  if (!P || !d_seq || !d_cost) return POYG_EINVAL;
+  if (P->n == 0) return POYG_OK;
+  if (P->triple) {
+    float *f = P->timing ? &P->fill_ms : nullptr, *t = P->timing ? &P->trace_ms : nullptr;
+    return poyg_triple_plan_run(P->triple, d_seq, d_cost, d_out, d_out_len,
+                                stream ? stream : (void *)P->ctx->stream, f, t);
+  }
+  const bool affine = P->mode >= POYG_MODE_AFFINE_COST;
+  const bool align = P->mode == POYG_MODE_AFFINE_ALIGN || P->mode == POYG_MODE_LINEAR_ALIGN;
+  if (align && (!d_out || !d_out_len)) {
+    set_error("poyg_plan_run: alignment modes need output buffers");
+    return POYG_EINVAL;
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : P->ctx->stream;
+  int *ctr = (int *)P->counters.p;
+  const PairMeta *meta = (const PairMeta *)P->meta.p;
+  const DevTcm *tcm = (const DevTcm *)P->tcm.p;
+  if (P->timing) HIP_TRY(hipEventRecord(P->ev[0], s));
+  HIP_TRY(launch_reset(ctr, 4, s));
+  HIP_TRY(launch_reset((int *)P->status.p, 4, s));
+  HIP_TRY(launch_prep(affine, std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * 8)), meta, P->n, d_seq, tcm,
+                      (int *)P->aux.p, s));
+  if (affine) {
+    if (P->n_lds)
+      HIP_TRY(launch_aff_fill(align, false, P->grid_lds, meta, (const int32_t *)P->order_lds.p, P->n_lds, ctr,
+                              d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, nullptr, 0, s));
+    if (P->n_glb)
+      HIP_TRY(launch_aff_fill(align, true, P->grid_glb, meta, (const int32_t *)P->order_glb.p, P->n_glb, ctr + 1,
+                              d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, (int *)P->rows.p,
+                              P->row_width, s));
+  } else {
+    if (P->n_lds)
+      HIP_TRY(launch_lin_fill(align, false, P->grid_lds, meta, (const int32_t *)P->order_lds.p, P->n_lds, ctr,
+                              d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, nullptr, 0, s));
+    if (P->n_glb)
+      HIP_TRY(launch_lin_fill(align, true, P->grid_glb, meta, (const int32_t *)P->order_glb.p, P->n_glb, ctr + 1,
+                              d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, (int *)P->rows.p,
+                              P->row_width, s));
+  }
+  if (P->timing) HIP_TRY(hipEventRecord(P->ev[1], s));
+  if (align) {
+    if (affine)
+      HIP_TRY(launch_aff_trace(meta, (const int32_t *)P->order_all.p, P->n, d_seq, tcm, (const uint8_t *)P->dm.p,
+                               d_out, d_out_len, (int32_t *)P->status.p, s));
+    else
+      HIP_TRY(launch_lin_trace(meta, (const int32_t *)P->order_all.p, P->n, d_seq, tcm, (const uint8_t *)P->dm.p,
+                               d_out, d_out_len, (int32_t *)P->status.p, s));
+  }
+  if (P->timing) {
+    HIP_TRY(hipEventRecord(P->ev[2], s));
+    HIP_TRY(hipEventSynchronize(P->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&P->fill_ms, P->ev[0], P->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&P->trace_ms, P->ev[1], P->ev[2]));
+  }
+  return POYG_OK;
+}
+
+// Number of backtraces of the last run that left the band (0 when sane).
+extern "C" int poyg_plan_status(poyg_plan *P, int32_t *bad) {
+  if (P->triple) return poyg_triple_plan_status(P->triple, bad);
+  HIP_TRY(hipMemcpyAsync(bad, P->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, P->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(P->ctx->stream));
+  return POYG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-call entry points: a one-pair plan over host data.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct OnePair {
+  DevBuf seq, cost, out, len;
+  std::vector<uint8_t> h_out;
+  std::vector<int32_t> h_len;
+  int32_t h_cost = 0;
+  poyg_plan *plan = nullptr;
+  ~OnePair() {
+    seq.release(); cost.release(); out.release(); len.release();
+    poyg_plan_destroy(plan);
+  }
+  int run(poyg_ctx *ctx, const poyg_cm *cm, int mode, const uint8_t *a, int la, const uint8_t *b, int lb,
+          int dw) {
+    if (!a || !b || la < 1 || lb < 1) {
+      set_error("sequence arguments must be non-empty (leading gap included)");
+      return POYG_EINVAL;
+    }
+    int64_t offa = 0, offb = round16(la);
+    int rc = poyg_plan_pairs(ctx, cm, mode, 1, &offa, &la, &offb, &lb, &dw, &plan);
+    if (rc) return rc;
+    if (seq.alloc((size_t)(offb + lb)) || cost.alloc(4)) return POYG_ENOMEM;
+    const int slots = plan->slots;
+    if (slots && (out.alloc((size_t)plan->out_bytes) || len.alloc(sizeof(int32_t) * slots))) return POYG_ENOMEM;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync((uint8_t *)seq.p + offa, a, la, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync((uint8_t *)seq.p + offb, b, lb, hipMemcpyHostToDevice, s));
+    rc = poyg_plan_run(plan, (const uint8_t *)seq.p, (int32_t *)cost.p, (uint8_t *)out.p, (int32_t *)len.p, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(&h_cost, cost.p, 4, hipMemcpyDeviceToHost, s));
+    if (slots) {
+      h_out.resize(plan->out_bytes);
+      h_len.resize(slots);
+      HIP_TRY(hipMemcpyAsync(h_out.data(), out.p, plan->out_bytes, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(h_len.data(), len.p, sizeof(int32_t) * slots, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    int32_t bad = 0;
+    poyg_plan_status(plan, &bad);
+    if (bad) {
+      set_error("backtrace left the filled band");
+      return POYG_EBAND;
+    }
+    return POYG_OK;
+  }
+  // copy slot k right-aligned into dst (capacity cap)
+  int take(int k, uint8_t *dst, int cap, int *olen) {
+    const int c = (int)plan->out_bytes / plan->slots;
+    const int l = h_len[k];
+    if (l > cap) {
+      set_error("output capacity %d is smaller than the result (%d)", cap, l);
+      return POYG_EINVAL;
+    }
+    memcpy(dst + cap - l, h_out.data() + (size_t)k * c + (c - l), l);
+    if (olen) *olen = l;
+    return POYG_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int poyg_algn_simple_2(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int l1, const uint8_t *s2, int l2,
+                       int deltawh, int *cost) {
+  if (l1 < l2) {
+    set_error("poyg_algn_simple_2: s1 must be at least as long as s2");
+    return POYG_EINVAL;
+  }
+  OnePair op;
+  int rc = op.run(ctx, cm, POYG_MODE_LINEAR_COST, s1, l1, s2, l2, deltawh);
+  if (!rc && cost) *cost = op.h_cost;
+  return rc;
+}
+
+int poyg_algn_align_2d(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int l1, const uint8_t *s2, int l2,
+                       int deltawh, int swaped, uint8_t *r1, uint8_t *r2, int cap, int *rlen, int *cost) {
+  // A pair plan puts the longer sequence on the rows and backtracks with
+  // swaped = (rows sequence is the caller's first), as
+  // Sequence.Align.create_edited_2 does.  The stub's (s1 longer, swaped)
+  // maps onto a plan of (s1, s2) when swaped, else of (s2, s1).
+  if (l1 < l2) {
+    set_error("poyg_algn_align_2d: s1 must be at least as long as s2");
+    return POYG_EINVAL;
+  }
+  if (!swaped && l1 == l2) {
+    set_error("poyg_algn_align_2d: swaped = false on equal lengths is not supported");
+    return POYG_EUNSUPPORTED;
+  }
+  OnePair op;
+  int rc = swaped ? op.run(ctx, cm, POYG_MODE_LINEAR_ALIGN, s1, l1, s2, l2, deltawh)
+                  : op.run(ctx, cm, POYG_MODE_LINEAR_ALIGN, s2, l2, s1, l1, deltawh);
+  if (rc) return rc;
+  if (cost) *cost = op.h_cost;
+  rc = op.take(swaped ? 0 : 1, r1, cap, rlen);
+  if (!rc) rc = op.take(swaped ? 1 : 0, r2, cap, nullptr);
+  return rc;
+}
+
+int poyg_algn_align_affine_3(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *si, int li, const uint8_t *sj,
+                             int lj, uint8_t *median, uint8_t *medianwg, uint8_t *resi, uint8_t *resj, int cap,
+                             int *lens, int *cost) {
+  OnePair op;
+  int rc = op.run(ctx, cm, POYG_MODE_AFFINE_ALIGN, si, li, sj, lj, 0);
+  if (rc) return rc;
+  if (cost) *cost = op.h_cost;
+  uint8_t *dst[4] = {median, medianwg, resi, resj};
+  for (int k = 0; k < 4 && !rc; ++k) rc = op.take(k, dst[k], cap, lens ? lens + k : nullptr);
+  return rc;
+}
+
+int poyg_algn_cost_affine_3(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *si, int li, const uint8_t *sj,
+                            int lj, int *cost) {
+  OnePair op;
+  int rc = op.run(ctx, cm, POYG_MODE_AFFINE_COST, si, li, sj, lj, 0);
+  if (!rc && cost) *cost = op.h_cost;
+  return rc;
+}
+
+int poyg_algn_align_3d(poyg_ctx *ctx, const poyg_cm3 *cm3, const uint8_t *s1, int l1, const uint8_t *s2, int l2,
+                       const uint8_t *s3, int l3, uint8_t *r1, uint8_t *r2, uint8_t *r3, int cap, int *rlen,
+                       int *cost) {
+  if (!ctx || !cm3 || !s1 || !s2 || !s3 || l1 < 1 || l2 < 1 || l3 < 1) return POYG_EINVAL;
+  int64_t o1 = 0, o2 = round16(l1), o3 = o2 + round16(l2);
+  poyg_plan *P = nullptr;
+  int rc = poyg_plan_triples(ctx, cm3, 1, &o1, &l1, &o2, &l2, &o3, &l3, &P);
+  if (rc) return rc;
+  DevBuf seq, dcost, out, len;
+  const int64_t ob = poyg_plan_out_bytes(P);
+  if (seq.alloc((size_t)(o3 + l3)) || dcost.alloc(4) || out.alloc((size_t)ob) || len.alloc(16)) {
+    poyg_plan_destroy(P);
+    return POYG_ENOMEM;
+  }
+  hipStream_t s = ctx->stream;
+  std::vector<uint8_t> h_out(ob);
+  int32_t h_len[4] = {0, 0, 0, 0}, h_cost = 0, bad = 0;
+  hipError_t e = hipMemcpyAsync((uint8_t *)seq.p + o1, s1, l1, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync((uint8_t *)seq.p + o2, s2, l2, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync((uint8_t *)seq.p + o3, s3, l3, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    rc = poyg_plan_run(P, (const uint8_t *)seq.p, (int32_t *)dcost.p, (uint8_t *)out.p, (int32_t *)len.p, s);
+    if (rc == POYG_OK) {
+      e = hipMemcpyAsync(&h_cost, dcost.p, 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(h_out.data(), out.p, ob, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(h_len, len.p, 16, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e == hipSuccess) rc = poyg_plan_status(P, &bad);
+    }
+  }
+  const int c4 = (int)(ob / 4);
+  if (e != hipSuccess) {
+    set_error("align_3d: %s", hipGetErrorString(e));
+    rc = POYG_EDEVICE;
+  } else if (rc == POYG_OK && bad) {
+    set_error("backtrack_3d left the cube");
+    rc = POYG_EBAND;
+  } else if (rc == POYG_OK && h_len[0] > cap) {
+    rc = POYG_EINVAL;
+  } else if (rc == POYG_OK) {
+    uint8_t *dst[3] = {r1, r2, r3};
+    for (int q = 0; q < 3; ++q) memcpy(dst[q] + cap - h_len[q], h_out.data() + (size_t)q * c4 + (c4 - h_len[q]), h_len[q]);
+    if (rlen) *rlen = h_len[0];
+    if (cost) *cost = h_cost;
+  }
+  seq.release(); dcost.release(); out.release(); len.release();
+  poyg_plan_destroy(P);
+  return rc;
+}
+
+// ---- host-side column rules (O(L) per call) ------------------------------
+static int put_rev(uint8_t *out, int cap, int &n, int v) {
+  if (n >= cap) return POYG_EINVAL;
+  out[cap - ++n] = (uint8_t)v;
+  return 0;
+}
+
+int poyg_algn_median_2_no_gaps(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len, uint8_t *out,
+                               int cap, int *olen) {
+  int n = 0;
+  for (int i = len - 1; i >= 0; --i) {
+    const int m = cm->median[(a[i] << cm->lcm) + b[i]];
+    if (m != cm->gap && put_rev(out, cap, n, m)) return POYG_EINVAL;
+  }
+  if (put_rev(out, cap, n, cm->gap)) return POYG_EINVAL;
+  *olen = n;
+  return POYG_OK;
+}
+
+int poyg_algn_median_2_with_gaps(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len, uint8_t *out,
+                                 int cap, int *olen) {
+  int n = 0;
+  for (int i = len - 1; i >= 0; --i)
+    if (put_rev(out, cap, n, cm->median[(a[i] << cm->lcm) + b[i]])) return POYG_EINVAL;
+  *olen = n;
+  return POYG_OK;
+}
+
+int poyg_algn_union(const uint8_t *a, const uint8_t *b, int len, uint8_t *out, int cap, int *olen) {
+  int n = 0;
+  for (int i = len - 1; i >= 0; --i)
+    if (put_rev(out, cap, n, a[i] | b[i])) return POYG_EINVAL;
+  *olen = n;
+  return POYG_OK;
+}
+
+// algn_ancestor_2 (src/algn.c:4126-4147) with algn_correct_blocks_affine
+// (:4080-4119) and algn_remove_gaps (:4058-4077).
+int poyg_algn_ancestor_2(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len, uint8_t *out, int cap,
+                         int *olen) {
+  const int gap = cm->gap;
+  const bool keep_all = cm->combinations && cm->cost_model == 1;
+  int n = 0;
+  for (int i = len - 1; i >= 0; --i) {
+    const int m = cm->median[(a[i] << cm->lcm) + b[i]];
+    if ((keep_all || m != gap) && put_rev(out, cap, n, m)) return POYG_EINVAL;
+  }
+  if (!keep_all) {
+    if (n == 0 || out[cap - n] != gap)
+      if (put_rev(out, cap, n, gap)) return POYG_EINVAL;
+    *olen = n;
+    return POYG_OK;
+  }
+  uint8_t *s = out + cap - n;
+  int extending = 0, inside = 0, prev_block = 0;
+  for (int i = 0; i < n; ++i) {
+    const int ab = a[i], bb = b[i];
+    int sb = s[i];
+    if (!inside && (!(ab & gap) || !(bb & gap))) inside = 0;
+    else if (inside && (!(ab & gap) || !(bb & gap))) inside = 0;
+    else if (((ab & gap) || (bb & gap)) && ((ab != gap) || (bb != gap))) inside = 1;
+    else inside = 0;
+    if (((gap & ab) || (gap & bb)) && !(sb & gap) && !extending) {
+      prev_block = inside;
+      extending = 1;
+    } else if ((gap & ab) && (gap & bb) && (sb & gap) && (sb != gap) && extending && inside && !prev_block) {
+      sb = (~gap) & sb;
+      prev_block = 0;
+    } else if ((gap & ab) && (gap & bb) && extending == 1) {
+      prev_block = inside;
+      extending = 0;
+    }
+    s[i] = (uint8_t)sb;
+  }
+  int src = cap - 1, dst = cap - 1, kept = 0;
+  for (int i = n - 1; i >= 0; --i, --src)
+    if (out[src] != gap) { out[dst--] = out[src]; ++kept; }
+  if (put_rev(out, cap, kept, gap)) return POYG_EINVAL;
+  *olen = kept;
+  return POYG_OK;
+}
+
+int poyg_algn_median_3(const poyg_cm3 *cm3, const uint8_t *a, const uint8_t *b, const uint8_t *c, int len,
+                       uint8_t *out, int cap, int *olen) {
+  // algn_get_median_3d (src/algn.c:4160-4173) never advances its end
+  // pointers: every position gets the median of the last column.
+  if (len > cap) return POYG_EINVAL;
+  const int m = len > 0 ? cm3->median[(((a[len - 1] << cm3->lcm) + b[len - 1]) << cm3->lcm) + c[len - 1]] : 0;
+  for (int k = 1; k <= len; ++k) out[cap - k] = (uint8_t)m;
+  *olen = len;
+  return POYG_OK;
+}
+
+static int calc_aligned(const poyg_cm *c, const uint8_t *s1, const uint8_t *s2, int len, const int *tab) {
+  // algn_calculate_from_2_aligned, src/algn.c:3316-3370
+  int res = 0, gap_row = 0;
+  const int gap = c->gap, go = c->gap_open;
+  const bool comb = c->combinations != 0;
+  if (len == 0) return 0;
+  int i = ((comb && (gap & s1[0]) && (gap & s2[0])) || (!comb && gap == s1[0] && gap == s2[0])) ? 1 : 0;
+  for (; i < len; ++i) {
+    const int x = s1[i], y = s2[i];
+    if (gap_row == 0) {
+      if ((comb && (x & gap) && !(y & gap)) || (!comb && x == gap)) { res += go; gap_row = 1; }
+      else if ((comb && (y & gap) && !(x & gap)) || (!comb && y == gap)) { res += go; gap_row = 2; }
+    } else if (gap_row == 1) {
+      if ((comb && !(x & gap)) || (!comb && x != gap)) {
+        if ((comb && (y & gap) && !(x & gap)) || (!comb && y == gap)) { res += go; gap_row = 2; }
+        else gap_row = 0;
+      }
+    } else {
+      if ((comb && !(y & gap)) || (!comb && y != gap)) {
+        if ((comb && (x & gap)) || (!comb && x == gap)) { res += go; gap_row = 1; }
+        else gap_row = 0;
+      }
+    }
+    res += tab[(x << c->lcm) + y];
+  }
+  return res;
+}
+
+int poyg_algn_worst_2(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len, int *cost) {
+  *cost = calc_aligned(cm, a, b, len, cm->worst);
+  return POYG_OK;
+}
+int poyg_algn_verify_2(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len, int *cost) {
+  *cost = calc_aligned(cm, a, b, len, cm->cost);
+  return POYG_OK;
+}
+
+// ---- Fitch ---------------------------------------------------------------
+int poyg_fitch_down(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_ia,
+                    const int32_t *d_ib, const int32_t *d_ires, int64_t *d_added, void *stream) {
+  if (!ctx || n < 0 || len < 0) return POYG_EINVAL;
+  if (n == 0) return POYG_OK;
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  HIP_TRY(launch_fitch_down(d_chars, len, n, d_ia, d_ib, d_ires, d_added, s));
+  return POYG_OK;
+}
+
+int poyg_fitch_up(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_iA,
+                  const int32_t *d_iN, const int32_t *d_iD1, const int32_t *d_iD2, const int32_t *d_iF,
+                  void *stream) {
+  if (!ctx || n < 0 || len < 0) return POYG_EINVAL;
+  if (n == 0) return POYG_OK;
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  HIP_TRY(launch_fitch_up(d_chars, len, n, d_iA, d_iN, d_iD1, d_iD2, d_iF, s));
+  return POYG_OK;
+}
+
+static int fitch_host(poyg_ctx *ctx, const uint8_t *const *in, int nin, uint8_t *res, int len, int64_t *added) {
+  DevBuf chars, idx, acc;
+  const int rows = nin + 1;
+  if (chars.alloc((size_t)rows * len) || idx.alloc(sizeof(int32_t) * 8) || acc.alloc(sizeof(int64_t))) {
+    chars.release(); idx.release(); acc.release();
+    return POYG_ENOMEM;
+  }
+  hipStream_t s = ctx->stream;
+  int rc = POYG_OK;
+  int32_t h_idx[8] = {0, 1, 2, 3, 4, 0, 0, 0};
+  int64_t zero = 0;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < nin && e == hipSuccess; ++k)
+    e = hipMemcpyAsync((uint8_t *)chars.p + (size_t)k * len, in[k], len, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(idx.p, h_idx, sizeof h_idx, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(acc.p, &zero, sizeof zero, hipMemcpyHostToDevice, s);
+  const int32_t *I = (const int32_t *)idx.p;
+  if (e == hipSuccess) {
+    if (nin == 2) e = launch_fitch_down((uint8_t *)chars.p, len, 1, I, I + 1, I + 2, (int64_t *)acc.p, s);
+    else e = launch_fitch_up((uint8_t *)chars.p, len, 1, I, I + 1, I + 2, I + 3, I + 4, s);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(res, (uint8_t *)chars.p + (size_t)nin * len, len, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && added) e = hipMemcpyAsync(added, acc.p, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    set_error("fitch: %s", hipGetErrorString(e));
+    rc = POYG_EDEVICE;
+  }
+  chars.release(); idx.release(); acc.release();
+  return rc;
+}
+
+int poyg_nonadd_median(poyg_ctx *ctx, const uint8_t *a, const uint8_t *b, uint8_t *res, int len, int64_t *added) {
+  const uint8_t *in[2] = {a, b};
+  return fitch_host(ctx, in, 2, res, len, added);
+}
+
+int poyg_nonadd_median_3(poyg_ctx *ctx, const uint8_t *A, const uint8_t *N, const uint8_t *D1, const uint8_t *D2,
+                         uint8_t *F, int len) {
+  const uint8_t *in[4] = {A, N, D1, D2};
+  return fitch_host(ctx, in, 4, F, len, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,896 @@
+// HIP kernels of the direct-optimization engine for gfx950 (MI355X).
+//
+// Pairwise band DP: one 64-lane wavefront aligns one pair at a time, taking
+// pairs from a length-sorted work list through an atomic counter
+// (persistent waves).  A row of the band is processed in chunks of 64
+// columns, one column per lane.  Everything a cell needs from the previous
+// row lives in two LDS row buffers (ring-indexed by column); the only
+// dependency inside a row -- the horizontal gap chain -- is a min-plus
+// recurrence x[j] = min(x[j-1] + a[j], b[j]) that the wave evaluates as a
+// prefix minimum over b[j] - A[j], with A the per-pair prefix sum of a,
+// precomputed once per pair.  All arithmetic is int32, exactly the
+// comparisons of the reference, so costs and tie masks are bit-identical.
+//
+// Direction bands are stored one byte per cell, row by row, starting at the
+// first filled column of the row; the backtraces run one thread per pair.
+#include <hip/hip_runtime.h>
+
+#include "poyg_internal.h"
+
+namespace poyg {
+
+constexpr int kHigh = kHighNum;
+constexpr int kBig = 0x3fffffff;
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Cross-lane operations use DPP (VALU) and v_readlane only: lane i receives
+// lane i-1's value (lane 0 receives `fill`).
+__device__ __forceinline__ int shfl_up1(int v, int fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+}
+
+// Inclusive prefix minimum over the 64 lanes: Hillis-Steele inside 16-lane
+// rows (row_shr 1, 2, 4, 8), then row_bcast:15 and row_bcast:31 carry the
+// row totals upward.  Disabled DPP lanes return the identity.
+__device__ __forceinline__ int wave_prefix_min(int v) {
+  constexpr int I = 0x7fffffff;
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x111, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x112, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x114, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x118, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x142, 0xa, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x143, 0xc, 0xf, false));
+  return v;
+}
+
+__device__ __forceinline__ int wave_prefix_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
+__device__ __forceinline__ int lane63(int v) { return __builtin_amdgcn_readlane(v, kWave - 1); }
+
+// Orders this wave's LDS traffic across lanes: every LDS access issued
+// before the call has completed (s_waitcnt lgkmcnt(0)) before any access
+// after it, and the compiler may not move memory operations across it.
+// Used between DP rows, where lane j reads what lane j-1 wrote.
+__device__ __forceinline__ void wave_sync_mem() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);   // gfx9 encoding of lgkmcnt(0), other counters unchanged
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ int next_item(int *counter) {
+  int v = 0;
+  if (lane_id() == 0) v = atomicAdd(counter, 1);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+__global__ void reset_counters_kernel(int *c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+// Right-to-left byte writer that stores whole dwords (slot bases and
+// capacities are multiples of 16 bytes, so a final partial dword only
+// touches unused capacity of the same slot).
+struct RevWriter {
+  uint8_t *base;
+  int pos;
+  uint32_t acc;
+  __device__ void init(uint8_t *b, int cap) { base = b; pos = cap; acc = 0; }
+  __device__ __forceinline__ void put(int v) {
+    --pos;
+    acc |= (uint32_t)(v & 0xff) << (8 * (pos & 3));
+    if ((pos & 3) == 0) {
+      *(uint32_t *)(base + pos) = acc;
+      acc = 0;
+    }
+  }
+  __device__ void flush() {
+    if (pos & 3) *(uint32_t *)(base + (pos & ~3)) = acc;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Affine band (algn_fill_plane_3_aff / _nobt, src/algn.c:2287-2548)
+// ---------------------------------------------------------------------------
+// Direction byte of the affine band, the fields backtrace_affine reads:
+//   bits 0-1: state entered from "todo" (0 H, 1 A, 2 V, 3 D) -- DO_* priority
+//   bit 2 END_VERTICAL, bit 3 END_HORIZONTAL, bit 4 END_BLOCK
+//   bits 5-6: state after an aligned column (0 stay, 1 H, 2 D, 3 V)
+constexpr uint8_t kAffLeft = 2 | 4;   // DO_VERTICAL | END_VERTICAL
+constexpr uint8_t kAffRight = 0 | 8;  // DO_HORIZONTAL | END_HORIZONTAL
+constexpr int kAffRB = 256;           // LDS ring length (columns) of a row buffer
+constexpr int kAffWaves = 4;          // waves per workgroup
+
+struct RowRef {
+  int *p;
+  int mask, bstride, sstride;
+  __device__ __forceinline__ int &at(int buf, int st, int j) const {
+    return p[buf * bstride + st * sstride + (j & mask)];
+  }
+};
+enum { ST_EH = 0, ST_EV = 1, ST_EB = 2, ST_CB = 3 };
+
+// Per-pair prefix sums over the columns sequence B (length lb+1):
+//   HX[j] = sum_{k<=j} sj_horizontal_extension[k]   (algn.c:2452-2457)
+//   G[j]  = sum_{k<=j} prepend[B[k]]                (row 0, algn.c:2201-2211)
+__device__ void aff_prefix(const uint8_t *B, int lb, const int *s_prep, int go, int *HX, int *G) {
+  const int lane = lane_id();
+  int carry_h = 0, carry_g = 0;
+  for (int j0 = 0; j0 <= lb; j0 += kWave) {
+    const int j = j0 + lane;
+    int hx = 0, gr = 0;
+    if (j >= 1 && j <= lb) {
+      const int jc = B[j] & 31, jp = B[j - 1] & 31;
+      gr = s_prep[jc];
+      const int sj_go = (!(kGap & jp) && (kGap & jc)) ? 0 : go;
+      hx = ((jp & kGap) && !(jc & kGap)) ? sj_go + gr : gr;
+      if (j == 1) hx = gr;
+    }
+    const int ph = wave_prefix_sum(hx) + carry_h;
+    const int pg = wave_prefix_sum(gr) + carry_g;
+    if (j <= lb) {
+      HX[j] = ph;
+      G[j] = pg;
+    }
+    carry_h = lane63(ph);
+    carry_g = lane63(pg);
+  }
+}
+
+// One wave per pair computes the prefix sums the fill reads (a kernel of its
+// own, so that the fill only reads them).
+__global__ __launch_bounds__(kWave * 4) void aff_prep_kernel(const PairMeta *__restrict__ meta, int n,
+                                                           const uint8_t *__restrict__ seq,
+                                                           const DevTcm *__restrict__ tcm, int *aux) {
+  __shared__ int s_prep[32];
+  if (threadIdx.x < 32) s_prep[threadIdx.x] = tcm->prepend[threadIdx.x];
+  __syncthreads();
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; p < n; p += waves) {
+    const PairMeta m = meta[p];
+    int *HX = aux + m.aux_off;
+    aff_prefix(seq + m.off_c, m.lc - 1, s_prep, tcm->gap_open, HX, HX + m.lc);
+  }
+}
+
+template <bool kBt, bool kGlobalRows>
+__device__ void aff_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq, const int *s_cost,
+                              const int *s_prep, int go, uint8_t *__restrict__ dm_base, int *aux,
+                              int32_t *__restrict__ cost_out, int pidx, RowRef R) {
+  const int lane = lane_id();
+  const uint8_t *A = seq + m.off_r;
+  const uint8_t *B = seq + m.off_c;
+  const int la = m.lr - 1, lb = m.lc - 1, E0 = m.p0, S = m.stride;
+  int *HX = aux + m.aux_off;
+  int *G = HX + (lb + 1);
+  uint8_t *dm = dm_base + m.dm_off;
+
+  // row 0 (initialize_matrices_affine, src/algn.c:2183-2248)
+  for (int j0 = 0; j0 <= E0; j0 += kWave) {
+    const int j = j0 + lane;
+    if (j <= E0) {
+      const int eh = j == 0 ? go : go + G[j];
+      R.at(0, ST_EH, j) = eh;
+      R.at(0, ST_EV, j) = j == 0 ? go : kHigh;
+      R.at(0, ST_EB, j) = j == 0 ? 0 : kHigh;
+      R.at(0, ST_CB, j) = j == 0 ? 0 : eh;
+    }
+  }
+  if (la == 0) {
+    if (lane == 0) cost_out[pidx] = lb == 0 ? 0 : go + G[lb];
+    return;
+  }
+  if (kGlobalRows) __threadfence();
+  else wave_sync_mem();
+
+  int ic = A[0] & 31;
+  for (int i = 1; i <= la; ++i) {
+    const int pb = (i - 1) & 1, cbuf = i & 1;
+    const int sp = 1 + (i > 40 ? i - 40 : 0);
+    const int ep = min(lb, E0 + i - 1);
+    const int ip = ic;
+    ic = A[i] & 31;
+    const int si_ge = s_cost[ic * 32 + kGap];
+    const int si_go = (!(kGap & ip) && (kGap & ic)) ? 0 : go;
+    const int si_ng = ic & 15;
+    const int sive = (i > 1 && (ip & kGap) && !(ic & kGap)) ? si_go + si_ge : si_ge;
+    const int ic_gap = ic & kGap;
+    const int r = R.at(pb, ST_EV, sp - 1) + sive;
+    uint8_t *drow = dm + (size_t)(i - 1) * S - (sp - 1);   // indexed by column j
+    if (lane == 0) {
+      R.at(cbuf, ST_EH, sp - 1) = kHigh;
+      R.at(cbuf, ST_EV, sp - 1) = r;
+      R.at(cbuf, ST_EB, sp - 1) = kHigh;
+      R.at(cbuf, ST_CB, sp - 1) = kHigh;
+      if (kBt) drow[sp - 1] = kAffLeft;
+    }
+    int carry_eh = kHigh, carry_cb = kHigh;
+    int carry_hx = HX[sp - 1];
+    const int *crow = s_cost + si_ng * 32;
+    for (int j0 = sp; j0 <= ep; j0 += kWave) {
+      const int j = j0 + lane;
+      const bool act = j <= ep;
+      int jc = 0, jp = 0, HXj = 0, pev_j = 0, pcb_j = 0, pev_l = 0, pcb_l = 0, peb_l = 0, peh_l = 0;
+      if (act) {
+        jc = B[j] & 31;
+        jp = B[j - 1] & 31;
+        HXj = HX[j];
+        pev_j = R.at(pb, ST_EV, j);
+        pcb_j = R.at(pb, ST_CB, j);
+        pev_l = R.at(pb, ST_EV, j - 1);
+        pcb_l = R.at(pb, ST_CB, j - 1);
+        peb_l = R.at(pb, ST_EB, j - 1);
+        peh_l = R.at(pb, ST_EH, j - 1);
+      }
+      const int gap_row = s_prep[jc];
+      const int sj_go = (!(kGap & jp) && (kGap & jc)) ? 0 : go;
+      int hx = ((jp & kGap) && !(jc & kGap)) ? sj_go + gap_row : gap_row;
+      if (j == 1) hx = gap_row;
+      // FILL_EXTEND_VERTICAL (algn.c:1812-1830)
+      int ext = pev_j + sive, open = pcb_j + si_go + si_ge;
+      const bool endv = !(ext < open);
+      const int ev = endv ? open : ext;
+      // FILL_EXTEND_BLOCK_DIAGONAL (bt :1861-1882, nobt :1837-1857)
+      const bool both_gap = ic_gap && (jc & kGap);
+      const int diag = both_gap ? 0 : kHigh;
+      ext = peb_l + diag;
+      if (kBt) {
+        open = pcb_l + diag;
+      } else {
+        const bool flag2 = !(kGap & ip) && !(kGap & jc);
+        open = pcb_l + (both_gap ? (flag2 ? 0 : 2 * go) : kHigh);
+      }
+      const bool endb = !(ext < open);
+      const int eb = endb ? open : ext;
+      // FILL_CLOSE_BLOCK_DIAGONAL (algn.c:1929-1977)
+      const int dg = crow[jc & 15];
+      const int xgo = (sj_go < si_go) ? si_go : sj_go;
+      const int fv = pev_l + dg + (ic_gap ? sj_go : 0);
+      const int fh = peh_l + dg + ((jc & kGap) ? si_go : 0);
+      const int fd = peb_l + dg + xgo;
+      int cb = pcb_l + dg, anext = 0;   // 0 = stay aligned
+      bool a2v = false, a2h = false, a2d = false;
+      if (cb >= fv) { if (cb > fv) { cb = fv; a2v = true; } else a2v = true; }
+      if (cb >= fh) { if (cb > fh) { cb = fh; a2v = false; a2h = true; } else a2h = true; }
+      if (cb >= fd) { if (cb > fd) { cb = fd; a2v = false; a2h = false; a2d = true; } else a2d = true; }
+      // a strict improvement replaces the whole mask, including ALIGN_TO_ALIGN
+      anext = a2h ? 1 : (a2d ? 2 : (a2v ? 3 : 0));
+      // FILL_EXTEND_HORIZONTAL (algn.c:1761-1787) as a prefix minimum
+      const int cb_left = shfl_up1(cb, carry_cb);
+      const int bval = cb_left + sj_go + gap_row;
+      int pm = wave_prefix_min(act ? bval - HXj : kBig);
+      pm = min(pm, carry_eh - carry_hx);
+      const int eh = HXj + pm;
+      const int eh_left = shfl_up1(eh, carry_eh);
+      const bool endh = !(eh_left + hx < bval);
+      // ASSIGN_MINIMUM (algn.c:2250-2280)
+      int f = eh;
+      bool dh = true, dv = false, dd = false, da = false;
+      if (f >= ev) { if (f > ev) { f = ev; dh = false; } dv = true; }
+      if (f >= eb) { if (f > eb) { f = eb; dh = dv = false; } dd = true; }
+      if (f >= cb) { if (f > cb) { f = cb; dh = dv = dd = false; } da = true; }
+      const int todo = dh ? 0 : (da ? 1 : (dv ? 2 : 3));
+      if (act) {
+        R.at(cbuf, ST_EH, j) = eh;
+        R.at(cbuf, ST_EV, j) = ev;
+        R.at(cbuf, ST_EB, j) = eb;
+        R.at(cbuf, ST_CB, j) = cb;
+        if (kBt) drow[j] = (uint8_t)(todo | (endv << 2) | (endh << 3) | (endb << 4) | (anext << 5));
+        if (i == la && j == lb) cost_out[pidx] = f;
+      }
+      carry_eh = lane63(eh);
+      carry_cb = lane63(cb);
+      carry_hx = lane63(HXj);
+    }
+    if (ep < lb && lane == 0) {
+      R.at(cbuf, ST_EH, ep + 1) = kHigh;
+      R.at(cbuf, ST_EV, ep + 1) = kHigh;
+      R.at(cbuf, ST_EB, ep + 1) = kHigh;
+      R.at(cbuf, ST_CB, ep + 1) = kHigh;
+      if (kBt) drow[ep + 1] = kAffRight;
+    }
+    if (kGlobalRows) __threadfence();   // row buffers in global memory
+    else wave_sync_mem();
+  }
+}
+
+// The ALIGN_TO mask logic above must equal the reference's: start with
+// ALIGN_TO_ALIGN; each candidate that is <= the running value either
+// replaces the whole mask (strictly smaller) or ORs its flag (equal).
+// backtrace_affine then tests ALIGN_TO_HORIZONTAL, ALIGN_TO_DIAGONAL,
+// ALIGN_TO_VERTICAL in that order.
+
+__device__ void load_tables(const DevTcm *tcm, int *s_cost, int *s_prep, uint8_t *s_med) {
+  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) {
+    s_cost[k] = tcm->cost[k];
+    if (s_med) s_med[k] = tcm->median[k];
+  }
+  for (int k = threadIdx.x; k < 32; k += blockDim.x) s_prep[k] = tcm->prepend[k];
+  __syncthreads();
+}
+
+template <bool kBt>
+__global__ __launch_bounds__(kWave * kAffWaves) void aff_fill_lds_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm, int *aux,
+    int32_t *cost_out) {
+  __shared__ int s_cost[32 * 32];
+  __shared__ int s_prep[32];
+  __shared__ int s_rows[kAffWaves][2][4][kAffRB];
+  load_tables(tcm, s_cost, s_prep, nullptr);
+  const int w = threadIdx.x / kWave;
+  const int go = tcm->gap_open;
+  RowRef R{&s_rows[w][0][0][0], kAffRB - 1, 4 * kAffRB, kAffRB};
+  for (;;) {
+    const int k = next_item(counter);
+    if (k >= n) break;
+    const int p = order[k];
+    const PairMeta m = meta[p];
+    aff_fill_pair<kBt, false>(m, seq, s_cost, s_prep, go, dm, aux, cost_out, p, R);
+  }
+}
+
+// Same fill with the two row buffers in global memory, for bands wider than
+// the LDS ring (very unequal lengths).  One wave per workgroup.
+template <bool kBt>
+__global__ __launch_bounds__(kWave) void aff_fill_glb_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm, int *aux,
+    int32_t *cost_out, int *rowscratch, int row_width) {
+  __shared__ int s_cost[32 * 32];
+  __shared__ int s_prep[32];
+  load_tables(tcm, s_cost, s_prep, nullptr);
+  const int go = tcm->gap_open;
+  int *mine = rowscratch + (size_t)blockIdx.x * 8 * row_width;
+  RowRef R{mine, -1, 4 * row_width, row_width};
+  for (;;) {
+    const int k = next_item(counter);
+    if (k >= n) break;
+    const int p = order[k];
+    const PairMeta m = meta[p];
+    aff_fill_pair<kBt, true>(m, seq, s_cost, s_prep, go, dm, aux, cost_out, p, R);
+  }
+}
+
+// backtrace_affine (src/algn.c:1982-2097), one thread per pair.  Outputs per
+// pair: median, medianwg, res of the caller's first and second sequence.
+__global__ void aff_trace_kernel(const PairMeta *__restrict__ meta, const int32_t *__restrict__ order,
+                                 int n, const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm,
+                                 const uint8_t *__restrict__ dm_base, uint8_t *out, int32_t *out_len,
+                                 int32_t *status) {
+  __shared__ uint8_t s_med[32 * 32];
+  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_med[k] = tcm->median[k];
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int p = order[t];
+  const PairMeta m = meta[p];
+  const uint8_t *A = seq + m.off_r;
+  const uint8_t *B = seq + m.off_c;
+  const int la = m.lr - 1, lb = m.lc - 1, E0 = m.p0, S = m.stride;
+  const uint8_t *dm = dm_base + m.dm_off;
+  uint8_t *o = out + m.out_off;
+  RevWriter wm, ww, wa, wb;
+  wm.init(o, m.cap);
+  ww.init(o + m.cap, m.cap);
+  // resa belongs to the rows sequence: slot 2 if it is the caller's first
+  wa.init(o + (m.swap ? 3 : 2) * m.cap, m.cap);
+  wb.init(o + (m.swap ? 2 : 3) * m.cap, m.cap);
+  int i = la, j = lb, mode = 0;   // 0 todo, 1 V, 2 H, 3 D, 4 A
+  int ic = A[i], jc = B[j];
+  int nm = 0, bad = 0;
+  int last_med = -1;
+  while (i != 0 && j != 0) {
+    const int sp = 1 + (i > 40 ? i - 40 : 0);
+    const int ep = min(lb, E0 + i - 1);
+    const int hi = ep < lb ? ep + 1 : ep;
+    if (j < sp - 1 || j > hi) { bad = 1; break; }
+    const int f = dm[(size_t)(i - 1) * S + (j - (sp - 1))];
+    if (mode == 0) {
+      const int td = f & 3;
+      mode = td == 0 ? 2 : (td == 1 ? 4 : (td == 2 ? 1 : 3));
+    } else if (mode == 1) {
+      if (f & 4) mode = 0;
+      if (!(ic & kGap)) { wm.put(ic | kGap); ww.put(ic | kGap); ++nm; last_med = ic | kGap; }
+      else ww.put(kGap);
+      wa.put(ic); wb.put(kGap);
+      --i; ic = A[i];
+    } else if (mode == 2) {
+      if (f & 8) mode = 0;
+      if (!(jc & kGap)) { wm.put(jc | kGap); ww.put(jc | kGap); ++nm; last_med = jc | kGap; }
+      else ww.put(kGap);
+      wa.put(kGap); wb.put(jc);
+      --j; jc = B[j];
+    } else if (mode == 3) {
+      if (f & 16) mode = 0;
+      wa.put(ic); wb.put(jc); ww.put(kGap);
+      --i; --j; jc = B[j]; ic = A[i];
+    } else {
+      const int an = (f >> 5) & 3;
+      if (an == 1) mode = 2; else if (an == 2) mode = 3; else if (an == 3) mode = 1;
+      const int prep = s_med[(ic & 15) * 32 + (jc & 15)];
+      wm.put(prep); ww.put(prep); ++nm; last_med = prep;
+      wa.put(ic); wb.put(jc);
+      --i; --j; jc = B[j]; ic = A[i];
+    }
+  }
+  if (!bad) {
+    while (i != 0) {
+      if (!(ic & kGap)) { wm.put(ic | kGap); ww.put(ic | kGap); ++nm; last_med = ic | kGap; }
+      else ww.put(kGap);
+      wa.put(ic); wb.put(kGap);
+      --i; ic = A[i];
+    }
+    while (j != 0) {
+      if (!(jc & kGap)) { wm.put(jc | kGap); ww.put(jc | kGap); ++nm; last_med = jc | kGap; }
+      else ww.put(kGap);
+      wa.put(kGap); wb.put(jc);
+      --j; jc = B[j];
+    }
+    wa.put(kGap); wb.put(kGap); ww.put(kGap);
+    if (nm == 0 || last_med != kGap) { wm.put(kGap); ++nm; }
+  }
+  wm.flush(); ww.flush(); wa.flush(); wb.flush();
+  out_len[4 * p + 0] = m.cap - wm.pos;
+  out_len[4 * p + 1] = m.cap - ww.pos;
+  out_len[4 * p + (m.swap ? 3 : 2)] = m.cap - wa.pos;
+  out_len[4 * p + (m.swap ? 2 : 3)] = m.cap - wb.pos;
+  if (bad) atomicAdd(status, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Linear band (algn_fill_plane_2 + algn_fill_row family, src/algn.c:87-968)
+// ---------------------------------------------------------------------------
+constexpr int kLinRB = 512;
+constexpr int kLinWaves = 4;
+enum { LK_FIRST_TAIL = 0, LK_FIRST_COST = 1, LK_LEFT = 2 };
+enum { RK_RIGHT = 0, RK_LAST = 1 };
+
+// Row layout of algn_fill_plane_2; m = |s1| >= n = |s2| (with the leading
+// gaps), w and h the clamped width/height, cs the case.
+__device__ __forceinline__ void lin_row(int i, int m, int n, int cs, int w, int h, int &lo, int &hi,
+                                        int &lk, int &rk) {
+  if (cs == kLinFull) {
+    lo = 0; hi = n - 1; lk = LK_FIRST_COST; rk = RK_LAST;
+    return;
+  }
+  if (cs == kLinCase2) {
+    if (i < h) { lo = 0; hi = w + i - 1; lk = LK_FIRST_TAIL; rk = RK_RIGHT; }
+    else if (i < m - (h - 1)) { lo = 1 + (i - h); hi = lo + w + h - 2; lk = LK_LEFT; rk = RK_RIGHT; }
+    else { lo = n - (w + h - 2) + (i - (m - (h - 1))); hi = n - 1; lk = LK_LEFT; rk = RK_LAST; }
+    return;
+  }
+  if (i < n - w + 1) { lo = 0; hi = w + i - 1; lk = LK_FIRST_TAIL; rk = RK_RIGHT; }
+  else if (i < m - (n - w) + 1) { lo = 0; hi = n - 1; lk = LK_FIRST_TAIL; rk = RK_LAST; }
+  else { lo = 1 + (i - (m - (n - w) + 1)); hi = n - 1; lk = LK_LEFT; rk = RK_LAST; }
+}
+
+__device__ __forceinline__ int lin_row0_hi(int n, int cs, int w) { return cs == kLinFull ? n - 1 : w - 1; }
+
+// Per-pair prefix sums over s2: GI[j] = sum cost(gap, s2[k]), k=1..j (the
+// INSERT chain, precalc row `gap`), GP[j] = sum prepend[s2[k]] (row 0).
+__device__ void lin_prefix(const uint8_t *s2, int n, const int *s_cost, const int *s_prep, int *GI,
+                           int *GP) {
+  const int lane = lane_id();
+  int ci = 0, cp = 0;
+  for (int j0 = 0; j0 < n; j0 += kWave) {
+    const int j = j0 + lane;
+    int gi = 0, gp = 0;
+    if (j >= 1 && j < n) {
+      const int x = s2[j] & 31;
+      gi = s_cost[kGap * 32 + x];
+      gp = s_prep[x];
+    }
+    const int a = wave_prefix_sum(gi) + ci;
+    const int b = wave_prefix_sum(gp) + cp;
+    if (j < n) { GI[j] = a; GP[j] = b; }
+    ci = lane63(a);
+    cp = lane63(b);
+  }
+}
+
+__global__ __launch_bounds__(kWave * 4) void lin_prep_kernel(const PairMeta *__restrict__ meta, int n,
+                                                           const uint8_t *__restrict__ seq,
+                                                           const DevTcm *__restrict__ tcm, int *aux) {
+  __shared__ int s_cost[32 * 32];
+  __shared__ int s_prep[32];
+  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_cost[k] = tcm->cost[k];
+  if (threadIdx.x < 32) s_prep[threadIdx.x] = tcm->prepend[threadIdx.x];
+  __syncthreads();
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; p < n; p += waves) {
+    const PairMeta m = meta[p];
+    int *GI = aux + m.aux_off;
+    lin_prefix(seq + m.off_c, m.lc, s_cost, s_prep, GI, GI + m.lc);
+  }
+}
+
+template <bool kDm, bool kGlobalRows>
+__device__ void lin_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq, const int *s_cost,
+                              const int *s_prep, const int *s_tail, uint8_t *__restrict__ dm_base, int *aux,
+                              int32_t *__restrict__ cost_out, int pidx, int *rows, int mask, int bstride) {
+  const int lane = lane_id();
+  const uint8_t *s1 = seq + m.off_r;   // the longer sequence (rows)
+  const uint8_t *s2 = seq + m.off_c;
+  const int M = m.lr, N = m.lc, cs = m.p0, w = m.p1, h = m.p2, S = m.stride;
+  int *GI = aux + m.aux_off;
+  int *GP = GI + N;
+  uint8_t *dm = dm_base + m.dm_off;
+  // row 0 (algn_fill_first_row, :584-607, or the first row of algn_fill_plane)
+  const int hi0 = lin_row0_hi(N, cs, w);
+  for (int j0 = 0; j0 <= hi0; j0 += kWave) {
+    const int j = j0 + lane;
+    if (j <= hi0) {
+      rows[j & mask] = GP[j];
+      if (kDm) dm[j] = j == 0 ? 1 : 2;   // ALIGN, then INSERT
+    }
+  }
+  if (M == 1) {
+    if (lane == 0) cost_out[pidx] = GP[N - 1];
+    return;
+  }
+  if (kGlobalRows) __threadfence();
+  else wave_sync_mem();
+  for (int i = 1; i < M; ++i) {
+    int lo, hi, lk, rk;
+    lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
+    const int *prv = rows + ((i - 1) & 1) * bstride;
+    int *cur = rows + (i & 1) * bstride;
+    const int si = s1[i] & 31;
+    const int c_del = s_cost[si * 32 + kGap];
+    const int tail = s_tail[si];
+    const int *crow = s_cost + si * 32;
+    uint8_t *drow = dm + (size_t)i * S - lo;
+    int carry_mm = kBig, carry_gi = 0;   // chain value before lo: none
+    for (int j0 = lo; j0 <= hi; j0 += kWave) {
+      const int j = j0 + lane;
+      const bool act = j <= hi;
+      int best = kBig, msk = 0, T = kBig, GIj = 0, t1 = kBig, t3 = kBig, gj = 0, cst = kBig;
+      const bool first = (j == lo) && lk != LK_LEFT;
+      const bool has_ins = act && !(j == lo);
+      if (act) {
+        GIj = GI[j];
+        if (first) {
+          T = prv[0 & mask] + (lk == LK_FIRST_TAIL ? tail : c_del);
+        } else {
+          const int x = s2[j] & 31;
+          gj = s_cost[kGap * 32 + x];
+          t3 = prv[(j - 1) & mask] + crow[x];
+          if (!(j == hi && rk == RK_RIGHT)) t1 = prv[j & mask] + c_del;
+          T = min(t1, t3);
+          if (j == hi && rk == RK_LAST && j > 0) {
+            cst = tail + prv[j & mask];
+            T = min(T, cst);
+          }
+        }
+      }
+      int pm = wave_prefix_min(act ? T - GIj : kBig);
+      pm = min(pm, carry_mm - carry_gi);
+      const int mm = GIj + pm;
+      const int mm_left = shfl_up1(mm, carry_mm);
+      if (act) {
+        if (first) {
+          best = T;
+          msk = 4;
+        } else {
+          const int t2 = has_ins ? mm_left + gj : kBig;
+          best = min(min(t1, t2), t3);
+          msk = (t3 == best ? 1 : 0) | (has_ins && t2 == best ? 2 : 0) | (t1 == best ? 4 : 0);
+          if (cst != kBig) {
+            if (cst < best) { best = cst; msk = 4; }
+            else if (cst == best) msk |= 4;
+          }
+        }
+        cur[j & mask] = best;
+        if (kDm) drow[j] = (uint8_t)msk;
+        if (i == M - 1 && j == N - 1) cost_out[pidx] = best;
+      }
+      carry_mm = lane63(mm);
+      carry_gi = lane63(GIj);
+    }
+    if (kGlobalRows) __threadfence();
+    else wave_sync_mem();
+  }
+}
+
+template <bool kDm>
+__global__ __launch_bounds__(kWave * kLinWaves) void lin_fill_lds_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm, int *aux,
+    int32_t *cost_out) {
+  __shared__ int s_cost[32 * 32];
+  __shared__ int s_prep[32];
+  __shared__ int s_tail[32];
+  __shared__ int s_rows[kLinWaves][2][kLinRB];
+  load_tables(tcm, s_cost, s_prep, nullptr);
+  for (int k = threadIdx.x; k < 32; k += blockDim.x) s_tail[k] = tcm->tail[k];
+  __syncthreads();
+  const int wv = threadIdx.x / kWave;
+  for (;;) {
+    const int k = next_item(counter);
+    if (k >= n) break;
+    const int p = order[k];
+    const PairMeta m = meta[p];
+    lin_fill_pair<kDm, false>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, &s_rows[wv][0][0],
+                       kLinRB - 1, kLinRB);
+  }
+}
+
+template <bool kDm>
+__global__ __launch_bounds__(kWave) void lin_fill_glb_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm, int *aux,
+    int32_t *cost_out, int *rowscratch, int row_width) {
+  __shared__ int s_cost[32 * 32];
+  __shared__ int s_prep[32];
+  __shared__ int s_tail[32];
+  load_tables(tcm, s_cost, s_prep, nullptr);
+  for (int k = threadIdx.x; k < 32; k += blockDim.x) s_tail[k] = tcm->tail[k];
+  __syncthreads();
+  int *mine = rowscratch + (size_t)blockIdx.x * 2 * row_width;
+  for (;;) {
+    const int k = next_item(counter);
+    if (k >= n) break;
+    const int p = order[k];
+    const PairMeta m = meta[p];
+    lin_fill_pair<kDm, true>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, mine, -1, row_width);
+  }
+}
+
+// backtrack_2d (src/algn.c:3551-3666) + median_2 (:4042-4055), one thread per
+// pair.  Slots: aligned s1 (caller's first), aligned s2, median_2.
+__global__ void lin_trace_kernel(const PairMeta *__restrict__ meta, const int32_t *__restrict__ order,
+                                 int n, const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm,
+                                 const uint8_t *__restrict__ dm_base, uint8_t *out, int32_t *out_len,
+                                 int32_t *status) {
+  __shared__ uint8_t s_med[32 * 32];
+  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_med[k] = tcm->median[k];
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int p = order[t];
+  const PairMeta m = meta[p];
+  const uint8_t *s1 = seq + m.off_r;
+  const uint8_t *s2 = seq + m.off_c;
+  const int M = m.lr, N = m.lc, cs = m.p0, w = m.p1, h = m.p2, S = m.stride;
+  const uint8_t *dm = dm_base + m.dm_off;
+  uint8_t *o = out + m.out_off;
+  // swap == 0: the caller's first sequence is s1 (the longer one), and the
+  // backtrack runs with swaped = true (Sequence.Align.create_edited_2)
+  const bool swaped = m.swap == 0;
+  RevWriter w1, w2, wmed;
+  w1.init(o + (swaped ? 0 : 1) * m.cap, m.cap);   // aligned s1
+  w2.init(o + (swaped ? 1 : 0) * m.cap, m.cap);   // aligned s2
+  wmed.init(o + 2 * m.cap, m.cap);
+  int i = M - 1, j = N - 1, len = 0, bad = 0;
+  while (i >= 0 && j >= 0) {
+    int lo, hi, lk, rk;
+    if (i == 0) { lo = 0; hi = lin_row0_hi(N, cs, w); }
+    else lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
+    if (j < lo || j > hi) { bad = 1; break; }
+    const int d = dm[(size_t)i * S + (j - lo)];
+    int a, b;
+    if (d & 1) { a = s1[i]; b = s2[j]; --i; --j; }
+    else if (swaped ? (d & 2) : !(d & 4)) { a = kGap; b = s2[j]; --j; }
+    else { a = s1[i]; b = kGap; --i; }
+    w1.put(a); w2.put(b);
+    // median of the caller-ordered pair, gaps dropped
+    const int med = swaped ? s_med[(a & 31) * 32 + (b & 31)] : s_med[(b & 31) * 32 + (a & 31)];
+    if (med != kGap) wmed.put(med);
+    ++len;
+    if (j < 0 && i >= 0) { bad = 1; break; }
+  }
+  wmed.put(kGap);
+  w1.flush(); w2.flush(); wmed.flush();
+  out_len[3 * p + 0] = len;
+  out_len[3 * p + 1] = len;
+  out_len[3 * p + 2] = m.cap - wmed.pos;
+  if (bad) atomicAdd(status, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Fitch over packed nonadditive characters (src/nonaddCSc.c:422-544):
+// HBM-bound bit operations, 16 characters per lane per step.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t empty_bytes(uint32_t v) {
+  // 0xFF in every byte of v that is zero
+  uint32_t t = ((v & 0x7f7f7f7fu) + 0x7f7f7f7fu) | v;
+  t = ~(t | 0x7f7f7f7fu);   // 0x80 where the byte is zero
+  return (t >> 7) * 0xffu;
+}
+
+__global__ void fitch_down_kernel(uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
+                                  const int32_t *ires, int64_t *added) {
+  const int node = blockIdx.y;
+  if (node >= n) return;
+  const uint8_t *a = chars + (size_t)ia[node] * len;
+  const uint8_t *b = chars + (size_t)ib[node] * len;
+  uint8_t *r = chars + (size_t)ires[node] * len;
+  long long cnt = 0;
+  const int nvec = (len % 16 == 0) ? len / 16 : 0;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nvec; v += gridDim.x * blockDim.x) {
+    const uint4 x = ((const uint4 *)a)[v];
+    const uint4 y = ((const uint4 *)b)[v];
+    uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w}, rs[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t in = xs[q] & ys[q], un = xs[q] | ys[q];
+      const uint32_t emp = empty_bytes(in);
+      cnt += __popc(emp) >> 3;
+      rs[q] = (emp & un) | in;
+    }
+    ((uint4 *)r)[v] = make_uint4(rs[0], rs[1], rs[2], rs[3]);
+  }
+  if (blockIdx.x == 0) {
+    for (int c = nvec * 16 + threadIdx.x; c < len; c += blockDim.x) {
+      const uint8_t in = a[c] & b[c];
+      if (!in) ++cnt;
+      r[c] = in ? in : (uint8_t)(a[c] | b[c]);
+    }
+  }
+  // block reduction through LDS, one atomic per block
+  __shared__ unsigned long long s_red[256];
+  s_red[threadIdx.x] = (unsigned long long)cnt;
+  __syncthreads();
+  for (int d = blockDim.x / 2; d > 0; d >>= 1) {
+    if ((int)threadIdx.x < d) s_red[threadIdx.x] += s_red[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && s_red[0]) atomicAdd((unsigned long long *)&added[node], s_red[0]);
+}
+
+__global__ void fitch_up_kernel(uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
+                                const int32_t *iD1, const int32_t *iD2, const int32_t *iF) {
+  const int node = blockIdx.y;
+  if (node >= n) return;
+  const uint8_t *A = chars + (size_t)iA[node] * len;
+  const uint8_t *N = chars + (size_t)iN[node] * len;
+  const uint8_t *D1 = chars + (size_t)iD1[node] * len;
+  const uint8_t *D2 = chars + (size_t)iD2[node] * len;
+  uint8_t *F = chars + (size_t)iF[node] * len;
+  const int nvec = (len % 16 == 0) ? len / 16 : 0;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nvec; v += gridDim.x * blockDim.x) {
+    const uint4 a4 = ((const uint4 *)A)[v], n4 = ((const uint4 *)N)[v];
+    const uint4 d14 = ((const uint4 *)D1)[v], d24 = ((const uint4 *)D2)[v];
+    uint32_t as[4] = {a4.x, a4.y, a4.z, a4.w}, ns[4] = {n4.x, n4.y, n4.z, n4.w};
+    uint32_t d1[4] = {d14.x, d14.y, d14.z, d14.w}, d2[4] = {d24.x, d24.y, d24.z, d24.w}, fs[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t inter = ns[q] | (as[q] & (d1[q] | d2[q]));
+      uint32_t temp = inter;
+      const uint32_t was_union = empty_bytes(d1[q] & d2[q]);
+      inter = (inter | as[q]) & was_union;
+      temp |= inter;
+      const uint32_t not_contained = ~empty_bytes((as[q] & ns[q]) ^ as[q]);   // 0xFF where A is not within N
+      fs[q] = (~not_contained & as[q]) | (not_contained & temp);
+    }
+    ((uint4 *)F)[v] = make_uint4(fs[0], fs[1], fs[2], fs[3]);
+  }
+  if (blockIdx.x == 0) {
+    for (int c = nvec * 16 + threadIdx.x; c < len; c += blockDim.x) {
+      const uint8_t a = A[c], nn = N[c], x1 = D1[c], x2 = D2[c];
+      uint8_t f;
+      if ((a & nn) == a) f = a;
+      else if ((x1 & x2) == 0) f = nn | a;
+      else f = nn | (a & (x1 | x2));
+      F[c] = f;
+    }
+  }
+}
+
+}  // namespace poyg
+
+// ---------------------------------------------------------------------------
+// launch wrappers used by the host side
+// ---------------------------------------------------------------------------
+namespace poyg {
+
+hipError_t launch_reset(int *counters, int n, hipStream_t s) {
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep(bool affine, int grid, const PairMeta *meta, int n, const uint8_t *seq, const DevTcm *tcm,
+                       int *aux, hipStream_t s) {
+  if (affine)
+    hipLaunchKernelGGL(aff_prep_kernel, dim3(grid), dim3(kWave * 4), 0, s, meta, n, seq, tcm, aux);
+  else
+    hipLaunchKernelGGL(lin_prep_kernel, dim3(grid), dim3(kWave * 4), 0, s, meta, n, seq, tcm, aux);
+  return hipGetLastError();
+}
+
+hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
+                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
+                           int32_t *cost, int *rowscratch, int row_width, hipStream_t s) {
+  if (!glb) {
+    if (bt)
+      hipLaunchKernelGGL(aff_fill_lds_kernel<true>, dim3(grid), dim3(kWave * kAffWaves), 0, s, meta, order, n,
+                         counter, seq, tcm, dm, aux, cost);
+    else
+      hipLaunchKernelGGL(aff_fill_lds_kernel<false>, dim3(grid), dim3(kWave * kAffWaves), 0, s, meta, order, n,
+                         counter, seq, tcm, dm, aux, cost);
+  } else {
+    if (bt)
+      hipLaunchKernelGGL(aff_fill_glb_kernel<true>, dim3(grid), dim3(kWave), 0, s, meta, order, n, counter, seq,
+                         tcm, dm, aux, cost, rowscratch, row_width);
+    else
+      hipLaunchKernelGGL(aff_fill_glb_kernel<false>, dim3(grid), dim3(kWave), 0, s, meta, order, n, counter, seq,
+                         tcm, dm, aux, cost, rowscratch, row_width);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
+                            const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
+                            hipStream_t s) {
+  const int bs = 64;
+  hipLaunchKernelGGL(aff_trace_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, s, meta, order, n, seq, tcm, dm, out,
+                     out_len, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
+                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
+                           int32_t *cost, int *rowscratch, int row_width, hipStream_t s) {
+  if (!glb) {
+    if (with_dm)
+      hipLaunchKernelGGL(lin_fill_lds_kernel<true>, dim3(grid), dim3(kWave * kLinWaves), 0, s, meta, order, n,
+                         counter, seq, tcm, dm, aux, cost);
+    else
+      hipLaunchKernelGGL(lin_fill_lds_kernel<false>, dim3(grid), dim3(kWave * kLinWaves), 0, s, meta, order, n,
+                         counter, seq, tcm, dm, aux, cost);
+  } else {
+    if (with_dm)
+      hipLaunchKernelGGL(lin_fill_glb_kernel<true>, dim3(grid), dim3(kWave), 0, s, meta, order, n, counter, seq,
+                         tcm, dm, aux, cost, rowscratch, row_width);
+    else
+      hipLaunchKernelGGL(lin_fill_glb_kernel<false>, dim3(grid), dim3(kWave), 0, s, meta, order, n, counter, seq,
+                         tcm, dm, aux, cost, rowscratch, row_width);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_lin_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
+                            const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
+                            hipStream_t s) {
+  const int bs = 64;
+  hipLaunchKernelGGL(lin_trace_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, s, meta, order, n, seq, tcm, dm, out,
+                     out_len, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_fitch_down(uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
+                             const int32_t *ires, int64_t *added, hipStream_t s) {
+  const int bs = 256;
+  const int nvec = len / 16;
+  int gx = (nvec + bs - 1) / bs;
+  if (gx < 1) gx = 1;
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(fitch_down_kernel, dim3(gx, n), dim3(bs), 0, s, chars, len, n, ia, ib, ires, added);
+  return hipGetLastError();
+}
+
+hipError_t launch_fitch_up(uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
+                           const int32_t *iD1, const int32_t *iD2, const int32_t *iF, hipStream_t s) {
+  const int bs = 256;
+  const int nvec = len / 16;
+  int gx = (nvec + bs - 1) / bs;
+  if (gx < 1) gx = 1;
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(fitch_up_kernel, dim3(gx, n), dim3(bs), 0, s, chars, len, n, iA, iN, iD1, iD2, iF);
+  return hipGetLastError();
+}
+
+}  // namespace poyg

@@ -1,0 +1,469 @@
+"""Python host side of the engine: a ctypes binding of libpoyg.so.
+
+Mirrors the reference's OCaml interface for the alignment hot path
+(Sequence.Align in src/sequence.ml and the Cost_matrix.Two_D constructors);
+every call goes through the C ABI declared in include/poyg.h and runs on the
+GPU.  There is no CPU fallback: loading fails loudly when the library or a
+GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpoyg.so")
+
+POYG_MODE_LINEAR_COST = 0
+POYG_MODE_LINEAR_ALIGN = 1
+POYG_MODE_AFFINE_COST = 2
+POYG_MODE_AFFINE_ALIGN = 3
+
+_lib = None
+u8p = C.POINTER(C.c_uint8)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+vp = C.c_void_p
+
+
+class PoygError(RuntimeError):
+    pass
+
+
+# exported symbols and signatures (kept in sync with include/poyg.h)
+SIGNATURES = {
+    "poyg_last_error": (C.c_char_p, []),
+    "poyg_version": (C.c_char_p, []),
+    "poyg_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "poyg_ctx_destroy": (None, [vp]),
+    "poyg_ctx_sync": (C.c_int, [vp]),
+    "poyg_ctx_stream": (vp, [vp]),
+    "poyg_dev_alloc": (C.c_int, [vp, C.c_int64, C.POINTER(vp)]),
+    "poyg_dev_free": (C.c_int, [vp, vp]),
+    "poyg_dev_upload": (C.c_int, [vp, vp, vp, C.c_int64]),
+    "poyg_dev_download": (C.c_int, [vp, vp, vp, C.c_int64]),
+    "poyg_plan_status": (C.c_int, [vp, i32p]),
+    "poyg_cm_create": (C.c_int, [C.c_int] * 5 + [C.POINTER(vp)]),
+    "poyg_cm_destroy": (None, [vp]),
+    "poyg_cm_set_cost": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "poyg_cm_set_worst": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "poyg_cm_set_median": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "poyg_cm_set_prepend": (C.c_int, [vp, C.c_int, C.c_int]),
+    "poyg_cm_set_tail": (C.c_int, [vp, C.c_int, C.c_int]),
+    "poyg_cm_set_affine_flag": (C.c_int, [vp, C.c_int, C.c_int]),
+    "poyg_cm_get_cost": (C.c_int, [vp, C.c_int, C.c_int]),
+    "poyg_cm_get_median": (C.c_int, [vp, C.c_int, C.c_int]),
+    "poyg_cm_get_worst": (C.c_int, [vp, C.c_int, C.c_int]),
+    "poyg_cm_get_prepend": (C.c_int, [vp, C.c_int]),
+    "poyg_cm_get_tail": (C.c_int, [vp, C.c_int]),
+    "poyg_cm_get_gap": (C.c_int, [vp]),
+    "poyg_cm_get_affine": (C.c_int, [vp]),
+    "poyg_cm_get_gap_opening": (C.c_int, [vp]),
+    "poyg_cm_of_list": (C.c_int, [C.POINTER(C.c_int), C.c_int, C.POINTER(vp)]),
+    "poyg_cm_set_affine": (C.c_int, [vp, C.c_int, C.c_int]),
+    "poyg_cm_of_sub_indel": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+    "poyg_cm3_of_two_dim": (C.c_int, [vp, C.POINTER(vp)]),
+    "poyg_cm3_destroy": (None, [vp]),
+    "poyg_cm3_get_cost": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "poyg_cm3_get_median": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "poyg_algn_simple_2": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, C.c_int, i32p]),
+    "poyg_algn_align_2d": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int, u8p, u8p,
+                                     C.c_int, i32p, i32p]),
+    "poyg_algn_align_affine_3": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, u8p, u8p, u8p, u8p,
+                                           C.c_int, i32p, i32p]),
+    "poyg_algn_cost_affine_3": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, i32p]),
+    "poyg_algn_align_3d": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, u8p, u8p, u8p,
+                                     C.c_int, i32p, i32p]),
+    "poyg_algn_median_2_no_gaps": (C.c_int, [vp, u8p, u8p, C.c_int, u8p, C.c_int, i32p]),
+    "poyg_algn_median_2_with_gaps": (C.c_int, [vp, u8p, u8p, C.c_int, u8p, C.c_int, i32p]),
+    "poyg_algn_ancestor_2": (C.c_int, [vp, u8p, u8p, C.c_int, u8p, C.c_int, i32p]),
+    "poyg_algn_union": (C.c_int, [u8p, u8p, C.c_int, u8p, C.c_int, i32p]),
+    "poyg_algn_median_3": (C.c_int, [vp, u8p, u8p, u8p, C.c_int, u8p, C.c_int, i32p]),
+    "poyg_algn_worst_2": (C.c_int, [vp, u8p, u8p, C.c_int, i32p]),
+    "poyg_algn_verify_2": (C.c_int, [vp, u8p, u8p, C.c_int, i32p]),
+    "poyg_plan_pairs": (C.c_int, [vp, vp, C.c_int, C.c_int32, i64p, i32p, i64p, i32p, i32p,
+                                  C.POINTER(vp)]),
+    "poyg_plan_destroy": (None, [vp]),
+    "poyg_plan_slots": (C.c_int, [vp]),
+    "poyg_plan_out_bytes": (C.c_int64, [vp]),
+    "poyg_plan_out_offsets": (C.c_int, [vp, i64p]),
+    "poyg_plan_cells": (C.c_int64, [vp]),
+    "poyg_plan_run": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "poyg_plan_last_times": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "poyg_plan_set_timing": (C.c_int, [vp, C.c_int]),
+    "poyg_plan_triples": (C.c_int, [vp, vp, C.c_int32, i64p, i32p, i64p, i32p, i64p, i32p,
+                                    C.POINTER(vp)]),
+    "poyg_fitch_down": (C.c_int, [vp, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp]),
+    "poyg_fitch_up": (C.c_int, [vp, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp]),
+    "poyg_nonadd_median": (C.c_int, [vp, u8p, u8p, u8p, C.c_int, i64p]),
+    "poyg_nonadd_median_3": (C.c_int, [vp, u8p, u8p, u8p, u8p, u8p, C.c_int]),
+}
+
+
+def load(path: str = LIB_PATH):
+    """Load libpoyg.so (no GPU needed); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise PoygError(f"{path} is missing: build it with `python -m poyd_amd.build`")
+        L = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().poyg_last_error().decode(errors="replace")
+        raise PoygError(f"{what} failed ({rc}): {msg}")
+
+
+def _u8(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint8))
+
+
+def _p8(a: np.ndarray):
+    return a.ctypes.data_as(u8p)
+
+
+class Context:
+    """A device context (one HIP stream on one GPU)."""
+
+    def __init__(self, device: int = 0):
+        h = vp()
+        _check(load().poyg_ctx_create(device, C.byref(h)), "poyg_ctx_create")
+        self.h = h
+
+    def sync(self):
+        _check(load().poyg_ctx_sync(self.h), "poyg_ctx_sync")
+
+    def close(self):
+        if self.h:
+            load().poyg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """A device allocation made through the engine (hipMalloc)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        h = vp()
+        _check(load().poyg_dev_alloc(ctx.h, self.nbytes, C.byref(h)), "poyg_dev_alloc")
+        self.ptr = h.value
+
+    @classmethod
+    def from_array(cls, ctx: Context, arr: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        b = cls(ctx, max(a.nbytes, 16))
+        b.upload(a)
+        return b
+
+    def upload(self, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        _check(load().poyg_dev_upload(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), "poyg_dev_upload")
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(load().poyg_dev_download(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), "poyg_dev_download")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().poyg_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CostMatrix:
+    """Cost_matrix.Two_D.m (src/cost_matrix.ml), held by the engine."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def of_sub_indel(cls, sub: int, indel: int, gap_open: int | None = None) -> "CostMatrix":
+        h = vp()
+        _check(load().poyg_cm_of_sub_indel(sub, indel, -1 if gap_open is None else gap_open, C.byref(h)),
+               "poyg_cm_of_sub_indel")
+        return cls(h)
+
+    @classmethod
+    def of_list(cls, mat) -> "CostMatrix":
+        m = np.ascontiguousarray(np.asarray(mat, dtype=np.int32))
+        h = vp()
+        _check(load().poyg_cm_of_list(m.ctypes.data_as(C.POINTER(C.c_int)), m.shape[0], C.byref(h)),
+               "poyg_cm_of_list")
+        return cls(h)
+
+    def set_affine(self, model: int, gap_open: int) -> None:
+        _check(load().poyg_cm_set_affine(self.h, model, gap_open), "poyg_cm_set_affine")
+
+    def cost(self, a, b):
+        return load().poyg_cm_get_cost(self.h, a, b)
+
+    def median(self, a, b):
+        return load().poyg_cm_get_median(self.h, a, b)
+
+    def worst(self, a, b):
+        return load().poyg_cm_get_worst(self.h, a, b)
+
+    def prepend(self, a):
+        return load().poyg_cm_get_prepend(self.h, a)
+
+    def tail(self, a):
+        return load().poyg_cm_get_tail(self.h, a)
+
+    @property
+    def affine(self) -> bool:
+        return load().poyg_cm_get_affine(self.h) == 1
+
+    @property
+    def gap_opening(self) -> int:
+        return load().poyg_cm_get_gap_opening(self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().poyg_cm_destroy(self.h)
+        except Exception:
+            pass
+
+
+class CostMatrix3:
+    def __init__(self, cm2: CostMatrix):
+        h = vp()
+        _check(load().poyg_cm3_of_two_dim(cm2.h, C.byref(h)), "poyg_cm3_of_two_dim")
+        self.h = h
+
+    def cost(self, a, b, c):
+        return load().poyg_cm3_get_cost(self.h, a, b, c)
+
+    def median(self, a, b, c):
+        return load().poyg_cm3_get_median(self.h, a, b, c)
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().poyg_cm3_destroy(self.h)
+        except Exception:
+            pass
+
+
+def _tail(buf, n):
+    return buf[len(buf) - n:].copy()
+
+
+def count_gaps(s, gap: int = 16) -> int:
+    """seq_CAML_count (src/seq.c:571-581): positions carrying the gap bit."""
+    return int((np.asarray(s) & gap).astype(bool).sum())
+
+
+def deltaw(s1, s2, dw: int | None = None, gap: int = 16) -> int:
+    """Band parameter of Sequence.Align.cost_2 (src/sequence.ml:691-713)."""
+    l1, l2 = len(s1), len(s2)
+    gaps = max(count_gaps(s1, gap), count_gaps(s2, gap))
+    longer, shorter = (l1, l2) if l1 >= l2 else (l2, l1)
+    dif = longer - shorter
+    lower = int(float(longer) * 0.10)
+    if dw is None:
+        extra = lower // 2 if dif < lower else 2
+    else:
+        extra = lower if dif < lower else dw
+    return gaps + extra
+
+
+class Align:
+    """Per-call entry points with the semantics of Sequence.Align
+    (src/sequence.ml:453-1141); each call is a one-pair GPU batch."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def cost_2(self, s1, s2, cm: CostMatrix, dw: int | None = None) -> int:
+        a, b = _u8(s1), _u8(s2)
+        out = C.c_int32(0)
+        if cm.affine:
+            _check(load().poyg_algn_cost_affine_3(self.ctx.h, cm.h, _p8(a), len(a), _p8(b), len(b),
+                                                  C.byref(out)), "cost_affine_3")
+            return out.value
+        d = deltaw(a, b, dw)
+        if len(a) < len(b):
+            a, b = b, a
+        _check(load().poyg_algn_simple_2(self.ctx.h, cm.h, _p8(a), len(a), _p8(b), len(b), d, C.byref(out)),
+               "algn_simple_2")
+        return out.value
+
+    def align_2(self, s1, s2, cm: CostMatrix):
+        """align_2 ~first_gap:true: (aligned s1, aligned s2, cost)."""
+        if cm.affine:
+            med, ri, rj, cost, mwg = self.align_affine_3(s1, s2, cm)
+            return ri, rj, cost
+        a, b = _u8(s1), _u8(s2)
+        d = deltaw(a, b)
+        swaped = len(a) >= len(b)
+        x, y = (a, b) if swaped else (b, a)
+        cap = len(a) + len(b) + 2
+        r1 = np.zeros(cap, np.uint8)
+        r2 = np.zeros(cap, np.uint8)
+        n = C.c_int32(0)
+        cost = C.c_int32(0)
+        # the stub receives the longer sequence first and the swap flag
+        _check(load().poyg_algn_align_2d(self.ctx.h, cm.h, _p8(x), len(x), _p8(y), len(y), d, int(swaped),
+                                         _p8(r1), _p8(r2), cap, C.byref(n), C.byref(cost)), "algn_align_2d")
+        o1, o2 = _tail(r1, n.value), _tail(r2, n.value)
+        return (o1, o2, cost.value) if swaped else (o2, o1, cost.value)
+
+    def align_affine_3(self, si, sj, cm: CostMatrix):
+        """(median, resi, resj, cost, medianwg) as Sequence.Align.align_affine_3."""
+        a, b = _u8(si), _u8(sj)
+        cap = len(a) + len(b) + 2
+        bufs = [np.zeros(cap, np.uint8) for _ in range(4)]
+        lens = np.zeros(4, np.int32)
+        cost = C.c_int32(0)
+        _check(load().poyg_algn_align_affine_3(self.ctx.h, cm.h, _p8(a), len(a), _p8(b), len(b),
+                                               *[_p8(x) for x in bufs], cap, lens.ctypes.data_as(i32p),
+                                               C.byref(cost)), "align_affine_3")
+        med, mwg, ri, rj = (_tail(bufs[k], int(lens[k])) for k in range(4))
+        return med, ri, rj, cost.value, mwg
+
+    def median_2(self, a, b, cm: CostMatrix):
+        x, y = _u8(a), _u8(b)
+        out = np.zeros(len(x) + 1, np.uint8)
+        n = C.c_int32(0)
+        _check(load().poyg_algn_median_2_no_gaps(cm.h, _p8(x), _p8(y), len(x), _p8(out), len(out), C.byref(n)),
+               "median_2")
+        return _tail(out, n.value)
+
+
+@dataclass
+class PairResult:
+    cost: np.ndarray
+    out: np.ndarray | None
+    out_len: np.ndarray | None
+    out_off: np.ndarray | None
+    cap: np.ndarray | None
+    slots: int
+
+    def seq(self, k: int, slot: int) -> np.ndarray:
+        cap = int(self.cap[k])
+        base = int(self.out_off[k]) + slot * cap
+        n = int(self.out_len[k * self.slots + slot])
+        return self.out[base + cap - n: base + cap]
+
+
+class PairPlan:
+    """A batched pairwise plan over device-resident sequences (torch tensors
+    on the GPU are used as plain device buffers)."""
+
+    def __init__(self, ctx: Context, cm: CostMatrix, mode: int, off_a, len_a, off_b, len_b, dws=None):
+        self.ctx = ctx
+        self.cm = cm
+        self.mode = mode
+        self.n = len(len_a)
+        self._keep = [np.ascontiguousarray(x, dtype=t) for x, t in
+                      ((off_a, np.int64), (len_a, np.int32), (off_b, np.int64), (len_b, np.int32))]
+        oa, la, ob, lb = self._keep
+        d = None
+        if dws is not None:
+            d = np.ascontiguousarray(dws, dtype=np.int32)
+            self._keep.append(d)
+        h = vp()
+        _check(load().poyg_plan_pairs(ctx.h, cm.h, mode, self.n, oa.ctypes.data_as(i64p), la.ctypes.data_as(i32p),
+                                      ob.ctypes.data_as(i64p), lb.ctypes.data_as(i32p),
+                                      d.ctypes.data_as(i32p) if d is not None else None, C.byref(h)),
+               "poyg_plan_pairs")
+        self.h = h
+        self.slots = load().poyg_plan_slots(h)
+        self.out_bytes = load().poyg_plan_out_bytes(h)
+        self.cells = load().poyg_plan_cells(h)
+        self.out_off = np.zeros(self.n, np.int64)
+        if self.n:
+            load().poyg_plan_out_offsets(h, self.out_off.ctypes.data_as(i64p))
+        self.cap = ((la.astype(np.int64) + lb + 2 + 15) // 16) * 16
+
+    def set_timing(self, on: bool = True):
+        _check(load().poyg_plan_set_timing(self.h, int(on)), "set_timing")
+
+    def last_times(self):
+        f, t = C.c_float(0), C.c_float(0)
+        load().poyg_plan_last_times(self.h, C.byref(f), C.byref(t))
+        return f.value, t.value
+
+    def run(self, d_seq, d_cost, d_out=None, d_out_len=None, stream=None) -> None:
+        """d_* are device pointers (ints) of caller-owned buffers."""
+        _check(load().poyg_plan_run(self.h, d_seq, d_cost, d_out or None, d_out_len or None, stream),
+               "poyg_plan_run")
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().poyg_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_deltaw(a_batch, b_batch) -> np.ndarray:
+    """deltaw of Sequence.Align.cost_2 for every pair, vectorised."""
+    la = a_batch.ln.astype(np.int64)
+    lb = b_batch.ln.astype(np.int64)
+    ga = np.add.reduceat((a_batch.seq & 16) != 0, a_batch.off) if len(la) else la
+    gb = np.add.reduceat((b_batch.seq & 16) != 0, b_batch.off) if len(lb) else lb
+    longer = np.maximum(la, lb)
+    dif = np.abs(la - lb)
+    lower = np.floor(longer.astype(np.float64) * 0.10).astype(np.int64)
+    extra = np.where(dif < lower, lower // 2, 2)
+    return (np.maximum(ga, gb) + extra).astype(np.int32)
+
+
+def run_pairs(ctx: Context, cm: CostMatrix, mode: int, a_batch, b_batch, dws=None,
+              check_band: bool = True) -> PairResult:
+    """Convenience: upload two SeqBatch-like objects (seq/off/ln), run a
+    plan and download every result."""
+    seq = np.concatenate([a_batch.seq, b_batch.seq])
+    off_a = a_batch.off
+    off_b = b_batch.off + len(a_batch.seq)
+    if dws is None and mode in (POYG_MODE_LINEAR_COST, POYG_MODE_LINEAR_ALIGN):
+        dws = batch_deltaw(a_batch, b_batch)
+    plan = PairPlan(ctx, cm, mode, off_a, a_batch.ln, off_b, b_batch.ln, dws)
+    d_seq = DeviceBuffer.from_array(ctx, seq)
+    d_cost = DeviceBuffer(ctx, 4 * max(plan.n, 1))
+    d_out = DeviceBuffer(ctx, max(plan.out_bytes, 16)) if plan.slots else None
+    d_len = DeviceBuffer(ctx, 4 * max(plan.n * plan.slots, 1)) if plan.slots else None
+    plan.run(d_seq.ptr, d_cost.ptr, d_out.ptr if d_out else None, d_len.ptr if d_len else None)
+    ctx.sync()
+    if check_band:
+        bad = C.c_int32(0)
+        _check(load().poyg_plan_status(plan.h, C.byref(bad)), "poyg_plan_status")
+        if bad.value:
+            raise PoygError(f"{bad.value} backtraces left the filled band")
+    res = PairResult(d_cost.download(np.int32, plan.n),
+                     d_out.download(np.uint8, plan.out_bytes) if d_out else None,
+                     d_len.download(np.int32, plan.n * plan.slots) if d_len else None,
+                     plan.out_off, plan.cap, plan.slots)
+    plan.close()
+    return res

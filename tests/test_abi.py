@@ -1,0 +1,96 @@
+"""The C-ABI library builds, loads without a GPU and exports every entry
+point include/poyg.h declares; the host-side cost matrices equal the
+oracle's (no device work is done here)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "poyg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(poyg_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_symbols_exported():
+    from poyd_amd import engine
+    lib = engine.load()
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    from poyd_amd import engine
+    names = set(declared_functions())
+    assert set(engine.SIGNATURES) <= names | {"poyg_plan_status"}
+
+
+@pytest.mark.parametrize("sub,indel,go", [(1, 2, None), (1, 1, None), (2, 1, 1), (3, 1, 5), (1, 1, 3)])
+def test_engine_tcm_equals_oracle(oracle, sub, indel, go):
+    from poyd_amd import engine
+    cm = engine.CostMatrix.of_sub_indel(sub, indel, go)
+    ref = oracle.CM2.sub_indel(sub, indel, go)
+    for a in range(1, 32):
+        assert cm.prepend(a) == ref.cost(16, a) and cm.tail(a) == ref.cost(a, 16)
+        for b in range(1, 32):
+            assert cm.cost(a, b) == ref.cost(a, b)
+            assert cm.median(a, b) == ref.median(a, b)
+            assert cm.worst(a, b) == ref.worst(a, b)
+
+
+def test_engine_tcm3_equals_oracle(oracle):
+    from poyd_amd import engine
+    cm3 = engine.CostMatrix3(engine.CostMatrix.of_sub_indel(1, 2))
+    ref = oracle.CM3(oracle.CM2.sub_indel(1, 2))
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        a, b, c = (int(x) for x in rng.integers(1, 32, 3))
+        assert cm3.cost(a, b, c) == ref.cost(a, b, c)
+        assert cm3.median(a, b, c) == ref.median(a, b, c)
+
+
+def test_engine_deltaw_equals_oracle(oracle):
+    from poyd_amd import engine
+    cm = oracle.CM2.sub_indel(1, 2)
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        a = np.concatenate([[16], rng.integers(1, 32, int(rng.integers(0, 400)))]).astype(np.uint8)
+        b = np.concatenate([[16], rng.integers(1, 32, int(rng.integers(0, 400)))]).astype(np.uint8)
+        assert engine.deltaw(a, b) == oracle.deltaw(a, b, cm)
+        assert engine.deltaw(a, b, 7) == oracle.deltaw(a, b, cm, 7)
+
+
+def test_host_column_rules_equal_oracle(oracle):
+    """median_2 / with_gaps / ancestor_2 / verify_2 / worst_2 are host code
+    in the engine; they run without a GPU."""
+    import ctypes as C
+    from poyd_amd import engine
+    L = engine.load()
+    rng = np.random.default_rng(2)
+    for go in (None, 3):
+        cm = engine.CostMatrix.of_sub_indel(1, 1, go)
+        ref = oracle.CM2.sub_indel(1, 1, go)
+        for _ in range(50):
+            n = int(rng.integers(1, 60))
+            a = rng.integers(1, 32, n).astype(np.uint8)
+            b = rng.integers(1, 32, n).astype(np.uint8)
+            a[0] = b[0] = 16
+            out = np.zeros(n + 1, np.uint8)
+            ln = C.c_int32(0)
+            for fn, rfn in ((L.poyg_algn_median_2_no_gaps, oracle.median_2),
+                            (L.poyg_algn_median_2_with_gaps, oracle.median_2_with_gaps),
+                            (L.poyg_algn_ancestor_2, oracle.ancestor_2)):
+                assert fn(cm.h, engine._p8(a), engine._p8(b), n, engine._p8(out), n + 1, C.byref(ln)) == 0
+                assert np.array_equal(out[n + 1 - ln.value:], rfn(a, b, ref))
+            v = C.c_int32(0)
+            L.poyg_algn_verify_2(cm.h, engine._p8(a), engine._p8(b), n, C.byref(v))
+            assert v.value == oracle.verify_2(a, b, ref)
+            L.poyg_algn_worst_2(cm.h, engine._p8(a), engine._p8(b), n, C.byref(v))
+            assert v.value == oracle.worst_2(a, b, ref)

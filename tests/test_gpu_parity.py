@@ -1,0 +1,333 @@
+"""GPU parity: every engine path against the CPU oracle, bit for bit.
+
+All calls go through the C ABI (libpoyg.so).  Inputs are seeded; each case
+prints its name before running so that a failing run names it.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from poyd_amd import synth
+
+pytestmark = pytest.mark.gpu
+GAP = 16
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from poyd_amd import engine
+    try:
+        return engine.Context(0)
+    except engine.PoygError as e:
+        pytest.skip(f"no GPU: {e}")
+
+
+def note(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def batch_of(seqs):
+    ln = np.array([len(s) for s in seqs], np.int32)
+    off = np.zeros(len(seqs), np.int64)
+    if len(seqs) > 1:
+        off[1:] = np.cumsum(ln[:-1])
+    return synth.SeqBatch(np.concatenate(seqs).astype(np.uint8) if seqs else np.zeros(0, np.uint8), off, ln)
+
+
+def rand_seq(rng, n, alphabet="dna"):
+    if alphabet == "dna":
+        body = rng.choice([1, 2, 4, 8], size=n)
+    elif alphabet == "ambig":
+        body = rng.integers(1, 16, size=n)
+    else:   # any 5-bit mask, gap-bearing symbols included
+        body = rng.integers(1, 32, size=n)
+    return np.concatenate([[GAP], body]).astype(np.uint8)
+
+
+def mutate(rng, s, ps, pi):
+    out = [GAP]
+    for x in s[1:]:
+        r = rng.random()
+        if r < pi / 2:
+            continue
+        if r < pi:
+            out.append(int(rng.choice([1, 2, 4, 8])))
+        out.append(int(rng.choice([1, 2, 4, 8])) if rng.random() < ps else int(x))
+    return np.array(out, np.uint8)
+
+
+def mixed_pairs(rng, n, alphabet="dna"):
+    A, B = [], []
+    edge = [(0, 0), (0, 5), (7, 0), (1, 1), (39, 40), (40, 40), (41, 41), (45, 120), (300, 12), (10, 700),
+            (700, 10), (250, 250), (1, 300)]
+    for la, lb in edge:
+        A.append(rand_seq(rng, la, alphabet))
+        B.append(rand_seq(rng, lb, alphabet))
+    for k in range(n):
+        L = int(rng.integers(20, 600))
+        a = rand_seq(rng, L, alphabet)
+        b = mutate(rng, a, 0.2, 0.08) if k % 4 else rand_seq(rng, int(rng.integers(1, 600)), alphabet)
+        if alphabet != "dna" and k % 2:
+            b = rand_seq(rng, len(b) - 1, alphabet)
+        A.append(a)
+        B.append(b)
+    return A, B
+
+
+AFFINE_MATRICES = [(1, 1, 3), (2, 1, 1), (3, 1, 5)]
+
+
+def test_concurrent_copies_agree(ctx, oracle):
+    """Thousands of copies of a few wide-band pairs in one batch: every copy
+    must give the oracle's result (many waves resident per workgroup)."""
+    from poyd_amd import engine
+    note("concurrent copies")
+    rng = np.random.default_rng(31)
+    base = [(rand_seq(rng, 388), rand_seq(rng, 542)), (rand_seq(rng, 300), rand_seq(rng, 420))]
+    a0 = rand_seq(rng, 900)
+    base.append((a0, mutate(rng, a0, 0.2, 0.06)))
+    A = [base[k % 3][0] for k in range(3000)]
+    B = [base[k % 3][1] for k in range(3000)]
+    cm = engine.CostMatrix.of_sub_indel(1, 1, 3)
+    ref = oracle.CM2.sub_indel(1, 1, 3)
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_AFFINE_ALIGN, batch_of(A), batch_of(B))
+    want = [oracle.align_affine_3(a, b, ref) for a, b in base]
+    bad = [k for k in range(3000) if res.cost[k] != want[k % 3][3]]
+    assert not bad, (len(bad), bad[:10])
+    for k in range(0, 3000, 7):
+        med, ri, rj, cost, mwg = want[k % 3]
+        for slot, w in enumerate((med, mwg, ri, rj)):
+            assert np.array_equal(res.seq(k, slot), w), (k, slot)
+    lcm = engine.CostMatrix.of_sub_indel(1, 2)
+    lref = oracle.CM2.sub_indel(1, 2)
+    res = engine.run_pairs(ctx, lcm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
+    lw = [oracle.align_2(a, b, lref) for a, b in base]
+    bad = [k for k in range(3000) if res.cost[k] != lw[k % 3][0]]
+    assert not bad, (len(bad), bad[:10])
+
+
+@pytest.mark.parametrize("sub,indel,go", AFFINE_MATRICES)
+@pytest.mark.parametrize("alphabet", ["dna", "any"])
+def test_affine_batch_matches_oracle(ctx, oracle, sub, indel, go, alphabet):
+    from poyd_amd import engine
+    note(f"affine batch sub={sub} indel={indel} go={go} alphabet={alphabet}")
+    rng = np.random.default_rng(100 + sub * 10 + go)
+    A, B = mixed_pairs(rng, 120, alphabet)
+    cm = engine.CostMatrix.of_sub_indel(sub, indel, go)
+    ref = oracle.CM2.sub_indel(sub, indel, go)
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_AFFINE_ALIGN, batch_of(A), batch_of(B))
+    cres = engine.run_pairs(ctx, cm, engine.POYG_MODE_AFFINE_COST, batch_of(A), batch_of(B))
+    for k, (a, b) in enumerate(zip(A, B)):
+        med, ri, rj, cost, mwg = oracle.align_affine_3(a, b, ref)
+        assert res.cost[k] == cost, (k, len(a), len(b))
+        assert cres.cost[k] == oracle.cost_affine_3(a, b, ref), (k, len(a), len(b))
+        for slot, want in enumerate((med, mwg, ri, rj)):
+            got = res.seq(k, slot)
+            assert np.array_equal(got, want), (k, slot, len(a), len(b))
+
+
+@pytest.mark.parametrize("sub,indel", [(1, 2), (1, 1), (2, 1)])
+@pytest.mark.parametrize("alphabet", ["dna", "ambig"])
+def test_linear_batch_matches_oracle(ctx, oracle, sub, indel, alphabet):
+    from poyd_amd import engine
+    note(f"linear batch sub={sub} indel={indel} alphabet={alphabet}")
+    rng = np.random.default_rng(200 + sub * 10 + indel)
+    A, B = mixed_pairs(rng, 120, alphabet)
+    cm = engine.CostMatrix.of_sub_indel(sub, indel)
+    ref = oracle.CM2.sub_indel(sub, indel)
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
+    cres = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_COST, batch_of(A), batch_of(B))
+    for k, (a, b) in enumerate(zip(A, B)):
+        cost, oa, ob = oracle.align_2(a, b, ref)
+        assert res.cost[k] == cost, (k, len(a), len(b))
+        assert cres.cost[k] == oracle.cost_2(a, b, ref), (k, len(a), len(b))
+        assert np.array_equal(res.seq(k, 0), oa), (k, len(a), len(b))
+        assert np.array_equal(res.seq(k, 1), ob), (k, len(a), len(b))
+        assert np.array_equal(res.seq(k, 2), oracle.median_2(oa, ob, ref)), k
+
+
+def test_linear_explicit_deltaw(ctx, oracle):
+    """DOS.distance passes ?deltaw (src/seqCS.ml:856-866); the batch takes the
+    resulting band parameter per pair."""
+    from poyd_amd import engine
+    note("linear explicit deltaw")
+    rng = np.random.default_rng(7)
+    A, B = mixed_pairs(rng, 40)
+    cm = engine.CostMatrix.of_sub_indel(1, 2)
+    ref = oracle.CM2.sub_indel(1, 2)
+    dws = [max(8, abs(len(a) - len(b))) for a, b in zip(A, B)]
+    bands = np.array([engine.deltaw(a, b, d) for a, b, d in zip(A, B, dws)], np.int32)
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_COST, batch_of(A), batch_of(B), bands)
+    for k, (a, b) in enumerate(zip(A, B)):
+        assert res.cost[k] == oracle.cost_2(a, b, ref, dws[k])
+
+
+def test_per_call_entry_points(ctx, oracle):
+    from poyd_amd import engine
+    note("per-call entry points")
+    rng = np.random.default_rng(9)
+    al = engine.Align(ctx)
+    cm = engine.CostMatrix.of_sub_indel(1, 2)
+    ref = oracle.CM2.sub_indel(1, 2)
+    cma = engine.CostMatrix.of_sub_indel(1, 1, 3)
+    refa = oracle.CM2.sub_indel(1, 1, 3)
+    for k in range(12):
+        a = rand_seq(rng, int(rng.integers(1, 200)))
+        b = mutate(rng, a, 0.1, 0.05) if k % 2 else rand_seq(rng, int(rng.integers(1, 200)))
+        assert al.cost_2(a, b, cm) == oracle.cost_2(a, b, ref)
+        x, y, c = al.align_2(a, b, cm)
+        c2, ox, oy = oracle.align_2(a, b, ref)
+        assert c == c2 and np.array_equal(x, ox) and np.array_equal(y, oy)
+        got = al.align_affine_3(a, b, cma)
+        want = oracle.align_affine_3(a, b, refa)
+        assert got[3] == want[3]
+        for g, w in zip(got, want):
+            if isinstance(g, np.ndarray):
+                assert np.array_equal(g, w)
+        assert al.cost_2(a, b, cma) == oracle.cost_affine_3(a, b, refa)
+
+
+def test_three_d_matches_oracle(ctx, oracle):
+    import ctypes as C
+    from poyd_amd import engine
+    note("three-dimensional alignment")
+    rng = np.random.default_rng(11)
+    cm3 = engine.CostMatrix3(engine.CostMatrix.of_sub_indel(1, 2))
+    ref = oracle.CM3(oracle.CM2.sub_indel(1, 2))
+    L = engine.load()
+    for k in range(16):
+        anc = rand_seq(rng, int(rng.integers(0, 40)))
+        s = [mutate(rng, anc, 0.15, 0.1) for _ in range(3)]
+        if k == 0:
+            s = [rand_seq(rng, 0), rand_seq(rng, 0), rand_seq(rng, 0)]
+        if k == 1:
+            s = [rand_seq(rng, 0), rand_seq(rng, 5), rand_seq(rng, 9)]
+        cap = sum(len(x) for x in s) + 1
+        outs = [np.zeros(cap, np.uint8) for _ in range(3)]
+        ln = C.c_int32(0)
+        cost = C.c_int32(0)
+        rc = L.poyg_algn_align_3d(ctx.h, cm3.h, *[v for x in s for v in (engine._p8(x), len(x))],
+                                  *[engine._p8(o) for o in outs], cap, C.byref(ln), C.byref(cost))
+        assert rc == 0, engine.load().poyg_last_error()
+        want = oracle.align_3(*s, ref)
+        assert cost.value == want[0], k
+        for o, w in zip(outs, want[1:]):
+            assert np.array_equal(o[cap - ln.value:], w), k
+
+
+def test_three_d_batch_matches_oracle(ctx, oracle):
+    from poyd_amd import engine
+    note("three-dimensional batch")
+    rng = np.random.default_rng(12)
+    cm2 = engine.CostMatrix.of_sub_indel(1, 2)
+    cm3 = engine.CostMatrix3(cm2)
+    ref = oracle.CM3(oracle.CM2.sub_indel(1, 2))
+    T = []
+    for k in range(40):
+        anc = rand_seq(rng, int(rng.integers(5, 60)))
+        T.append([mutate(rng, anc, 0.15, 0.06) for _ in range(3)])
+    seqs = [x for t in T for x in t]
+    bt = batch_of(seqs)
+    off = bt.off.reshape(-1, 3)
+    ln = bt.ln.reshape(-1, 3)
+    import ctypes as C
+    L = engine.load()
+    h = engine.vp()
+    cols = [np.ascontiguousarray(off[:, q]) for q in range(3)] + [np.ascontiguousarray(ln[:, q]) for q in range(3)]
+    rc = L.poyg_plan_triples(ctx.h, cm3.h, len(T), cols[0].ctypes.data_as(engine.i64p), cols[3].ctypes.data_as(engine.i32p),
+                             cols[1].ctypes.data_as(engine.i64p), cols[4].ctypes.data_as(engine.i32p),
+                             cols[2].ctypes.data_as(engine.i64p), cols[5].ctypes.data_as(engine.i32p), C.byref(h))
+    assert rc == 0
+    ob = L.poyg_plan_out_bytes(h)
+    oo = np.zeros(len(T), np.int64)
+    L.poyg_plan_out_offsets(h, oo.ctypes.data_as(engine.i64p))
+    d_seq = engine.DeviceBuffer.from_array(ctx, bt.seq)
+    d_cost = engine.DeviceBuffer(ctx, 4 * len(T))
+    d_out = engine.DeviceBuffer(ctx, ob)
+    d_len = engine.DeviceBuffer(ctx, 16 * len(T))
+    assert L.poyg_plan_run(h, d_seq.ptr, d_cost.ptr, d_out.ptr, d_len.ptr, None) == 0
+    ctx.sync()
+    bad = C.c_int32(0)
+    L.poyg_plan_status(h, C.byref(bad))
+    assert bad.value == 0
+    cost = d_cost.download(np.int32, len(T))
+    out = d_out.download(np.uint8, ob)
+    lens = d_len.download(np.int32, 4 * len(T))
+    L.poyg_plan_destroy(h)
+    for k, t in enumerate(T):
+        want = oracle.align_3(*t, ref)
+        assert cost[k] == want[0], k
+        cap = ((sum(len(x) for x in t) + 1 + 15) // 16) * 16
+        for q in range(3):
+            n = lens[4 * k + q]
+            base = oo[k] + q * cap
+            assert np.array_equal(out[base + cap - n: base + cap], want[1 + q]), (k, q)
+        n = lens[4 * k + 3]
+        base = oo[k] + 3 * cap
+        assert np.array_equal(out[base + cap - n: base + cap], oracle.median_3(*want[1:], ref)), k
+
+
+def test_fitch_matches_oracle(ctx, oracle):
+    from poyd_amd import engine
+    note("fitch down/up")
+    rng = np.random.default_rng(13)
+    for length in (1, 17, 10000, 4096):
+        a, b, c, d = (rng.integers(1, 256, length).astype(np.uint8) for _ in range(4))
+        res = np.zeros(length, np.uint8)
+        import ctypes as C
+        added = C.c_int64(0)
+        L = engine.load()
+        assert L.poyg_nonadd_median(ctx.h, engine._p8(a), engine._p8(b), engine._p8(res), length, C.byref(added)) == 0
+        want, wadd = oracle.nonadd_median(a, b)
+        assert np.array_equal(res, want) and added.value == wadd
+        assert L.poyg_nonadd_median_3(ctx.h, engine._p8(a), engine._p8(b), engine._p8(c), engine._p8(d),
+                                      engine._p8(res), length) == 0
+        assert np.array_equal(res, oracle.nonadd_median_3(a, b, c, d))
+
+
+def test_workload_pairwise_linear_1000(ctx, oracle):
+    """The 1,000-pair linear workload (L 200-400, 10% subs, 2% indels),
+    every pair bit-exact: cost, both aligned sequences and the median."""
+    from poyd_amd import engine
+    note("workload pairwise-linear-1000")
+    w = synth.workload_c1()
+    cm = engine.CostMatrix.of_sub_indel(1, 2)
+    ref = oracle.CM2.sub_indel(1, 2)
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, w.a, w.b)
+    for k in range(len(w.a)):
+        a, b = w.a.get(k), w.b.get(k)
+        cost, oa, ob = oracle.align_2(a, b, ref)
+        assert res.cost[k] == cost
+        assert np.array_equal(res.seq(k, 0), oa) and np.array_equal(res.seq(k, 1), ob)
+        assert np.array_equal(res.seq(k, 2), oracle.median_2(oa, ob, ref))
+
+
+def test_workload_pairwise_affine_sample(ctx, oracle):
+    """The 65,536-pair affine workload: a seeded sample of 1,500 pairs
+    bit-exact (all four outputs and the cost) and, for all pairs, the
+    size-independent properties: recost of the alignment equals the cost and
+    stripping gaps returns the inputs."""
+    from poyd_amd import engine
+    note("workload pairwise-affine-65536")
+    w = synth.workload_c2()
+    cm = engine.CostMatrix.of_sub_indel(1, 1, 3)
+    ref = oracle.CM2.sub_indel(1, 1, 3)
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_AFFINE_ALIGN, w.a, w.b)
+    rng = np.random.default_rng(5)
+    sample = rng.choice(len(w.a), size=1500, replace=False)
+    for k in sample:
+        a, b = w.a.get(k), w.b.get(k)
+        med, ri, rj, cost, mwg = oracle.align_affine_3(a, b, ref)
+        assert res.cost[k] == cost, k
+        for slot, want in enumerate((med, mwg, ri, rj)):
+            assert np.array_equal(res.seq(k, slot), want), (k, slot)
+    bad = 0
+    for k in range(len(w.a)):
+        ri, rj = res.seq(k, 2), res.seq(k, 3)
+        if oracle.verify_2(ri, rj, ref) != res.cost[k]:
+            bad += 1
+        elif not (np.array_equal(ri[ri != GAP], w.a.get(k)[1:]) and np.array_equal(rj[rj != GAP], w.b.get(k)[1:])):
+            bad += 1
+    assert bad == 0
