@@ -1,0 +1,226 @@
+"""Benchmark of the direct-optimization engine on one MI355X.
+
+Default workload: 65,536 DNA pairs, lengths log-uniform on [50, 5000],
+divergence U[5%, 30%] (substitutions d, indels d/5), affine gaps (open 3,
+extend 1, substitution 1).  One step = one batched align_affine_3 over all
+pairs: cost, both aligned sequences, median and median-with-gaps, with the
+sequences resident in HBM before timing starts.  The metric is band-GCUPS:
+DP cells the reference algorithm evaluates (its band) per second.
+
+Prints one JSON line (see README of the bench contract in DESIGN.md).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DP GCUPS (batched NW); TBR joins/s surrogate+exact vs exhaustive, same cost"
+
+# int32 VALU operations per DP cell counted from the reference recurrences
+# (src/algn.c:1740-1977 affine, :87-453 linear, :2812-3055 cube).
+OPS_PER_CELL = {"affine": 30, "linear": 8, "cube": 20}
+# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz int32 VALU ops; HBM3E 8 TB/s.
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def workload(name: str):
+    from poyd_amd import synth
+    if name == "affine":
+        return synth.workload_c2(), "affine"
+    if name == "linear":
+        return synth.workload_c1(), "linear"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(w, kind: str, budget_s: float):
+    """Single-core oracle (the C restatement of the reference algorithm,
+    built -O2 without -march like the reference) over a seeded sample of
+    the same pairs, timed until budget_s of CPU work has been spent."""
+    from oracle import pyoracle as po
+    cm = po.CM2.sub_indel(1, 1, 3) if kind == "affine" else po.CM2.sub_indel(1, 2)
+    n = len(w.a)
+    rng = np.random.default_rng(7)
+    order = rng.permutation(n)
+    try:
+        old_aff = os.sched_getaffinity(0)
+        core = min(old_aff)
+        os.sched_setaffinity(0, {core})
+    except (AttributeError, OSError):
+        old_aff = None
+    cells = 0
+    done = 0
+    t0 = time.perf_counter()
+    for k in order:
+        a, b = w.a.get(int(k)), w.b.get(int(k))
+        if kind == "affine":
+            po.align_affine_3(a, b, cm)
+            cells += po.affine_cells(len(a), len(b))
+        else:
+            po.align_2(a, b, cm)
+            l1, l2 = max(len(a), len(b)), min(len(a), len(b))
+            cells += po.algn_nw_cells(l1, l2, po.deltaw(a, b, cm))
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    if old_aff is not None:
+        os.sched_setaffinity(0, old_aff)
+    return {
+        "value": cells / dt / 1e9,
+        "unit": "GCUPS",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} of {n} pairs (seeded random order), {cells} band cells in {dt:.1f} s, "
+                  f"align + median per pair, 1 pinned core of {cpu_model()}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="affine", choices=["affine", "linear"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(pairs are independent: N GPUs would each take 1/N of the batch)", file=sys.stderr)
+        return 2
+
+    from poyd_amd import engine
+    w, kind = workload(args.workload)
+    ctx = engine.Context(0)
+    if kind == "affine":
+        cm = engine.CostMatrix.of_sub_indel(1, 1, 3)
+        mode = engine.POYG_MODE_AFFINE_ALIGN
+        dws = None
+    else:
+        cm = engine.CostMatrix.of_sub_indel(1, 2)
+        mode = engine.POYG_MODE_LINEAR_ALIGN
+        dws = engine.batch_deltaw(w.a, w.b)
+    seq = np.concatenate([w.a.seq, w.b.seq])
+    plan = engine.PairPlan(ctx, cm, mode, w.a.off, w.a.ln, w.b.off + len(w.a.seq), w.b.ln, dws)
+    d_seq = engine.DeviceBuffer.from_array(ctx, seq)
+    d_cost = engine.DeviceBuffer(ctx, 4 * plan.n)
+    d_out = engine.DeviceBuffer(ctx, plan.out_bytes)
+    d_len = engine.DeviceBuffer(ctx, 4 * plan.n * plan.slots)
+    stream = load_stream(ctx)
+
+    def step():
+        plan.run(d_seq.ptr, d_cost.ptr, d_out.ptr, d_len.ptr, stream)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    first_costs = d_cost.download(np.int32, plan.n)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    costs = d_cost.download(np.int32, plan.n)
+    deterministic = bool(np.array_equal(costs, first_costs))
+
+    # kernel timings from HIP events recorded on the run stream
+    plan.set_timing(True)
+    fills, traces = [], []
+    for _ in range(max(3, min(args.steps, 10))):
+        step()
+        f, t = plan.last_times()
+        fills.append(f)
+        traces.append(t)
+    plan.set_timing(False)
+    fill_ms = float(np.mean(fills))
+    trace_ms = float(np.mean(traces))
+
+    cells = plan.cells
+    gcups = cells / (ms / 1e3) / 1e9
+    ops = OPS_PER_CELL[kind]
+    valu_tops = cells * ops / (fill_ms / 1e3) / 1e12
+    # algorithmic HBM bytes of the fill: one direction byte per band cell,
+    # both sequences read once, one int32 cost written per pair
+    alg_bytes = cells + int(w.a.ln.sum() + w.b.ln.sum()) + 4 * plan.n
+    fill_gbs = alg_bytes / (fill_ms / 1e3) / 1e9
+    matrix_cells = int(((w.a.ln.astype(np.int64) - 1) * (w.b.ln.astype(np.int64) - 1)).sum())
+
+    out = {
+        "metric": METRIC,
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": w.name,
+            "pairs": plan.n,
+            "band_cells": int(cells),
+            "matrix_equivalent_gcups": round(matrix_cells / (ms / 1e3) / 1e9, 3),
+            "cost_model": "affine open 3 extend 1 sub 1" if kind == "affine" else "linear sub 1 indel 2",
+            "outputs": "cost, aligned pair, median" + (", median with gaps" if kind == "affine" else ""),
+            "fill_ms": round(fill_ms, 3),
+            "backtrace_ms": round(trace_ms, 3),
+            "deterministic": deterministic,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(fill_gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(fill_gbs / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "kernel": "aff_fill_lds_kernel" if kind == "affine" else "lin_fill_lds_kernel",
+        },
+        "valu_roofline": {
+            "bound": "valu",
+            "achieved": round(valu_tops, 3),
+            "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "TOP/s (int32)",
+            "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
+            "ops_per_cell": ops,
+        },
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(w, kind, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["config"]["gpu_vs_cpu_1core"] = round(gcups / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+    plan.close()
+    return 0
+
+
+def load_stream(ctx):
+    from poyd_amd import engine
+    return engine.load().poyg_ctx_stream(ctx.h)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
