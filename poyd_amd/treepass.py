@@ -1,0 +1,299 @@
+"""Tree evaluation of dynamic (DO) characters: the composition POY's
+AllDirChar performs around the alignment kernels, batched so that every
+alignment of one tree level runs as one GPU plan.
+
+What is computed (reference behaviour, restated):
+
+* Directional medians (AllDirChar.internal_downpass,
+  src/allDirChar.ml:722-785): every interior vertex v holds, for each
+  neighbour u, the median of the two subtrees on its other sides
+  ("v not_with u"). A median of two vertices orders them by the smallest
+  taxon code in their subtrees (Node.cs_median, src/node.ml:341-347), and
+  per character is DOS.median (src/seqCS.ml:747-776): an empty side passes
+  the other through at cost 0; otherwise linear models align (align_2) and
+  take ancestor_2, affine models take align_affine_3's median.  A subtree's
+  cost is the sum of its medians' costs (Node.median, src/node.ml:638-678).
+* Edge medians (create_lazy_edge, src/allDirChar.ml:58-73) and the root
+  (create_root / pick_best_root, src/allDirChar.ml:97-125, 836-861): the
+  first root is the handle's edge; then the edges, oriented away from the
+  handle and sorted by decreasing (a, b), replace it whenever their total
+  downpass cost is strictly smaller.
+* Single assignment (assign_single, src/allDirChar.ml:288-409): the root
+  single is closest(root median, handle side); every vertex then gets
+  closest(parent single, its directional median away from the parent)
+  (DOS.to_single, src/seqCS.ml:730-745; Sequence.Align.closest,
+  src/sequence.ml:985-1033; Cost_matrix.Two_D.get_closest,
+  src/cost_matrix.ml:681-700).
+* Tree cost (check_cost, src/allDirChar.ml:185-213): the sum over edges of
+  DOS.distance between the two singles (src/seqCS.ml:818-864: band
+  deltaw = max(8, length difference) for linear models, cost_2_affine for
+  affine ones; 0 when a side is empty).
+
+The alignments go to a backend with three batched operations (median,
+align, distance).  EngineBackend runs them on the GPU through libpoyg.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .treedata import GAP, Dataset, Tree
+
+
+def is_empty(s: np.ndarray) -> bool:
+    """Sequence.is_empty (src/sequence.ml:442-450): every element is a gap."""
+    return bool((s == GAP).all())
+
+
+def closest_table(cost: np.ndarray) -> np.ndarray:
+    """Cost_matrix.Two_D.get_closest for all (a, b) of a combination
+    alphabet with gap bit 16: the first bit of b (lowest first) whose cost
+    against a is strictly below all earlier ones, after b is reduced to the
+    gap (both carry it) or stripped of it.  Entries that leave no bit are 0
+    (the reference fails there)."""
+    out = np.zeros((32, 32), np.uint8)
+    for a in range(32):
+        for b in range(32):
+            if a == GAP or b == GAP:
+                bb = b
+            elif (a & GAP) and (b & GAP):
+                bb = GAP
+            else:
+                bb = b & ~GAP
+            best, best_cost = a, None
+            for bit in range(5):
+                x = 1 << bit
+                if bb & x:
+                    c = int(cost[a, x])
+                    if best_cost is None or c < best_cost:
+                        best, best_cost = x, c
+            out[a, b] = best if best_cost is not None else 0
+    return out
+
+
+def remove_gaps(s: np.ndarray) -> np.ndarray:
+    """closest's remove_gaps: drop gap elements, then prepend one gap."""
+    body = s[s != GAP]
+    return np.concatenate([np.array([GAP], np.uint8), body])
+
+
+@dataclass
+class DirData:
+    """A vertex seen from one neighbour: one sequence per character, the
+    subtree's downpass cost and the smallest taxon code inside it."""
+
+    seqs: list
+    cost: int
+    mcc: int
+
+
+@dataclass
+class Diagnosis:
+    cost: int                 # Adjusted tree cost
+    downpass: int             # total downpass cost at the chosen root
+    root: tuple[int, int]
+    singles: dict
+    alignments: int           # pairwise alignments issued
+
+
+class TreeEval:
+    """All-directions diagnosis of one tree over a Dataset of DO characters."""
+
+    def __init__(self, ds: Dataset, tree: Tree, backend):
+        self.ds, self.t, self.be = ds, tree, backend
+        self.closest = closest_table(backend.cost_table())
+        self.n_align = 0
+
+    # ---------------------------------------------------------- medians
+    def _medians(self, jobs):
+        """jobs: list of (x: DirData, y: DirData).  Returns DirData list."""
+        pairs, where = [], []
+        out = []
+        for j, (x, y) in enumerate(jobs):
+            first, second = (x, y) if x.mcc < y.mcc else (y, x)
+            seqs, cost = [], 0
+            for c in range(len(self.ds.chars)):
+                a, b = first.seqs[c], second.seqs[c]
+                if is_empty(a):
+                    seqs.append(b)
+                elif is_empty(b):
+                    seqs.append(a)
+                else:
+                    seqs.append(None)
+                    pairs.append((a, b))
+                    where.append((j, c))
+            out.append(DirData(seqs, x.cost + y.cost, min(x.mcc, y.mcc)))
+        if pairs:
+            res = self.be.median(pairs)
+            self.n_align += len(pairs)
+            for (j, c), (m, cost) in zip(where, res):
+                out[j].seqs[c] = m
+                out[j].cost += int(cost)
+        return out
+
+    def directional(self):
+        """dir[(u, v)] = v not_with u, for every ordered adjacent pair."""
+        t = self.t
+        leaf = {v: DirData([ch[v - 1] for ch in self.ds.chars], 0, v) for v in t.leaves}
+        height = {}
+
+        def h(u, v):
+            key = (u, v)
+            if key in height:
+                return height[key]
+            stack = [key]
+            while stack:
+                uu, vv = stack[-1]
+                if vv in t.leaves:
+                    height[(uu, vv)] = 0
+                    stack.pop()
+                    continue
+                kids = [(vv, w) for w in t.others(vv, uu)]
+                todo = [k for k in kids if k not in height]
+                if todo:
+                    stack.extend(todo)
+                    continue
+                height[(uu, vv)] = 1 + max(height[k] for k in kids)
+                stack.pop()
+            return height[key]
+
+        keys = [(u, v) for v in t.nbrs for u in t.nbrs[v]]
+        for k in keys:
+            h(*k)
+        d = {}
+        for (u, v), hv in height.items():
+            if hv == 0:
+                d[(u, v)] = leaf[v]
+        for level in range(1, max(height.values()) + 1):
+            batch = [k for k, hv in height.items() if hv == level]
+            batch.sort()
+            jobs = []
+            for (u, v) in batch:
+                x, y = t.others(v, u)
+                jobs.append((d[(v, x)], d[(v, y)]))
+            for k, r in zip(batch, self._medians(jobs)):
+                d[k] = r
+        return d
+
+    # ---------------------------------------------------------- singles
+    def _to_single(self, jobs):
+        """jobs: list of (parent seqs, mine seqs) -> list of single seqs
+        (DOS.to_single per character)."""
+        out = [list(m) for _, m in jobs]
+        pairs, where = [], []
+        for j, (p, m) in enumerate(jobs):
+            for c in range(len(m)):
+                mine = m[c]
+                if is_empty(mine):
+                    continue
+                par = mine if is_empty(p[c]) else p[c]
+                if par.shape == mine.shape and np.array_equal(par, mine):
+                    # both equal: drop the gap bit after position 0
+                    s1 = par.copy()
+                    s1[1:] &= ~np.uint8(GAP)
+                    out[j][c] = remove_gaps(self.closest[s1, s1])
+                else:
+                    pairs.append((par, mine))
+                    where.append((j, c))
+        if pairs:
+            res = self.be.align(pairs)
+            self.n_align += len(pairs)
+            for (j, c), (a1, a2) in zip(where, res):
+                out[j][c] = remove_gaps(self.closest[a1, a2])
+        return out
+
+    def diagnose(self) -> Diagnosis:
+        t = self.t
+        d = self.directional()
+        # edge medians
+        pre = t.pre_order_edges()
+        jobs = [(d[(b, a)], d[(a, b)]) for (a, b) in pre]
+        emed = dict(zip(pre, self._medians(jobs)))
+        # root: the handle's edge first, then decreasing (a, b)
+        h = t.handle
+        root = (h, t.nbrs[h][0])
+        emed_any = {frozenset(k): v for k, v in emed.items()}
+        best = emed_any[frozenset(root)].cost
+        for e in sorted(pre, reverse=True):
+            c = emed[e].cost
+            if abs(best) > abs(c):
+                best, root = c, e
+        a, b = root
+        rmed = emed_any[frozenset(root)]
+        rsingle = self._to_single([(rmed.seqs, d[(b, a)].seqs)])[0]
+        singles = {}
+        frontier = [(rsingle, b, a), (rsingle, a, b)]   # (parent single, parent, current)
+        while frontier:
+            res = self._to_single([(ps, d[(p, v)].seqs) for ps, p, v in frontier])
+            nxt = []
+            for (ps, p, v), s in zip(frontier, res):
+                singles[v] = s
+                for w in t.others(v, p):
+                    nxt.append((s, v, w))
+            frontier = nxt
+        # adjusted cost: distances over every edge
+        pairs = []
+        for (p, v) in pre:
+            for c in range(len(self.ds.chars)):
+                x, y = singles[p][c], singles[v][c]
+                if is_empty(x) or is_empty(y):
+                    continue
+                pairs.append((x, y))
+        total = 0
+        if pairs:
+            total = int(sum(int(x) for x in self.be.distance(pairs)))
+            self.n_align += len(pairs)
+        return Diagnosis(total, int(best), root, singles, self.n_align)
+
+
+# ---------------------------------------------------------------- backends
+
+class EngineBackend:
+    """Batched alignments on the GPU through libpoyg plans."""
+
+    def __init__(self, ctx, cm):
+        from . import engine
+        self.engine = engine
+        self.ctx, self.cm = ctx, cm
+        self.affine = cm.affine
+
+    def cost_table(self) -> np.ndarray:
+        return np.array([[self.cm.cost(a, b) for b in range(32)] for a in range(32)], np.int64)
+
+    @staticmethod
+    def _batch(seqs):
+        from .synth import SeqBatch
+        ln = np.array([len(s) for s in seqs], np.int32)
+        off = np.zeros(len(seqs), np.int64)
+        if len(seqs) > 1:
+            off[1:] = np.cumsum(ln[:-1], dtype=np.int64)
+        return SeqBatch(np.concatenate(seqs).astype(np.uint8), off, ln)
+
+    def _run(self, mode, pairs, dws=None):
+        A = self._batch([p[0] for p in pairs])
+        B = self._batch([p[1] for p in pairs])
+        return self.engine.run_pairs(self.ctx, self.cm, mode, A, B, dws)
+
+    def median(self, pairs):
+        e = self.engine
+        if self.affine:
+            r = self._run(e.POYG_MODE_AFFINE_ALIGN, pairs)
+            return [(r.seq(k, 0).copy(), int(r.cost[k])) for k in range(len(pairs))]
+        r = self._run(e.POYG_MODE_LINEAR_ALIGN, pairs)
+        return [(r.seq(k, 2).copy(), int(r.cost[k])) for k in range(len(pairs))]
+
+    def align(self, pairs):
+        e = self.engine
+        if self.affine:
+            r = self._run(e.POYG_MODE_AFFINE_ALIGN, pairs)
+            return [(r.seq(k, 2).copy(), r.seq(k, 3).copy()) for k in range(len(pairs))]
+        r = self._run(e.POYG_MODE_LINEAR_ALIGN, pairs)
+        return [(r.seq(k, 0).copy(), r.seq(k, 1).copy()) for k in range(len(pairs))]
+
+    def distance(self, pairs):
+        e = self.engine
+        if self.affine:
+            return self._run(e.POYG_MODE_AFFINE_COST, pairs).cost.tolist()
+        dws = np.array([e.deltaw(a, b, max(8, abs(len(a) - len(b)))) for a, b in pairs], np.int32)
+        return self._run(e.POYG_MODE_LINEAR_COST, pairs, dws).cost.tolist()

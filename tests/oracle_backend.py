@@ -1,0 +1,58 @@
+"""Tree-pass backend that runs every alignment on the CPU oracle (test
+infrastructure: the checker for poyd_amd.treepass on the GPU)."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import pyoracle as po
+
+
+class OracleBackend:
+    def __init__(self, cm: po.CM2, affine: bool):
+        self.cm, self.affine = cm, affine
+
+    def cost_table(self) -> np.ndarray:
+        return np.array([[self.cm.cost(a, b) for b in range(32)] for a in range(32)], np.int64)
+
+    def median(self, pairs):
+        out = []
+        for a, b in pairs:
+            if self.affine:
+                med, ri, rj, cost, mwg = po.align_affine_3(a, b, self.cm)
+                out.append((med, cost))
+            else:
+                cost, oa, ob = po.align_2(a, b, self.cm)
+                out.append((po.ancestor_2(oa, ob, self.cm), cost))
+        return out
+
+    def align(self, pairs):
+        out = []
+        for a, b in pairs:
+            if self.affine:
+                med, ri, rj, cost, mwg = po.align_affine_3(a, b, self.cm)
+                out.append((ri, rj))
+            else:
+                cost, oa, ob = po.align_2(a, b, self.cm)
+                out.append((oa, ob))
+        return out
+
+    def distance(self, pairs):
+        if self.affine:
+            return [po.cost_affine_3(a, b, self.cm) for a, b in pairs]
+        return [po.cost_2(a, b, self.cm, max(8, abs(len(a) - len(b)))) for a, b in pairs]
+
+
+# the four scripts of the reference's tree-cost checks: cc (default DNA
+# matrix), cc1 tcm:(1,1), cc2 tcm:(2,1) gap_opening:1, cc3 tcm:(3,1) gap_opening:5
+MODELS = {
+    "cc": (1, 2, None),
+    "cc1": (1, 1, None),
+    "cc2": (2, 1, 1),
+    "cc3": (3, 1, 5),
+}
+TREE_SUFFIX = {"cc": ".tree", "cc1": ".1.tree", "cc2": ".2.tree", "cc3": ".3.tree"}
+
+
+def model_cm(name: str) -> po.CM2:
+    sub, indel, go = MODELS[name]
+    return po.CM2.sub_indel(sub, indel, go)
