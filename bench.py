@@ -197,7 +197,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(fill_gbs / HBM_PEAK_GBS, 5),
             "traffic": None,
-            "kernel": "aff_fill_lds_kernel" if kind == "affine" else "lin_fill_lds_kernel",
+            "kernel": "aff_diag_kernel" if kind == "affine" else "lin_fill_lds_kernel",
         },
         "valu_roofline": {
             "bound": "valu",

@@ -17,6 +17,9 @@ hipError_t launch_prep(bool affine, int grid, const PairMeta *meta, int n, const
 hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
                            int32_t *cost, int *rowscratch, int row_width, hipStream_t s);
+hipError_t launch_aff_diag(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
+                           hipStream_t s);
 hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
                             const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
                             hipStream_t s);
@@ -94,15 +97,16 @@ struct poyg_plan {
   int64_t out_bytes = 0;
   std::vector<int64_t> out_off;
   // device
-  DevBuf tcm, meta, order_all, order_lds, order_glb, dm, aux, rows, counters, status;
-  int n_lds = 0, n_glb = 0;
-  int grid_lds = 0, grid_glb = 0, row_width = 0;
+  DevBuf tcm, meta, order_all, order_lds, order_glb, order_diag, dm, aux, rows, counters, status, sink;
+  int n_lds = 0, n_glb = 0, n_diag = 0;
+  int grid_lds = 0, grid_glb = 0, grid_diag = 0, row_width = 0;
   bool timing = false;
   float fill_ms = 0.f, trace_ms = 0.f;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   ~poyg_plan() {
     if (triple) poyg_triple_plan_destroy(triple);
-    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &dm, &aux, &rows, &counters, &status})
+    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_diag, &dm, &aux, &rows, &counters,
+                      &status, &sink})
       b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -281,7 +285,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   P->slots = align ? (affine ? 4 : 3) : 0;
   std::vector<PairMeta> meta(n);
   std::vector<int64_t> cells(n);
-  std::vector<char> glb(n, 0);
+  std::vector<char> glb(n, 0), diag(n, 0);
   int64_t dm_bytes = 0, aux_ints = 0, out_bytes = 0;
   int row_width = 0;
   P->out_off.resize(n);
@@ -308,12 +312,28 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       const int ra = m.lr - 1, cb = m.lc - 1;
       const int e0 = std::min(cb, std::max(40, cb - ra + 8));
       m.p0 = e0;
-      m.stride = (int32_t)((std::min(cb + 1, e0 + 41) + 3) & ~3);
-      m.aux_off = aux_ints;
-      aux_ints += (2 * (int64_t)(cb + 1) + 63) & ~(int64_t)63;   // 256-byte aligned regions
-      if (align) {
-        m.dm_off = dm_bytes;
-        dm_bytes += (int64_t)ra * m.stride;
+      const int lanes = ((e0 + 40) >> 1) + 1;   // band diagonal pairs
+      if (lanes <= kWave) {
+        // anti-diagonal wavefront: one dword per lane and 4 steps
+        diag[k] = 1;
+        m.p1 = lanes;
+        m.p2 = 1;
+        if (align && ra > 0) {
+          m.dm_off = dm_bytes;
+          dm_bytes += (((int64_t)ra + cb + 4) >> 2) * 4 * lanes;
+        }
+      } else {
+        m.stride = (int32_t)((std::min(cb + 1, e0 + 41) + 3) & ~3);
+        m.aux_off = aux_ints;
+        aux_ints += (2 * (int64_t)(cb + 1) + 63) & ~(int64_t)63;   // 256-byte aligned regions
+        if (align) {
+          m.dm_off = dm_bytes;
+          dm_bytes += (int64_t)ra * m.stride;
+        }
+        if (m.stride > kAffRBHost) {
+          glb[k] = 1;
+          row_width = std::max(row_width, (cb + 2 + 63) & ~63);
+        }
       }
       int64_t c = 0;
       for (int i = 1; i <= ra; ++i) {
@@ -321,10 +341,6 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
         c += ep - sp + 1;
       }
       cells[k] = c;
-      if (m.stride > kAffRBHost) {
-        glb[k] = 1;
-        row_width = std::max(row_width, (cb + 2 + 63) & ~63);
-      }
     } else {
       // Sequence.Align.cost_2 / create_edited_2: the longer sequence goes first
       m.swap = la < lb;
@@ -364,10 +380,12 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a] > cells[b]; });
-  std::vector<int32_t> o_lds, o_glb;
-  for (int k : order) (glb[k] ? o_glb : o_lds).push_back(k);
+  std::vector<int32_t> o_lds, o_glb, o_diag;
+  for (int k : order) (diag[k] ? o_diag : glb[k] ? o_glb : o_lds).push_back(k);
   P->n_lds = (int)o_lds.size();
   P->n_glb = (int)o_glb.size();
+  P->n_diag = (int)o_diag.size();
+  P->grid_diag = std::max(1, std::min((P->n_diag + 3) / 4, ctx->cus * 8));
   P->row_width = row_width;
   const int per_cu = affine ? 4 : 8;
   P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
@@ -380,11 +398,13 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   rc |= P->order_all.alloc(sizeof(int32_t) * (size_t)n);
   rc |= P->order_lds.alloc(sizeof(int32_t) * (size_t)P->n_lds);
   rc |= P->order_glb.alloc(sizeof(int32_t) * (size_t)P->n_glb);
+  rc |= P->order_diag.alloc(sizeof(int32_t) * (size_t)P->n_diag);
   rc |= P->dm.alloc((size_t)dm_bytes);
   rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
   rc |= P->rows.alloc(P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16);
   rc |= P->counters.alloc(sizeof(int) * 4);
   rc |= P->status.alloc(sizeof(int32_t) * 4);
+  rc |= P->sink.alloc(sizeof(uint32_t) * kWave);
   if (rc) {
     delete P;
     return POYG_ENOMEM;
@@ -398,6 +418,8 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     e = hipMemcpyAsync(P->order_lds.p, o_lds.data(), sizeof(int32_t) * P->n_lds, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && P->n_glb)
     e = hipMemcpyAsync(P->order_glb.p, o_glb.data(), sizeof(int32_t) * P->n_glb, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && P->n_diag)
+    e = hipMemcpyAsync(P->order_diag.p, o_diag.data(), sizeof(int32_t) * P->n_diag, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete P;
@@ -430,9 +452,13 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[0], s));
   HIP_TRY(launch_reset(ctr, 4, s));
   HIP_TRY(launch_reset((int *)P->status.p, 4, s));
-  HIP_TRY(launch_prep(affine, std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * 8)), meta, P->n, d_seq, tcm,
-                      (int *)P->aux.p, s));
+  if (!affine || P->n_lds + P->n_glb)
+    HIP_TRY(launch_prep(affine, std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * 8)), meta, P->n, d_seq, tcm,
+                        (int *)P->aux.p, s));
   if (affine) {
+    if (P->n_diag)
+      HIP_TRY(launch_aff_diag(align, P->grid_diag, meta, (const int32_t *)P->order_diag.p, P->n_diag, ctr + 2, d_seq,
+                              tcm, (uint8_t *)P->dm.p, d_cost, (uint32_t *)P->sink.p, s));
     if (P->n_lds)
       HIP_TRY(launch_aff_fill(align, false, P->grid_lds, meta, (const int32_t *)P->order_lds.p, P->n_lds, ctr,
                               d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, nullptr, 0, s));

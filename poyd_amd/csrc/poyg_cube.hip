@@ -29,6 +29,12 @@ constexpr int kP1 = 1, kP2 = 2, kP3 = 4, kS1 = 8, kS2 = 16, kS3 = 32, kSS = 64;
 
 __device__ __forceinline__ int c_lane() { return (int)__lane_id(); }
 
+__device__ __forceinline__ int64_t c_uniform64(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // LDS ordering across lanes of one wave (see wave_sync_mem in poyg_kernels.hip).
 __device__ __forceinline__ void c_lds_sync() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -38,7 +44,16 @@ __device__ __forceinline__ void c_lds_sync() {
 }
 
 // DPP-only cross-lane helpers (see poyg_kernels.hip).
+// Eight wait states and a scheduling barrier between any EXEC change and the
+// DPP moves that follow (a DPP issued shortly after an EXEC restore has been
+// seen to treat enabled source lanes as disabled).
+__device__ __forceinline__ void c_dpp_guard() {
+  asm volatile("s_nop 7");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 __device__ __forceinline__ int c_prefix_min(int v) {
+  c_dpp_guard();
   constexpr int I = 0x7fffffff;
   v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x111, 0xf, 0xf, false));
   v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x112, 0xf, 0xf, false));
@@ -49,6 +64,7 @@ __device__ __forceinline__ int c_prefix_min(int v) {
   return v;
 }
 __device__ __forceinline__ int c_prefix_sum(int v) {
+  c_dpp_guard();
   v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
   v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
   v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
@@ -71,6 +87,7 @@ __device__ __forceinline__ ChainOut c_chain(int v, bool act, int G3k, int g, int
   int pm = c_prefix_min(act ? v - G3k : big);
   pm = min(pm, carry_mm - carry_g3);
   const int mm = G3k + pm;
+  c_dpp_guard();
   const int left = __builtin_amdgcn_update_dpp(carry_mm, mm, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
   ChainOut o;
   o.ss = act && !k0 && (left + g < v);
@@ -96,8 +113,19 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
     if (lane == 0) kk = atomicAdd(counter, 1);
     kk = __builtin_amdgcn_readfirstlane(kk);
     if (kk >= n) break;
-    const int t = order[kk];
-    const TripleMeta m = meta[t];
+    // the descriptor is wave-uniform: keep it in scalar registers so that the
+    // loops over it compile to scalar branches
+    const int t = __builtin_amdgcn_readfirstlane(order[kk]);
+    TripleMeta m = meta[t];
+    m.off1 = c_uniform64(m.off1);
+    m.off2 = c_uniform64(m.off2);
+    m.off3 = c_uniform64(m.off3);
+    m.out_off = c_uniform64(m.out_off);
+    m.dm_off = c_uniform64(m.dm_off);
+    m.l1 = __builtin_amdgcn_readfirstlane(m.l1);
+    m.l2 = __builtin_amdgcn_readfirstlane(m.l2);
+    m.l3 = __builtin_amdgcn_readfirstlane(m.l3);
+    m.cap = __builtin_amdgcn_readfirstlane(m.cap);
     const uint8_t *s1 = seq + m.off1, *s2 = seq + m.off2, *s3 = seq + m.off3;
     const int n1 = m.l1, n2 = m.l2, n3 = m.l3, W = n1 + n2;
     int *G3 = ring + (size_t)W * n3;
@@ -242,11 +270,11 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
         }
       }
     }
-    // the reference returns the last cell of the last row written
-    if (lane == 0) {
-      __threadfence();
-      cost_out[t] = ROW(n1 * n2 - 1)[n3 - 1];
-    }
+    // the reference returns the last cell of the last row written; every
+    // lane stores the same value (no lane-divergent code at the end of the
+    // item loop body, which keeps the loop scalar)
+    __threadfence();
+    cost_out[t] = ROW(n1 * n2 - 1)[n3 - 1];
     __threadfence();
 #undef ROW
   }

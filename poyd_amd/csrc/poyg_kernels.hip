@@ -15,6 +15,8 @@
 // first filled column of the row; the backtraces run one thread per pair.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "poyg_internal.h"
 
 namespace poyg {
@@ -27,9 +29,20 @@ constexpr int kBig = 0x3fffffff;
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Separates DPP moves from any EXEC change before them: eight wait states,
+// and the scheduler may move nothing across the barrier.  A DPP issued a few
+// instructions after an EXEC restore has been seen to treat enabled source
+// lanes as disabled (returning the fill value), depending on how waves
+// interleave; every DPP sequence below starts with this guard.
+__device__ __forceinline__ void dpp_exec_guard() {
+  asm volatile("s_nop 7");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 // Cross-lane operations use DPP (VALU) and v_readlane only: lane i receives
 // lane i-1's value (lane 0 receives `fill`).
 __device__ __forceinline__ int shfl_up1(int v, int fill) {
+  dpp_exec_guard();
   return __builtin_amdgcn_update_dpp(fill, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
 }
 
@@ -38,6 +51,7 @@ __device__ __forceinline__ int shfl_up1(int v, int fill) {
 // row totals upward.  Disabled DPP lanes return the identity.
 __device__ __forceinline__ int wave_prefix_min(int v) {
   constexpr int I = 0x7fffffff;
+  dpp_exec_guard();
   v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x111, 0xf, 0xf, false));
   v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x112, 0xf, 0xf, false));
   v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x114, 0xf, 0xf, false));
@@ -48,6 +62,7 @@ __device__ __forceinline__ int wave_prefix_min(int v) {
 }
 
 __device__ __forceinline__ int wave_prefix_sum(int v) {
+  dpp_exec_guard();
   v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
   v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
   v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
@@ -59,6 +74,14 @@ __device__ __forceinline__ int wave_prefix_sum(int v) {
 
 __device__ __forceinline__ int lane63(int v) { return __builtin_amdgcn_readlane(v, kWave - 1); }
 
+// Index of this wave inside its workgroup as a wave-uniform (scalar) value.
+// Per-wave LDS regions are addressed from it, so that their base lives in
+// scalar registers: a per-lane copy of the base has been seen to go stale in
+// some lanes across the divergent code of the fill loops.
+__device__ __forceinline__ int wave_in_group() {
+  return __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+}
+
 // Orders this wave's LDS traffic across lanes: every LDS access issued
 // before the call has completed (s_waitcnt lgkmcnt(0)) before any access
 // after it, and the compiler may not move memory operations across it.
@@ -68,6 +91,40 @@ __device__ __forceinline__ void wave_sync_mem() {
   __builtin_amdgcn_s_waitcnt(0xC07F);   // gfx9 encoding of lgkmcnt(0), other counters unchanged
   __builtin_amdgcn_wave_barrier();
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Wave-uniform copies of loaded values.  A work item's descriptor is the
+// same in every lane; reading it through v_readfirstlane lets the compiler
+// keep it in scalar registers and turn every branch on it into a scalar
+// branch, instead of EXEC-masked code for control flow that never diverges.
+// Together with the rule that no early exit (return, break) of a per-item
+// body sits inside lane-divergent code, this keeps the persistent item loops
+// scalar.  When that rule was broken (`if (lane == 0) store; return;`), the
+// compiler built the item loop as a divergent loop that refreshed VGPR copies
+// of the item's geometry under a partial EXEC mask, so some lanes computed
+// with the previous item's geometry.
+__device__ __forceinline__ int uniform32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ PairMeta uniform_meta(const PairMeta *meta, int p) {
+  PairMeta m = meta[p];
+  m.off_r = uniform64(m.off_r);
+  m.off_c = uniform64(m.off_c);
+  m.out_off = uniform64(m.out_off);
+  m.dm_off = uniform64(m.dm_off);
+  m.aux_off = uniform64(m.aux_off);
+  m.lr = uniform32(m.lr);
+  m.lc = uniform32(m.lc);
+  m.stride = uniform32(m.stride);
+  m.cap = uniform32(m.cap);
+  m.swap = uniform32(m.swap);
+  m.p0 = uniform32(m.p0);
+  m.p1 = uniform32(m.p1);
+  m.p2 = uniform32(m.p2);
+  return m;
 }
 
 __device__ __forceinline__ int next_item(int *counter) {
@@ -158,24 +215,28 @@ __global__ __launch_bounds__(kWave * 4) void aff_prep_kernel(const PairMeta *__r
   if (threadIdx.x < 32) s_prep[threadIdx.x] = tcm->prepend[threadIdx.x];
   __syncthreads();
   const int waves = gridDim.x * (blockDim.x / kWave);
-  for (int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; p < n; p += waves) {
-    const PairMeta m = meta[p];
+  for (int p = blockIdx.x * (blockDim.x / kWave) + wave_in_group(); p < n; p += waves) {
+    const PairMeta m = uniform_meta(meta, p);
+    if (m.p2) continue;   // anti-diagonal pairs need no prefix sums
     int *HX = aux + m.aux_off;
     aff_prefix(seq + m.off_c, m.lc - 1, s_prep, tcm->gap_open, HX, HX + m.lc);
   }
 }
 
 template <bool kBt, bool kGlobalRows>
-__device__ void aff_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq, const int *s_cost,
-                              const int *s_prep, int go, uint8_t *__restrict__ dm_base, int *aux,
-                              int32_t *__restrict__ cost_out, int pidx, RowRef R) {
+__device__ __forceinline__ void aff_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq, const int *s_cost,
+                                              const int *s_prep, int go, uint8_t *__restrict__ dm_base, int *aux,
+                                              int32_t *__restrict__ cost_out, int pidx, RowRef R) {
   const int lane = lane_id();
-  const uint8_t *A = seq + m.off_r;
-  const uint8_t *B = seq + m.off_c;
-  const int la = m.lr - 1, lb = m.lc - 1, E0 = m.p0, S = m.stride;
-  int *HX = aux + m.aux_off;
+  // the pair's geometry is wave-uniform (scalar registers, scalar branches)
+  const uint8_t *A = seq + uniform64(m.off_r);
+  const uint8_t *B = seq + uniform64(m.off_c);
+  const int la = uniform32(m.lr) - 1, lb = uniform32(m.lc) - 1, E0 = uniform32(m.p0), S = uniform32(m.stride);
+  pidx = uniform32(pidx);
+  go = uniform32(go);
+  int *HX = aux + uniform64(m.aux_off);
   int *G = HX + (lb + 1);
-  uint8_t *dm = dm_base + m.dm_off;
+  uint8_t *dm = dm_base + uniform64(m.dm_off);
 
   // row 0 (initialize_matrices_affine, src/algn.c:2183-2248)
   for (int j0 = 0; j0 <= E0; j0 += kWave) {
@@ -189,7 +250,9 @@ __device__ void aff_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq
     }
   }
   if (la == 0) {
-    if (lane == 0) cost_out[pidx] = lb == 0 ? 0 : go + G[lb];
+    // every lane stores the same value: an early exit must not sit inside
+    // lane-divergent code (see uniform32)
+    cost_out[pidx] = lb == 0 ? 0 : go + G[lb];
     return;
   }
   if (kGlobalRows) __threadfence();
@@ -330,14 +393,14 @@ __global__ __launch_bounds__(kWave * kAffWaves) void aff_fill_lds_kernel(
   __shared__ int s_prep[32];
   __shared__ int s_rows[kAffWaves][2][4][kAffRB];
   load_tables(tcm, s_cost, s_prep, nullptr);
-  const int w = threadIdx.x / kWave;
+  const int w = wave_in_group();
   const int go = tcm->gap_open;
   RowRef R{&s_rows[w][0][0][0], kAffRB - 1, 4 * kAffRB, kAffRB};
   for (;;) {
     const int k = next_item(counter);
     if (k >= n) break;
-    const int p = order[k];
-    const PairMeta m = meta[p];
+    const int p = uniform32(order[k]);
+    const PairMeta m = uniform_meta(meta, p);
     aff_fill_pair<kBt, false>(m, seq, s_cost, s_prep, go, dm, aux, cost_out, p, R);
   }
 }
@@ -358,9 +421,255 @@ __global__ __launch_bounds__(kWave) void aff_fill_glb_kernel(
   for (;;) {
     const int k = next_item(counter);
     if (k >= n) break;
-    const int p = order[k];
-    const PairMeta m = meta[p];
+    const int p = uniform32(order[k]);
+    const PairMeta m = uniform_meta(meta, p);
     aff_fill_pair<kBt, true>(m, seq, s_cost, s_prep, go, dm, aux, cost_out, p, R);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Affine band as an anti-diagonal wavefront (bands of at most 64 diagonal
+// pairs, i.e. E0 <= 87).
+// ---------------------------------------------------------------------------
+// Lane l owns the band diagonals d = j - i = 2l - 40 and 2l - 39.  Step t
+// computes the cells with i + j = t: the even diagonal on even steps, the odd
+// one on odd steps, so a lane's cell is (i, j) = (s + 20 - l, s - 20 + l + p)
+// at step t = 2s + p.  Of a cell's neighbours, (i-1, j-1) was computed two
+// steps earlier by the same lane; (i, j-1) and (i-1, j) one step earlier, by
+// the same lane or by lane l-1 (even steps, left) / lane l+1 (odd steps, up),
+// which one DPP move fetches.  The horizontal gap chain of a row therefore
+// advances in time inside one lane instead of across lanes.  Row characters
+// move one lane up every two steps and column characters one lane down; the
+// character entering at lane 0 / lane 63 is read from a 64-entry window held
+// across the wave.  Direction bytes are stored per step and lane: dword
+// (t >> 2) * W + l of the pair's band, byte t & 3 (W = lanes in use).
+__device__ __forceinline__ int dpp_from_below(int v, int fill) {   // lane l <- lane l-1
+  return __builtin_amdgcn_update_dpp(fill, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_from_above(int v, int fill) {   // lane l <- lane l+1
+  return __builtin_amdgcn_update_dpp(fill, v, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+}
+
+// Branch-free bounded read: the index is clamped, the value then masked, so
+// no lane is switched off (see aff_diag_kernel).
+__device__ __forceinline__ int load_or0(const uint8_t *s, int idx, int len) {
+  const int v = s[min(max(idx, 0), len)] & 31;
+  return (idx >= 0 && idx <= len) ? v : 0;
+}
+
+struct AffDiag {
+  int eh1, ev1, eb1, cb1;   // this lane's cell of the previous step
+  int eh2, ev2, eb2, cb2;   // ... and of the step before
+};
+
+// One cell of the band (FILL_* of src/algn.c:1740-1977 and ASSIGN_MINIMUM
+// :2250-2280 with the band rules of :2287-2548).  Returns the direction byte.
+// kSteady: every cell of the step has 41 <= i <= la and j <= lb, so the band
+// edges are the fixed lanes d == -40 (left) and d == E0 (right), given as
+// flags; no row 0 and no final cell occur.
+template <int P, bool kBt, bool kSteady>
+__device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int i, int j, int la, int lb, int E0, int go,
+                                                  const int *s_cost, int ic, int ip, int si_ge, int si_go,
+                                                  int sive, int jc, int gap_row, int sj_go, int hx,
+                                                  int &fcost, bool st_valid, bool st_left,
+                                                  bool st_right) {
+  int eh_l, cb_l, ev_u, cb_u;
+  if (P == 0) {
+    eh_l = dpp_from_below(S.eh1, kHigh);
+    cb_l = dpp_from_below(S.cb1, kHigh);
+    ev_u = S.ev1;
+    cb_u = S.cb1;
+  } else {
+    eh_l = S.eh1;
+    cb_l = S.cb1;
+    ev_u = dpp_from_above(S.ev1, kHigh);
+    cb_u = dpp_from_above(S.cb1, kHigh);
+  }
+  const int d = j - i;
+  // FILL_EXTEND_VERTICAL
+  const int ext_v = ev_u + sive, open_v = cb_u + si_go + si_ge;
+  const bool endv = !(ext_v < open_v);
+  int ev = endv ? open_v : ext_v;
+  // FILL_EXTEND_BLOCK_DIAGONAL (bt / nobt)
+  const bool ic_gap = ic & kGap, jc_gap = jc & kGap;
+  const bool both_gap = ic_gap && jc_gap;
+  const int diag = both_gap ? 0 : kHigh;
+  const int ext_b = S.eb2 + diag;
+  int open_b;
+  if (kBt) {
+    open_b = S.cb2 + diag;
+  } else {
+    const bool flag2 = !(kGap & ip) && !jc_gap;
+    open_b = S.cb2 + (both_gap ? (flag2 ? 0 : 2 * go) : kHigh);
+  }
+  const bool endb = !(ext_b < open_b);
+  int eb = endb ? open_b : ext_b;
+  // FILL_CLOSE_BLOCK_DIAGONAL
+  const int dg = s_cost[(ic & 15) * 32 + (jc & 15)];
+  const int xgo = (sj_go < si_go) ? si_go : sj_go;
+  const int fv = S.ev2 + dg + (ic_gap ? sj_go : 0);
+  const int fh = S.eh2 + dg + (jc_gap ? si_go : 0);
+  const int fd = S.eb2 + dg + xgo;
+  int cb = S.cb2 + dg;
+  bool a2v = false, a2h = false, a2d = false;
+  if (cb >= fv) { if (cb > fv) cb = fv; a2v = true; }
+  if (cb >= fh) { if (cb > fh) { cb = fh; a2v = false; } a2h = true; }
+  if (cb >= fd) { if (cb > fd) { cb = fd; a2v = false; a2h = false; } a2d = true; }
+  const int anext = a2h ? 1 : (a2d ? 2 : (a2v ? 3 : 0));
+  // FILL_EXTEND_HORIZONTAL
+  const int bval = cb_l + sj_go + gap_row, ext_h = eh_l + hx;
+  const bool endh = !(ext_h < bval);
+  int eh = endh ? bval : ext_h;
+  // ASSIGN_MINIMUM
+  int f = eh;
+  bool dh = true, dv = false, dd = false, da = false;
+  if (f >= ev) { if (f > ev) { f = ev; dh = false; } dv = true; }
+  if (f >= eb) { if (f > eb) { f = eb; dh = dv = false; } dd = true; }
+  if (f >= cb) { if (f > cb) { f = cb; dh = dv = dd = false; } da = true; }
+  const int todo = dh ? 0 : (da ? 1 : (dv ? 2 : 3));
+  uint32_t byte = (uint32_t)(todo | (endv << 2) | (endh << 3) | (endb << 4) | (anext << 5));
+  // band edges: the left boundary column carries only the vertical state,
+  // the cell right of the band is closed (initialize_matrices_affine and the
+  // per-row boundary writes of algn_fill_plane_3_aff)
+  // selects only: no lane may be switched off in this loop (see the kernel)
+  if (kSteady) {
+    const bool closed = st_left || st_right || !st_valid;
+    ev = closed ? (st_left ? ext_v : kHigh) : ev;
+    eh = closed ? kHigh : eh;
+    eb = closed ? kHigh : eb;
+    cb = closed ? kHigh : cb;
+    byte = closed ? (st_left ? kAffLeft : kAffRight) : byte;
+  } else {
+    const bool valid = i >= 0 && i <= la && j >= 0 && j <= lb && d <= E0;
+    const bool lbnd = valid && i >= 1 && (j == 0 || d == -40);
+    const bool rbnd = valid && i >= 1 && d == E0;
+    const bool closed = lbnd || rbnd || !valid;
+    ev = closed ? (lbnd ? ext_v : kHigh) : ev;
+    eh = closed ? kHigh : eh;
+    eb = closed ? kHigh : eb;
+    cb = closed ? kHigh : cb;
+    byte = closed ? (lbnd ? kAffLeft : kAffRight) : byte;
+    const bool row0 = valid && i == 0, j0 = j == 0;   // row 0
+    const int r0eh = j0 ? go : eh_l + gap_row;
+    eh = row0 ? r0eh : eh;
+    ev = row0 ? (j0 ? go : kHigh) : ev;
+    eb = row0 ? (j0 ? 0 : kHigh) : eb;
+    cb = row0 ? (j0 ? 0 : r0eh) : cb;
+    fcost = (valid && i == la && j == lb) ? f : fcost;
+  }
+  S.eh2 = S.eh1; S.ev2 = S.ev1; S.eb2 = S.eb1; S.cb2 = S.cb1;
+  S.eh1 = eh; S.ev1 = ev; S.eb1 = eb; S.cb1 = cb;
+  return byte;
+}
+
+// Every half-step starts with dpp_exec_guard (see there), and the loop avoids
+// divergence where it can: edges are selects, out-of-band lanes store their
+// direction dword to `sink`, the cost is taken with v_readlane after the
+// loop, and character reads clamp their index.
+template <bool kBt>
+__global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, int32_t *cost_out,
+    uint32_t *sink) {
+  __shared__ int s_cost[32 * 32];
+  __shared__ int s_prep[32];
+  load_tables(tcm, s_cost, s_prep, nullptr);
+  const int go = tcm->gap_open;
+  const int lane = lane_id();
+  for (;;) {
+    const int k = next_item(counter);
+    if (k >= n) break;
+    const int p = uniform32(order[k]);
+    const PairMeta m = uniform_meta(meta, p);
+    const uint8_t *A = seq + m.off_r;
+    const uint8_t *B = seq + m.off_c;
+    const int la = m.lr - 1, lb = m.lc - 1, E0 = m.p0, W = m.p1;
+    if (la == 0) {   // only row 0: go + prefix of prepend costs
+      int sum = 0;
+      for (int j0 = 1; j0 <= lb; j0 += kWave) {
+        const int j = j0 + lane;
+        const int v = s_prep[B[min(j, lb)] & 31];
+        sum += j <= lb ? v : 0;
+      }
+      sum = lane63(wave_prefix_sum(sum));
+      cost_out[p] = lb == 0 ? 0 : go + sum;   // every lane stores the same value
+      continue;
+    }
+    uint8_t *dm = dm_base + m.dm_off;
+    const int i0 = 20 - lane, j0 = lane - 20;
+    int ic = load_or0(A, 19 - lane, la), ip = 0;
+    int jc = load_or0(B, lane - 20, lb), jp = load_or0(B, lane - 21, lb);
+    int awin = load_or0(A, 20 + lane, la), anext = load_or0(A, 84 + lane, la);
+    int bwin = load_or0(B, 44 + lane, lb), bnext = load_or0(B, 108 + lane, lb);
+    int si_ge = 0, si_go = 0, sive = 0;
+    int gap_row = s_prep[jc], sj_go = 0, hx = 0;
+    {
+      const int j = j0;
+      sj_go = (!(kGap & jp) && (kGap & jc)) ? 0 : go;
+      hx = ((jp & kGap) && !(jc & kGap)) ? sj_go + gap_row : gap_row;
+      if (j == 1) hx = gap_row;
+    }
+    AffDiag S{kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh};
+    const int groups = (la + lb + 4) >> 2;   // ceil((la + lb + 1) / 4)
+    int fcost = 0;
+    uint32_t *dmw = lane < W ? (uint32_t *)dm + lane : sink + lane;
+    const int dmw_step = lane < W ? W : 0;
+    // steady steps: t in [E0 + 82, min(2 lb - E0, 2 la - 40)]
+    const int t_lo = E0 + 82, t_hi = min(2 * lb - E0, 2 * la - 40);
+    int g_lo = (t_lo + 3) >> 2, g_hi = t_hi >= 3 ? (t_hi - 3) / 4 + 1 : 0;
+    g_lo = min(g_lo, groups);
+    g_hi = max(g_lo, min(g_hi, groups));
+    const int d0 = 2 * lane - 40;   // even diagonal of this lane; odd is d0 + 1
+    const bool v0 = d0 <= E0, v1 = d0 + 1 <= E0;
+    const bool l0 = lane == 0;
+    const bool r0 = d0 == E0, r1 = d0 + 1 == E0;
+    auto group = [&](int g, auto steady_tag) {
+      constexpr bool kSt = decltype(steady_tag)::value;
+      if (g && (g & 31) == 0) {
+        const int base = 2 * g;   // first pair-step of the new window
+        awin = anext;
+        bwin = bnext;
+        anext = load_or0(A, 20 + base + 64 + lane, la);
+        bnext = load_or0(B, 44 + base + 64 + lane, lb);
+      }
+      uint32_t acc = 0;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int s = 2 * g + h;
+        const int i = i0 + s;
+        {   // even step: next row character
+          dpp_exec_guard();
+          ip = ic;
+          ic = dpp_from_below(ic, __builtin_amdgcn_readlane(awin, s & 63));
+          si_ge = s_cost[ic * 32 + kGap];
+          si_go = (!(kGap & ip) && (kGap & ic)) ? 0 : go;
+          sive = (i > 1 && (ip & kGap) && !(ic & kGap)) ? si_go + si_ge : si_ge;
+          acc |= aff_diag_cell<0, kBt, kSt>(S, i, j0 + s, la, lb, E0, go, s_cost, ic, ip, si_ge, si_go, sive, jc,
+                                            gap_row, sj_go, hx, fcost, v0, l0, r0)
+                 << (16 * h);
+        }
+        {   // odd step: next column character
+          dpp_exec_guard();
+          const int j = j0 + s + 1;
+          jp = jc;
+          jc = dpp_from_above(jc, __builtin_amdgcn_readlane(bwin, s & 63));
+          gap_row = s_prep[jc];
+          sj_go = (!(kGap & jp) && (kGap & jc)) ? 0 : go;
+          hx = ((jp & kGap) && !(jc & kGap)) ? sj_go + gap_row : gap_row;
+          if (j == 1) hx = gap_row;
+          acc |= aff_diag_cell<1, kBt, kSt>(S, i, j, la, lb, E0, go, s_cost, ic, ip, si_ge, si_go, sive, jc,
+                                            gap_row, sj_go, hx, fcost, v1, false, r1)
+                 << (16 * h + 8);
+        }
+      }
+      if (kBt) dmw[(size_t)g * dmw_step] = acc;
+    };
+    int g = 0;
+    for (; g < g_lo; ++g) group(g, std::false_type{});
+    for (; g < g_hi; ++g) group(g, std::true_type{});
+    for (; g < groups; ++g) group(g, std::false_type{});
+    // cell (la, lb) sits on diagonal lb - la, i.e. lane (lb - la + 40) / 2
+    cost_out[p] = __builtin_amdgcn_readlane(fcost, (lb - la + 40) >> 1);
   }
 }
 
@@ -397,7 +706,10 @@ __global__ void aff_trace_kernel(const PairMeta *__restrict__ meta, const int32_
     const int ep = min(lb, E0 + i - 1);
     const int hi = ep < lb ? ep + 1 : ep;
     if (j < sp - 1 || j > hi) { bad = 1; break; }
-    const int f = dm[(size_t)(i - 1) * S + (j - (sp - 1))];
+    // band layouts: rows (stride S from the first filled column) or the
+    // anti-diagonal steps of aff_diag_kernel (W lanes of 4 steps per dword)
+    const int f = m.p2 ? dm[(size_t)((i + j) >> 2) * (4 * m.p1) + (((j - i + 40) >> 1) << 2) + ((i + j) & 3)]
+                       : dm[(size_t)(i - 1) * S + (j - (sp - 1))];
     if (mode == 0) {
       const int td = f & 3;
       mode = td == 0 ? 2 : (td == 1 ? 4 : (td == 2 ? 1 : 3));
@@ -510,24 +822,28 @@ __global__ __launch_bounds__(kWave * 4) void lin_prep_kernel(const PairMeta *__r
   if (threadIdx.x < 32) s_prep[threadIdx.x] = tcm->prepend[threadIdx.x];
   __syncthreads();
   const int waves = gridDim.x * (blockDim.x / kWave);
-  for (int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; p < n; p += waves) {
-    const PairMeta m = meta[p];
+  for (int p = blockIdx.x * (blockDim.x / kWave) + wave_in_group(); p < n; p += waves) {
+    const PairMeta m = uniform_meta(meta, p);
     int *GI = aux + m.aux_off;
     lin_prefix(seq + m.off_c, m.lc, s_cost, s_prep, GI, GI + m.lc);
   }
 }
 
 template <bool kDm, bool kGlobalRows>
-__device__ void lin_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq, const int *s_cost,
-                              const int *s_prep, const int *s_tail, uint8_t *__restrict__ dm_base, int *aux,
-                              int32_t *__restrict__ cost_out, int pidx, int *rows, int mask, int bstride) {
+__device__ __forceinline__ void lin_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq, const int *s_cost,
+                                              const int *s_prep, const int *s_tail, uint8_t *__restrict__ dm_base,
+                                              int *aux, int32_t *__restrict__ cost_out, int pidx, int *rows,
+                                              int mask, int bstride) {
   const int lane = lane_id();
-  const uint8_t *s1 = seq + m.off_r;   // the longer sequence (rows)
-  const uint8_t *s2 = seq + m.off_c;
-  const int M = m.lr, N = m.lc, cs = m.p0, w = m.p1, h = m.p2, S = m.stride;
-  int *GI = aux + m.aux_off;
+  // the pair's geometry is wave-uniform (scalar registers, scalar branches)
+  const uint8_t *s1 = seq + uniform64(m.off_r);   // the longer sequence (rows)
+  const uint8_t *s2 = seq + uniform64(m.off_c);
+  const int M = uniform32(m.lr), N = uniform32(m.lc), cs = uniform32(m.p0), w = uniform32(m.p1),
+            h = uniform32(m.p2), S = uniform32(m.stride);
+  pidx = uniform32(pidx);
+  int *GI = aux + uniform64(m.aux_off);
   int *GP = GI + N;
-  uint8_t *dm = dm_base + m.dm_off;
+  uint8_t *dm = dm_base + uniform64(m.dm_off);
   // row 0 (algn_fill_first_row, :584-607, or the first row of algn_fill_plane)
   const int hi0 = lin_row0_hi(N, cs, w);
   for (int j0 = 0; j0 <= hi0; j0 += kWave) {
@@ -538,7 +854,9 @@ __device__ void lin_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq
     }
   }
   if (M == 1) {
-    if (lane == 0) cost_out[pidx] = GP[N - 1];
+    // every lane stores the same value: an early exit must not sit inside
+    // lane-divergent code (see uniform32)
+    cost_out[pidx] = GP[N - 1];
     return;
   }
   if (kGlobalRows) __threadfence();
@@ -546,6 +864,10 @@ __device__ void lin_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq
   for (int i = 1; i < M; ++i) {
     int lo, hi, lk, rk;
     lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
+    lo = uniform32(lo);
+    hi = uniform32(hi);
+    lk = uniform32(lk);
+    rk = uniform32(rk);
     const int *prv = rows + ((i - 1) & 1) * bstride;
     int *cur = rows + (i & 1) * bstride;
     const int si = s1[i] & 31;
@@ -555,44 +877,41 @@ __device__ void lin_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq
     uint8_t *drow = dm + (size_t)i * S - lo;
     int carry_mm = kBig, carry_gi = 0;   // chain value before lo: none
     for (int j0 = lo; j0 <= hi; j0 += kWave) {
+      // Every lane loads and computes (clamped column jj); only the stores
+      // are conditional.  Keeping the body free of divergent branches keeps
+      // the per-lane state of inactive lanes out of reach of the branches.
       const int j = j0 + lane;
       const bool act = j <= hi;
-      int best = kBig, msk = 0, T = kBig, GIj = 0, t1 = kBig, t3 = kBig, gj = 0, cst = kBig;
+      const int jj = min(j, hi);
       const bool first = (j == lo) && lk != LK_LEFT;
       const bool has_ins = act && !(j == lo);
-      if (act) {
-        GIj = GI[j];
-        if (first) {
-          T = prv[0 & mask] + (lk == LK_FIRST_TAIL ? tail : c_del);
-        } else {
-          const int x = s2[j] & 31;
-          gj = s_cost[kGap * 32 + x];
-          t3 = prv[(j - 1) & mask] + crow[x];
-          if (!(j == hi && rk == RK_RIGHT)) t1 = prv[j & mask] + c_del;
-          T = min(t1, t3);
-          if (j == hi && rk == RK_LAST && j > 0) {
-            cst = tail + prv[j & mask];
-            T = min(T, cst);
-          }
-        }
-      }
+      const bool right = j == hi && rk == RK_RIGHT;
+      const bool last = j == hi && rk == RK_LAST && j > 0;
+      const int GIraw = GI[jj];
+      const int GIj = act ? GIraw : 0;
+      const int x = s2[jj] & 31;
+      const int gj = s_cost[kGap * 32 + x];
+      const int p_left = prv[(jj - 1) & mask];   // unused (first) when jj == lo
+      const int p_up = prv[jj & mask];
+      const int p_first = prv[0 & mask];
+      const int t3 = first ? kBig : p_left + crow[x];
+      const int t1 = (first || right) ? kBig : p_up + c_del;
+      const int cst = (!first && last) ? tail + p_up : kBig;
+      int T = first ? p_first + (lk == LK_FIRST_TAIL ? tail : c_del) : min(min(t1, t3), cst);
+      T = act ? T : kBig;
       int pm = wave_prefix_min(act ? T - GIj : kBig);
       pm = min(pm, carry_mm - carry_gi);
       const int mm = GIj + pm;
       const int mm_left = shfl_up1(mm, carry_mm);
+      const int t2 = has_ins ? mm_left + gj : kBig;
+      int best = min(min(t1, t2), t3);
+      int msk = (t3 == best ? 1 : 0) | (has_ins && t2 == best ? 2 : 0) | (t1 == best ? 4 : 0);
+      const bool cst_lt = cst < best, cst_eq = cst == best && cst != kBig;
+      best = cst_lt ? cst : best;
+      msk = cst_lt ? 4 : (cst_eq ? (msk | 4) : msk);
+      best = first ? T : best;
+      msk = first ? 4 : msk;
       if (act) {
-        if (first) {
-          best = T;
-          msk = 4;
-        } else {
-          const int t2 = has_ins ? mm_left + gj : kBig;
-          best = min(min(t1, t2), t3);
-          msk = (t3 == best ? 1 : 0) | (has_ins && t2 == best ? 2 : 0) | (t1 == best ? 4 : 0);
-          if (cst != kBig) {
-            if (cst < best) { best = cst; msk = 4; }
-            else if (cst == best) msk |= 4;
-          }
-        }
         cur[j & mask] = best;
         if (kDm) drow[j] = (uint8_t)msk;
         if (i == M - 1 && j == N - 1) cost_out[pidx] = best;
@@ -617,12 +936,12 @@ __global__ __launch_bounds__(kWave * kLinWaves) void lin_fill_lds_kernel(
   load_tables(tcm, s_cost, s_prep, nullptr);
   for (int k = threadIdx.x; k < 32; k += blockDim.x) s_tail[k] = tcm->tail[k];
   __syncthreads();
-  const int wv = threadIdx.x / kWave;
+  const int wv = wave_in_group();
   for (;;) {
     const int k = next_item(counter);
     if (k >= n) break;
-    const int p = order[k];
-    const PairMeta m = meta[p];
+    const int p = uniform32(order[k]);
+    const PairMeta m = uniform_meta(meta, p);
     lin_fill_pair<kDm, false>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, &s_rows[wv][0][0],
                        kLinRB - 1, kLinRB);
   }
@@ -643,8 +962,8 @@ __global__ __launch_bounds__(kWave) void lin_fill_glb_kernel(
   for (;;) {
     const int k = next_item(counter);
     if (k >= n) break;
-    const int p = order[k];
-    const PairMeta m = meta[p];
+    const int p = uniform32(order[k]);
+    const PairMeta m = uniform_meta(meta, p);
     lin_fill_pair<kDm, true>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, mine, -1, row_width);
   }
 }
@@ -829,6 +1148,18 @@ hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, co
       hipLaunchKernelGGL(aff_fill_glb_kernel<false>, dim3(grid), dim3(kWave), 0, s, meta, order, n, counter, seq,
                          tcm, dm, aux, cost, rowscratch, row_width);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_aff_diag(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
+                           hipStream_t s) {
+  if (bt)
+    hipLaunchKernelGGL(aff_diag_kernel<true>, dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq, tcm,
+                       dm, cost, sink);
+  else
+    hipLaunchKernelGGL(aff_diag_kernel<false>, dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq, tcm,
+                       dm, cost, sink);
   return hipGetLastError();
 }
 
