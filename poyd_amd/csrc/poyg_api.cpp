@@ -17,9 +17,14 @@ hipError_t launch_prep(bool affine, int grid, const PairMeta *meta, int n, const
 hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
                            int32_t *cost, int *rowscratch, int row_width, hipStream_t s);
-hipError_t launch_aff_diag(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
-                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
-                           hipStream_t s);
+int aff_diag_blocks_per_cu(bool bt, int sets);
+hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, const int32_t *order, int n,
+                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
+                           uint32_t *sink, hipStream_t s);
+int aff_trace_wave_blocks_per_cu();
+hipError_t launch_aff_trace_wave(int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                                 const uint8_t *seq, const DevTcm *tcm, const uint8_t *dm, uint8_t *out,
+                                 int32_t *out_len, int32_t *status, hipStream_t s);
 hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
                             const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
                             hipStream_t s);
@@ -64,6 +69,7 @@ struct poyg_ctx {
   int device = 0;
   int cus = 256;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;   // second stream: wide-band pairs run beside the main fill
 };
 
 // A device buffer owned by a plan.
@@ -97,19 +103,25 @@ struct poyg_plan {
   int64_t out_bytes = 0;
   std::vector<int64_t> out_off;
   // device
-  DevBuf tcm, meta, order_all, order_lds, order_glb, order_diag, dm, aux, rows, counters, status, sink;
-  int n_lds = 0, n_glb = 0, n_diag = 0;
-  int grid_lds = 0, grid_glb = 0, grid_diag = 0, row_width = 0;
+  // work lists (longest first): row kernels (LDS / global rows), the
+  // anti-diagonal kernel per lane-set count (1, 2, 4), and the backtraces
+  DevBuf tcm, meta, order_all, order_lds, order_glb, order_diag[3], order_rows, order_tr, dm, aux, rows, counters,
+      status, sink;
+  int n_lds = 0, n_glb = 0, n_diag[3] = {0, 0, 0}, n_rows = 0, n_tr = 0;
+  int grid_lds = 0, grid_glb = 0, grid_diag[3] = {0, 0, 0}, grid_tr = 0, row_width = 0;
   bool timing = false;
   float fill_ms = 0.f, trace_ms = 0.f;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join = nullptr;   // side-stream hand-off of the wide-band fills
   ~poyg_plan() {
     if (triple) poyg_triple_plan_destroy(triple);
-    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_diag, &dm, &aux, &rows, &counters,
-                      &status, &sink})
+    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_diag[0], &order_diag[1],
+                      &order_diag[2], &order_rows, &order_tr, &dm, &aux, &rows, &counters, &status, &sink})
       b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
   }
 };
 
@@ -130,7 +142,9 @@ int poyg_ctx_create(int device, poyg_ctx **out) {
   c->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     set_error("poyg_ctx_create: stream creation failed");
     return POYG_EDEVICE;
@@ -142,6 +156,7 @@ int poyg_ctx_create(int device, poyg_ctx **out) {
 void poyg_ctx_destroy(poyg_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   delete ctx;
 }
 
@@ -313,9 +328,10 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       const int e0 = std::min(cb, std::max(40, cb - ra + 8));
       m.p0 = e0;
       const int lanes = ((e0 + 40) >> 1) + 1;   // band diagonal pairs
-      if (lanes <= kWave) {
-        // anti-diagonal wavefront: one dword per lane and 4 steps
-        diag[k] = 1;
+      if (lanes <= 4 * kWave) {
+        // anti-diagonal wavefront (1, 2 or 4 lane sets): one dword per
+        // diagonal pair and 4 steps
+        diag[k] = lanes <= kWave ? 1 : (lanes <= 2 * kWave ? 2 : 4);
         m.p1 = lanes;
         m.p2 = 1;
         if (align && ra > 0) {
@@ -380,12 +396,28 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a] > cells[b]; });
-  std::vector<int32_t> o_lds, o_glb, o_diag;
-  for (int k : order) (diag[k] ? o_diag : glb[k] ? o_glb : o_lds).push_back(k);
+  std::vector<int32_t> o_lds, o_glb, o_diag[3], o_rows, o_tr;
+  for (int k : order) {
+    if (diag[k]) {
+      o_diag[diag[k] == 1 ? 0 : (diag[k] == 2 ? 1 : 2)].push_back(k);
+      o_tr.push_back(k);
+    } else {
+      (glb[k] ? o_glb : o_lds).push_back(k);
+      o_rows.push_back(k);
+    }
+  }
   P->n_lds = (int)o_lds.size();
   P->n_glb = (int)o_glb.size();
-  P->n_diag = (int)o_diag.size();
-  P->grid_diag = std::max(1, std::min((P->n_diag + 3) / 4, ctx->cus * 8));
+  P->n_rows = (int)o_rows.size();
+  P->n_tr = (int)o_tr.size();
+  // persistent grids: exactly the workgroups that can be resident at once
+  for (int q = 0; q < 3; ++q) {
+    P->n_diag[q] = (int)o_diag[q].size();
+    if (P->n_diag[q])
+      P->grid_diag[q] = std::max(1, std::min((P->n_diag[q] + 3) / 4,
+                                             ctx->cus * aff_diag_blocks_per_cu(align, q == 0 ? 1 : 2 * q)));
+  }
+  if (P->n_tr) P->grid_tr = std::max(1, std::min((P->n_tr + 3) / 4, ctx->cus * aff_trace_wave_blocks_per_cu()));
   P->row_width = row_width;
   const int per_cu = affine ? 4 : 8;
   P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
@@ -398,16 +430,22 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   rc |= P->order_all.alloc(sizeof(int32_t) * (size_t)n);
   rc |= P->order_lds.alloc(sizeof(int32_t) * (size_t)P->n_lds);
   rc |= P->order_glb.alloc(sizeof(int32_t) * (size_t)P->n_glb);
-  rc |= P->order_diag.alloc(sizeof(int32_t) * (size_t)P->n_diag);
+  for (int q = 0; q < 3; ++q) rc |= P->order_diag[q].alloc(sizeof(int32_t) * (size_t)P->n_diag[q]);
+  rc |= P->order_rows.alloc(sizeof(int32_t) * (size_t)P->n_rows);
+  rc |= P->order_tr.alloc(sizeof(int32_t) * (size_t)P->n_tr);
   rc |= P->dm.alloc((size_t)dm_bytes);
   rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
   rc |= P->rows.alloc(P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16);
-  rc |= P->counters.alloc(sizeof(int) * 4);
+  rc |= P->counters.alloc(sizeof(int) * 8);
   rc |= P->status.alloc(sizeof(int32_t) * 4);
   rc |= P->sink.alloc(sizeof(uint32_t) * kWave);
+  if (!rc && affine && P->n_diag[0] && (P->n_lds || P->n_glb || P->n_diag[1] || P->n_diag[2]) &&
+      (hipEventCreateWithFlags(&P->fork, hipEventDisableTiming) != hipSuccess ||
+       hipEventCreateWithFlags(&P->join, hipEventDisableTiming) != hipSuccess))
+    rc = POYG_EDEVICE;
   if (rc) {
     delete P;
-    return POYG_ENOMEM;
+    return rc;
   }
   hipStream_t s = ctx->stream;
   hipError_t e = hipSuccess;
@@ -418,8 +456,14 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     e = hipMemcpyAsync(P->order_lds.p, o_lds.data(), sizeof(int32_t) * P->n_lds, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && P->n_glb)
     e = hipMemcpyAsync(P->order_glb.p, o_glb.data(), sizeof(int32_t) * P->n_glb, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && P->n_diag)
-    e = hipMemcpyAsync(P->order_diag.p, o_diag.data(), sizeof(int32_t) * P->n_diag, hipMemcpyHostToDevice, s);
+  for (int q = 0; q < 3; ++q)
+    if (e == hipSuccess && P->n_diag[q])
+      e = hipMemcpyAsync(P->order_diag[q].p, o_diag[q].data(), sizeof(int32_t) * P->n_diag[q],
+                         hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && P->n_rows)
+    e = hipMemcpyAsync(P->order_rows.p, o_rows.data(), sizeof(int32_t) * P->n_rows, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && P->n_tr)
+    e = hipMemcpyAsync(P->order_tr.p, o_tr.data(), sizeof(int32_t) * P->n_tr, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete P;
@@ -450,22 +494,40 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   const PairMeta *meta = (const PairMeta *)P->meta.p;
   const DevTcm *tcm = (const DevTcm *)P->tcm.p;
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[0], s));
-  HIP_TRY(launch_reset(ctr, 4, s));
+  HIP_TRY(launch_reset(ctr, 8, s));
   HIP_TRY(launch_reset((int *)P->status.p, 4, s));
   if (!affine || P->n_lds + P->n_glb)
     HIP_TRY(launch_prep(affine, std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * 8)), meta, P->n, d_seq, tcm,
                         (int *)P->aux.p, s));
   if (affine) {
-    if (P->n_diag)
-      HIP_TRY(launch_aff_diag(align, P->grid_diag, meta, (const int32_t *)P->order_diag.p, P->n_diag, ctr + 2, d_seq,
-                              tcm, (uint8_t *)P->dm.p, d_cost, (uint32_t *)P->sink.p, s));
+    // The wide-band pairs (row kernels) are the longest; they run on the side
+    // stream, launched first, beside the anti-diagonal kernel on the main one.
+    const bool fork = P->fork != nullptr;
+    hipStream_t rs = s;
+    if (fork) {
+      rs = P->ctx->side;
+      HIP_TRY(hipEventRecord(P->fork, s));
+      HIP_TRY(hipStreamWaitEvent(rs, P->fork, 0));
+    }
+    for (int q = 2; q >= 1; --q)   // the widest bands: 4 and 2 lane sets
+      if (P->n_diag[q])
+        HIP_TRY(launch_aff_diag(align, 2 * q, P->grid_diag[q], meta, (const int32_t *)P->order_diag[q].p,
+                                P->n_diag[q], ctr + 2 + q, d_seq, tcm, (uint8_t *)P->dm.p, d_cost,
+                                (uint32_t *)P->sink.p, rs));
     if (P->n_lds)
       HIP_TRY(launch_aff_fill(align, false, P->grid_lds, meta, (const int32_t *)P->order_lds.p, P->n_lds, ctr,
-                              d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, nullptr, 0, s));
+                              d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, nullptr, 0, rs));
     if (P->n_glb)
       HIP_TRY(launch_aff_fill(align, true, P->grid_glb, meta, (const int32_t *)P->order_glb.p, P->n_glb, ctr + 1,
                               d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, (int *)P->rows.p,
-                              P->row_width, s));
+                              P->row_width, rs));
+    if (P->n_diag[0])
+      HIP_TRY(launch_aff_diag(align, 1, P->grid_diag[0], meta, (const int32_t *)P->order_diag[0].p, P->n_diag[0],
+                              ctr + 2, d_seq, tcm, (uint8_t *)P->dm.p, d_cost, (uint32_t *)P->sink.p, s));
+    if (fork) {
+      HIP_TRY(hipEventRecord(P->join, rs));
+      HIP_TRY(hipStreamWaitEvent(s, P->join, 0));
+    }
   } else {
     if (P->n_lds)
       HIP_TRY(launch_lin_fill(align, false, P->grid_lds, meta, (const int32_t *)P->order_lds.p, P->n_lds, ctr,
@@ -477,9 +539,14 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   }
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[1], s));
   if (align) {
-    if (affine)
-      HIP_TRY(launch_aff_trace(meta, (const int32_t *)P->order_all.p, P->n, d_seq, tcm, (const uint8_t *)P->dm.p,
-                               d_out, d_out_len, (int32_t *)P->status.p, s));
+    if (affine) {
+      if (P->n_tr)
+        HIP_TRY(launch_aff_trace_wave(P->grid_tr, meta, (const int32_t *)P->order_tr.p, P->n_tr, ctr + 5, d_seq, tcm,
+                                      (const uint8_t *)P->dm.p, d_out, d_out_len, (int32_t *)P->status.p, s));
+      if (P->n_rows)
+        HIP_TRY(launch_aff_trace(meta, (const int32_t *)P->order_rows.p, P->n_rows, d_seq, tcm,
+                                 (const uint8_t *)P->dm.p, d_out, d_out_len, (int32_t *)P->status.p, s));
+    }
     else
       HIP_TRY(launch_lin_trace(meta, (const int32_t *)P->order_all.p, P->n, d_seq, tcm, (const uint8_t *)P->dm.p,
                                d_out, d_out_len, (int32_t *)P->status.p, s));

@@ -464,27 +464,17 @@ struct AffDiag {
 
 // One cell of the band (FILL_* of src/algn.c:1740-1977 and ASSIGN_MINIMUM
 // :2250-2280 with the band rules of :2287-2548).  Returns the direction byte.
+// (eh_l, cb_l) are the states of the left neighbour (i, j-1), (ev_u, cb_u)
+// those of the upper one (i-1, j); the diagonal one is S's step-before-last.
 // kSteady: every cell of the step has 41 <= i <= la and j <= lb, so the band
 // edges are the fixed lanes d == -40 (left) and d == E0 (right), given as
 // flags; no row 0 and no final cell occur.
-template <int P, bool kBt, bool kSteady>
-__device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int i, int j, int la, int lb, int E0, int go,
-                                                  const int *s_cost, int ic, int ip, int si_ge, int si_go,
-                                                  int sive, int jc, int gap_row, int sj_go, int hx,
-                                                  int &fcost, bool st_valid, bool st_left,
+template <bool kBt, bool kSteady>
+__device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int eh_l, int cb_l, int ev_u, int cb_u, int i, int j,
+                                                  int la, int lb, int E0, int go, const int *s_cost, int ic, int ip,
+                                                  int si_ge, int si_go, int sive, int jc, int gap_row, int sj_go,
+                                                  int hx, int &fcost, bool st_valid, bool st_left,
                                                   bool st_right) {
-  int eh_l, cb_l, ev_u, cb_u;
-  if (P == 0) {
-    eh_l = dpp_from_below(S.eh1, kHigh);
-    cb_l = dpp_from_below(S.cb1, kHigh);
-    ev_u = S.ev1;
-    cb_u = S.cb1;
-  } else {
-    eh_l = S.eh1;
-    cb_l = S.cb1;
-    ev_u = dpp_from_above(S.ev1, kHigh);
-    cb_u = dpp_from_above(S.cb1, kHigh);
-  }
   const int d = j - i;
   // FILL_EXTEND_VERTICAL
   const int ext_v = ev_u + sive, open_v = cb_u + si_go + si_ge;
@@ -566,7 +556,20 @@ __device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int i, int j, int 
 // divergence where it can: edges are selects, out-of-band lanes store their
 // direction dword to `sink`, the cost is taken with v_readlane after the
 // loop, and character reads clamp their index.
-template <bool kBt>
+//
+// R lane sets: for bands of up to 64 R diagonal pairs every lane holds R
+// cells per step, lane l of set r owning the diagonal pair 64 r + l.  Values
+// cross from set to set at lanes 63 / 0 through v_readlane.
+template <int R>
+struct AffDiagLanes {
+  AffDiag S[R];
+  int ic[R], ip[R], jc[R], jp[R];
+  int si_ge[R], si_go[R], sive[R];
+  int gap_row[R], sj_go[R], hx[R];
+  int fcost[R];
+};
+
+template <bool kBt, int R>
 __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
     const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, int32_t *cost_out,
@@ -596,33 +599,46 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
       continue;
     }
     uint8_t *dm = dm_base + m.dm_off;
-    const int i0 = 20 - lane, j0 = lane - 20;
-    int ic = load_or0(A, 19 - lane, la), ip = 0;
-    int jc = load_or0(B, lane - 20, lb), jp = load_or0(B, lane - 21, lb);
-    int awin = load_or0(A, 20 + lane, la), anext = load_or0(A, 84 + lane, la);
-    int bwin = load_or0(B, 44 + lane, lb), bnext = load_or0(B, 108 + lane, lb);
-    int si_ge = 0, si_go = 0, sive = 0;
-    int gap_row = s_prep[jc], sj_go = 0, hx = 0;
-    {
-      const int j = j0;
-      sj_go = (!(kGap & jp) && (kGap & jc)) ? 0 : go;
-      hx = ((jp & kGap) && !(jc & kGap)) ? sj_go + gap_row : gap_row;
-      if (j == 1) hx = gap_row;
+    AffDiagLanes<R> L;
+    int i0[R], j0[R];
+    uint32_t *dmw[R];
+    int dmw_step[R];
+    bool v0[R], v1[R], r0[R], r1[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int lr = lane + 64 * r;   // this lane's diagonal pair in set r
+      i0[r] = 20 - lr;
+      j0[r] = lr - 20;
+      L.ic[r] = load_or0(A, 19 - lr, la);
+      L.ip[r] = 0;
+      L.jc[r] = load_or0(B, lr - 20, lb);
+      L.jp[r] = load_or0(B, lr - 21, lb);
+      L.si_ge[r] = L.si_go[r] = L.sive[r] = 0;
+      L.gap_row[r] = s_prep[L.jc[r]];
+      L.sj_go[r] = (!(kGap & L.jp[r]) && (kGap & L.jc[r])) ? 0 : go;
+      L.hx[r] = ((L.jp[r] & kGap) && !(L.jc[r] & kGap)) ? L.sj_go[r] + L.gap_row[r] : L.gap_row[r];
+      if (j0[r] == 1) L.hx[r] = L.gap_row[r];
+      L.S[r] = AffDiag{kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh};
+      L.fcost[r] = 0;
+      dmw[r] = lr < W ? (uint32_t *)dm + lr : sink + lane;
+      dmw_step[r] = lr < W ? W : 0;
+      const int d0 = 2 * lr - 40;   // even diagonal of this lane in set r; odd is d0 + 1
+      v0[r] = d0 <= E0;
+      v1[r] = d0 + 1 <= E0;
+      r0[r] = d0 == E0;
+      r1[r] = d0 + 1 == E0;
     }
-    AffDiag S{kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh};
+    const bool l0 = lane == 0;
+    // characters entering at set 0 lane 0 (rows) and set R-1 lane 63 (columns)
+    int awin = load_or0(A, 20 + lane, la), anext = load_or0(A, 84 + lane, la);
+    const int bbase = 44 + 64 * (R - 1);
+    int bwin = load_or0(B, bbase + lane, lb), bnext = load_or0(B, bbase + 64 + lane, lb);
     const int groups = (la + lb + 4) >> 2;   // ceil((la + lb + 1) / 4)
-    int fcost = 0;
-    uint32_t *dmw = lane < W ? (uint32_t *)dm + lane : sink + lane;
-    const int dmw_step = lane < W ? W : 0;
     // steady steps: t in [E0 + 82, min(2 lb - E0, 2 la - 40)]
     const int t_lo = E0 + 82, t_hi = min(2 * lb - E0, 2 * la - 40);
     int g_lo = (t_lo + 3) >> 2, g_hi = t_hi >= 3 ? (t_hi - 3) / 4 + 1 : 0;
     g_lo = min(g_lo, groups);
     g_hi = max(g_lo, min(g_hi, groups));
-    const int d0 = 2 * lane - 40;   // even diagonal of this lane; odd is d0 + 1
-    const bool v0 = d0 <= E0, v1 = d0 + 1 <= E0;
-    const bool l0 = lane == 0;
-    const bool r0 = d0 == E0, r1 = d0 + 1 == E0;
     auto group = [&](int g, auto steady_tag) {
       constexpr bool kSt = decltype(steady_tag)::value;
       if (g && (g & 31) == 0) {
@@ -630,46 +646,98 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
         awin = anext;
         bwin = bnext;
         anext = load_or0(A, 20 + base + 64 + lane, la);
-        bnext = load_or0(B, 44 + base + 64 + lane, lb);
+        bnext = load_or0(B, bbase + base + 64 + lane, lb);
       }
-      uint32_t acc = 0;
+      uint32_t acc[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = 0;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int s = 2 * g + h;
-        const int i = i0 + s;
-        {   // even step: next row character
+        {   // even step: next row character; left neighbours come from lane - 1
+          // all DPP moves of the step first, pinned right after the guard
+          // (no EXEC change of the cell code may come between them)
           dpp_exec_guard();
-          ip = ic;
-          ic = dpp_from_below(ic, __builtin_amdgcn_readlane(awin, s & 63));
-          si_ge = s_cost[ic * 32 + kGap];
-          si_go = (!(kGap & ip) && (kGap & ic)) ? 0 : go;
-          sive = (i > 1 && (ip & kGap) && !(ic & kGap)) ? si_go + si_ge : si_ge;
-          acc |= aff_diag_cell<0, kBt, kSt>(S, i, j0 + s, la, lb, E0, go, s_cost, ic, ip, si_ge, si_go, sive, jc,
-                                            gap_row, sj_go, hx, fcost, v0, l0, r0)
-                 << (16 * h);
+          const int fresh = __builtin_amdgcn_readlane(awin, s & 63);
+          int top_ic[R], top_eh[R], top_cb[R], eh_ls[R], cb_ls[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            top_ic[r] = __builtin_amdgcn_readlane(L.ic[r], 63);
+            top_eh[r] = __builtin_amdgcn_readlane(L.S[r].eh1, 63);
+            top_cb[r] = __builtin_amdgcn_readlane(L.S[r].cb1, 63);
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            L.ip[r] = L.ic[r];
+            L.ic[r] = dpp_from_below(L.ic[r], r == 0 ? fresh : top_ic[r > 0 ? r - 1 : 0]);
+            eh_ls[r] = dpp_from_below(L.S[r].eh1, r == 0 ? kHigh : top_eh[r > 0 ? r - 1 : 0]);
+            cb_ls[r] = dpp_from_below(L.S[r].cb1, r == 0 ? kHigh : top_cb[r > 0 ? r - 1 : 0]);
+            asm volatile("" : "+v"(L.ic[r]), "+v"(eh_ls[r]), "+v"(cb_ls[r]));
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int i = i0[r] + s;
+            const int ic = L.ic[r], ip = L.ip[r];
+            L.si_ge[r] = s_cost[ic * 32 + kGap];
+            L.si_go[r] = (!(kGap & ip) && (kGap & ic)) ? 0 : go;
+            L.sive[r] = (i > 1 && (ip & kGap) && !(ic & kGap)) ? L.si_go[r] + L.si_ge[r] : L.si_ge[r];
+            const int eh_l = eh_ls[r], cb_l = cb_ls[r];
+            acc[r] |= aff_diag_cell<kBt, kSt>(L.S[r], eh_l, cb_l, L.S[r].ev1, L.S[r].cb1, i, j0[r] + s, la, lb, E0,
+                                              go, s_cost, ic, ip, L.si_ge[r], L.si_go[r], L.sive[r], L.jc[r],
+                                              L.gap_row[r], L.sj_go[r], L.hx[r], L.fcost[r], v0[r],
+                                              r == 0 && l0, r0[r])
+                      << (16 * h);
+          }
         }
-        {   // odd step: next column character
+        {   // odd step: next column character; upper neighbours come from lane + 1
           dpp_exec_guard();
-          const int j = j0 + s + 1;
-          jp = jc;
-          jc = dpp_from_above(jc, __builtin_amdgcn_readlane(bwin, s & 63));
-          gap_row = s_prep[jc];
-          sj_go = (!(kGap & jp) && (kGap & jc)) ? 0 : go;
-          hx = ((jp & kGap) && !(jc & kGap)) ? sj_go + gap_row : gap_row;
-          if (j == 1) hx = gap_row;
-          acc |= aff_diag_cell<1, kBt, kSt>(S, i, j, la, lb, E0, go, s_cost, ic, ip, si_ge, si_go, sive, jc,
-                                            gap_row, sj_go, hx, fcost, v1, false, r1)
-                 << (16 * h + 8);
+          const int fresh = __builtin_amdgcn_readlane(bwin, s & 63);
+          int bot_jc[R], bot_ev[R], bot_cb[R], ev_us[R], cb_us[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            bot_jc[r] = __builtin_amdgcn_readlane(L.jc[r], 0);
+            bot_ev[r] = __builtin_amdgcn_readlane(L.S[r].ev1, 0);
+            bot_cb[r] = __builtin_amdgcn_readlane(L.S[r].cb1, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            L.jp[r] = L.jc[r];
+            L.jc[r] = dpp_from_above(L.jc[r], r == R - 1 ? fresh : bot_jc[r < R - 1 ? r + 1 : 0]);
+            ev_us[r] = dpp_from_above(L.S[r].ev1, r == R - 1 ? kHigh : bot_ev[r < R - 1 ? r + 1 : 0]);
+            cb_us[r] = dpp_from_above(L.S[r].cb1, r == R - 1 ? kHigh : bot_cb[r < R - 1 ? r + 1 : 0]);
+            asm volatile("" : "+v"(L.jc[r]), "+v"(ev_us[r]), "+v"(cb_us[r]));
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int i = i0[r] + s, j = j0[r] + s + 1;
+            const int jc = L.jc[r], jp = L.jp[r];
+            L.gap_row[r] = s_prep[jc];
+            L.sj_go[r] = (!(kGap & jp) && (kGap & jc)) ? 0 : go;
+            L.hx[r] = ((jp & kGap) && !(jc & kGap)) ? L.sj_go[r] + L.gap_row[r] : L.gap_row[r];
+            if (j == 1) L.hx[r] = L.gap_row[r];
+            const int ev_u = ev_us[r], cb_u = cb_us[r];
+            acc[r] |= aff_diag_cell<kBt, kSt>(L.S[r], L.S[r].eh1, L.S[r].cb1, ev_u, cb_u, i, j, la, lb, E0, go,
+                                              s_cost, L.ic[r], L.ip[r], L.si_ge[r], L.si_go[r], L.sive[r], jc,
+                                              L.gap_row[r], L.sj_go[r], L.hx[r], L.fcost[r], v1[r], false, r1[r])
+                      << (16 * h + 8);
+          }
         }
       }
-      if (kBt) dmw[(size_t)g * dmw_step] = acc;
+      if (kBt) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) dmw[r][(size_t)g * dmw_step[r]] = acc[r];
+      }
     };
     int g = 0;
     for (; g < g_lo; ++g) group(g, std::false_type{});
     for (; g < g_hi; ++g) group(g, std::true_type{});
     for (; g < groups; ++g) group(g, std::false_type{});
-    // cell (la, lb) sits on diagonal lb - la, i.e. lane (lb - la + 40) / 2
-    cost_out[p] = __builtin_amdgcn_readlane(fcost, (lb - la + 40) >> 1);
+    // cell (la, lb) sits on diagonal lb - la, i.e. diagonal pair (lb - la + 40) / 2
+    const int q = (lb - la + 40) >> 1, qr = q >> 6;
+    int fc = L.fcost[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r) fc = qr == r ? L.fcost[r] : fc;
+    cost_out[p] = __builtin_amdgcn_readlane(fc, q & 63);
   }
 }
 
@@ -760,6 +828,163 @@ __global__ void aff_trace_kernel(const PairMeta *__restrict__ meta, const int32_
   out_len[4 * p + (m.swap ? 3 : 2)] = m.cap - wa.pos;
   out_len[4 * p + (m.swap ? 2 : 3)] = m.cap - wb.pos;
   if (bad) atomicAdd(status, 1);
+}
+
+// backtrace_affine over the anti-diagonal band layout, one wave per pair.
+// The walk only moves to smaller anti-diagonals, so the band is read in
+// tiles of consecutive 4-step groups (contiguous in memory), copied to LDS
+// by the whole wave together with the window of both sequences the tile's
+// cells can touch; the walk itself is wave-uniform (every lane follows it,
+// the values read from LDS are made scalar), so it runs on scalar branches
+// at LDS latency instead of one dependent HBM load per step.
+constexpr int kTrTileBytes = 8192;   // direction bytes staged per wave
+constexpr int kTrWin = 512;          // sequence bytes staged per wave and side
+
+__global__ __launch_bounds__(kWave * 4) void aff_trace_wave_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, const uint8_t *__restrict__ dm_base,
+    uint8_t *out, int32_t *out_len, int32_t *status) {
+  constexpr int kTileDw = kTrTileBytes / 4;   // dwords per tile
+  constexpr int kPerLane = kTileDw / kWave;   // staged dwords per lane
+  constexpr int kWinPerLane = kTrWin / kWave; // staged sequence bytes per lane and side
+  __shared__ uint8_t s_med[32 * 32];
+  __shared__ uint32_t s_tile[4][kTileDw];
+  __shared__ uint8_t s_win[4][2][kTrWin];
+  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_med[k] = tcm->median[k];
+  __syncthreads();
+  const int w = wave_in_group();
+  const int lane = lane_id();
+  uint32_t *tile = s_tile[w];
+  const uint8_t *tile8 = (const uint8_t *)tile;
+  uint8_t *win_a = s_win[w][0];
+  uint8_t *win_b = s_win[w][1];
+  for (;;) {
+    const int k = next_item(counter);
+    if (k >= n) break;
+    const int p = uniform32(order[k]);
+    const PairMeta m = uniform_meta(meta, p);
+    const uint8_t *A = seq + m.off_r;
+    const uint8_t *B = seq + m.off_c;
+    const int la = m.lr - 1, lb = m.lc - 1, E0 = m.p0, W = m.p1;
+    const uint32_t *dmw = (const uint32_t *)(dm_base + m.dm_off);
+    // groups per tile; at most 90 so that a tile's cells stay inside the
+    // sequence windows: a tile of ng groups spans at most 2 ng + W + 3 rows
+    // (and columns), which is below kTrWin for W <= 256
+    const int ng = min(kTrTileBytes / (4 * W), 90);
+    uint8_t *o = out + m.out_off;
+    RevWriter wm, ww, wa, wb;
+    wm.init(o, m.cap);
+    ww.init(o + m.cap, m.cap);
+    wa.init(o + (m.swap ? 3 : 2) * m.cap, m.cap);
+    wb.init(o + (m.swap ? 2 : 3) * m.cap, m.cap);
+    int i = la, j = lb, mode = 0;   // 0 todo, 1 V, 2 H, 3 D, 4 A
+    int nm = 0, bad = 0, last_med = -1;
+    // Tiles: the walk leaves a tile only into the group just below it, so
+    // the next tile (groups [tg0 - ng, tg0 - 1]) is loaded into registers
+    // while the current one is walked, then committed to LDS in one batch.
+    uint32_t pre[kPerLane];
+    int prea[kWinPerLane], preb[kWinPerLane];
+    int ntg0 = 0, ntg1 = -1, nia0 = 0, njb0 = 0;   // the prefetched tile
+    auto fetch = [&](int g1) {
+      ntg1 = g1;
+      ntg0 = max(0, g1 - ng + 1);
+      const int nd = (ntg1 - ntg0 + 1) * W;
+#pragma unroll
+      for (int u = 0; u < kPerLane; ++u) pre[u] = dmw[(size_t)ntg0 * W + min(u * kWave + lane, nd - 1)];
+      nia0 = max(0, ((4 * ntg0 - E0) >> 1) - 1);
+      njb0 = max(0, ((4 * ntg0 - 40) >> 1) - 1);
+#pragma unroll
+      for (int u = 0; u < kWinPerLane; ++u) {
+        prea[u] = A[min(nia0 + u * kWave + lane, la)];
+        preb[u] = B[min(njb0 + u * kWave + lane, lb)];
+      }
+    };
+    int tg0 = 0x7fffffff, ia0 = 0, jb0 = 0;
+    auto commit = [&]() {
+      wave_sync_mem();   // the walk's reads of the previous tile are done
+      // slots past the tile's groups receive copies of its last dword and
+      // are never read (the tile buffer holds kTileDw dwords)
+#pragma unroll
+      for (int u = 0; u < kPerLane; ++u) tile[u * kWave + lane] = pre[u];
+#pragma unroll
+      for (int u = 0; u < kWinPerLane; ++u) {
+        win_a[u * kWave + lane] = (uint8_t)prea[u];
+        win_b[u * kWave + lane] = (uint8_t)preb[u];
+      }
+      wave_sync_mem();
+      tg0 = ntg0;
+      ia0 = nia0;
+      jb0 = njb0;
+    };
+    if (i != 0 && j != 0) fetch((i + j) >> 2);
+    while (i != 0 && j != 0) {
+      const int t = i + j, g = t >> 2;
+      if (g < tg0) {
+        commit();
+        if (tg0 > 0) fetch(tg0 - 1);
+      }
+      const int sp = 1 + (i > 40 ? i - 40 : 0);
+      const int ep = min(lb, E0 + i - 1);
+      const int hi = ep < lb ? ep + 1 : ep;
+      if (j < sp - 1 || j > hi) { bad = 1; break; }
+      const int f = uniform32(tile8[(g - tg0) * 4 * W + (((j - i + 40) >> 1) << 2) + (t & 3)]);
+      const int ic = uniform32(win_a[i - ia0]);
+      const int jc = uniform32(win_b[j - jb0]);
+      if (mode == 0) {   // entering a state reads the same cell again: do it now
+        const int td = f & 3;
+        mode = td == 0 ? 2 : (td == 1 ? 4 : (td == 2 ? 1 : 3));
+      }
+      if (mode == 1) {
+        if (f & 4) mode = 0;
+        if (!(ic & kGap)) { wm.put(ic | kGap); ww.put(ic | kGap); ++nm; last_med = ic | kGap; }
+        else ww.put(kGap);
+        wa.put(ic); wb.put(kGap);
+        --i;
+      } else if (mode == 2) {
+        if (f & 8) mode = 0;
+        if (!(jc & kGap)) { wm.put(jc | kGap); ww.put(jc | kGap); ++nm; last_med = jc | kGap; }
+        else ww.put(kGap);
+        wa.put(kGap); wb.put(jc);
+        --j;
+      } else if (mode == 3) {
+        if (f & 16) mode = 0;
+        wa.put(ic); wb.put(jc); ww.put(kGap);
+        --i; --j;
+      } else {
+        const int an = (f >> 5) & 3;
+        if (an == 1) mode = 2; else if (an == 2) mode = 3; else if (an == 3) mode = 1;
+        const int prep = uniform32(s_med[(ic & 15) * 32 + (jc & 15)]);
+        wm.put(prep); ww.put(prep); ++nm; last_med = prep;
+        wa.put(ic); wb.put(jc);
+        --i; --j;
+      }
+    }
+    if (!bad) {
+      while (i != 0) {
+        const int ic = uniform32(A[i]);
+        if (!(ic & kGap)) { wm.put(ic | kGap); ww.put(ic | kGap); ++nm; last_med = ic | kGap; }
+        else ww.put(kGap);
+        wa.put(ic); wb.put(kGap);
+        --i;
+      }
+      while (j != 0) {
+        const int jc = uniform32(B[j]);
+        if (!(jc & kGap)) { wm.put(jc | kGap); ww.put(jc | kGap); ++nm; last_med = jc | kGap; }
+        else ww.put(kGap);
+        wa.put(kGap); wb.put(jc);
+        --j;
+      }
+      wa.put(kGap); wb.put(kGap); ww.put(kGap);
+      if (nm == 0 || last_med != kGap) { wm.put(kGap); ++nm; }
+    }
+    // every lane stores the same values (no lane-divergent code in the loop)
+    wm.flush(); ww.flush(); wa.flush(); wb.flush();
+    out_len[4 * p + 0] = m.cap - wm.pos;
+    out_len[4 * p + 1] = m.cap - ww.pos;
+    out_len[4 * p + (m.swap ? 3 : 2)] = m.cap - wa.pos;
+    out_len[4 * p + (m.swap ? 2 : 3)] = m.cap - wb.pos;
+    if (bad) atomicAdd(status, lane == 0 ? 1 : 0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1151,15 +1376,52 @@ hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, co
   return hipGetLastError();
 }
 
-hipError_t launch_aff_diag(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
-                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
-                           hipStream_t s) {
+// Workgroups of aff_diag_kernel<bt, R> one CU holds at once (register and
+// LDS limits), for a persistent grid that is exactly resident.
+template <bool kBt, int R>
+static int diag_occupancy() {
+  int nb = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_diag_kernel<kBt, R>, kWave * 4, 0);
+  return (e == hipSuccess && nb > 0) ? nb : 2;
+}
+int aff_diag_blocks_per_cu(bool bt, int sets) {
+  if (sets == 1) return bt ? diag_occupancy<true, 1>() : diag_occupancy<false, 1>();
+  if (sets == 2) return bt ? diag_occupancy<true, 2>() : diag_occupancy<false, 2>();
+  return bt ? diag_occupancy<true, 4>() : diag_occupancy<false, 4>();
+}
+
+template <int R>
+static void launch_diag_r(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                          const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
+                          hipStream_t s) {
   if (bt)
-    hipLaunchKernelGGL(aff_diag_kernel<true>, dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq, tcm,
-                       dm, cost, sink);
+    hipLaunchKernelGGL((aff_diag_kernel<true, R>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
+                       tcm, dm, cost, sink);
   else
-    hipLaunchKernelGGL(aff_diag_kernel<false>, dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq, tcm,
-                       dm, cost, sink);
+    hipLaunchKernelGGL((aff_diag_kernel<false, R>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
+                       tcm, dm, cost, sink);
+}
+
+hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, const int32_t *order, int n,
+                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
+                           uint32_t *sink, hipStream_t s) {
+  if (sets == 1) launch_diag_r<1>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, s);
+  else if (sets == 2) launch_diag_r<2>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, s);
+  else launch_diag_r<4>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, s);
+  return hipGetLastError();
+}
+
+int aff_trace_wave_blocks_per_cu() {
+  int nb = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_trace_wave_kernel, kWave * 4, 0);
+  return (e == hipSuccess && nb > 0) ? nb : 2;
+}
+
+hipError_t launch_aff_trace_wave(int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                                 const uint8_t *seq, const DevTcm *tcm, const uint8_t *dm, uint8_t *out,
+                                 int32_t *out_len, int32_t *status, hipStream_t s) {
+  hipLaunchKernelGGL(aff_trace_wave_kernel, dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq, tcm, dm,
+                     out, out_len, status);
   return hipGetLastError();
 }
 
