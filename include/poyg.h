@@ -163,7 +163,8 @@ int poyg_plan_out_offsets(const poyg_plan *plan, int64_t *out_off);
 int64_t poyg_plan_cells(const poyg_plan *plan);
 /* Runs the plan on `stream` (hipStream_t, or NULL for the context stream).
  * d_seq: device sequences; d_cost: n int32; d_out / d_out_len as above
- * (ignored by cost modes). */
+ * (ignored by cost modes; d_out 16-byte aligned, as hipMalloc returns it).
+ * Bytes of a slot left of its right-aligned sequence are unspecified. */
 int poyg_plan_run(poyg_plan *plan, const uint8_t *d_seq, int32_t *d_cost, uint8_t *d_out,
                   int32_t *d_out_len, void *stream);
 /* Kernel timings of the last run (ms, measured with HIP events on the run's

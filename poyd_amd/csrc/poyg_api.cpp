@@ -489,6 +489,10 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
     set_error("poyg_plan_run: alignment modes need output buffers");
     return POYG_EINVAL;
   }
+  if (align && ((uintptr_t)d_out & 15) != 0) {
+    set_error("poyg_plan_run: the output buffer must be 16-byte aligned");
+    return POYG_EINVAL;
+  }
   hipStream_t s = stream ? (hipStream_t)stream : P->ctx->stream;
   int *ctr = (int *)P->counters.p;
   const PairMeta *meta = (const PairMeta *)P->meta.p;

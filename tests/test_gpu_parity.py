@@ -128,6 +128,47 @@ def test_affine_batch_matches_oracle(ctx, oracle, sub, indel, go, alphabet):
             assert np.array_equal(got, want), (k, slot, len(a), len(b))
 
 
+def block_indel_pairs(rng):
+    """Pairs whose alignment path crosses many band lanes: long sequences
+    with block insertions and deletions (band widths of 40 to 250 diagonal
+    pairs), long runs of one state, several 1,024-step output batches, and
+    sequences where one side ends early (tail gaps)."""
+    A, B = [], []
+    for k, (L, ins, dele) in enumerate([(1500, 150, 0), (1500, 0, 140), (2200, 300, 0), (900, 60, 60),
+                                         (3000, 420, 30), (1200, 5, 5), (2500, 0, 0), (400, 200, 0),
+                                         (1800, 90, 90), (700, 0, 380)]):
+        a = rand_seq(rng, L)
+        body = list(a[1:])
+        if dele:
+            at = int(rng.integers(0, L - dele))
+            del body[at:at + dele]
+        if ins:
+            at = int(rng.integers(0, len(body)))
+            body[at:at] = list(rng.choice([1, 2, 4, 8], size=ins))
+        b = mutate(rng, np.array([GAP] + body, np.uint8), 0.1, 0.02)
+        A.append(a)
+        B.append(b)
+        A.append(b)   # the same pair with the sides swapped
+        B.append(a)
+    return A, B
+
+
+@pytest.mark.parametrize("sub,indel,go", AFFINE_MATRICES)
+def test_affine_block_indels(ctx, oracle, sub, indel, go):
+    from poyd_amd import engine
+    note(f"affine block indels sub={sub} indel={indel} go={go}")
+    rng = np.random.default_rng(700 + sub * 10 + go)
+    A, B = block_indel_pairs(rng)
+    cm = engine.CostMatrix.of_sub_indel(sub, indel, go)
+    ref = oracle.CM2.sub_indel(sub, indel, go)
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_AFFINE_ALIGN, batch_of(A), batch_of(B))
+    for k, (a, b) in enumerate(zip(A, B)):
+        med, ri, rj, cost, mwg = oracle.align_affine_3(a, b, ref)
+        assert res.cost[k] == cost, (k, len(a), len(b))
+        for slot, want in enumerate((med, mwg, ri, rj)):
+            assert np.array_equal(res.seq(k, slot), want), (k, slot, len(a), len(b))
+
+
 @pytest.mark.parametrize("sub,indel", [(1, 2), (1, 1), (2, 1)])
 @pytest.mark.parametrize("alphabet", ["dna", "ambig"])
 def test_linear_batch_matches_oracle(ctx, oracle, sub, indel, alphabet):
