@@ -18,6 +18,10 @@ hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, co
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
                            int32_t *cost, int *rowscratch, int row_width, hipStream_t s);
 int aff_diag_blocks_per_cu(bool bt, int sets);
+int aff_quad_blocks_per_cu(bool bt, int h);
+hipError_t launch_aff_quad(bool bt, int h, int grid, const PairMeta *meta, const int32_t *order, int n,
+                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
+                           uint32_t *sink, hipStream_t s);
 hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
                            uint32_t *sink, hipStream_t s);
@@ -69,7 +73,8 @@ struct poyg_ctx {
   int device = 0;
   int cus = 256;
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;   // second stream: wide-band pairs run beside the main fill
+  hipStream_t side = nullptr;    // second stream: fills beside the main one
+  hipStream_t side2 = nullptr;   // third stream: the widest bands (lane sets, row kernels)
 };
 
 // A device buffer owned by a plan.
@@ -104,28 +109,37 @@ struct poyg_plan {
   std::vector<int64_t> out_off;
   // device
   // work lists (longest first): row kernels (LDS / global rows), the
-  // anti-diagonal kernel per lane-set count (1, 2, 4), and the backtraces
-  DevBuf tcm, meta, order_all, order_lds, order_glb, order_diag[3], order_rows, order_tr, dm, aux, rows, counters,
-      status, sink;
-  int n_lds = 0, n_glb = 0, n_diag[3] = {0, 0, 0}, n_rows = 0, n_tr = 0;
-  int grid_lds = 0, grid_glb = 0, grid_diag[3] = {0, 0, 0}, grid_tr = 0, row_width = 0;
+  // four-pairs-per-wave kernel (H = 3, 4), the anti-diagonal kernel per
+  // lane-set count (1, 2, 4), and the backtraces
+  DevBuf tcm, meta, order_all, order_lds, order_glb, order_quad[2], order_diag[3], order_rows, order_tr, dm, aux,
+      rows, counters, status, sink;
+  int n_lds = 0, n_glb = 0, n_quad[2] = {0, 0}, n_diag[3] = {0, 0, 0}, n_rows = 0, n_tr = 0;
+  int grid_lds = 0, grid_glb = 0, grid_quad[2] = {0, 0}, grid_diag[3] = {0, 0, 0}, grid_tr = 0, row_width = 0;
+  // the fill with the most cells runs on the main stream ("main": 1 the
+  // H = 3 four-pair kernel, 2 the one-set anti-diagonal kernel, 0 none); every
+  // other fill runs on the side stream, followed by a second launch of the
+  // main kernel (grid_help workgroups) that pulls from the same work list
+  int main_fill = 0, grid_help = 0;
   bool timing = false;
   float fill_ms = 0.f, trace_ms = 0.f;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  hipEvent_t fork = nullptr, join = nullptr;   // side-stream hand-off of the wide-band fills
+  hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;   // hand-off to and from the side streams
   ~poyg_plan() {
     if (triple) poyg_triple_plan_destroy(triple);
-    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_diag[0], &order_diag[1],
-                      &order_diag[2], &order_rows, &order_tr, &dm, &aux, &rows, &counters, &status, &sink})
+    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_quad[0], &order_quad[1], &order_diag[0],
+                      &order_diag[1], &order_diag[2], &order_rows, &order_tr, &dm, &aux, &rows, &counters, &status,
+                      &sink})
       b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
+    if (join2) (void)hipEventDestroy(join2);
   }
 };
 
 static int64_t round16(int64_t v) { return (v + 15) & ~(int64_t)15; }
+static constexpr int kQuadH = 3;   // diagonal pairs per lane of the four-pair kernel
 
 extern "C" {
 
@@ -143,8 +157,10 @@ int poyg_ctx_create(int device, poyg_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->side) (void)hipStreamDestroy(c->side);
     delete c;
     set_error("poyg_ctx_create: stream creation failed");
     return POYG_EDEVICE;
@@ -157,6 +173,7 @@ void poyg_ctx_destroy(poyg_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
+  if (ctx->side2) (void)hipStreamDestroy(ctx->side2);
   delete ctx;
 }
 
@@ -300,7 +317,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   P->slots = align ? (affine ? 4 : 3) : 0;
   std::vector<PairMeta> meta(n);
   std::vector<int64_t> cells(n);
-  std::vector<char> glb(n, 0), diag(n, 0);
+  std::vector<char> glb(n, 0), diag(n, 0), quad(n, 0);
   int64_t dm_bytes = 0, aux_ints = 0, out_bytes = 0;
   int row_width = 0;
   P->out_off.resize(n);
@@ -332,6 +349,11 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
         // anti-diagonal wavefront (1, 2 or 4 lane sets): one dword per
         // diagonal pair and 4 steps
         diag[k] = lanes <= kWave ? 1 : (lanes <= 2 * kWave ? 2 : 4);
+        // four pairs per wave: 16 lanes of H = 3 diagonal pairs (row 0
+        // alone, ra == 0, stays with the one-set kernel).  Wider bands belong
+        // to long pairs, whose latency the one-pair-per-wave kernel keeps
+        // lower (a lane of it steps one diagonal pair instead of three).
+        if (ra > 0 && lanes <= 16 * kQuadH) quad[k] = 3;
         m.p1 = lanes;
         m.p2 = 1;
         if (align && ra > 0) {
@@ -396,10 +418,16 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a] > cells[b]; });
-  std::vector<int32_t> o_lds, o_glb, o_diag[3], o_rows, o_tr;
+  std::vector<int32_t> o_lds, o_glb, o_quad[2], o_diag[3], o_rows, o_tr;
+  int64_t c_quad3 = 0, c_diag1 = 0;
   for (int k : order) {
-    if (diag[k]) {
+    if (quad[k]) {
+      o_quad[quad[k] - 3].push_back(k);
+      if (quad[k] == 3) c_quad3 += cells[k];
+      o_tr.push_back(k);
+    } else if (diag[k]) {
       o_diag[diag[k] == 1 ? 0 : (diag[k] == 2 ? 1 : 2)].push_back(k);
+      if (diag[k] == 1) c_diag1 += cells[k];
       o_tr.push_back(k);
     } else {
       (glb[k] ? o_glb : o_lds).push_back(k);
@@ -417,11 +445,46 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       P->grid_diag[q] = std::max(1, std::min((P->n_diag[q] + 3) / 4,
                                              ctx->cus * aff_diag_blocks_per_cu(align, q == 0 ? 1 : 2 * q)));
   }
+  for (int h = 0; h < 2; ++h) {
+    P->n_quad[h] = (int)o_quad[h].size();
+    if (P->n_quad[h])   // four pairs per wave, four waves per workgroup
+      P->grid_quad[h] = std::max(1, std::min((P->n_quad[h] + 15) / 16, ctx->cus * aff_quad_blocks_per_cu(align, h + 3)));
+  }
   if (P->n_tr) P->grid_tr = std::max(1, std::min((P->n_tr + 3) / 4, ctx->cus * aff_trace_wave_blocks_per_cu()));
   P->row_width = row_width;
   const int per_cu = affine ? 4 : 8;
   P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
   P->grid_glb = std::max(1, std::min(P->n_glb, 256));
+  if (affine) {
+    // The fill with the most cells is the main kernel.  The side stream's
+    // kernels are persistent too; the main kernel leaves them whole CUs, or
+    // whichever kernel the hardware dispatches first would hold every slot
+    // and the other would wait for it to drain.  Once the side work is done,
+    // a second launch of the main kernel on the side stream pulls from the
+    // same work list.
+    P->main_fill = P->n_quad[0] && c_quad3 >= c_diag1 ? 1 : (P->n_diag[0] ? 2 : (P->n_quad[0] ? 1 : 0));
+    auto cus_of = [](int grid, int bpc) { return (grid + bpc - 1) / std::max(1, bpc); };
+    int side = 0;
+    for (int q = 1; q < 3; ++q)
+      if (P->n_diag[q]) side += cus_of(P->grid_diag[q], aff_diag_blocks_per_cu(align, 2 * q));
+    if (P->n_quad[1]) side += cus_of(P->grid_quad[1], aff_quad_blocks_per_cu(align, 4));
+    if (P->n_quad[0] && P->main_fill != 1) side += cus_of(P->grid_quad[0], aff_quad_blocks_per_cu(align, 3));
+    if (P->n_diag[0] && P->main_fill != 2) side += cus_of(P->grid_diag[0], aff_diag_blocks_per_cu(align, 1));
+    if (P->n_lds) side += cus_of(P->grid_lds, per_cu);
+    if (P->n_glb) side += cus_of(P->grid_glb, 8);
+    if (side && P->main_fill) {
+      side = std::min(side, ctx->cus / 2);
+      if (P->main_fill == 1) {
+        const int bpc = aff_quad_blocks_per_cu(align, 3);
+        P->grid_quad[0] = std::max(1, std::min(P->grid_quad[0], (ctx->cus - side) * bpc));
+        P->grid_help = side * bpc;
+      } else {
+        const int bpc = aff_diag_blocks_per_cu(align, 1);
+        P->grid_diag[0] = std::max(1, std::min(P->grid_diag[0], (ctx->cus - side) * bpc));
+        P->grid_help = side * bpc;
+      }
+    }
+  }
   DevTcm dt;
   to_dev_tcm(cm, &dt);
   int rc = 0;
@@ -431,6 +494,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   rc |= P->order_lds.alloc(sizeof(int32_t) * (size_t)P->n_lds);
   rc |= P->order_glb.alloc(sizeof(int32_t) * (size_t)P->n_glb);
   for (int q = 0; q < 3; ++q) rc |= P->order_diag[q].alloc(sizeof(int32_t) * (size_t)P->n_diag[q]);
+  for (int h = 0; h < 2; ++h) rc |= P->order_quad[h].alloc(sizeof(int32_t) * (size_t)P->n_quad[h]);
   rc |= P->order_rows.alloc(sizeof(int32_t) * (size_t)P->n_rows);
   rc |= P->order_tr.alloc(sizeof(int32_t) * (size_t)P->n_tr);
   rc |= P->dm.alloc((size_t)dm_bytes);
@@ -439,9 +503,10 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   rc |= P->counters.alloc(sizeof(int) * 8);
   rc |= P->status.alloc(sizeof(int32_t) * 4);
   rc |= P->sink.alloc(sizeof(uint32_t) * kWave);
-  if (!rc && affine && P->n_diag[0] && (P->n_lds || P->n_glb || P->n_diag[1] || P->n_diag[2]) &&
+  if (!rc && affine && P->grid_help &&
       (hipEventCreateWithFlags(&P->fork, hipEventDisableTiming) != hipSuccess ||
-       hipEventCreateWithFlags(&P->join, hipEventDisableTiming) != hipSuccess))
+       hipEventCreateWithFlags(&P->join, hipEventDisableTiming) != hipSuccess ||
+       hipEventCreateWithFlags(&P->join2, hipEventDisableTiming) != hipSuccess))
     rc = POYG_EDEVICE;
   if (rc) {
     delete P;
@@ -456,6 +521,10 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     e = hipMemcpyAsync(P->order_lds.p, o_lds.data(), sizeof(int32_t) * P->n_lds, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && P->n_glb)
     e = hipMemcpyAsync(P->order_glb.p, o_glb.data(), sizeof(int32_t) * P->n_glb, hipMemcpyHostToDevice, s);
+  for (int h = 0; h < 2; ++h)
+    if (e == hipSuccess && P->n_quad[h])
+      e = hipMemcpyAsync(P->order_quad[h].p, o_quad[h].data(), sizeof(int32_t) * P->n_quad[h],
+                         hipMemcpyHostToDevice, s);
   for (int q = 0; q < 3; ++q)
     if (e == hipSuccess && P->n_diag[q])
       e = hipMemcpyAsync(P->order_diag[q].p, o_diag[q].data(), sizeof(int32_t) * P->n_diag[q],
@@ -504,33 +573,51 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
     HIP_TRY(launch_prep(affine, std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * 8)), meta, P->n, d_seq, tcm,
                         (int *)P->aux.p, s));
   if (affine) {
-    // The wide-band pairs (row kernels) are the longest; they run on the side
-    // stream, launched first, beside the anti-diagonal kernel on the main one.
+    // The main kernel runs on the main stream; the other one-set and
+    // four-pair fills on the side stream, followed by a second launch of the
+    // main kernel there; the widest bands (lane sets, row kernels: few, long
+    // pairs) on a third stream, so that they start at once.
     const bool fork = P->fork != nullptr;
-    hipStream_t rs = s;
+    hipStream_t rs = s, rs2 = s;
     if (fork) {
       rs = P->ctx->side;
+      rs2 = P->ctx->side2;
       HIP_TRY(hipEventRecord(P->fork, s));
       HIP_TRY(hipStreamWaitEvent(rs, P->fork, 0));
+      HIP_TRY(hipStreamWaitEvent(rs2, P->fork, 0));
     }
+    uint8_t *dm = (uint8_t *)P->dm.p;
+    uint32_t *sink = (uint32_t *)P->sink.p;
+    auto quad = [&](int h, int grid, hipStream_t st) {
+      return launch_aff_quad(align, h + 3, grid, meta, (const int32_t *)P->order_quad[h].p, P->n_quad[h], ctr + 6 + h,
+                             d_seq, tcm, dm, d_cost, sink, st);
+    };
+    auto diag = [&](int q, int grid, hipStream_t st) {
+      return launch_aff_diag(align, q == 0 ? 1 : 2 * q, grid, meta, (const int32_t *)P->order_diag[q].p,
+                             P->n_diag[q], ctr + 2 + q, d_seq, tcm, dm, d_cost, sink, st);
+    };
     for (int q = 2; q >= 1; --q)   // the widest bands: 4 and 2 lane sets
-      if (P->n_diag[q])
-        HIP_TRY(launch_aff_diag(align, 2 * q, P->grid_diag[q], meta, (const int32_t *)P->order_diag[q].p,
-                                P->n_diag[q], ctr + 2 + q, d_seq, tcm, (uint8_t *)P->dm.p, d_cost,
-                                (uint32_t *)P->sink.p, rs));
+      if (P->n_diag[q]) HIP_TRY(diag(q, P->grid_diag[q], rs2));
     if (P->n_lds)
       HIP_TRY(launch_aff_fill(align, false, P->grid_lds, meta, (const int32_t *)P->order_lds.p, P->n_lds, ctr,
-                              d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, nullptr, 0, rs));
+                              d_seq, tcm, dm, (int *)P->aux.p, d_cost, nullptr, 0, rs2));
     if (P->n_glb)
       HIP_TRY(launch_aff_fill(align, true, P->grid_glb, meta, (const int32_t *)P->order_glb.p, P->n_glb, ctr + 1,
-                              d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, (int *)P->rows.p,
-                              P->row_width, rs));
-    if (P->n_diag[0])
-      HIP_TRY(launch_aff_diag(align, 1, P->grid_diag[0], meta, (const int32_t *)P->order_diag[0].p, P->n_diag[0],
-                              ctr + 2, d_seq, tcm, (uint8_t *)P->dm.p, d_cost, (uint32_t *)P->sink.p, s));
+                              d_seq, tcm, dm, (int *)P->aux.p, d_cost, (int *)P->rows.p, P->row_width, rs2));
+    if (P->n_quad[1]) HIP_TRY(quad(1, P->grid_quad[1], rs));
+    if (P->n_quad[0] && P->main_fill != 1) HIP_TRY(quad(0, P->grid_quad[0], rs));
+    if (P->n_diag[0] && P->main_fill != 2) HIP_TRY(diag(0, P->grid_diag[0], rs));
+    if (fork && P->grid_help) {
+      if (P->main_fill == 1) HIP_TRY(quad(0, P->grid_help, rs));
+      else HIP_TRY(diag(0, P->grid_help, rs));
+    }
+    if (P->main_fill == 1) HIP_TRY(quad(0, P->grid_quad[0], s));
+    else if (P->main_fill == 2) HIP_TRY(diag(0, P->grid_diag[0], s));
     if (fork) {
       HIP_TRY(hipEventRecord(P->join, rs));
+      HIP_TRY(hipEventRecord(P->join2, rs2));
       HIP_TRY(hipStreamWaitEvent(s, P->join, 0));
+      HIP_TRY(hipStreamWaitEvent(s, P->join2, 0));
     }
   } else {
     if (P->n_lds)

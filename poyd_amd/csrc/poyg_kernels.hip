@@ -127,6 +127,11 @@ __device__ __forceinline__ PairMeta uniform_meta(const PairMeta *meta, int p) {
   return m;
 }
 
+// The one-pair-per-wave fills take the longest pairs, whose run time is
+// a chain of dependent steps; when they share SIMDs with the four-pair
+// kernel, their waves issue first.
+__device__ __forceinline__ void wave_priority_long_pairs() { __builtin_amdgcn_s_setprio(2); }
+
 __device__ __forceinline__ int next_item(int *counter) {
   int v = 0;
   if (lane_id() == 0) v = atomicAdd(counter, 1);
@@ -393,6 +398,7 @@ __global__ __launch_bounds__(kWave * kAffWaves) void aff_fill_lds_kernel(
   __shared__ int s_prep[32];
   __shared__ int s_rows[kAffWaves][2][4][kAffRB];
   load_tables(tcm, s_cost, s_prep, nullptr);
+  wave_priority_long_pairs();
   const int w = wave_in_group();
   const int go = tcm->gap_open;
   RowRef R{&s_rows[w][0][0][0], kAffRB - 1, 4 * kAffRB, kAffRB};
@@ -415,6 +421,7 @@ __global__ __launch_bounds__(kWave) void aff_fill_glb_kernel(
   __shared__ int s_cost[32 * 32];
   __shared__ int s_prep[32];
   load_tables(tcm, s_cost, s_prep, nullptr);
+  wave_priority_long_pairs();
   const int go = tcm->gap_open;
   int *mine = rowscratch + (size_t)blockIdx.x * 8 * row_width;
   RowRef R{mine, -1, 4 * row_width, row_width};
@@ -473,78 +480,102 @@ template <bool kBt, bool kSteady>
 __device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int eh_l, int cb_l, int ev_u, int cb_u, int i, int j,
                                                   int la, int lb, int E0, int go, const int *s_cost, int ic, int ip,
                                                   int si_ge, int si_go, int sive, int jc, int gap_row, int sj_go,
-                                                  int hx, int &fcost, bool st_valid, bool st_left,
-                                                  bool st_right) {
+                                                  int hx, int &fcost, int edge) {
+  // Every decision below is a select on int values: the cell code must not
+  // branch (no lane may be switched off between the DPP moves of a step).
   const int d = j - i;
   // FILL_EXTEND_VERTICAL
   const int ext_v = ev_u + sive, open_v = cb_u + si_go + si_ge;
-  const bool endv = !(ext_v < open_v);
-  int ev = endv ? open_v : ext_v;
+  const int endv = ext_v >= open_v;
+  int ev = min(ext_v, open_v);
   // FILL_EXTEND_BLOCK_DIAGONAL (bt / nobt)
-  const bool ic_gap = ic & kGap, jc_gap = jc & kGap;
-  const bool both_gap = ic_gap && jc_gap;
+  const int ic_gap = ic & kGap, jc_gap = jc & kGap;
+  const int both_gap = (ic & jc) & kGap;
   const int diag = both_gap ? 0 : kHigh;
   const int ext_b = S.eb2 + diag;
   int open_b;
   if (kBt) {
     open_b = S.cb2 + diag;
   } else {
-    const bool flag2 = !(kGap & ip) && !jc_gap;
+    const int flag2 = !(kGap & ip) && !jc_gap;
     open_b = S.cb2 + (both_gap ? (flag2 ? 0 : 2 * go) : kHigh);
   }
-  const bool endb = !(ext_b < open_b);
-  int eb = endb ? open_b : ext_b;
-  // FILL_CLOSE_BLOCK_DIAGONAL
+  const int endb = ext_b >= open_b;
+  int eb = min(ext_b, open_b);
+  // FILL_CLOSE_BLOCK_DIAGONAL: the aligned state and where it came from
   const int dg = s_cost[(ic & 15) * 32 + (jc & 15)];
-  const int xgo = (sj_go < si_go) ? si_go : sj_go;
+  const int xgo = max(si_go, sj_go);
   const int fv = S.ev2 + dg + (ic_gap ? sj_go : 0);
   const int fh = S.eh2 + dg + (jc_gap ? si_go : 0);
   const int fd = S.eb2 + dg + xgo;
   int cb = S.cb2 + dg;
-  bool a2v = false, a2h = false, a2d = false;
-  if (cb >= fv) { if (cb > fv) cb = fv; a2v = true; }
-  if (cb >= fh) { if (cb > fh) { cb = fh; a2v = false; } a2h = true; }
-  if (cb >= fd) { if (cb > fd) { cb = fd; a2v = false; a2h = false; } a2d = true; }
+  int a2v = cb >= fv;
+  cb = min(cb, fv);
+  const int gt_h = cb > fh;
+  int a2h = cb >= fh;
+  a2v = gt_h ? 0 : a2v;
+  cb = min(cb, fh);
+  const int gt_d = cb > fd;
+  const int a2d = cb >= fd;
+  a2v = gt_d ? 0 : a2v;
+  a2h = gt_d ? 0 : a2h;
+  cb = min(cb, fd);
   const int anext = a2h ? 1 : (a2d ? 2 : (a2v ? 3 : 0));
   // FILL_EXTEND_HORIZONTAL
   const int bval = cb_l + sj_go + gap_row, ext_h = eh_l + hx;
-  const bool endh = !(ext_h < bval);
-  int eh = endh ? bval : ext_h;
-  // ASSIGN_MINIMUM
+  const int endh = ext_h >= bval;
+  int eh = min(ext_h, bval);
+  // ASSIGN_MINIMUM: ties prefer H, then V, then D, then A
   int f = eh;
-  bool dh = true, dv = false, dd = false, da = false;
-  if (f >= ev) { if (f > ev) { f = ev; dh = false; } dv = true; }
-  if (f >= eb) { if (f > eb) { f = eb; dh = dv = false; } dd = true; }
-  if (f >= cb) { if (f > cb) { f = cb; dh = dv = dd = false; } da = true; }
+  int dh = f <= ev;          // f > ev clears H
+  int dv = f >= ev;
+  f = min(f, ev);
+  const int gt_b = f > eb;
+  int dd = f >= eb;
+  dh = gt_b ? 0 : dh;
+  dv = gt_b ? 0 : dv;
+  f = min(f, eb);
+  const int gt_c = f > cb;
+  const int da = f >= cb;
+  dh = gt_c ? 0 : dh;
+  dv = gt_c ? 0 : dv;
+  dd = gt_c ? 0 : dd;
+  f = min(f, cb);
   const int todo = dh ? 0 : (da ? 1 : (dv ? 2 : 3));
+  (void)dd;
   uint32_t byte = (uint32_t)(todo | (endv << 2) | (endh << 3) | (endb << 4) | (anext << 5));
+  // assemble the byte now: left to the scheduler, the comparison results of
+  // all cells of a step stay alive as lane masks until the stores
+  if (kBt) asm volatile("" : "+v"(byte));
   // band edges: the left boundary column carries only the vertical state,
   // the cell right of the band is closed (initialize_matrices_affine and the
   // per-row boundary writes of algn_fill_plane_3_aff)
-  // selects only: no lane may be switched off in this loop (see the kernel)
   if (kSteady) {
-    const bool closed = st_left || st_right || !st_valid;
-    ev = closed ? (st_left ? ext_v : kHigh) : ev;
-    eh = closed ? kHigh : eh;
-    eb = closed ? kHigh : eb;
-    cb = closed ? kHigh : cb;
-    byte = closed ? (st_left ? kAffLeft : kAffRight) : byte;
+    // edge: 0 inside the band, 1 the left boundary column, 2 closed
+    ev = edge ? (edge == 1 ? ext_v : kHigh) : ev;
+    eh = edge ? kHigh : eh;
+    eb = edge ? kHigh : eb;
+    cb = edge ? kHigh : cb;
+    byte = edge ? (edge == 1 ? kAffLeft : kAffRight) : byte;
   } else {
-    const bool valid = i >= 0 && i <= la && j >= 0 && j <= lb && d <= E0;
-    const bool lbnd = valid && i >= 1 && (j == 0 || d == -40);
-    const bool rbnd = valid && i >= 1 && d == E0;
-    const bool closed = lbnd || rbnd || !valid;
-    ev = closed ? (lbnd ? ext_v : kHigh) : ev;
-    eh = closed ? kHigh : eh;
-    eb = closed ? kHigh : eb;
-    cb = closed ? kHigh : cb;
-    byte = closed ? (lbnd ? kAffLeft : kAffRight) : byte;
-    const bool row0 = valid && i == 0, j0 = j == 0;   // row 0
+    (void)edge;
+    const int valid = i >= 0 && i <= la && j >= 0 && j <= lb && d <= E0;
+    const int lbnd = valid && i >= 1 && (j == 0 || d == -40);
+    const int rbnd = valid && i >= 1 && d == E0;
+    const int closed = lbnd || rbnd || !valid;
+    const int row0 = valid && i == 0, j0 = j == 0;   // row 0
+    // one code per cell (0 open, 1 left boundary, 2 closed, 3 row 0), made
+    // opaque so that the selects below share it instead of each keeping its
+    // own condition alive
+    int code = row0 ? 3 : (closed ? (lbnd ? 1 : 2) : 0);
+    asm volatile("" : "+v"(code));
     const int r0eh = j0 ? go : eh_l + gap_row;
-    eh = row0 ? r0eh : eh;
-    ev = row0 ? (j0 ? go : kHigh) : ev;
-    eb = row0 ? (j0 ? 0 : kHigh) : eb;
-    cb = row0 ? (j0 ? 0 : r0eh) : cb;
+    const int open = code == 0;
+    ev = open ? ev : (code == 1 ? ext_v : (code == 3 && j0 ? go : kHigh));
+    eh = open ? eh : (code == 3 ? r0eh : kHigh);
+    eb = open ? eb : (code == 3 && j0 ? 0 : kHigh);
+    cb = open ? cb : (code == 3 ? (j0 ? 0 : r0eh) : kHigh);
+    byte = (code == 1 || code == 2) ? (code == 1 ? kAffLeft : kAffRight) : byte;
     fcost = (valid && i == la && j == lb) ? f : fcost;
   }
   S.eh2 = S.eh1; S.ev2 = S.ev1; S.eb2 = S.eb1; S.cb2 = S.cb1;
@@ -577,6 +608,7 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
   __shared__ int s_cost[32 * 32];
   __shared__ int s_prep[32];
   load_tables(tcm, s_cost, s_prep, nullptr);
+  wave_priority_long_pairs();
   const int go = tcm->gap_open;
   const int lane = lane_id();
   for (;;) {
@@ -684,8 +716,8 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
             const int eh_l = eh_ls[r], cb_l = cb_ls[r];
             acc[r] |= aff_diag_cell<kBt, kSt>(L.S[r], eh_l, cb_l, L.S[r].ev1, L.S[r].cb1, i, j0[r] + s, la, lb, E0,
                                               go, s_cost, ic, ip, L.si_ge[r], L.si_go[r], L.sive[r], L.jc[r],
-                                              L.gap_row[r], L.sj_go[r], L.hx[r], L.fcost[r], v0[r],
-                                              r == 0 && l0, r0[r])
+                                              L.gap_row[r], L.sj_go[r], L.hx[r], L.fcost[r],
+                                              (r == 0 && l0) ? 1 : ((r0[r] || !v0[r]) ? 2 : 0))
                       << (16 * h);
           }
         }
@@ -718,7 +750,8 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
             const int ev_u = ev_us[r], cb_u = cb_us[r];
             acc[r] |= aff_diag_cell<kBt, kSt>(L.S[r], L.S[r].eh1, L.S[r].cb1, ev_u, cb_u, i, j, la, lb, E0, go,
                                               s_cost, L.ic[r], L.ip[r], L.si_ge[r], L.si_go[r], L.sive[r], jc,
-                                              L.gap_row[r], L.sj_go[r], L.hx[r], L.fcost[r], v1[r], false, r1[r])
+                                              L.gap_row[r], L.sj_go[r], L.hx[r], L.fcost[r],
+                                              (r1[r] || !v1[r]) ? 2 : 0)
                       << (16 * h + 8);
           }
         }
@@ -738,6 +771,245 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
 #pragma unroll
     for (int r = 1; r < R; ++r) fc = qr == r ? L.fcost[r] : fc;
     cost_out[p] = __builtin_amdgcn_readlane(fc, q & 63);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Affine band, four pairs per wave (bands of at most 16 H diagonal pairs).
+// ---------------------------------------------------------------------------
+// The same anti-diagonal recurrence as aff_diag_kernel, with each lane
+// holding H consecutive diagonal pairs ("virtual lanes" L = H q + k) and each
+// 16-lane row of the wave working on its own pair.  A band of E0 <= 55 fills
+// 41..48 of the 48 virtual lanes of H = 3 (against 41..64 of the 64 lanes in
+// aff_diag_kernel), and the per-step overhead (DPP moves, incoming
+// characters, stores) is shared by H cells per lane.  Between the virtual
+// lanes of one lane values move by register; across lanes by DPP row_shr:1
+// (from the lane below) and row_shl:1 (from the lane above), which stop at
+// the 16-lane row boundaries and there return the band-edge value (kHigh) or
+// the incoming character, exactly what lanes 0 / 63 receive in
+// aff_diag_kernel.  Incoming characters come from a 16-entry window per row
+// that rotates by one lane per step (row_ror:15), so the current entry is
+// always at lane 0 (rows) / lane 15 (columns).  Row constants (gap costs of a
+// row) move with their row character; only the row entering a lane is
+// evaluated.
+__device__ __forceinline__ int dpp_row_from_below(int v, int fill) {   // lane q <- q-1 in the row
+  return __builtin_amdgcn_update_dpp(fill, v, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_row_from_above(int v, int fill) {   // lane q <- q+1 in the row
+  return __builtin_amdgcn_update_dpp(fill, v, 0x101 /* row_shl:1 */, 0xf, 0xf, false);
+}
+// lane q <- lane (q + 1) mod 16 of its row
+__device__ __forceinline__ int dpp_row_rotate_down(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x12f /* row_ror:15 */, 0xf, 0xf, false);
+}
+
+template <int H>
+struct QuadLanes {
+  AffDiag S[H];
+  int ic[H], ip[H], si_ge[H], si_go[H], sive[H];   // row i = s + 20 - L and its gap costs
+  int jc[H], jp[H], gap_row[H], sj_go[H], hx[H];   // column and its gap costs
+  int fcost[H];
+};
+
+template <bool kBt, int H>
+__global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, int32_t *cost_out,
+    uint32_t *sink) {
+  __shared__ int s_cost[32 * 32];
+  __shared__ int s_prep[32];
+  load_tables(tcm, s_cost, s_prep, nullptr);
+  const int go = tcm->gap_open;
+  const int lane = lane_id();
+  const int q = lane & 15;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(counter, 4);
+    base = uniform32(base);
+    if (base >= n) break;
+    // row r of the wave takes item base + r; rows past the list repeat the
+    // last item and store nothing
+    const int item = base + (lane >> 4);
+    const bool active = item < n;
+    const int p = order[min(item, n - 1)];
+    const PairMeta m = meta[p];
+    const uint8_t *A = seq + m.off_r;
+    const uint8_t *B = seq + m.off_c;
+    const int la = m.lr - 1, lb = m.lc - 1, E0 = m.p0, W = m.p1;
+    uint32_t *dmw = (uint32_t *)(dm_base + m.dm_off);
+    QuadLanes<H> Q;
+    int Lk[H], edge0[H], edge1[H];   // edges of the even / odd cell (0 open, 1 left column, 2 closed)
+    auto row_consts = [&](int ic, int ip, int i, int &ge, int &gop, int &ve) {
+      ge = s_cost[ic * 32 + kGap];
+      gop = (!(kGap & ip) && (kGap & ic)) ? 0 : go;
+      ve = (i > 1 && (ip & kGap) && !(ic & kGap)) ? gop + ge : ge;
+    };
+    auto col_consts = [&](int jc, int jp, int j, int &gr, int &gop, int &x) {
+      gr = s_prep[jc];
+      gop = (!(kGap & jp) && (kGap & jc)) ? 0 : go;
+      x = ((jp & kGap) && !(jc & kGap)) ? gop + gr : gr;
+      x = j == 1 ? gr : x;
+    };
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const int L = H * q + k;
+      Lk[k] = L;
+      // the state before step 0: row 19 - L, column L - 20
+      Q.ic[k] = load_or0(A, 19 - L, la);
+      Q.ip[k] = load_or0(A, 18 - L, la);
+      row_consts(Q.ic[k], Q.ip[k], 19 - L, Q.si_ge[k], Q.si_go[k], Q.sive[k]);
+      Q.jc[k] = load_or0(B, L - 20, lb);
+      Q.jp[k] = load_or0(B, L - 21, lb);
+      col_consts(Q.jc[k], Q.jp[k], L - 20, Q.gap_row[k], Q.sj_go[k], Q.hx[k]);
+      Q.S[k] = AffDiag{kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh};
+      Q.fcost[k] = 0;
+      const int d0 = 2 * L - 40;
+      edge0[k] = L == 0 ? 1 : (d0 >= E0 ? 2 : 0);
+      edge1[k] = d0 + 1 >= E0 ? 2 : 0;
+    }
+    const int Gs = (la + lb + 4) >> 2;   // groups of 4 steps of this row's pair
+    // steady groups of this row's pair (all cells inside the sequences)
+    const int t_lo = E0 + 82, t_hi = min(2 * lb - E0, 2 * la - 40);
+    const int glo = active ? (t_lo + 3) >> 2 : 0;
+    const int ghi = active ? (t_hi >= 3 ? (t_hi - 3) / 4 + 1 : 0) : 0x3fffffff;
+    int groups = 0, g_lo = 0, g_hi = 0x3fffffff;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      groups = max(groups, __builtin_amdgcn_readlane(Gs, 16 * r));
+      g_lo = max(g_lo, __builtin_amdgcn_readlane(glo, 16 * r));
+      g_hi = min(g_hi, __builtin_amdgcn_readlane(ghi, 16 * r));
+    }
+    // characters entering the row at pair-step s: A[s + 20] at lane 0 and
+    // B[s - 20 + 16 H] at lane 15.  Windows of 16 steps: before the
+    // rotations of a window's steps, lane q holds step 16 w + q (rows) and
+    // step 16 w + ((q + 1) & 15) (columns)
+    int awin = load_or0(A, 20 + q, la);
+    int bwin = load_or0(B, 16 * H - 20 + ((q + 1) & 15), lb);
+    int anext = 0, bnext = 0;
+    auto group = [&](int g, auto steady_tag) {
+      constexpr bool kSt = decltype(steady_tag)::value;
+      if ((g & 7) == 0) {   // a new 16-step window; prefetch the one after it
+        if (g) {
+          awin = anext;
+          bwin = bnext;
+        }
+        anext = load_or0(A, 2 * g + 16 + 20 + q, la);
+        bnext = load_or0(B, 2 * g + 16 + 16 * H - 20 + ((q + 1) & 15), lb);
+      }
+      {
+        // per-pair values pass through an opaque move at every group, so
+        // that the compiler cannot hoist the band-edge comparisons of all
+        // cells out of the loop (as lane masks, that exhausts the scalar
+        // registers and spills)
+        int lag = la, lbg = lb, E0g = E0, Lg[H], e0g[H], e1g[H];
+        asm volatile("" : "+v"(lag), "+v"(lbg), "+v"(E0g));
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          Lg[k] = Lk[k];
+          e0g[k] = edge0[k];
+          e1g[k] = edge1[k];
+          asm volatile("" : "+v"(Lg[k]), "+v"(e0g[k]), "+v"(e1g[k]));
+        }
+        uint32_t acc[H];
+#pragma unroll
+        for (int k = 0; k < H; ++k) acc[k] = 0u;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int s = 2 * g + h;
+          {   // even step: next row character into every lane's virtual lane 0
+            dpp_exec_guard();
+            int nic = dpp_row_from_below(Q.ic[H - 1], awin);
+            int eh_l0 = dpp_row_from_below(Q.S[H - 1].eh1, kHigh);
+            int cb_l0 = dpp_row_from_below(Q.S[H - 1].cb1, kHigh);
+            awin = dpp_row_rotate_down(awin);   // the next step's entry to lane 0
+            asm volatile("" : "+v"(nic), "+v"(eh_l0), "+v"(cb_l0), "+v"(awin));
+            const int old0 = Q.ic[0];
+#pragma unroll
+            for (int k = H - 1; k > 0; --k) {
+              Q.ip[k] = Q.ic[k];
+              Q.ic[k] = Q.ic[k - 1];
+              Q.si_ge[k] = Q.si_ge[k - 1];
+              Q.si_go[k] = Q.si_go[k - 1];
+              Q.sive[k] = Q.sive[k - 1];
+            }
+            Q.ip[0] = old0;
+            Q.ic[0] = nic;
+            row_consts(nic, old0, s + 20 - Lg[0], Q.si_ge[0], Q.si_go[0], Q.sive[0]);
+            // descending k: virtual lane k reads k-1's odd cell before it changes
+#pragma unroll
+            for (int k = H - 1; k >= 0; --k) {
+              const int eh_l = k == 0 ? eh_l0 : Q.S[k > 0 ? k - 1 : 0].eh1;
+              const int cb_l = k == 0 ? cb_l0 : Q.S[k > 0 ? k - 1 : 0].cb1;
+              acc[k] |= aff_diag_cell<kBt, kSt>(Q.S[k], eh_l, cb_l, Q.S[k].ev1, Q.S[k].cb1, s + 20 - Lg[k],
+                                                s - 20 + Lg[k], lag, lbg, E0g, go, s_cost, Q.ic[k], Q.ip[k],
+                                                Q.si_ge[k], Q.si_go[k], Q.sive[k], Q.jc[k], Q.gap_row[k], Q.sj_go[k],
+                                                Q.hx[k], Q.fcost[k], e0g[k])
+                        << (16 * h);
+            }
+          }
+          {   // odd step: next column character into every lane's virtual lane H-1
+            dpp_exec_guard();
+            int njc = dpp_row_from_above(Q.jc[0], bwin);
+            int ev_u0 = dpp_row_from_above(Q.S[0].ev1, kHigh);
+            int cb_u0 = dpp_row_from_above(Q.S[0].cb1, kHigh);
+            bwin = dpp_row_rotate_down(bwin);   // the next step's entry to lane 15
+            asm volatile("" : "+v"(njc), "+v"(ev_u0), "+v"(cb_u0), "+v"(bwin));
+            const int oldt = Q.jc[H - 1];
+#pragma unroll
+            for (int k = 0; k < H - 1; ++k) {
+              Q.jp[k] = Q.jc[k];
+              Q.jc[k] = Q.jc[k + 1];
+              Q.gap_row[k] = Q.gap_row[k + 1];
+              Q.sj_go[k] = Q.sj_go[k + 1];
+              Q.hx[k] = Q.hx[k + 1];
+            }
+            Q.jp[H - 1] = oldt;
+            Q.jc[H - 1] = njc;
+            col_consts(njc, oldt, s - 19 + Lg[H - 1], Q.gap_row[H - 1], Q.sj_go[H - 1], Q.hx[H - 1]);
+            // ascending k: virtual lane k reads k+1's even cell before it changes
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+              const int ev_u = k == H - 1 ? ev_u0 : Q.S[k < H - 1 ? k + 1 : 0].ev1;
+              const int cb_u = k == H - 1 ? cb_u0 : Q.S[k < H - 1 ? k + 1 : 0].cb1;
+              acc[k] |= aff_diag_cell<kBt, kSt>(Q.S[k], Q.S[k].eh1, Q.S[k].cb1, ev_u, cb_u, s + 20 - Lg[k],
+                                                s - 19 + Lg[k], lag, lbg, E0g, go, s_cost, Q.ic[k], Q.ip[k],
+                                                Q.si_ge[k], Q.si_go[k], Q.sive[k], Q.jc[k], Q.gap_row[k], Q.sj_go[k],
+                                                Q.hx[k], Q.fcost[k], e1g[k])
+                        << (16 * h + 8);
+            }
+          }
+        }
+        if (kBt) {
+          const bool gin = active && g < Gs;
+#pragma unroll
+          for (int k = 0; k < H; ++k) {
+            uint32_t *dst = (gin && Lk[k] < W) ? dmw + (size_t)g * W + Lk[k] : sink + lane;
+            *dst = acc[k];
+          }
+        }
+      }
+    };
+    // three loops (edge groups, steady groups, edge groups), so that no code
+    // of a group is shared with -- and hoisted above -- a branch between the
+    // two variants
+    const int gs_lo = min(g_lo, groups), gs_hi = max(gs_lo, min(g_hi, groups));
+    int g = 0;
+    for (; g < gs_lo; ++g) group(g, std::false_type{});
+    for (; g < gs_hi; ++g) group(g, std::true_type{});
+    for (; g < groups; ++g) group(g, std::false_type{});
+    // cell (la, lb): diagonal pair (lb - la + 40) / 2 = H qs + ks
+    const int Ls = (lb - la + 40) >> 1;
+    const int qs = Ls / H, ks = Ls - H * (Ls / H);
+    int fc = Q.fcost[0];
+#pragma unroll
+    for (int k = 1; k < H; ++k) fc = ks == k ? Q.fcost[k] : fc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int src = 16 * r + __builtin_amdgcn_readlane(qs, 16 * r);
+      const int c = __builtin_amdgcn_readlane(fc, src);
+      const int pr = __builtin_amdgcn_readlane(p, 16 * r);
+      if (__builtin_amdgcn_readlane((int)active, 16 * r)) cost_out[pr] = c;   // every lane stores the same value
+    }
   }
 }
 
@@ -1582,6 +1854,38 @@ hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, co
   if (sets == 1) launch_diag_r<1>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, s);
   else if (sets == 2) launch_diag_r<2>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, s);
   else launch_diag_r<4>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, s);
+  return hipGetLastError();
+}
+
+template <bool kBt, int H>
+static int quad_occupancy() {
+  int nb = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_quad_kernel<kBt, H>, kWave * 4, 0);
+  return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+int aff_quad_blocks_per_cu(bool bt, int h) {
+  if (h == 3) return bt ? quad_occupancy<true, 3>() : quad_occupancy<false, 3>();
+  return bt ? quad_occupancy<true, 4>() : quad_occupancy<false, 4>();
+}
+
+hipError_t launch_aff_quad(bool bt, int h, int grid, const PairMeta *meta, const int32_t *order, int n,
+                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
+                           uint32_t *sink, hipStream_t s) {
+  if (h == 3) {
+    if (bt)
+      hipLaunchKernelGGL((aff_quad_kernel<true, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
+                         tcm, dm, cost, sink);
+    else
+      hipLaunchKernelGGL((aff_quad_kernel<false, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter,
+                         seq, tcm, dm, cost, sink);
+  } else {
+    if (bt)
+      hipLaunchKernelGGL((aff_quad_kernel<true, 4>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
+                         tcm, dm, cost, sink);
+    else
+      hipLaunchKernelGGL((aff_quad_kernel<false, 4>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter,
+                         seq, tcm, dm, cost, sink);
+  }
   return hipGetLastError();
 }
 
