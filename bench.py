@@ -103,7 +103,12 @@ def main() -> int:
     ap.add_argument("--workload", default="affine", choices=["affine", "linear"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device once (the reference dispatch that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               f"(pairs are independent: N GPUs would each take 1/N of the batch)", file=sys.stderr)
@@ -214,6 +219,19 @@ def main() -> int:
         out["config"]["gpu_vs_cpu_1core"] = round(gcups / cb["value"], 1)
     print(json.dumps(out), flush=True)
     plan.close()
+    return 0
+
+
+def copy_calibration(mib: int) -> int:
+    """One plain device copy of known size (reads and writes mib MiB)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "checksum_ok": bool(dst[-1].item() == 1)}), flush=True)
     return 0
 
 

@@ -18,10 +18,12 @@ hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, co
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
                            int32_t *cost, int *rowscratch, int row_width, hipStream_t s);
 int aff_diag_blocks_per_cu(bool bt, int sets);
-int aff_quad_blocks_per_cu(bool bt, int h);
-hipError_t launch_aff_quad(bool bt, int h, int grid, const PairMeta *meta, const int32_t *order, int n,
-                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
-                           uint32_t *sink, hipStream_t s);
+int aff_quad_blocks_per_cu(bool bt);
+hipError_t launch_pair_pure(int grid, const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
+                            uint8_t *pure, hipStream_t s);
+hipError_t launch_aff_quad(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
+                           const uint8_t *pure, hipStream_t s);
 hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
                            uint32_t *sink, hipStream_t s);
@@ -109,14 +111,15 @@ struct poyg_plan {
   std::vector<int64_t> out_off;
   // device
   // work lists (longest first): row kernels (LDS / global rows), the
-  // four-pairs-per-wave kernel (H = 3, 4), the anti-diagonal kernel per
-  // lane-set count (1, 2, 4), and the backtraces
-  DevBuf tcm, meta, order_all, order_lds, order_glb, order_quad[2], order_diag[3], order_rows, order_tr, dm, aux,
-      rows, counters, status, sink;
-  int n_lds = 0, n_glb = 0, n_quad[2] = {0, 0}, n_diag[3] = {0, 0, 0}, n_rows = 0, n_tr = 0;
-  int grid_lds = 0, grid_glb = 0, grid_quad[2] = {0, 0}, grid_diag[3] = {0, 0, 0}, grid_tr = 0, row_width = 0;
+  // four-pairs-per-wave kernel, the anti-diagonal kernel per lane-set count
+  // (1, 2, 4), and the backtraces; pure: per pair, one sequence without
+  // gap-bearing characters (written by a scan before the fills)
+  DevBuf tcm, meta, order_all, order_lds, order_glb, order_quad, order_diag[3], order_rows, order_tr, dm, aux,
+      rows, counters, status, sink, pure;
+  int n_lds = 0, n_glb = 0, n_quad = 0, n_diag[3] = {0, 0, 0}, n_rows = 0, n_tr = 0;
+  int grid_lds = 0, grid_glb = 0, grid_quad = 0, grid_diag[3] = {0, 0, 0}, grid_tr = 0, row_width = 0;
   // the fill with the most cells runs on the main stream ("main": 1 the
-  // H = 3 four-pair kernel, 2 the one-set anti-diagonal kernel, 0 none); every
+  // four-pair kernel, 2 the one-set anti-diagonal kernel, 0 none); every
   // other fill runs on the side stream, followed by a second launch of the
   // main kernel (grid_help workgroups) that pulls from the same work list
   int main_fill = 0, grid_help = 0;
@@ -126,9 +129,8 @@ struct poyg_plan {
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;   // hand-off to and from the side streams
   ~poyg_plan() {
     if (triple) poyg_triple_plan_destroy(triple);
-    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_quad[0], &order_quad[1], &order_diag[0],
-                      &order_diag[1], &order_diag[2], &order_rows, &order_tr, &dm, &aux, &rows, &counters, &status,
-                      &sink})
+    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_quad, &order_diag[0], &order_diag[1],
+                      &order_diag[2], &order_rows, &order_tr, &dm, &aux, &rows, &counters, &status, &sink, &pure})
       b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -418,12 +420,12 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a] > cells[b]; });
-  std::vector<int32_t> o_lds, o_glb, o_quad[2], o_diag[3], o_rows, o_tr;
+  std::vector<int32_t> o_lds, o_glb, o_quad, o_diag[3], o_rows, o_tr;
   int64_t c_quad3 = 0, c_diag1 = 0;
   for (int k : order) {
     if (quad[k]) {
-      o_quad[quad[k] - 3].push_back(k);
-      if (quad[k] == 3) c_quad3 += cells[k];
+      o_quad.push_back(k);
+      c_quad3 += cells[k];
       o_tr.push_back(k);
     } else if (diag[k]) {
       o_diag[diag[k] == 1 ? 0 : (diag[k] == 2 ? 1 : 2)].push_back(k);
@@ -445,11 +447,9 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       P->grid_diag[q] = std::max(1, std::min((P->n_diag[q] + 3) / 4,
                                              ctx->cus * aff_diag_blocks_per_cu(align, q == 0 ? 1 : 2 * q)));
   }
-  for (int h = 0; h < 2; ++h) {
-    P->n_quad[h] = (int)o_quad[h].size();
-    if (P->n_quad[h])   // four pairs per wave, four waves per workgroup
-      P->grid_quad[h] = std::max(1, std::min((P->n_quad[h] + 15) / 16, ctx->cus * aff_quad_blocks_per_cu(align, h + 3)));
-  }
+  P->n_quad = (int)o_quad.size();
+  if (P->n_quad)   // four pairs per wave, four waves per workgroup
+    P->grid_quad = std::max(1, std::min((P->n_quad + 15) / 16, ctx->cus * aff_quad_blocks_per_cu(align)));
   if (P->n_tr) P->grid_tr = std::max(1, std::min((P->n_tr + 3) / 4, ctx->cus * aff_trace_wave_blocks_per_cu()));
   P->row_width = row_width;
   const int per_cu = affine ? 4 : 8;
@@ -462,21 +462,20 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     // and the other would wait for it to drain.  Once the side work is done,
     // a second launch of the main kernel on the side stream pulls from the
     // same work list.
-    P->main_fill = P->n_quad[0] && c_quad3 >= c_diag1 ? 1 : (P->n_diag[0] ? 2 : (P->n_quad[0] ? 1 : 0));
+    P->main_fill = P->n_quad && c_quad3 >= c_diag1 ? 1 : (P->n_diag[0] ? 2 : (P->n_quad ? 1 : 0));
     auto cus_of = [](int grid, int bpc) { return (grid + bpc - 1) / std::max(1, bpc); };
     int side = 0;
     for (int q = 1; q < 3; ++q)
       if (P->n_diag[q]) side += cus_of(P->grid_diag[q], aff_diag_blocks_per_cu(align, 2 * q));
-    if (P->n_quad[1]) side += cus_of(P->grid_quad[1], aff_quad_blocks_per_cu(align, 4));
-    if (P->n_quad[0] && P->main_fill != 1) side += cus_of(P->grid_quad[0], aff_quad_blocks_per_cu(align, 3));
+    if (P->n_quad && P->main_fill != 1) side += cus_of(P->grid_quad, aff_quad_blocks_per_cu(align));
     if (P->n_diag[0] && P->main_fill != 2) side += cus_of(P->grid_diag[0], aff_diag_blocks_per_cu(align, 1));
     if (P->n_lds) side += cus_of(P->grid_lds, per_cu);
     if (P->n_glb) side += cus_of(P->grid_glb, 8);
     if (side && P->main_fill) {
       side = std::min(side, ctx->cus / 2);
       if (P->main_fill == 1) {
-        const int bpc = aff_quad_blocks_per_cu(align, 3);
-        P->grid_quad[0] = std::max(1, std::min(P->grid_quad[0], (ctx->cus - side) * bpc));
+        const int bpc = aff_quad_blocks_per_cu(align);
+        P->grid_quad = std::max(1, std::min(P->grid_quad, (ctx->cus - side) * bpc));
         P->grid_help = side * bpc;
       } else {
         const int bpc = aff_diag_blocks_per_cu(align, 1);
@@ -487,6 +486,28 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   }
   DevTcm dt;
   to_dev_tcm(cm, &dt);
+  if (affine) {
+    // The affine kernels rank equal minima by packing (value << 2 | rank);
+    // every DP value is non-negative and at most 3 kHigh plus the cost of
+    // the whole alignment, which must stay below 2^29.
+    int64_t cmax = std::max(0, dt.gap_open), longest = 0;
+    bool negative = dt.gap_open < 0;
+    for (int x = 0; x < 32 * 32; ++x) {
+      negative |= dt.cost[x] < 0;
+      cmax = std::max<int64_t>(cmax, dt.cost[x]);
+    }
+    for (int x = 0; x < 32; ++x) {
+      negative |= dt.prepend[x] < 0;
+      cmax = std::max<int64_t>(cmax, dt.prepend[x]);
+    }
+    for (int k = 0; k < n; ++k) longest = std::max<int64_t>(longest, (int64_t)len_a[k] + len_b[k] + 2);
+    if (negative || 3 * (int64_t)kHighNum + 2 * cmax * longest >= ((int64_t)1 << 29)) {
+      delete P;
+      set_error("poyg_plan_pairs: affine costs must be non-negative and small enough that an alignment costs "
+                "less than 2^28 (largest entry %lld, longest pair %lld)", (long long)cmax, (long long)longest);
+      return POYG_EINVAL;
+    }
+  }
   int rc = 0;
   rc |= P->tcm.alloc(sizeof(DevTcm));
   rc |= P->meta.alloc(sizeof(PairMeta) * (size_t)n);
@@ -494,7 +515,8 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   rc |= P->order_lds.alloc(sizeof(int32_t) * (size_t)P->n_lds);
   rc |= P->order_glb.alloc(sizeof(int32_t) * (size_t)P->n_glb);
   for (int q = 0; q < 3; ++q) rc |= P->order_diag[q].alloc(sizeof(int32_t) * (size_t)P->n_diag[q]);
-  for (int h = 0; h < 2; ++h) rc |= P->order_quad[h].alloc(sizeof(int32_t) * (size_t)P->n_quad[h]);
+  rc |= P->order_quad.alloc(sizeof(int32_t) * (size_t)P->n_quad);
+  rc |= P->pure.alloc((size_t)n);
   rc |= P->order_rows.alloc(sizeof(int32_t) * (size_t)P->n_rows);
   rc |= P->order_tr.alloc(sizeof(int32_t) * (size_t)P->n_tr);
   rc |= P->dm.alloc((size_t)dm_bytes);
@@ -521,10 +543,8 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     e = hipMemcpyAsync(P->order_lds.p, o_lds.data(), sizeof(int32_t) * P->n_lds, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && P->n_glb)
     e = hipMemcpyAsync(P->order_glb.p, o_glb.data(), sizeof(int32_t) * P->n_glb, hipMemcpyHostToDevice, s);
-  for (int h = 0; h < 2; ++h)
-    if (e == hipSuccess && P->n_quad[h])
-      e = hipMemcpyAsync(P->order_quad[h].p, o_quad[h].data(), sizeof(int32_t) * P->n_quad[h],
-                         hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && P->n_quad)
+    e = hipMemcpyAsync(P->order_quad.p, o_quad.data(), sizeof(int32_t) * P->n_quad, hipMemcpyHostToDevice, s);
   for (int q = 0; q < 3; ++q)
     if (e == hipSuccess && P->n_diag[q])
       e = hipMemcpyAsync(P->order_diag[q].p, o_diag[q].data(), sizeof(int32_t) * P->n_diag[q],
@@ -577,6 +597,10 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
     // four-pair fills on the side stream, followed by a second launch of the
     // main kernel there; the widest bands (lane sets, row kernels: few, long
     // pairs) on a third stream, so that they start at once.
+    // purity of the four-pair kernel's pairs, before any fill is released
+    if (P->n_quad)
+      HIP_TRY(launch_pair_pure(std::max(1, std::min((P->n_quad + 3) / 4, P->ctx->cus * 8)), meta,
+                               (const int32_t *)P->order_quad.p, P->n_quad, d_seq, (uint8_t *)P->pure.p, s));
     const bool fork = P->fork != nullptr;
     hipStream_t rs = s, rs2 = s;
     if (fork) {
@@ -588,9 +612,9 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
     }
     uint8_t *dm = (uint8_t *)P->dm.p;
     uint32_t *sink = (uint32_t *)P->sink.p;
-    auto quad = [&](int h, int grid, hipStream_t st) {
-      return launch_aff_quad(align, h + 3, grid, meta, (const int32_t *)P->order_quad[h].p, P->n_quad[h], ctr + 6 + h,
-                             d_seq, tcm, dm, d_cost, sink, st);
+    auto quad = [&](int grid, hipStream_t st) {
+      return launch_aff_quad(align, grid, meta, (const int32_t *)P->order_quad.p, P->n_quad, ctr + 6, d_seq, tcm, dm,
+                             d_cost, sink, (const uint8_t *)P->pure.p, st);
     };
     auto diag = [&](int q, int grid, hipStream_t st) {
       return launch_aff_diag(align, q == 0 ? 1 : 2 * q, grid, meta, (const int32_t *)P->order_diag[q].p,
@@ -604,14 +628,13 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
     if (P->n_glb)
       HIP_TRY(launch_aff_fill(align, true, P->grid_glb, meta, (const int32_t *)P->order_glb.p, P->n_glb, ctr + 1,
                               d_seq, tcm, dm, (int *)P->aux.p, d_cost, (int *)P->rows.p, P->row_width, rs2));
-    if (P->n_quad[1]) HIP_TRY(quad(1, P->grid_quad[1], rs));
-    if (P->n_quad[0] && P->main_fill != 1) HIP_TRY(quad(0, P->grid_quad[0], rs));
+    if (P->n_quad && P->main_fill != 1) HIP_TRY(quad(P->grid_quad, rs));
     if (P->n_diag[0] && P->main_fill != 2) HIP_TRY(diag(0, P->grid_diag[0], rs));
     if (fork && P->grid_help) {
-      if (P->main_fill == 1) HIP_TRY(quad(0, P->grid_help, rs));
+      if (P->main_fill == 1) HIP_TRY(quad(P->grid_help, rs));
       else HIP_TRY(diag(0, P->grid_help, rs));
     }
-    if (P->main_fill == 1) HIP_TRY(quad(0, P->grid_quad[0], s));
+    if (P->main_fill == 1) HIP_TRY(quad(P->grid_quad, s));
     else if (P->main_fill == 2) HIP_TRY(diag(0, P->grid_diag[0], s));
     if (fork) {
       HIP_TRY(hipEventRecord(P->join, rs));

@@ -457,6 +457,8 @@ __device__ __forceinline__ int dpp_from_above(int v, int fill) {   // lane l <- 
   return __builtin_amdgcn_update_dpp(fill, v, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
 }
 
+__device__ __forceinline__ int min3_i32(int a, int b, int c) { return min(min(a, b), c); }
+
 // Branch-free bounded read: the index is clamped, the value then masked, so
 // no lane is switched off (see aff_diag_kernel).
 __device__ __forceinline__ int load_or0(const uint8_t *s, int idx, int len) {
@@ -476,11 +478,11 @@ struct AffDiag {
 // kSteady: every cell of the step has 41 <= i <= la and j <= lb, so the band
 // edges are the fixed lanes d == -40 (left) and d == E0 (right), given as
 // flags; no row 0 and no final cell occur.
-template <bool kBt, bool kSteady>
+template <bool kBt, bool kSteady, bool kPure = false>
 __device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int eh_l, int cb_l, int ev_u, int cb_u, int i, int j,
                                                   int la, int lb, int E0, int go, const int *s_cost, int ic, int ip,
                                                   int si_ge, int si_go, int sive, int jc, int gap_row, int sj_go,
-                                                  int hx, int &fcost, int edge) {
+                                                  int hx, int &fcost, int open_mask, int left_mask) {
   // Every decision below is a select on int values: the cell code must not
   // branch (no lane may be switched off between the DPP moves of a step).
   const int d = j - i;
@@ -488,61 +490,56 @@ __device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int eh_l, int cb_l
   const int ext_v = ev_u + sive, open_v = cb_u + si_go + si_ge;
   const int endv = ext_v >= open_v;
   int ev = min(ext_v, open_v);
-  // FILL_EXTEND_BLOCK_DIAGONAL (bt / nobt)
   const int ic_gap = ic & kGap, jc_gap = jc & kGap;
-  const int both_gap = (ic & jc) & kGap;
-  const int diag = both_gap ? 0 : kHigh;
-  const int ext_b = S.eb2 + diag;
-  int open_b;
-  if (kBt) {
-    open_b = S.cb2 + diag;
-  } else {
-    const int flag2 = !(kGap & ip) && !jc_gap;
-    open_b = S.cb2 + (both_gap ? (flag2 ? 0 : 2 * go) : kHigh);
-  }
-  const int endb = ext_b >= open_b;
-  int eb = min(ext_b, open_b);
-  // FILL_CLOSE_BLOCK_DIAGONAL: the aligned state and where it came from
   const int dg = s_cost[(ic & 15) * 32 + (jc & 15)];
-  const int xgo = max(si_go, sj_go);
   const int fv = S.ev2 + dg + (ic_gap ? sj_go : 0);
   const int fh = S.eh2 + dg + (jc_gap ? si_go : 0);
-  const int fd = S.eb2 + dg + xgo;
-  int cb = S.cb2 + dg;
-  int a2v = cb >= fv;
-  cb = min(cb, fv);
-  const int gt_h = cb > fh;
-  int a2h = cb >= fh;
-  a2v = gt_h ? 0 : a2v;
-  cb = min(cb, fh);
-  const int gt_d = cb > fd;
-  const int a2d = cb >= fd;
-  a2v = gt_d ? 0 : a2v;
-  a2h = gt_d ? 0 : a2h;
-  cb = min(cb, fd);
-  const int anext = a2h ? 1 : (a2d ? 2 : (a2v ? 3 : 0));
+  // The reference takes each minimum step by step (strict and non-strict
+  // comparisons); a state is flagged exactly when it equals the final
+  // minimum, with the precedence of its tests.  All values are non-negative
+  // and below 2^29, so the minimum of (value << 2 | rank) gives both the
+  // minimum and the first state reaching it.
+  int eb, endb, cb, anext;
+  if (kPure) {
+    // No cell of the group pairs two gap-bearing characters (one sequence
+    // has none past its leading gap), so the block-diagonal state is at
+    // least kHigh everywhere past (0, 0): it is never a minimum, its
+    // extension test always holds, and kHigh stands in for its value.
+    eb = kHigh;
+    endb = 1;
+    const int kc = min3_i32((fh << 2) | 0, (fv << 2) | 2, ((S.cb2 + dg) << 2) | 3);
+    cb = kc >> 2;
+    anext = (kc + 1) & 3;
+  } else {
+    // FILL_EXTEND_BLOCK_DIAGONAL (bt / nobt)
+    const int both_gap = (ic & jc) & kGap;
+    const int diag = both_gap ? 0 : kHigh;
+    const int ext_b = S.eb2 + diag;
+    int open_b;
+    if (kBt) {
+      open_b = S.cb2 + diag;
+    } else {
+      const int flag2 = !(kGap & ip) && !jc_gap;
+      open_b = S.cb2 + (both_gap ? (flag2 ? 0 : 2 * go) : kHigh);
+    }
+    endb = ext_b >= open_b;
+    eb = min(ext_b, open_b);
+    // FILL_CLOSE_BLOCK_DIAGONAL: the aligned state and where it came from
+    // (H, then D, then V)
+    const int fd = S.eb2 + dg + max(si_go, sj_go);
+    const int kc = min(min((fh << 2) | 0, (fd << 2) | 1), min((fv << 2) | 2, ((S.cb2 + dg) << 2) | 3));
+    cb = kc >> 2;
+    anext = (kc + 1) & 3;   // rank 0 H -> 1, 1 D -> 2, 2 V -> 3, 3 none -> 0
+  }
   // FILL_EXTEND_HORIZONTAL
   const int bval = cb_l + sj_go + gap_row, ext_h = eh_l + hx;
   const int endh = ext_h >= bval;
   int eh = min(ext_h, bval);
-  // ASSIGN_MINIMUM: ties prefer H, then V, then D, then A
-  int f = eh;
-  int dh = f <= ev;          // f > ev clears H
-  int dv = f >= ev;
-  f = min(f, ev);
-  const int gt_b = f > eb;
-  int dd = f >= eb;
-  dh = gt_b ? 0 : dh;
-  dv = gt_b ? 0 : dv;
-  f = min(f, eb);
-  const int gt_c = f > cb;
-  const int da = f >= cb;
-  dh = gt_c ? 0 : dh;
-  dv = gt_c ? 0 : dv;
-  dd = gt_c ? 0 : dd;
-  f = min(f, cb);
-  const int todo = dh ? 0 : (da ? 1 : (dv ? 2 : 3));
-  (void)dd;
+  // ASSIGN_MINIMUM: likewise, ties prefer H, then A, then V, then D
+  const int kf = kPure ? min3_i32((eh << 2) | 0, (cb << 2) | 1, (ev << 2) | 2)
+                       : min(min((eh << 2) | 0, (cb << 2) | 1), min((ev << 2) | 2, (eb << 2) | 3));
+  const int f = kf >> 2;
+  const int todo = kf & 3;
   uint32_t byte = (uint32_t)(todo | (endv << 2) | (endh << 3) | (endb << 4) | (anext << 5));
   // assemble the byte now: left to the scheduler, the comparison results of
   // all cells of a step stay alive as lane masks until the stores
@@ -551,14 +548,18 @@ __device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int eh_l, int cb_l
   // the cell right of the band is closed (initialize_matrices_affine and the
   // per-row boundary writes of algn_fill_plane_3_aff)
   if (kSteady) {
-    // edge: 0 inside the band, 1 the left boundary column, 2 closed
-    ev = edge ? (edge == 1 ? ext_v : kHigh) : ev;
-    eh = edge ? kHigh : eh;
-    eb = edge ? kHigh : eb;
-    cb = edge ? kHigh : cb;
-    byte = edge ? (edge == 1 ? kAffLeft : kAffRight) : byte;
+    // open_mask: all ones for a cell inside the band; left_mask: all ones on
+    // the left boundary column (bit selects, no lane masks)
+    const int ev_closed = (left_mask & ext_v) | (~left_mask & kHigh);
+    ev = (open_mask & ev) | (~open_mask & ev_closed);
+    eh = (open_mask & eh) | (~open_mask & kHigh);
+    eb = (open_mask & eb) | (~open_mask & kHigh);
+    cb = (open_mask & cb) | (~open_mask & kHigh);
+    const uint32_t byte_closed = ((uint32_t)left_mask & kAffLeft) | (~(uint32_t)left_mask & kAffRight);
+    byte = ((uint32_t)open_mask & byte) | (~(uint32_t)open_mask & byte_closed);
   } else {
-    (void)edge;
+    (void)open_mask;
+    (void)left_mask;
     const int valid = i >= 0 && i <= la && j >= 0 && j <= lb && d <= E0;
     const int lbnd = valid && i >= 1 && (j == 0 || d == -40);
     const int rbnd = valid && i >= 1 && d == E0;
@@ -717,7 +718,7 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
             acc[r] |= aff_diag_cell<kBt, kSt>(L.S[r], eh_l, cb_l, L.S[r].ev1, L.S[r].cb1, i, j0[r] + s, la, lb, E0,
                                               go, s_cost, ic, ip, L.si_ge[r], L.si_go[r], L.sive[r], L.jc[r],
                                               L.gap_row[r], L.sj_go[r], L.hx[r], L.fcost[r],
-                                              (r == 0 && l0) ? 1 : ((r0[r] || !v0[r]) ? 2 : 0))
+                                              (r == 0 && l0) || r0[r] || !v0[r] ? 0 : -1, (r == 0 && l0) ? -1 : 0)
                       << (16 * h);
           }
         }
@@ -751,7 +752,7 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
             acc[r] |= aff_diag_cell<kBt, kSt>(L.S[r], L.S[r].eh1, L.S[r].cb1, ev_u, cb_u, i, j, la, lb, E0, go,
                                               s_cost, L.ic[r], L.ip[r], L.si_ge[r], L.si_go[r], L.sive[r], jc,
                                               L.gap_row[r], L.sj_go[r], L.hx[r], L.fcost[r],
-                                              (r1[r] || !v1[r]) ? 2 : 0)
+                                              (r1[r] || !v1[r]) ? 0 : -1, 0)
                       << (16 * h + 8);
           }
         }
@@ -803,6 +804,25 @@ __device__ __forceinline__ int dpp_row_rotate_down(int v) {
   return __builtin_amdgcn_update_dpp(0, v, 0x12f /* row_ror:15 */, 0xf, 0xf, false);
 }
 
+// Per pair: 1 if one of its sequences has no gap-bearing character past the
+// leading gap, so that no DP cell pairs two of them (the block-diagonal
+// state then never matters; see aff_diag_cell).
+__global__ __launch_bounds__(kWave * 4) void pair_pure_kernel(const PairMeta *__restrict__ meta,
+                                                            const int32_t *__restrict__ order, int n,
+                                                            const uint8_t *__restrict__ seq, uint8_t *pure) {
+  const int lane = lane_id();
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int w = blockIdx.x * (blockDim.x / kWave) + wave_in_group(); w < n; w += waves) {
+    const int p = uniform32(order[w]);
+    const PairMeta m = uniform_meta(meta, p);
+    int ga = 0, gb = 0;
+    for (int x = 1 + lane; x < m.lr; x += kWave) ga |= seq[m.off_r + x];
+    for (int x = 1 + lane; x < m.lc; x += kWave) gb |= seq[m.off_c + x];
+    const bool any_a = __ballot(ga & kGap) != 0, any_b = __ballot(gb & kGap) != 0;
+    pure[p] = (!any_a || !any_b) ? 1 : 0;   // every lane stores the same value
+  }
+}
+
 template <int H>
 struct QuadLanes {
   AffDiag S[H];
@@ -815,7 +835,7 @@ template <bool kBt, int H>
 __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
     const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, int32_t *cost_out,
-    uint32_t *sink) {
+    uint32_t *sink, const uint8_t *__restrict__ pure) {
   __shared__ int s_cost[32 * 32];
   __shared__ int s_prep[32];
   load_tables(tcm, s_cost, s_prep, nullptr);
@@ -838,7 +858,10 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
     const int la = m.lr - 1, lb = m.lc - 1, E0 = m.p0, W = m.p1;
     uint32_t *dmw = (uint32_t *)(dm_base + m.dm_off);
     QuadLanes<H> Q;
-    int Lk[H], edge0[H], edge1[H];   // edges of the even / odd cell (0 open, 1 left column, 2 closed)
+    // per diagonal pair: all ones if the even / odd cell is inside the band
+    // (steady groups); lmask: all ones on lane 0 (the left boundary column)
+    int Lk[H], open0[H], open1[H];
+    const int lmask = q == 0 ? -1 : 0;
     auto row_consts = [&](int ic, int ip, int i, int &ge, int &gop, int &ve) {
       ge = s_cost[ic * 32 + kGap];
       gop = (!(kGap & ip) && (kGap & ic)) ? 0 : go;
@@ -864,17 +887,19 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
       Q.S[k] = AffDiag{kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh, kHigh};
       Q.fcost[k] = 0;
       const int d0 = 2 * L - 40;
-      edge0[k] = L == 0 ? 1 : (d0 >= E0 ? 2 : 0);
-      edge1[k] = d0 + 1 >= E0 ? 2 : 0;
+      open0[k] = (L == 0 || d0 >= E0) ? 0 : -1;
+      open1[k] = d0 + 1 >= E0 ? 0 : -1;
     }
     const int Gs = (la + lb + 4) >> 2;   // groups of 4 steps of this row's pair
+    const int pf = active ? pure[p] : 1;
     // steady groups of this row's pair (all cells inside the sequences)
     const int t_lo = E0 + 82, t_hi = min(2 * lb - E0, 2 * la - 40);
     const int glo = active ? (t_lo + 3) >> 2 : 0;
     const int ghi = active ? (t_hi >= 3 ? (t_hi - 3) / 4 + 1 : 0) : 0x3fffffff;
-    int groups = 0, g_lo = 0, g_hi = 0x3fffffff;
+    int groups = 0, g_lo = 0, g_hi = 0x3fffffff, all_pure = 1;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
+      all_pure &= __builtin_amdgcn_readlane(pf, 16 * r);
       groups = max(groups, __builtin_amdgcn_readlane(Gs, 16 * r));
       g_lo = max(g_lo, __builtin_amdgcn_readlane(glo, 16 * r));
       g_hi = min(g_hi, __builtin_amdgcn_readlane(ghi, 16 * r));
@@ -886,8 +911,11 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
     int awin = load_or0(A, 20 + q, la);
     int bwin = load_or0(B, 16 * H - 20 + ((q + 1) & 15), lb);
     int anext = 0, bnext = 0;
-    auto group = [&](int g, auto steady_tag) {
-      constexpr bool kSt = decltype(steady_tag)::value;
+    // variant: 0 edge groups, 1 steady groups, 2 steady groups of four pure
+    // pairs (see aff_diag_cell)
+    auto group = [&](int g, auto variant_tag) {
+      constexpr int kVar = decltype(variant_tag)::value;
+      constexpr bool kSt = kVar >= 1, kPu = kVar == 2;
       if ((g & 7) == 0) {   // a new 16-step window; prefetch the one after it
         if (g) {
           awin = anext;
@@ -901,13 +929,13 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
         // that the compiler cannot hoist the band-edge comparisons of all
         // cells out of the loop (as lane masks, that exhausts the scalar
         // registers and spills)
-        int lag = la, lbg = lb, E0g = E0, Lg[H], e0g[H], e1g[H];
-        asm volatile("" : "+v"(lag), "+v"(lbg), "+v"(E0g));
+        int lag = la, lbg = lb, E0g = E0, Lg[H], e0g[H], e1g[H], lmg = lmask;
+        asm volatile("" : "+v"(lag), "+v"(lbg), "+v"(E0g), "+v"(lmg));
 #pragma unroll
         for (int k = 0; k < H; ++k) {
           Lg[k] = Lk[k];
-          e0g[k] = edge0[k];
-          e1g[k] = edge1[k];
+          e0g[k] = open0[k];
+          e1g[k] = open1[k];
           asm volatile("" : "+v"(Lg[k]), "+v"(e0g[k]), "+v"(e1g[k]));
         }
         uint32_t acc[H];
@@ -940,10 +968,10 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
             for (int k = H - 1; k >= 0; --k) {
               const int eh_l = k == 0 ? eh_l0 : Q.S[k > 0 ? k - 1 : 0].eh1;
               const int cb_l = k == 0 ? cb_l0 : Q.S[k > 0 ? k - 1 : 0].cb1;
-              acc[k] |= aff_diag_cell<kBt, kSt>(Q.S[k], eh_l, cb_l, Q.S[k].ev1, Q.S[k].cb1, s + 20 - Lg[k],
+              acc[k] |= aff_diag_cell<kBt, kSt, kPu>(Q.S[k], eh_l, cb_l, Q.S[k].ev1, Q.S[k].cb1, s + 20 - Lg[k],
                                                 s - 20 + Lg[k], lag, lbg, E0g, go, s_cost, Q.ic[k], Q.ip[k],
                                                 Q.si_ge[k], Q.si_go[k], Q.sive[k], Q.jc[k], Q.gap_row[k], Q.sj_go[k],
-                                                Q.hx[k], Q.fcost[k], e0g[k])
+                                                Q.hx[k], Q.fcost[k], e0g[k], k == 0 ? lmg : 0)
                         << (16 * h);
             }
           }
@@ -971,10 +999,10 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
             for (int k = 0; k < H; ++k) {
               const int ev_u = k == H - 1 ? ev_u0 : Q.S[k < H - 1 ? k + 1 : 0].ev1;
               const int cb_u = k == H - 1 ? cb_u0 : Q.S[k < H - 1 ? k + 1 : 0].cb1;
-              acc[k] |= aff_diag_cell<kBt, kSt>(Q.S[k], Q.S[k].eh1, Q.S[k].cb1, ev_u, cb_u, s + 20 - Lg[k],
+              acc[k] |= aff_diag_cell<kBt, kSt, kPu>(Q.S[k], Q.S[k].eh1, Q.S[k].cb1, ev_u, cb_u, s + 20 - Lg[k],
                                                 s - 19 + Lg[k], lag, lbg, E0g, go, s_cost, Q.ic[k], Q.ip[k],
                                                 Q.si_ge[k], Q.si_go[k], Q.sive[k], Q.jc[k], Q.gap_row[k], Q.sj_go[k],
-                                                Q.hx[k], Q.fcost[k], e1g[k])
+                                                Q.hx[k], Q.fcost[k], e1g[k], 0)
                         << (16 * h + 8);
             }
           }
@@ -994,9 +1022,13 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
     // two variants
     const int gs_lo = min(g_lo, groups), gs_hi = max(gs_lo, min(g_hi, groups));
     int g = 0;
-    for (; g < gs_lo; ++g) group(g, std::false_type{});
-    for (; g < gs_hi; ++g) group(g, std::true_type{});
-    for (; g < groups; ++g) group(g, std::false_type{});
+    for (; g < gs_lo; ++g) group(g, std::integral_constant<int, 0>{});
+    if (all_pure) {
+      for (; g < gs_hi; ++g) group(g, std::integral_constant<int, 2>{});
+    } else {
+      for (; g < gs_hi; ++g) group(g, std::integral_constant<int, 1>{});
+    }
+    for (; g < groups; ++g) group(g, std::integral_constant<int, 0>{});
     // cell (la, lb): diagonal pair (lb - la + 40) / 2 = H qs + ks
     const int Ls = (lb - la + 40) >> 1;
     const int qs = Ls / H, ks = Ls - H * (Ls / H);
@@ -1857,35 +1889,29 @@ hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, co
   return hipGetLastError();
 }
 
-template <bool kBt, int H>
+template <bool kBt>
 static int quad_occupancy() {
   int nb = 0;
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_quad_kernel<kBt, H>, kWave * 4, 0);
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_quad_kernel<kBt, 3>, kWave * 4, 0);
   return (e == hipSuccess && nb > 0) ? nb : 1;
 }
-int aff_quad_blocks_per_cu(bool bt, int h) {
-  if (h == 3) return bt ? quad_occupancy<true, 3>() : quad_occupancy<false, 3>();
-  return bt ? quad_occupancy<true, 4>() : quad_occupancy<false, 4>();
+int aff_quad_blocks_per_cu(bool bt) { return bt ? quad_occupancy<true>() : quad_occupancy<false>(); }
+
+hipError_t launch_pair_pure(int grid, const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
+                            uint8_t *pure, hipStream_t s) {
+  hipLaunchKernelGGL(pair_pure_kernel, dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, seq, pure);
+  return hipGetLastError();
 }
 
-hipError_t launch_aff_quad(bool bt, int h, int grid, const PairMeta *meta, const int32_t *order, int n,
-                           int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
-                           uint32_t *sink, hipStream_t s) {
-  if (h == 3) {
-    if (bt)
-      hipLaunchKernelGGL((aff_quad_kernel<true, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
-                         tcm, dm, cost, sink);
-    else
-      hipLaunchKernelGGL((aff_quad_kernel<false, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter,
-                         seq, tcm, dm, cost, sink);
-  } else {
-    if (bt)
-      hipLaunchKernelGGL((aff_quad_kernel<true, 4>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
-                         tcm, dm, cost, sink);
-    else
-      hipLaunchKernelGGL((aff_quad_kernel<false, 4>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter,
-                         seq, tcm, dm, cost, sink);
-  }
+hipError_t launch_aff_quad(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
+                           const uint8_t *pure, hipStream_t s) {
+  if (bt)
+    hipLaunchKernelGGL((aff_quad_kernel<true, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
+                       tcm, dm, cost, sink, pure);
+  else
+    hipLaunchKernelGGL((aff_quad_kernel<false, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
+                       tcm, dm, cost, sink, pure);
   return hipGetLastError();
 }
 
