@@ -775,6 +775,359 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
   }
 }
 
+// backtrace_affine over the anti-diagonal band layout, one wave per pair.
+//
+// The walk (src/algn.c:2380-2470) is a state machine over the cells of the
+// band: in state A it moves diagonally until a cell whose bits 5-6 name
+// another state; V, H and D move one cell and may fall back to "todo",
+// which re-reads the same cell's bits 0-1.  Almost all of an alignment is
+// long runs of A, so the walk is split in two phases:
+//
+// * The walker (wave-uniform, scalar registers) runs over tiles of K
+//   groups (4K anti-diagonals).  A tile is held by the 64 lanes as K dwords
+//   of the 64 band lanes around the walk; it is copied to LDS (for the
+//   single-cell reads of V, H, D and todo) and reduced per lane to masks of
+//   the cells that end an A run, ordered along each of the lane's two
+//   diagonals.  A whole A run inside a tile costs one v_readlane and one
+//   find-last-bit.  The walker records one 2-bit operation per step
+//   (A 0, D 1, V 2, H 3) in an op log held one 16-step word per lane.
+// * Every 1024 steps (and at the end) the lanes expand their 16 steps of
+//   the log into the four outputs: positions follow from prefix sums of the
+//   per-lane i and j decrements, characters are read from the sequences,
+//   the aligned sequences and the median with gaps are stored as one
+//   16-byte store per lane and output, and the gap-free median is
+//   compacted with a prefix sum of the per-lane emit counts.
+//
+// Window (K = 8, bands wider than 64 diagonal pairs): the next tile is
+// prefetched when the walk enters the current one, at most 63
+// anti-diagonals before the walk leaves the next tile; the walk then stays
+// within lanes ((d + 1) >> 1) - 32 .. +31 of its band lane (d = j - i + 40),
+// which is where the prefetched window starts.  Bands of at most 64
+// diagonal pairs fit the window whole, and use K = 16.
+constexpr int kTrStep = 16;    // op-log steps per lane
+constexpr int kTrBatch = kTrStep * kWave;
+
+__device__ __forceinline__ uint32_t tr_spread(uint32_t x) {
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+
+// One pair's backtrace by the calling wave (every lane).  s_med: the median
+// table in LDS; tile: this wave's LDS tile (64 K dwords).  K: groups per
+// tile, 8 in general, 16 when the band has at most 64 diagonal pairs (the
+// window then always holds the whole band, and twice the anti-diagonals per
+// tile halve the tile changes and the A runs they cut).
+template <int K>
+__device__ __forceinline__ void aff_trace_pair(const PairMeta &m, int p, const uint8_t *__restrict__ seq,
+                                               const uint8_t *__restrict__ dm_base, const uint8_t *s_med,
+                                               uint32_t *tile, uint8_t *out, int32_t *out_len, int32_t *status) {
+  static_assert(K == 8 || K == 16, "tiles of 8 or 16 groups");
+  const int lane = lane_id();
+  const uint8_t *tile8 = (const uint8_t *)tile;
+  const uint8_t *A = seq + m.off_r;
+  const uint8_t *B = seq + m.off_c;
+  const int la = m.lr - 1, lb = m.lc - 1, E0 = m.p0, W = m.p1, cap = m.cap;
+  const int G = (la + lb + 4) >> 2;   // groups of the band
+  const uint32_t *dmw = (const uint32_t *)(dm_base + m.dm_off);
+  uint8_t *o = out + m.out_off;
+  uint8_t *om = o;
+  uint8_t *ow = o + cap;
+  uint8_t *oa = o + (m.swap ? 3 : 2) * cap;   // the rows sequence's slot
+  uint8_t *ob = o + (m.swap ? 2 : 3) * cap;
+
+  // ---- op log and the emit phase
+  uint32_t oplog = 0;   // lane l: steps 16l .. 16l+15 of the batch
+  uint32_t acc = 0;     // the word being filled (scalar)
+  int s = 0;            // steps in the batch
+  int S0 = 0, ib = la, jb = lb, wmpos = cap, nm = 0, last = -1, final_step = -1;
+
+  auto expand = [&](int nst) {
+    const uint32_t c = oplog;
+    const int v = min(max(nst - kTrStep * lane, 0), kTrStep);
+    const uint32_t vm = v >= kTrStep ? 0xffffffffu : ((1u << (2 * v)) - 1u);
+    const uint32_t lo = c & 0x55555555u & vm, hi = (c >> 1) & 0x55555555u & vm;
+    const int nI = v - __popc(lo & hi), nJ = v - __popc(hi & ~lo);
+    const int cI = wave_prefix_sum(nI);
+    const int cJ = wave_prefix_sum(nJ);
+    uint32_t wa[4] = {0u, 0u, 0u, 0u}, wb[4] = {0u, 0u, 0u, 0u}, ww[4] = {0u, 0u, 0u, 0u};
+    uint32_t em = 0u;
+    int lastv = 0;
+    const int gbase = S0 + kTrStep * lane;
+    int ii = ib - (cI - nI), jj = jb - (cJ - nJ);
+    // two halves of 8 steps keep the loaded characters of one half live
+#pragma unroll
+    for (int h = 0; h < kTrStep; h += 8) {
+      int av[8], bv[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t code = (c >> (2 * (h + q))) & 3u;
+        av[q] = A[min(max(ii, 0), la)];
+        bv[q] = B[min(max(jj, 0), lb)];
+        ii -= code != 3u;
+        jj -= code != 2u;
+      }
+#pragma unroll
+      for (int q0 = 0; q0 < 8; ++q0) {
+        const int q = h + q0;
+        const uint32_t code = (c >> (2 * q)) & 3u;
+        const int a = av[q0], b = bv[q0];
+        const bool isA = code == 0u, isD = code == 1u, isV = code == 2u;
+        const int md = s_med[(a & 15) * 32 + (b & 15)];
+        const int gv = (a & kGap) ? kGap : (a | kGap);
+        const int gh = (b & kGap) ? kGap : (b | kGap);
+        int va = code == 3u ? kGap : a;
+        int vb = isV ? kGap : b;
+        int vw = isA ? md : (isD ? kGap : (isV ? gv : gh));
+        bool e = isA || (isV && !(a & kGap)) || (code == 3u && !(b & kGap));
+        if (gbase + q == final_step) {   // the closing all-gap column
+          va = kGap;
+          vb = kGap;
+          vw = kGap;
+        }
+        if (q >= v) {
+          va = vb = vw = 0;
+          e = false;
+        }
+        const int byte = kTrStep - 1 - q;   // steps are written right to left
+        wa[byte >> 2] |= (uint32_t)va << (8 * (byte & 3));
+        wb[byte >> 2] |= (uint32_t)vb << (8 * (byte & 3));
+        ww[byte >> 2] |= (uint32_t)vw << (8 * (byte & 3));
+        em |= (uint32_t)e << q;
+        lastv = e ? vw : lastv;
+      }
+    }
+    const int P = cap - kTrStep - gbase;   // 16-byte aligned: cap and S0 are multiples of 16
+    if (v > 0 && P >= 0) {
+      *(uint4 *)(oa + P) = make_uint4(wa[0], wa[1], wa[2], wa[3]);
+      *(uint4 *)(ob + P) = make_uint4(wb[0], wb[1], wb[2], wb[3]);
+      *(uint4 *)(ow + P) = make_uint4(ww[0], ww[1], ww[2], ww[3]);
+    }
+    const int ne = __popc(em);
+    const int cE = wave_prefix_sum(ne);
+    const int wtop = wmpos - 1 - (cE - ne);
+    int rank = 0;
+#pragma unroll
+    for (int q = 0; q < kTrStep; ++q) {
+      const int byte = kTrStep - 1 - q;
+      if ((em >> q) & 1u) om[wtop - rank] = (uint8_t)(ww[byte >> 2] >> (8 * (byte & 3)));
+      rank += (em >> q) & 1u;
+    }
+    const int tot = lane63(cE);
+    if (tot) {
+      const uint64_t bal = __ballot(ne > 0);
+      const int ll = 63 - __clzll((long long)bal);
+      last = __builtin_amdgcn_readlane(lastv, ll);
+    }
+    wmpos -= tot;
+    nm += tot;
+    ib -= lane63(cI);
+    jb -= lane63(cJ);
+    S0 += nst;
+    oplog = 0u;
+  };
+  // ---- the walker: each pass yields one run of an operation (0 A, 1 D,
+  // 2 V, 3 H), then the run is appended to the op log (the only place the
+  // log is expanded, so the emit phase is instantiated once)
+  int i = la, j = lb, mode = 0, bad = 0;   // mode: 0 todo, 1 V, 2 H, 3 D, 4 A
+  int stage = 0;                            // 0 walk, 1 tail V, 2 tail H, 3 closing column, 4 done
+  uint32_t pre[K];
+  uint32_t maskA = 0u, maskB = 0u;   // A-run ends (layout below)
+  int g0 = 0, Lb = 0, g0n = 0, Lbn = 0;
+  auto lbase = [&](int d) { return min(max(((d + 1) >> 1) - kWave / 2, 0), max(0, W - kWave)); };
+  // Lanes past the band load (and then ignore) copies of its last lane;
+  // groups outside the band are clamped to its ends: the walker never reads
+  // them (anti-diagonals above the start or below row / column 0).
+  auto fetch = [&](int gs, int lb0) {
+    const int Lc = min(lb0 + lane, W - 1);
+    if (gs >= 0 && gs + K <= G) {
+      const uint32_t *src = dmw + (size_t)gs * W + Lc;
+#pragma unroll
+      for (int u = 0; u < K; ++u) pre[u] = src[(size_t)u * W];
+    } else {
+#pragma unroll
+      for (int u = 0; u < K; ++u) pre[u] = dmw[(size_t)min(max(gs + u, 0), G - 1) * W + Lc];
+    }
+  };
+  auto commit = [&]() {
+    wave_sync_mem();   // the walker's reads of the previous tile are done
+    uint4 *dst = (uint4 *)(tile + lane * K);
+#pragma unroll
+    for (int v = 0; v < K / 4; ++v) dst[v] = make_uint4(pre[4 * v], pre[4 * v + 1], pre[4 * v + 2], pre[4 * v + 3]);
+    // bit 8b + u of T[h]: the cell of group 8h + u, byte b ends an A run
+    // (bits 5-6 set)
+    uint32_t T[K / 8];
+#pragma unroll
+    for (int h = 0; h < K / 8; ++h) {
+      T[h] = 0u;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t x = pre[8 * h + u];
+        const uint32_t y = (x | (x << 1)) & 0x40404040u;
+        T[h] |= u <= 6 ? (y >> (6 - u)) : (y << (u - 6));
+      }
+    }
+    if (K == 8) {
+      // interleaved to bit 16 (b & 1) + 2u + (b >> 1): the cells of each of
+      // the lane's two diagonals in anti-diagonal order
+      const uint32_t z0 = (T[0] & 0xFFu) | ((T[0] & 0xFF00u) << 8);
+      const uint32_t z1 = ((T[0] >> 16) & 0xFFu) | ((T[0] >> 8) & 0xFF0000u);
+      maskA = tr_spread(z0) | (tr_spread(z1) << 1);
+    } else {
+      // one word per diagonal: bit 2u + (b >> 1) of mask (b & 1)
+      auto row = [&](int b) { return ((T[0] >> (8 * b)) & 0xFFu) | (((T[K / 8 - 1] >> (8 * b)) & 0xFFu) << 8); };
+      auto spread16 = [](uint32_t x) {
+        x = (x | (x << 8)) & 0x00FF00FFu;
+        return tr_spread(x);
+      };
+      maskA = spread16(row(0)) | (spread16(row(2)) << 1);
+      maskB = spread16(row(1)) | (spread16(row(3)) << 1);
+    }
+    wave_sync_mem();
+  };
+  if (i > 0 && j > 0) {
+    g0 = max(0, ((i + j) >> 2) - K + 1);
+    Lb = lbase(j - i + 40);
+    fetch(g0, Lb);
+    commit();
+    g0n = g0 - K;
+    Lbn = Lb;
+    if (g0n + K > 0) fetch(g0n, Lbn);
+  }
+  for (;;) {
+    int code = 0, r = 0;
+    if (stage == 0) {
+      if (i > 0 && j > 0) {
+        const int t = i + j, d = j - i + 40;
+        if ((unsigned)d > (unsigned)(E0 + 40)) {   // left the band
+          bad = 1;
+          stage = 4;
+        } else {
+          if (t < 4 * g0) {
+            commit();
+            g0 = g0n;
+            Lb = Lbn;
+            g0n = g0 - K;
+            Lbn = lbase(d);
+            if (g0n + K > 0) fetch(g0n, Lbn);
+          }
+          const int kk = (d >> 1) - Lb;   // window lane, 0..63
+          const int pos = t - 4 * g0;      // anti-diagonal in the tile, 0 .. 4K-1
+          int f = 0;
+          if (mode != 4) f = uniform32(tile8[kk * (4 * K) + pos]);
+          if (mode == 0) mode = (0x3142 >> ((f & 3) * 4)) & 15;   // todo: 0 H, 1 A, 2 V, 3 D
+          if (mode == 4) {
+            const int par = pos & 1, key = pos >> 1;
+            uint32_t mk;
+            if (K == 8) {
+              mk = (uint32_t)__builtin_amdgcn_readlane((int)maskA, kk) >> (16 * par);
+            } else {
+              mk = par ? (uint32_t)__builtin_amdgcn_readlane((int)maskB, kk)
+                       : (uint32_t)__builtin_amdgcn_readlane((int)maskA, kk);
+            }
+            const uint32_t cand = mk & ((2u << key) - 1u);   // key <= 31: 2u << 31 wraps to 0
+            int ks = -1;
+            r = key + 1;
+            if (cand) {
+              ks = 31 - __clz((int)cand);
+              r = key - ks + 1;
+            }
+            const int avail = min(i, j);
+            if (r > avail) {
+              r = avail;
+              ks = -1;
+            }
+            code = 0;
+            i -= r;
+            j -= r;
+            if (ks >= 0) {   // the run's last cell names the next state: 1 H, 2 D, 3 V
+              const int f2 = uniform32(tile8[kk * (4 * K) + 2 * ks + par]);
+              mode = (0x1324 >> (((f2 >> 5) & 3) * 4)) & 15;
+            }
+          } else if (mode == 1) {
+            code = 2;
+            r = 1;
+            if (f & 4) mode = 0;
+            --i;
+          } else if (mode == 2) {
+            code = 3;
+            r = 1;
+            if (f & 8) mode = 0;
+            --j;
+          } else {
+            code = 1;
+            r = 1;
+            if (f & 16) mode = 0;
+            --i;
+            --j;
+          }
+        }
+      } else {
+        stage = 1;
+      }
+    } else if (stage == 1) {
+      code = 2;
+      r = i;
+      i = 0;
+      stage = 2;
+    } else if (stage == 2) {
+      code = 3;
+      r = j;
+      j = 0;
+      stage = 3;
+    } else if (stage == 3) {
+      final_step = S0 + s;
+      code = 1;
+      r = 1;
+      stage = 4;
+    }
+    const bool finish = stage == 4;
+    const uint32_t pat = code == 0 ? 0u : (code == 1 ? 0x55555555u : (code == 2 ? 0xAAAAAAAAu : 0xFFFFFFFFu));
+    while (r > 0 || (finish && s > 0)) {
+      if (r > 0) {
+        int take;
+        if (code == 0) {   // A codes are zero: only completed words are stored
+          take = min(r, kTrBatch - s);
+          if (((s + take) >> 4) != (s >> 4)) {
+            oplog = lane == (s >> 4) ? acc : oplog;
+            acc = 0u;
+          }
+        } else {
+          const int wq = s & 15;
+          take = min(r, 16 - wq);
+          const uint32_t msk = take >= 16 ? 0xffffffffu : ((1u << (2 * take)) - 1u);
+          acc |= (pat & msk) << (2 * wq);
+          if (((s + take) & 15) == 0) {
+            oplog = lane == (s >> 4) ? acc : oplog;
+            acc = 0u;
+          }
+        }
+        s += take;
+        r -= take;
+      }
+      if (s == kTrBatch || (finish && r == 0 && s > 0)) {
+        if (s & 15) {
+          oplog = lane == (s >> 4) ? acc : oplog;
+          acc = 0u;
+        }
+        expand(s);
+        s = 0;
+      }
+    }
+    if (finish) break;
+  }
+  if (!bad && (nm == 0 || last != kGap)) {
+    --wmpos;
+    om[wmpos] = (uint8_t)kGap;   // every lane stores the same byte
+  }
+  out_len[4 * p + 0] = cap - wmpos;
+  out_len[4 * p + 1] = S0;
+  out_len[4 * p + 2] = S0;
+  out_len[4 * p + 3] = S0;
+  if (bad) atomicAdd(status, lane == 0 ? 1 : 0);
+}
+
 // ---------------------------------------------------------------------------
 // Affine band, four pairs per wave (bands of at most 16 H diagonal pairs).
 // ---------------------------------------------------------------------------
@@ -1134,334 +1487,22 @@ __global__ void aff_trace_kernel(const PairMeta *__restrict__ meta, const int32_
   if (bad) atomicAdd(status, 1);
 }
 
-// backtrace_affine over the anti-diagonal band layout, one wave per pair.
-//
-// The walk (src/algn.c:2380-2470) is a state machine over the cells of the
-// band: in state A it moves diagonally until a cell whose bits 5-6 name
-// another state; V, H and D move one cell and may fall back to "todo",
-// which re-reads the same cell's bits 0-1.  Almost all of an alignment is
-// long runs of A, so the walk is split in two phases:
-//
-// * The walker (wave-uniform, scalar registers) runs over tiles of 8 groups
-//   (32 anti-diagonals).  A tile is held by the 64 lanes as 8 dwords of the
-//   64 band lanes around the walk; it is copied to LDS (for the single-cell
-//   reads of V, H, D and todo) and reduced per lane to a 32-bit mask of the
-//   cells that end an A run, ordered along each of the lane's two
-//   diagonals.  A whole A run inside a tile costs one v_readlane and one
-//   find-last-bit.  The walker records one 2-bit operation per step
-//   (A 0, D 1, V 2, H 3) in an op log held one 16-step word per lane.
-// * Every 1024 steps (and at the end) the lanes expand their 16 steps of
-//   the log into the four outputs: positions follow from prefix sums of the
-//   per-lane i and j decrements, characters are read from the sequences,
-//   the aligned sequences and the median with gaps are stored as one
-//   16-byte store per lane and output, and the gap-free median is
-//   compacted with a prefix sum of the per-lane emit counts.
-//
-// Window: the next tile is prefetched when the walk enters the current one,
-// at most 63 anti-diagonals before the walk leaves the next tile; the walk
-// then stays within lanes ((d + 1) >> 1) - 32 .. +31 of its band lane
-// (d = j - i + 40), which is where the prefetched window starts.
-constexpr int kTrK = 8;        // groups per tile
-constexpr int kTrStep = 16;    // op-log steps per lane
-constexpr int kTrBatch = kTrStep * kWave;
-
-__device__ __forceinline__ uint32_t tr_spread(uint32_t x) {
-  x = (x | (x << 4)) & 0x0F0F0F0Fu;
-  x = (x | (x << 2)) & 0x33333333u;
-  x = (x | (x << 1)) & 0x55555555u;
-  return x;
-}
-
 __global__ __launch_bounds__(kWave * 4) void aff_trace_wave_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
     const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, const uint8_t *__restrict__ dm_base,
     uint8_t *out, int32_t *out_len, int32_t *status) {
   __shared__ uint8_t s_med[32 * 32];
-  __shared__ uint32_t s_tile[4][kWave * kTrK];   // lane-major: window lane k at [8k, 8k + 8)
+  __shared__ uint32_t s_tile[4][kWave * 16];   // lane-major: window lane k at [Kk, Kk + K)
   for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_med[k] = tcm->median[k];
   __syncthreads();
-  const int w = wave_in_group();
-  const int lane = lane_id();
-  uint32_t *tile = s_tile[w];
-  const uint8_t *tile8 = (const uint8_t *)tile;
+  uint32_t *tile = s_tile[wave_in_group()];
   for (;;) {
     const int item = next_item(counter);
     if (item >= n) break;
     const int p = uniform32(order[item]);
     const PairMeta m = uniform_meta(meta, p);
-    const uint8_t *A = seq + m.off_r;
-    const uint8_t *B = seq + m.off_c;
-    const int la = m.lr - 1, lb = m.lc - 1, E0 = m.p0, W = m.p1, cap = m.cap;
-    const int G = (la + lb + 4) >> 2;   // groups of the band
-    const uint32_t *dmw = (const uint32_t *)(dm_base + m.dm_off);
-    uint8_t *o = out + m.out_off;
-    uint8_t *om = o;
-    uint8_t *ow = o + cap;
-    uint8_t *oa = o + (m.swap ? 3 : 2) * cap;   // the rows sequence's slot
-    uint8_t *ob = o + (m.swap ? 2 : 3) * cap;
-
-    // ---- op log and the emit phase
-    uint32_t oplog = 0;   // lane l: steps 16l .. 16l+15 of the batch
-    uint32_t acc = 0;     // the word being filled (scalar)
-    int s = 0;            // steps in the batch
-    int S0 = 0, ib = la, jb = lb, wmpos = cap, nm = 0, last = -1, final_step = -1;
-
-    auto expand = [&](int nst) {
-      const uint32_t c = oplog;
-      const int v = min(max(nst - kTrStep * lane, 0), kTrStep);
-      const uint32_t vm = v >= kTrStep ? 0xffffffffu : ((1u << (2 * v)) - 1u);
-      const uint32_t lo = c & 0x55555555u & vm, hi = (c >> 1) & 0x55555555u & vm;
-      const int nI = v - __popc(lo & hi), nJ = v - __popc(hi & ~lo);
-      const int cI = wave_prefix_sum(nI);
-      const int cJ = wave_prefix_sum(nJ);
-      uint32_t wa[4] = {0u, 0u, 0u, 0u}, wb[4] = {0u, 0u, 0u, 0u}, ww[4] = {0u, 0u, 0u, 0u};
-      uint32_t em = 0u;
-      int lastv = 0;
-      const int gbase = S0 + kTrStep * lane;
-      int ii = ib - (cI - nI), jj = jb - (cJ - nJ);
-      // two halves of 8 steps keep the loaded characters of one half live
-#pragma unroll
-      for (int h = 0; h < kTrStep; h += 8) {
-        int av[8], bv[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const uint32_t code = (c >> (2 * (h + q))) & 3u;
-          av[q] = A[min(max(ii, 0), la)];
-          bv[q] = B[min(max(jj, 0), lb)];
-          ii -= code != 3u;
-          jj -= code != 2u;
-        }
-#pragma unroll
-        for (int q0 = 0; q0 < 8; ++q0) {
-          const int q = h + q0;
-          const uint32_t code = (c >> (2 * q)) & 3u;
-          const int a = av[q0], b = bv[q0];
-          const bool isA = code == 0u, isD = code == 1u, isV = code == 2u;
-          const int md = s_med[(a & 15) * 32 + (b & 15)];
-          const int gv = (a & kGap) ? kGap : (a | kGap);
-          const int gh = (b & kGap) ? kGap : (b | kGap);
-          int va = code == 3u ? kGap : a;
-          int vb = isV ? kGap : b;
-          int vw = isA ? md : (isD ? kGap : (isV ? gv : gh));
-          bool e = isA || (isV && !(a & kGap)) || (code == 3u && !(b & kGap));
-          if (gbase + q == final_step) {   // the closing all-gap column
-            va = kGap;
-            vb = kGap;
-            vw = kGap;
-          }
-          if (q >= v) {
-            va = vb = vw = 0;
-            e = false;
-          }
-          const int byte = kTrStep - 1 - q;   // steps are written right to left
-          wa[byte >> 2] |= (uint32_t)va << (8 * (byte & 3));
-          wb[byte >> 2] |= (uint32_t)vb << (8 * (byte & 3));
-          ww[byte >> 2] |= (uint32_t)vw << (8 * (byte & 3));
-          em |= (uint32_t)e << q;
-          lastv = e ? vw : lastv;
-        }
-      }
-      const int P = cap - kTrStep - gbase;   // 16-byte aligned: cap and S0 are multiples of 16
-      if (v > 0 && P >= 0) {
-        *(uint4 *)(oa + P) = make_uint4(wa[0], wa[1], wa[2], wa[3]);
-        *(uint4 *)(ob + P) = make_uint4(wb[0], wb[1], wb[2], wb[3]);
-        *(uint4 *)(ow + P) = make_uint4(ww[0], ww[1], ww[2], ww[3]);
-      }
-      const int ne = __popc(em);
-      const int cE = wave_prefix_sum(ne);
-      const int wtop = wmpos - 1 - (cE - ne);
-      int rank = 0;
-#pragma unroll
-      for (int q = 0; q < kTrStep; ++q) {
-        const int byte = kTrStep - 1 - q;
-        if ((em >> q) & 1u) om[wtop - rank] = (uint8_t)(ww[byte >> 2] >> (8 * (byte & 3)));
-        rank += (em >> q) & 1u;
-      }
-      const int tot = lane63(cE);
-      if (tot) {
-        const uint64_t bal = __ballot(ne > 0);
-        const int ll = 63 - __clzll((long long)bal);
-        last = __builtin_amdgcn_readlane(lastv, ll);
-      }
-      wmpos -= tot;
-      nm += tot;
-      ib -= lane63(cI);
-      jb -= lane63(cJ);
-      S0 += nst;
-      oplog = 0u;
-    };
-    // ---- the walker: each pass yields one run of an operation (0 A, 1 D,
-    // 2 V, 3 H), then the run is appended to the op log (the only place the
-    // log is expanded, so the emit phase is instantiated once)
-    int i = la, j = lb, mode = 0, bad = 0;   // mode: 0 todo, 1 V, 2 H, 3 D, 4 A
-    int stage = 0;                            // 0 walk, 1 tail V, 2 tail H, 3 closing column, 4 done
-    uint32_t pre[kTrK];
-    uint32_t maskA = 0u;
-    int g0 = 0, Lb = 0, g0n = 0, Lbn = 0;
-    auto lbase = [&](int d) { return min(max(((d + 1) >> 1) - kWave / 2, 0), max(0, W - kWave)); };
-    auto fetch = [&](int gs, int lb0) {
-      const int L = lb0 + lane;
-      const int Lc = min(L, W - 1);
-#pragma unroll
-      for (int u = 0; u < kTrK; ++u) {
-        const int g = gs + u;
-        const uint32_t x = dmw[(size_t)min(max(g, 0), G - 1) * W + Lc];
-        pre[u] = (g >= 0 && g < G && L < W) ? x : 0u;
-      }
-    };
-    auto commit = [&]() {
-      wave_sync_mem();   // the walker's reads of the previous tile are done
-      uint4 *dst = (uint4 *)(tile + lane * kTrK);
-      dst[0] = make_uint4(pre[0], pre[1], pre[2], pre[3]);
-      dst[1] = make_uint4(pre[4], pre[5], pre[6], pre[7]);
-      // bit 8b + u: the cell of group u, byte b ends an A run (bits 5-6 set)
-      uint32_t T = 0u;
-#pragma unroll
-      for (int u = 0; u < kTrK; ++u) {
-        const uint32_t y = (pre[u] | (pre[u] << 1)) & 0x40404040u;
-        T |= u <= 6 ? (y >> (6 - u)) : (y << (u - 6));
-      }
-      // interleaved to bit 16 (b & 1) + 2u + (b >> 1): the cells of each of
-      // the lane's two diagonals in anti-diagonal order
-      const uint32_t z0 = (T & 0xFFu) | ((T & 0xFF00u) << 8);
-      const uint32_t z1 = ((T >> 16) & 0xFFu) | ((T >> 8) & 0xFF0000u);
-      maskA = tr_spread(z0) | (tr_spread(z1) << 1);
-      wave_sync_mem();
-    };
-    if (i > 0 && j > 0) {
-      g0 = max(0, ((i + j) >> 2) - kTrK + 1);
-      Lb = lbase(j - i + 40);
-      fetch(g0, Lb);
-      commit();
-      g0n = g0 - kTrK;
-      Lbn = Lb;
-      if (g0n + kTrK > 0) fetch(g0n, Lbn);
-    }
-    for (;;) {
-      int code = 0, r = 0;
-      if (stage == 0) {
-        if (i > 0 && j > 0) {
-          const int t = i + j, d = j - i + 40;
-          if ((unsigned)d > (unsigned)(E0 + 40)) {   // left the band
-            bad = 1;
-            stage = 4;
-          } else {
-            if (t < 4 * g0) {
-              commit();
-              g0 = g0n;
-              Lb = Lbn;
-              g0n = g0 - kTrK;
-              Lbn = lbase(d);
-              if (g0n + kTrK > 0) fetch(g0n, Lbn);
-            }
-            const int kk = (d >> 1) - Lb;   // window lane, 0..63
-            const int pos = t - 4 * g0;      // anti-diagonal in the tile, 0..31
-            int f = 0;
-            if (mode != 4) f = uniform32(tile8[kk * (4 * kTrK) + pos]);
-            if (mode == 0) mode = (0x3142 >> ((f & 3) * 4)) & 15;   // todo: 0 H, 1 A, 2 V, 3 D
-            if (mode == 4) {
-              const uint32_t mk = (uint32_t)__builtin_amdgcn_readlane((int)maskA, kk);
-              const int par = pos & 1, key = pos >> 1;
-              const uint32_t cand = (mk >> (16 * par)) & ((2u << key) - 1u);
-              int ks = -1;
-              r = key + 1;
-              if (cand) {
-                ks = 31 - __clz((int)cand);
-                r = key - ks + 1;
-              }
-              const int avail = min(i, j);
-              if (r > avail) {
-                r = avail;
-                ks = -1;
-              }
-              code = 0;
-              i -= r;
-              j -= r;
-              if (ks >= 0) {   // the run's last cell names the next state: 1 H, 2 D, 3 V
-                const int f2 = uniform32(tile8[kk * (4 * kTrK) + 2 * ks + par]);
-                mode = (0x1324 >> (((f2 >> 5) & 3) * 4)) & 15;
-              }
-            } else if (mode == 1) {
-              code = 2;
-              r = 1;
-              if (f & 4) mode = 0;
-              --i;
-            } else if (mode == 2) {
-              code = 3;
-              r = 1;
-              if (f & 8) mode = 0;
-              --j;
-            } else {
-              code = 1;
-              r = 1;
-              if (f & 16) mode = 0;
-              --i;
-              --j;
-            }
-          }
-        } else {
-          stage = 1;
-        }
-      } else if (stage == 1) {
-        code = 2;
-        r = i;
-        i = 0;
-        stage = 2;
-      } else if (stage == 2) {
-        code = 3;
-        r = j;
-        j = 0;
-        stage = 3;
-      } else if (stage == 3) {
-        final_step = S0 + s;
-        code = 1;
-        r = 1;
-        stage = 4;
-      }
-      const bool finish = stage == 4;
-      const uint32_t pat = code == 0 ? 0u : (code == 1 ? 0x55555555u : (code == 2 ? 0xAAAAAAAAu : 0xFFFFFFFFu));
-      while (r > 0 || (finish && s > 0)) {
-        if (r > 0) {
-          int take;
-          if (code == 0) {   // A codes are zero: only completed words are stored
-            take = min(r, kTrBatch - s);
-            if (((s + take) >> 4) != (s >> 4)) {
-              oplog = lane == (s >> 4) ? acc : oplog;
-              acc = 0u;
-            }
-          } else {
-            const int wq = s & 15;
-            take = min(r, 16 - wq);
-            const uint32_t msk = take >= 16 ? 0xffffffffu : ((1u << (2 * take)) - 1u);
-            acc |= (pat & msk) << (2 * wq);
-            if (((s + take) & 15) == 0) {
-              oplog = lane == (s >> 4) ? acc : oplog;
-              acc = 0u;
-            }
-          }
-          s += take;
-          r -= take;
-        }
-        if (s == kTrBatch || (finish && r == 0 && s > 0)) {
-          if (s & 15) {
-            oplog = lane == (s >> 4) ? acc : oplog;
-            acc = 0u;
-          }
-          expand(s);
-          s = 0;
-        }
-      }
-      if (finish) break;
-    }
-    if (!bad && (nm == 0 || last != kGap)) {
-      --wmpos;
-      om[wmpos] = (uint8_t)kGap;   // every lane stores the same byte
-    }
-    out_len[4 * p + 0] = cap - wmpos;
-    out_len[4 * p + 1] = S0;
-    out_len[4 * p + 2] = S0;
-    out_len[4 * p + 3] = S0;
-    if (bad) atomicAdd(status, lane == 0 ? 1 : 0);
+    if (m.p1 <= kWave) aff_trace_pair<16>(m, p, seq, dm_base, s_med, tile, out, out_len, status);
+    else aff_trace_pair<8>(m, p, seq, dm_base, s_med, tile, out, out_len, status);
   }
 }
 
