@@ -478,7 +478,7 @@ struct AffDiag {
 // kSteady: every cell of the step has 41 <= i <= la and j <= lb, so the band
 // edges are the fixed lanes d == -40 (left) and d == E0 (right), given as
 // flags; no row 0 and no final cell occur.
-template <bool kBt, bool kSteady, bool kPure = false>
+template <bool kBt, bool kSteady, bool kPure = false, bool kGapFree = false>
 __device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int eh_l, int cb_l, int ev_u, int cb_u, int i, int j,
                                                   int la, int lb, int E0, int go, const int *s_cost, int ic, int ip,
                                                   int si_ge, int si_go, int sive, int jc, int gap_row, int sj_go,
@@ -486,11 +486,20 @@ __device__ __forceinline__ uint32_t aff_diag_cell(AffDiag &S, int eh_l, int cb_l
   // Every decision below is a select on int values: the cell code must not
   // branch (no lane may be switched off between the DPP moves of a step).
   const int d = j - i;
+  // kGapFree (with kPure): neither character of any cell carries the gap
+  // bit, so every gap-opening term is go and every extension the plain
+  // indel cost (sive = si_ge, hx = gap_row)
+  if (kGapFree) {
+    si_go = go;
+    sj_go = go;
+    sive = si_ge;
+    hx = gap_row;
+  }
   // FILL_EXTEND_VERTICAL
   const int ext_v = ev_u + sive, open_v = cb_u + si_go + si_ge;
   const int endv = ext_v >= open_v;
   int ev = min(ext_v, open_v);
-  const int ic_gap = ic & kGap, jc_gap = jc & kGap;
+  const int ic_gap = kGapFree ? 0 : ic & kGap, jc_gap = kGapFree ? 0 : jc & kGap;
   const int dg = s_cost[(ic & 15) * 32 + (jc & 15)];
   const int fv = S.ev2 + dg + (ic_gap ? sj_go : 0);
   const int fh = S.eh2 + dg + (jc_gap ? si_go : 0);
@@ -1157,9 +1166,10 @@ __device__ __forceinline__ int dpp_row_rotate_down(int v) {
   return __builtin_amdgcn_update_dpp(0, v, 0x12f /* row_ror:15 */, 0xf, 0xf, false);
 }
 
-// Per pair: 1 if one of its sequences has no gap-bearing character past the
-// leading gap, so that no DP cell pairs two of them (the block-diagonal
-// state then never matters; see aff_diag_cell).
+// Per pair: bit 0 / bit 1 set if the rows / columns sequence has no
+// gap-bearing character past the leading gap.  One bit: no DP cell pairs two
+// of them (the block-diagonal state never matters); both bits: no cell holds
+// one (see aff_diag_cell).
 __global__ __launch_bounds__(kWave * 4) void pair_pure_kernel(const PairMeta *__restrict__ meta,
                                                             const int32_t *__restrict__ order, int n,
                                                             const uint8_t *__restrict__ seq, uint8_t *pure) {
@@ -1172,7 +1182,7 @@ __global__ __launch_bounds__(kWave * 4) void pair_pure_kernel(const PairMeta *__
     for (int x = 1 + lane; x < m.lr; x += kWave) ga |= seq[m.off_r + x];
     for (int x = 1 + lane; x < m.lc; x += kWave) gb |= seq[m.off_c + x];
     const bool any_a = __ballot(ga & kGap) != 0, any_b = __ballot(gb & kGap) != 0;
-    pure[p] = (!any_a || !any_b) ? 1 : 0;   // every lane stores the same value
+    pure[p] = (any_a ? 0 : 1) | (any_b ? 0 : 2);   // every lane stores the same value
   }
 }
 
@@ -1244,15 +1254,17 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
       open1[k] = d0 + 1 >= E0 ? 0 : -1;
     }
     const int Gs = (la + lb + 4) >> 2;   // groups of 4 steps of this row's pair
-    const int pf = active ? pure[p] : 1;
+    const int pf = active ? pure[p] : 3;
     // steady groups of this row's pair (all cells inside the sequences)
     const int t_lo = E0 + 82, t_hi = min(2 * lb - E0, 2 * la - 40);
     const int glo = active ? (t_lo + 3) >> 2 : 0;
     const int ghi = active ? (t_hi >= 3 ? (t_hi - 3) / 4 + 1 : 0) : 0x3fffffff;
-    int groups = 0, g_lo = 0, g_hi = 0x3fffffff, all_pure = 1;
+    int groups = 0, g_lo = 0, g_hi = 0x3fffffff, all_pure = 1, all_free = 1;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      all_pure &= __builtin_amdgcn_readlane(pf, 16 * r);
+      const int f = __builtin_amdgcn_readlane(pf, 16 * r);
+      all_pure &= f != 0;
+      all_free &= f == 3;
       groups = max(groups, __builtin_amdgcn_readlane(Gs, 16 * r));
       g_lo = max(g_lo, __builtin_amdgcn_readlane(glo, 16 * r));
       g_hi = min(g_hi, __builtin_amdgcn_readlane(ghi, 16 * r));
@@ -1265,10 +1277,10 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
     int bwin = load_or0(B, 16 * H - 20 + ((q + 1) & 15), lb);
     int anext = 0, bnext = 0;
     // variant: 0 edge groups, 1 steady groups, 2 steady groups of four pure
-    // pairs (see aff_diag_cell)
+    // pairs, 3 steady groups of four gap-free pairs (see aff_diag_cell)
     auto group = [&](int g, auto variant_tag) {
       constexpr int kVar = decltype(variant_tag)::value;
-      constexpr bool kSt = kVar >= 1, kPu = kVar == 2;
+      constexpr bool kSt = kVar >= 1, kPu = kVar >= 2, kGF = kVar == 3;
       if ((g & 7) == 0) {   // a new 16-step window; prefetch the one after it
         if (g) {
           awin = anext;
@@ -1310,18 +1322,21 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
               Q.ip[k] = Q.ic[k];
               Q.ic[k] = Q.ic[k - 1];
               Q.si_ge[k] = Q.si_ge[k - 1];
-              Q.si_go[k] = Q.si_go[k - 1];
-              Q.sive[k] = Q.sive[k - 1];
+              if (!kGF) {
+                Q.si_go[k] = Q.si_go[k - 1];
+                Q.sive[k] = Q.sive[k - 1];
+              }
             }
             Q.ip[0] = old0;
             Q.ic[0] = nic;
-            row_consts(nic, old0, s + 20 - Lg[0], Q.si_ge[0], Q.si_go[0], Q.sive[0]);
+            if (kGF) Q.si_ge[0] = s_cost[nic * 32 + kGap];
+            else row_consts(nic, old0, s + 20 - Lg[0], Q.si_ge[0], Q.si_go[0], Q.sive[0]);
             // descending k: virtual lane k reads k-1's odd cell before it changes
 #pragma unroll
             for (int k = H - 1; k >= 0; --k) {
               const int eh_l = k == 0 ? eh_l0 : Q.S[k > 0 ? k - 1 : 0].eh1;
               const int cb_l = k == 0 ? cb_l0 : Q.S[k > 0 ? k - 1 : 0].cb1;
-              acc[k] |= aff_diag_cell<kBt, kSt, kPu>(Q.S[k], eh_l, cb_l, Q.S[k].ev1, Q.S[k].cb1, s + 20 - Lg[k],
+              acc[k] |= aff_diag_cell<kBt, kSt, kPu, kGF>(Q.S[k], eh_l, cb_l, Q.S[k].ev1, Q.S[k].cb1, s + 20 - Lg[k],
                                                 s - 20 + Lg[k], lag, lbg, E0g, go, s_cost, Q.ic[k], Q.ip[k],
                                                 Q.si_ge[k], Q.si_go[k], Q.sive[k], Q.jc[k], Q.gap_row[k], Q.sj_go[k],
                                                 Q.hx[k], Q.fcost[k], e0g[k], k == 0 ? lmg : 0)
@@ -1341,18 +1356,21 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
               Q.jp[k] = Q.jc[k];
               Q.jc[k] = Q.jc[k + 1];
               Q.gap_row[k] = Q.gap_row[k + 1];
-              Q.sj_go[k] = Q.sj_go[k + 1];
-              Q.hx[k] = Q.hx[k + 1];
+              if (!kGF) {
+                Q.sj_go[k] = Q.sj_go[k + 1];
+                Q.hx[k] = Q.hx[k + 1];
+              }
             }
             Q.jp[H - 1] = oldt;
             Q.jc[H - 1] = njc;
-            col_consts(njc, oldt, s - 19 + Lg[H - 1], Q.gap_row[H - 1], Q.sj_go[H - 1], Q.hx[H - 1]);
+            if (kGF) Q.gap_row[H - 1] = s_prep[njc];
+            else col_consts(njc, oldt, s - 19 + Lg[H - 1], Q.gap_row[H - 1], Q.sj_go[H - 1], Q.hx[H - 1]);
             // ascending k: virtual lane k reads k+1's even cell before it changes
 #pragma unroll
             for (int k = 0; k < H; ++k) {
               const int ev_u = k == H - 1 ? ev_u0 : Q.S[k < H - 1 ? k + 1 : 0].ev1;
               const int cb_u = k == H - 1 ? cb_u0 : Q.S[k < H - 1 ? k + 1 : 0].cb1;
-              acc[k] |= aff_diag_cell<kBt, kSt, kPu>(Q.S[k], Q.S[k].eh1, Q.S[k].cb1, ev_u, cb_u, s + 20 - Lg[k],
+              acc[k] |= aff_diag_cell<kBt, kSt, kPu, kGF>(Q.S[k], Q.S[k].eh1, Q.S[k].cb1, ev_u, cb_u, s + 20 - Lg[k],
                                                 s - 19 + Lg[k], lag, lbg, E0g, go, s_cost, Q.ic[k], Q.ip[k],
                                                 Q.si_ge[k], Q.si_go[k], Q.sive[k], Q.jc[k], Q.gap_row[k], Q.sj_go[k],
                                                 Q.hx[k], Q.fcost[k], e1g[k], 0)
@@ -1376,7 +1394,18 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
     const int gs_lo = min(g_lo, groups), gs_hi = max(gs_lo, min(g_hi, groups));
     int g = 0;
     for (; g < gs_lo; ++g) group(g, std::integral_constant<int, 0>{});
-    if (all_pure) {
+    if (all_free) {
+      for (; g < gs_hi; ++g) group(g, std::integral_constant<int, 3>{});
+      // the gap-free groups kept only si_ge and gap_row; for rows and columns
+      // without gap-bearing characters the other costs follow from them
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        Q.si_go[k] = go;
+        Q.sive[k] = Q.si_ge[k];
+        Q.sj_go[k] = go;
+        Q.hx[k] = Q.gap_row[k];
+      }
+    } else if (all_pure) {
       for (; g < gs_hi; ++g) group(g, std::integral_constant<int, 2>{});
     } else {
       for (; g < gs_hi; ++g) group(g, std::integral_constant<int, 1>{});
