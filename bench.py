@@ -201,8 +201,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(fill_gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
-            "kernel": "aff_diag_kernel" if kind == "affine" else "lin_fill_lds_kernel",
+            "traffic": measured_traffic(kind),
+            "kernel": ("fill stage: aff_quad_kernel (main and side-stream launches) with the one-set and "
+                       "lane-set fills, one span of HIP events on the main stream") if kind == "affine"
+                      else "lin_fill_lds_kernel",
         },
         "valu_roofline": {
             "bound": "valu",
@@ -220,6 +222,22 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     plan.close()
     return 0
+
+
+def measured_traffic(kind: str):
+    """HBM bytes per step of the fill stage from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by a calibration copy), or None."""
+    if kind != "affine":
+        return None
+    path = os.path.join(ROOT, "profiles", "affine_65536_fill_traffic_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_step:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration(mib: int) -> int:

@@ -122,7 +122,7 @@ struct poyg_plan {
   // four-pair kernel, 2 the one-set anti-diagonal kernel, 0 none); every
   // other fill runs on the side stream, followed by a second launch of the
   // main kernel (grid_help workgroups) that pulls from the same work list
-  int main_fill = 0, grid_help = 0;
+  int main_fill = 0, grid_help = 0, grid_help2 = 0;
   bool timing = false;
   float fill_ms = 0.f, trace_ms = 0.f;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -464,24 +464,23 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     // same work list.
     P->main_fill = P->n_quad && c_quad3 >= c_diag1 ? 1 : (P->n_diag[0] ? 2 : (P->n_quad ? 1 : 0));
     auto cus_of = [](int grid, int bpc) { return (grid + bpc - 1) / std::max(1, bpc); };
-    int side = 0;
+    int side = 0, side2 = 0;   // CUs of the side stream and of the widest-band stream
     for (int q = 1; q < 3; ++q)
-      if (P->n_diag[q]) side += cus_of(P->grid_diag[q], aff_diag_blocks_per_cu(align, 2 * q));
+      if (P->n_diag[q]) side2 += cus_of(P->grid_diag[q], aff_diag_blocks_per_cu(align, 2 * q));
+    if (P->n_lds) side2 += cus_of(P->grid_lds, per_cu);
+    if (P->n_glb) side2 += cus_of(P->grid_glb, 8);
     if (P->n_quad && P->main_fill != 1) side += cus_of(P->grid_quad, aff_quad_blocks_per_cu(align));
     if (P->n_diag[0] && P->main_fill != 2) side += cus_of(P->grid_diag[0], aff_diag_blocks_per_cu(align, 1));
-    if (P->n_lds) side += cus_of(P->grid_lds, per_cu);
-    if (P->n_glb) side += cus_of(P->grid_glb, 8);
-    if (side && P->main_fill) {
-      side = std::min(side, ctx->cus / 2);
-      if (P->main_fill == 1) {
-        const int bpc = aff_quad_blocks_per_cu(align);
-        P->grid_quad = std::max(1, std::min(P->grid_quad, (ctx->cus - side) * bpc));
-        P->grid_help = side * bpc;
-      } else {
-        const int bpc = aff_diag_blocks_per_cu(align, 1);
-        P->grid_diag[0] = std::max(1, std::min(P->grid_diag[0], (ctx->cus - side) * bpc));
-        P->grid_help = side * bpc;
-      }
+    if ((side || side2) && P->main_fill) {
+      side = std::min(side, ctx->cus / 4);
+      side2 = std::min(side2, ctx->cus / 4);
+      const int bpc = P->main_fill == 1 ? aff_quad_blocks_per_cu(align) : aff_diag_blocks_per_cu(align, 1);
+      int &grid_main = P->main_fill == 1 ? P->grid_quad : P->grid_diag[0];
+      grid_main = std::max(1, std::min(grid_main, (ctx->cus - side - side2) * bpc));
+      // a second (third) launch of the main kernel follows the side work on
+      // each side stream and pulls from the same work list
+      P->grid_help = std::max(1, side * bpc);
+      P->grid_help2 = side2 ? side2 * bpc : 0;
     }
   }
   DevTcm dt;
@@ -633,6 +632,10 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
     if (fork && P->grid_help) {
       if (P->main_fill == 1) HIP_TRY(quad(P->grid_help, rs));
       else HIP_TRY(diag(0, P->grid_help, rs));
+    }
+    if (fork && P->grid_help2) {
+      if (P->main_fill == 1) HIP_TRY(quad(P->grid_help2, rs2));
+      else HIP_TRY(diag(0, P->grid_help2, rs2));
     }
     if (P->main_fill == 1) HIP_TRY(quad(P->grid_quad, s));
     else if (P->main_fill == 2) HIP_TRY(diag(0, P->grid_diag[0], s));
