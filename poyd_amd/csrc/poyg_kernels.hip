@@ -1388,6 +1388,11 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
         }
       }
     };
+    // The longest items set the end of the kernel: their waves issue first
+    // (the work list is longest-first, so they start first too).
+    if (groups >= 1536) __builtin_amdgcn_s_setprio(2);
+    else if (groups >= 768) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
     // three loops (edge groups, steady groups, edge groups), so that no code
     // of a group is shared with -- and hoisted above -- a branch between the
     // two variants
