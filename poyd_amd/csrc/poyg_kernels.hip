@@ -1092,6 +1092,25 @@ __device__ __forceinline__ void aff_trace_pair(const PairMeta &m, int p, const u
       stage = 4;
     }
     const bool finish = stage == 4;
+    if (!finish && r > 0 && s + r < kTrBatch) {   // the walk's runs: no batch ends inside
+      if (code == 0) {
+        if (((s + r) >> 4) != (s >> 4)) {   // A codes are zero: store the word it leaves
+          oplog = lane == (s >> 4) ? acc : oplog;
+          acc = 0u;
+        }
+        s += r;
+        continue;
+      }
+      if (r == 1) {
+        acc |= (uint32_t)code << (2 * (s & 15));
+        ++s;
+        if ((s & 15) == 0) {
+          oplog = lane == ((s - 1) >> 4) ? acc : oplog;
+          acc = 0u;
+        }
+        continue;
+      }
+    }
     const uint32_t pat = code == 0 ? 0u : (code == 1 ? 0x55555555u : (code == 2 ? 0xAAAAAAAAu : 0xFFFFFFFFu));
     while (r > 0 || (finish && s > 0)) {
       if (r > 0) {
