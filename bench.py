@@ -49,6 +49,8 @@ def workload(name: str):
         return synth.workload_c2(), "affine"
     if name == "linear":
         return synth.workload_c1(), "linear"
+    if name == "cube":
+        return synth.workload_c3(), "cube"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -58,7 +60,11 @@ def cpu_baseline(w, kind: str, budget_s: float):
     the same pairs, timed until budget_s of CPU work has been spent."""
     from oracle import pyoracle as po
     cm = po.CM2.sub_indel(1, 1, 3) if kind == "affine" else po.CM2.sub_indel(1, 2)
-    n = len(w.a)
+    if kind == "cube":
+        cm3 = po.CM3(cm)
+        n = len(w.ln[0])
+    else:
+        n = len(w.a)
     rng = np.random.default_rng(7)
     order = rng.permutation(n)
     try:
@@ -71,6 +77,14 @@ def cpu_baseline(w, kind: str, budget_s: float):
     done = 0
     t0 = time.perf_counter()
     for k in order:
+        if kind == "cube":
+            x = [w.get(q, int(k)) for q in range(3)]
+            po.align_3(*x, cm3)
+            cells += len(x[0]) * len(x[1]) * len(x[2])
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+            continue
         a, b = w.a.get(int(k)), w.b.get(int(k))
         if kind == "affine":
             po.align_affine_3(a, b, cm)
@@ -90,8 +104,10 @@ def cpu_baseline(w, kind: str, budget_s: float):
         "unit": "GCUPS",
         "cores": 1,
         "kind": "port",
-        "sample": f"{done} of {n} pairs (seeded random order), {cells} band cells in {dt:.1f} s, "
-                  f"align + median per pair, 1 pinned core of {cpu_model()}",
+        "sample": f"{done} of {n} {'triples' if kind == 'cube' else 'pairs'} (seeded random order), "
+                  f"{cells} {'cube' if kind == 'cube' else 'band'} cells in {dt:.1f} s, "
+                  f"{'align_3 + median_3 per triple' if kind == 'cube' else 'align + median per pair'}, "
+                  f"1 pinned core of {cpu_model()}",
     }
 
 
@@ -100,7 +116,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="affine", choices=["affine", "linear"])
+    ap.add_argument("--workload", default="affine", choices=["affine", "linear", "cube"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -117,7 +133,14 @@ def main() -> int:
     from poyd_amd import engine
     w, kind = workload(args.workload)
     ctx = engine.Context(0)
-    if kind == "affine":
+    if kind == "cube":
+        cm = engine.CostMatrix.of_sub_indel(1, 2)
+        cm3 = engine.CostMatrix3(cm)
+        seq = np.concatenate([b.seq for b in w.batches])
+        base = np.cumsum([0] + [len(b.seq) for b in w.batches])
+        plan = engine.TriplePlan(ctx, cm3, *[x for q, b in enumerate(w.batches) for x in (b.off + base[q], b.ln)])
+        seq_lens = [b.ln for b in w.batches]
+    elif kind == "affine":
         cm = engine.CostMatrix.of_sub_indel(1, 1, 3)
         mode = engine.POYG_MODE_AFFINE_ALIGN
         dws = None
@@ -125,8 +148,10 @@ def main() -> int:
         cm = engine.CostMatrix.of_sub_indel(1, 2)
         mode = engine.POYG_MODE_LINEAR_ALIGN
         dws = engine.batch_deltaw(w.a, w.b)
-    seq = np.concatenate([w.a.seq, w.b.seq])
-    plan = engine.PairPlan(ctx, cm, mode, w.a.off, w.a.ln, w.b.off + len(w.a.seq), w.b.ln, dws)
+    if kind != "cube":
+        seq = np.concatenate([w.a.seq, w.b.seq])
+        plan = engine.PairPlan(ctx, cm, mode, w.a.off, w.a.ln, w.b.off + len(w.a.seq), w.b.ln, dws)
+        seq_lens = [w.a.ln, w.b.ln]
     d_seq = engine.DeviceBuffer.from_array(ctx, seq)
     d_cost = engine.DeviceBuffer(ctx, 4 * plan.n)
     d_out = engine.DeviceBuffer(ctx, plan.out_bytes)
@@ -167,9 +192,12 @@ def main() -> int:
     valu_tops = cells * ops / (fill_ms / 1e3) / 1e12
     # algorithmic HBM bytes of the fill: one direction byte per band cell,
     # both sequences read once, one int32 cost written per pair
-    alg_bytes = cells + int(w.a.ln.sum() + w.b.ln.sum()) + 4 * plan.n
+    alg_bytes = cells + int(sum(int(x.sum()) for x in seq_lens)) + 4 * plan.n
     fill_gbs = alg_bytes / (fill_ms / 1e3) / 1e9
-    matrix_cells = int(((w.a.ln.astype(np.int64) - 1) * (w.b.ln.astype(np.int64) - 1)).sum())
+    if kind == "cube":
+        matrix_cells = cells
+    else:
+        matrix_cells = int(((w.a.ln.astype(np.int64) - 1) * (w.b.ln.astype(np.int64) - 1)).sum())
 
     out = {
         "metric": METRIC,
@@ -186,11 +214,12 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": w.name,
-            "pairs": plan.n,
-            "band_cells": int(cells),
+            ("triples" if kind == "cube" else "pairs"): plan.n,
+            ("cube_cells" if kind == "cube" else "band_cells"): int(cells),
             "matrix_equivalent_gcups": round(matrix_cells / (ms / 1e3) / 1e9, 3),
             "cost_model": "affine open 3 extend 1 sub 1" if kind == "affine" else "linear sub 1 indel 2",
-            "outputs": "cost, aligned pair, median" + (", median with gaps" if kind == "affine" else ""),
+            "outputs": ("cost, three aligned sequences, median" if kind == "cube" else
+                        "cost, aligned pair, median" + (", median with gaps" if kind == "affine" else "")),
             "fill_ms": round(fill_ms, 3),
             "backtrace_ms": round(trace_ms, 3),
             "deterministic": deterministic,
@@ -204,7 +233,7 @@ def main() -> int:
             "traffic": measured_traffic(kind),
             "kernel": ("fill stage: aff_quad_kernel (main and side-stream launches) with the one-set and "
                        "lane-set fills, one span of HIP events on the main stream") if kind == "affine"
-                      else "lin_fill_lds_kernel",
+                      else "cube_fill_kernel" if kind == "cube" else "lin_fill_lds_kernel",
         },
         "valu_roofline": {
             "bound": "valu",

@@ -228,8 +228,17 @@ int poyg_plan_triples(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64
                       const int64_t *off_2, const int32_t *len_2, const int64_t *off_3, const int32_t *len_3,
                       poyg_plan **out) {
   if (!ctx || !out) return POYG_EINVAL;
+  // direction cubes of one batch: POYG_CUBE_DM_BUDGET_MB, else a little over
+  // half of the free device memory (at least 1 GiB)
   const char *b = getenv("POYG_CUBE_DM_BUDGET_MB");
-  const int64_t budget = (b ? atoll(b) : 16384) << 20;
+  int64_t budget = int64_t(1) << 30;
+  if (b) {
+    budget = atoll(b) << 20;
+  } else {
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) budget = std::max(budget, (int64_t)(free_b / 16 * 9));
+  }
   void *tp = nullptr;
   int rc = poyg_triple_plan_create(ctx, cm3, n, off_1, len_1, off_2, len_2, off_3, len_3, ctx->device, ctx->cus,
                                    ctx->stream, budget, &tp);

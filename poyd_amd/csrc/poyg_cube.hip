@@ -23,9 +23,15 @@ namespace poyg {
 struct TripleMeta {
   int64_t off1, off2, off3, out_off, dm_off;
   int32_t l1, l2, l3, cap;
+  int32_t dstride;   // bytes per row of the direction cube
+  int32_t flags;     // bit 0: two 4-bit codes per byte (else one per byte); bits 1-2: wave priority
 };
 
-constexpr int kP1 = 1, kP2 = 2, kP3 = 4, kS1 = 8, kS2 = 16, kS3 = 32, kSS = 64;
+// Direction codes, one byte per cell: the index of the winning move in the
+// reference's evaluation order (fill_parallel P3, P1, P2, then fill_moved S3,
+// S1, S2, then the self pass SS; src/algn.c:2812-2858, 3041-3048).  With
+// strict-< first-wins updates the winner is the first minimum in this order.
+constexpr int kP3 = 0, kP1 = 1, kP2 = 2, kS3 = 3, kS1 = 4, kS2 = 5, kSS = 6;
 
 __device__ __forceinline__ int c_lane() { return (int)__lane_id(); }
 
@@ -42,6 +48,11 @@ __device__ __forceinline__ void c_lds_sync() {
   __builtin_amdgcn_wave_barrier();
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
+
+// Ordering of one wave's own global stores before its later loads of the
+// same rows (every cube kernel runs one wave per workgroup, so workgroup
+// scope suffices: no L2 write-back or invalidate is needed).
+__device__ __forceinline__ void c_wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
 // DPP-only cross-lane helpers (see poyg_kernels.hip).
 // Eight wait states and a scheduling barrier between any EXEC change and the
@@ -142,7 +153,7 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
         carry = __builtin_amdgcn_readlane(ps, 63);
       }
     }
-    __threadfence();
+    c_wave_fence();
 #define ROW(R) (ring + (size_t)((R) % W) * n3)
     // plane 0, line 0: 0, then the self-gap chain
     for (int k0 = 0; k0 < n3; k0 += 64) {
@@ -152,7 +163,7 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
         dm[k] = k == 0 ? kS2 : kSS;
       }
     }
-    __threadfence();
+    c_wave_fence();
     int fenced = 0;   // rows [0, fenced] are visible to loads
     // plane 0, lines r >= 1
     for (int r = 1; r < n2; ++r) {
@@ -185,7 +196,7 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
           dc[k] = (uint8_t)(o.ss ? kSS : code);
         }
       }
-      __threadfence();
+      c_wave_fence();
       fenced = r;
     }
     // planes p >= 1
@@ -198,7 +209,7 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
       // line 0: reads the row the "diagonal" pointer holds, out-(n2+p-1)
       {
         const int out = p * n2;
-        if (out - (n2 + p - 1) > fenced) { __threadfence(); fenced = out - 1; }
+        if (out - (n2 + p - 1) > fenced) { c_wave_fence(); fenced = out - 1; }
         const int *dg = ROW(out - (n2 + p - 1));
         int *cur = ROW(out);
         uint8_t *dc = dm + (size_t)out * n3;
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
           s_r[3][lane] = tcm->cost[(a * 32 + b) * 32 + lane];
         }
         c_lds_sync();
-        if (out - p - 1 > fenced) { __threadfence(); fenced = out - 1; }
+        if (out - p - 1 > fenced) { c_wave_fence(); fenced = out - 1; }
         const int *dg = ROW(out - n2 - p), *up = ROW(out - n2 - p + 1), *pv = ROW(out - p - 1);
         int *cur = ROW(out);
         uint8_t *dc = dm + (size_t)out * n3;
@@ -273,10 +284,332 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
     // the reference returns the last cell of the last row written; every
     // lane stores the same value (no lane-divergent code at the end of the
     // item loop body, which keeps the loop scalar)
-    __threadfence();
+    c_wave_fence();
     cost_out[t] = ROW(n1 * n2 - 1)[n3 - 1];
-    __threadfence();
+    c_wave_fence();
 #undef ROW
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row kernel: lane L owns the R consecutive cells k = L*R .. L*R+R-1 of every
+// row (n3 <= 64*R), so a row is one pass: six candidate moves per cell as
+// ranked keys (value*8 + move index; the minimum key is the reference's
+// first minimum), then the self pass along s3 as one prefix minimum
+// (mm[k] = min(v[k], mm[k-1] + g[k]) equals G3[k] + min_{j<=k}(v[j] - G3[j])
+// with G3 the prefix sums of g; the self move wins exactly where
+// mm[k-1] + g[k] < v[k]).
+//
+// The reference's row pointers lag (see the top of this file): row `out` of
+// plane p >= 1 reads rows out-(n2+p-1) ("upper", or "diagonal" on line 0),
+// out-(n2+p) ("diagonal") and out-(p+1) ("previous").  Every input of row
+// out+1 is therefore complete once row out-1 is stored, so the loads of the
+// next row are issued before the current row is computed, and the
+// "diagonal" row of line r is the "upper" row of line r-1 (line 1 reuses the
+// row of line 0), already in registers.  Rows live in a per-wave ring of
+// n1+n2 rows; T is int16_t when every cell value is provably below 2^15
+// (checked on the host), else int32_t.
+constexpr int kBig3 = 1 << 28;
+constexpr int kTop3 = 0x7fffffff;
+
+__device__ __forceinline__ int c_shr1(int v, int fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+}
+
+template <typename T, int R>
+struct RingRow {
+  static constexpr int kWords = R * (int)sizeof(T) / 4;
+  uint32_t w[kWords];
+  __device__ __forceinline__ void load(const T *p) {
+    static_assert(kWords % 2 == 0, "rows are loaded in 8-byte pieces");
+    const uint2 *q = (const uint2 *)p;
+#pragma unroll
+    for (int i = 0; i < kWords / 2; ++i) {
+      const uint2 v = q[i];
+      w[2 * i] = v.x;
+      w[2 * i + 1] = v.y;
+    }
+  }
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int i = 0; i < kWords; ++i) w[i] = 0;
+  }
+  // cell j as a non-negative int (inactive cells may hold anything)
+  __device__ __forceinline__ int get(int j) const {
+    if constexpr (sizeof(T) == 2) return (int)((w[j >> 1] >> (16 * (j & 1))) & 0xffffu);
+    else return (int)w[j];
+  }
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void store_ring(T *p, const int (&mm)[R]) {
+  constexpr int kWords = R * (int)sizeof(T) / 4;
+  uint32_t w[kWords];
+#pragma unroll
+  for (int i = 0; i < kWords; ++i) {
+    if constexpr (sizeof(T) == 2)
+      w[i] = __builtin_amdgcn_perm((uint32_t)mm[2 * i + 1], (uint32_t)mm[2 * i], 0x05040100u);
+    else
+      w[i] = (uint32_t)mm[i];
+  }
+  uint2 *q = (uint2 *)p;
+#pragma unroll
+  for (int i = 0; i < kWords / 2; ++i) q[i] = make_uint2(w[2 * i], w[2 * i + 1]);
+}
+
+// R 4-bit codes (R/2 bytes at p, aligned to R/2 bytes)
+template <int R>
+__device__ __forceinline__ void store_codes(uint8_t *p, const uint32_t (&c)[(R + 7) / 8]) {
+  if constexpr (R == 4) {
+    *(uint16_t *)p = (uint16_t)c[0];
+  } else if constexpr (R == 8) {
+    *(uint32_t *)p = c[0];
+  } else if constexpr (R == 12) {
+    uint16_t *q = (uint16_t *)p;
+    q[0] = (uint16_t)c[0];
+    q[1] = (uint16_t)(c[0] >> 16);
+    q[2] = (uint16_t)c[1];
+  } else {
+    static_assert(R == 16, "row widths are 4, 8, 12 or 16 cells per lane");
+    *(uint2 *)p = make_uint2(c[0], c[1]);
+  }
+}
+
+// One row.  kKind 0: plane 0, line >= 1 (previous-row moves only); 1: line 0
+// of a plane >= 1 (upper moves only, the upper row being the reference's
+// diagonal pointer there); 2: every other row.  U, D, P: raw values of the
+// upper, diagonal and previous rows; tab: the LDS cost table row of s2[r].
+// Out: mm (raw values) and the packed move codes.
+template <int R, int kKind>
+__device__ __forceinline__ void cube_row(const int (&U)[R], const int (&D)[R], const int (&P)[R],
+                                         const int (&G3s)[R], const int (&yo)[R], const int4 *tab, int cP3,
+                                         int cP1, int cP2, int (&mm)[R], uint32_t (&codes)[(R + 7) / 8]) {
+  // cell k-1 of lane 0's first cell does not exist: the fill value makes its
+  // moves lose to every real key
+  constexpr int kNone = kBig3 >> 3;
+  int Ul = 0, Dl = 0, Pl = 0;
+  c_dpp_guard();
+  if (kKind != 0) Ul = c_shr1(U[R - 1], kNone);
+  if (kKind == 2) Dl = c_shr1(D[R - 1], kNone);
+  if (kKind != 1) Pl = c_shr1(P[R - 1], kNone);
+  int key[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int4 tb = tab[yo[j]];
+    const int ul = j ? U[j - 1] : Ul, dl = j ? D[j - 1] : Dl, pl = j ? P[j - 1] : Pl;
+    int k;
+    if (kKind == 0) {
+      k = min((P[j] << 3) + cP1, (pl << 3) + tb.x);
+    } else if (kKind == 1) {
+      k = min((U[j] << 3) + cP3, (ul << 3) + tb.z);
+    } else {
+      k = min(min((U[j] << 3) + cP3, (P[j] << 3) + cP1), (D[j] << 3) + cP2);
+      k = min(min(k, (ul << 3) + tb.z), min((pl << 3) + tb.x, (dl << 3) + tb.y));
+    }
+    key[j] = k;
+  }
+  // self pass along s3
+  int wv[R], lp[R];
+  int run = kTop3;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    wv[j] = (key[j] & ~7) - G3s[j];
+    run = min(run, wv[j]);
+    lp[j] = run;
+  }
+  const int incl = c_prefix_min(run);
+  c_dpp_guard();
+  const int excl = c_shr1(incl, kTop3);
+  int prev = excl;
+#pragma unroll
+  for (int j = 0; j < (R + 7) / 8; ++j) codes[j] = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int pm = min(excl, lp[j]);
+    const bool ss = prev < wv[j];
+    mm[j] = (G3s[j] + pm) >> 3;
+    const uint32_t c = ss ? (uint32_t)kSS : (uint32_t)(key[j] & 7);
+    codes[j >> 3] |= c << (4 * (j & 7));
+    prev = pm;
+  }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(64) void cube_rows_kernel(const TripleMeta *__restrict__ meta,
+                                                       const int32_t *__restrict__ order, int n, int *counter,
+                                                       const uint8_t *__restrict__ seq,
+                                                       const DevTcm3 *__restrict__ tcm, uint8_t *dm_base,
+                                                       T *scratch, int64_t scratch_per_wave,
+                                                       int32_t *cost_out) {
+  // [b][y]: x = cost(gap, b, y)*8 + S1, y = cost(a, b, y)*8 + S2, z = cost(a, gap, y)*8 + S3 for the
+  // plane's s1 symbol a
+  __shared__ int4 s_tab[32 * 32];
+  constexpr int RS = 64 * R;   // ring row stride (elements)
+  const int lane = c_lane();
+  T *ring = scratch + (size_t)blockIdx.x * scratch_per_wave;
+  const int gap = tcm->gap;
+  for (int e = lane; e < 1024; e += 64) s_tab[e].x = (tcm->cost[gap * 1024 + e] << 3) | kS1;
+  c_lds_sync();
+  for (;;) {
+    int kk = 0;
+    if (lane == 0) kk = atomicAdd(counter, 1);
+    kk = __builtin_amdgcn_readfirstlane(kk);
+    if (kk >= n) break;
+    const int t = __builtin_amdgcn_readfirstlane(order[kk]);
+    const int64_t off1 = c_uniform64(meta[t].off1), off2 = c_uniform64(meta[t].off2),
+                  off3 = c_uniform64(meta[t].off3), dm_off = c_uniform64(meta[t].dm_off);
+    const int n1 = __builtin_amdgcn_readfirstlane(meta[t].l1), n2 = __builtin_amdgcn_readfirstlane(meta[t].l2),
+              n3 = __builtin_amdgcn_readfirstlane(meta[t].l3),
+              dstride = __builtin_amdgcn_readfirstlane(meta[t].dstride);
+    // the triples with the most rows bound the launch: their waves issue first
+    const int prio = (__builtin_amdgcn_readfirstlane(meta[t].flags) >> 1) & 3;
+    if (prio >= 2) __builtin_amdgcn_s_setprio(2);
+    else if (prio == 1) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+    const uint8_t *s1 = seq + off1, *s2 = seq + off2, *s3 = seq + off3;
+    uint8_t *dm = dm_base + dm_off;
+    const int W = n1 + n2;
+    const int s30 = __builtin_amdgcn_readfirstlane((int)(s3[0] & 31));
+    const bool act = lane * R < n3;   // lane holds at least one cell
+    // per-cell symbols of s3 (inactive cells: the gap) and prefix sums of
+    // cost(gap, gap, s3[k]) for k >= 1
+    int yo[R], G3s[R];
+    {
+      int g[R], sum = 0;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int k = lane * R + j;
+        const int y = k < n3 ? (int)(s3[k] & 31) : gap;
+        yo[j] = y;
+        g[j] = (k >= 1 && k < n3) ? (s_tab[gap * 32 + y].x >> 3) : 0;
+        sum += g[j];
+        G3s[j] = sum;
+      }
+      const int ex = c_prefix_sum(sum) - sum;
+#pragma unroll
+      for (int j = 0; j < R; ++j) G3s[j] = (G3s[j] + ex) << 3;
+    }
+    int mm[R];
+    uint32_t codes[(R + 7) / 8];
+    // plane 0, line 0: the self pass from 0
+#pragma unroll
+    for (int j = 0; j < R; ++j) mm[j] = G3s[j] >> 3;
+#pragma unroll
+    for (int j = 0; j < (R + 7) / 8; ++j) codes[j] = kSS * 0x11111111u;
+    if (lane == 0) codes[0] = (codes[0] & ~0xfu) | kS2;   // the origin's code is never read
+    if (act) {
+      store_ring<T, R>(ring + lane * R, mm);
+      store_codes<R>(dm + lane * (R / 2), codes);
+    }
+    int cur = 0;   // ring row of the row just written
+    // plane 0, lines r >= 1 (the previous row is the one just computed)
+    for (int r = 1; r < n2; ++r) {
+      const int b = s2[r] & 31;
+      const int4 *tab = s_tab + b * 32;
+      const int cP1 = __builtin_amdgcn_readfirstlane(tab[s30].x) - (kS1 - kP1);
+      int P[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) P[j] = mm[j];
+      cube_row<R, 0>(P, P, P, G3s, yo, tab, 0, cP1, 0, mm, codes);
+      cur = cur + 1 == W ? 0 : cur + 1;
+      if (act) {
+        store_ring<T, R>(ring + (size_t)cur * RS + lane * R, mm);
+        store_codes<R>(dm + (size_t)r * dstride + lane * (R / 2), codes);
+      }
+    }
+    int last = 0;   // cost: cell n3-1 of the last row
+    {
+      const int jl = (n3 - 1) % R;
+      int v = 0;
+#pragma unroll
+      for (int j = 0; j < R; ++j) v = j == jl ? mm[j] : v;
+      last = __builtin_amdgcn_readlane(v, (n3 - 1) / R);
+    }
+    if (n1 > 1) {
+      // planes p >= 1: rows are prefetched one ahead
+      RingRow<T, R> xn, pn;
+      int bn = 0;   // s2 symbol of the next row (loaded with its rows)
+      c_wave_fence();
+      {
+        int xi = cur + 1 - (n2 + 1 - 1);
+        xi += xi < 0 ? W : 0;
+        if (act) xn.load(ring + (size_t)xi * RS + lane * R); else xn.zero();
+        pn.zero();
+      }
+      int U[R], D[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) U[j] = D[j] = 0;
+      for (int p = 1; p < n1; ++p) {
+        const int a = __builtin_amdgcn_readfirstlane((int)(s1[p] & 31));
+        // plane table: cost(a, b, y) and cost(a, gap, y); all loads first
+        {
+          int v[16];
+#pragma unroll
+          for (int q = 0; q < 16; ++q) v[q] = tcm->cost[a * 1024 + q * 64 + lane];
+          const int z = (tcm->cost[(a * 32 + gap) * 32 + (lane & 31)] << 3) | kS3;
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            s_tab[q * 64 + lane].y = (v[q] << 3) | kS2;
+            s_tab[q * 64 + lane].z = z;
+          }
+        }
+        c_lds_sync();
+        const int cP3 = __builtin_amdgcn_readfirstlane(s_tab[gap * 32 + s30].z) - (kS3 - kP3);
+        for (int r = 0; r < n2; ++r) {
+          const int out_ring = cur + 1 == W ? 0 : cur + 1;
+          c_wave_fence();   // the prefetched rows have arrived; earlier stores are done
+          RingRow<T, R> xc = xn, pc = pn;
+          const int b = __builtin_amdgcn_readfirstlane(bn) & 31;
+          // prefetch the next row's inputs
+          {
+            int pn_p = p, rn = r + 1;
+            if (rn == n2) { pn_p = p + 1; rn = 0; }
+            bn = s2[rn];   // in range on every row; consumed one row later
+            if (pn_p < n1) {
+              const int nxt = out_ring + 1 == W ? 0 : out_ring + 1;
+              int xi = nxt - (n2 + pn_p - 1);
+              xi += xi < 0 ? W : 0;
+              if (act) xn.load(ring + (size_t)xi * RS + lane * R);
+              if (rn >= 1) {
+                int pi = nxt - (pn_p + 1);
+                pi += pi < 0 ? W : 0;
+                if (act) pn.load(ring + (size_t)pi * RS + lane * R);
+              }
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            D[j] = U[j];
+            U[j] = xc.get(j);
+          }
+          if (r == 0) {
+            cube_row<R, 1>(U, U, U, G3s, yo, s_tab + gap * 32, cP3, 0, 0, mm, codes);
+          } else {
+            const int4 *tab = s_tab + b * 32;
+            const int4 c0 = tab[s30];
+            const int cP1 = __builtin_amdgcn_readfirstlane(c0.x) - (kS1 - kP1);
+            const int cP2 = __builtin_amdgcn_readfirstlane(c0.y) - (kS2 - kP2);
+            int P[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) P[j] = pc.get(j);
+            cube_row<R, 2>(U, D, P, G3s, yo, tab, cP3, cP1, cP2, mm, codes);
+          }
+          cur = out_ring;
+          if (act) {
+            store_ring<T, R>(ring + (size_t)cur * RS + lane * R, mm);
+            store_codes<R>(dm + ((size_t)p * n2 + r) * dstride + lane * (R / 2), codes);
+          }
+        }
+      }
+      const int jl = (n3 - 1) % R;
+      int v = 0;
+#pragma unroll
+      for (int j = 0; j < R; ++j) v = j == jl ? mm[j] : v;
+      last = __builtin_amdgcn_readlane(v, (n3 - 1) / R);
+    }
+    // every lane stores the same value (keeps the item loop scalar)
+    cost_out[t] = last;
+    c_wave_fence();
   }
 }
 
@@ -307,23 +640,21 @@ __global__ void cube_trace_kernel(const TripleMeta *__restrict__ meta, const int
   uint8_t *o = out + m.out_off;
   int pos[3] = {m.cap, m.cap, m.cap};
   uint32_t acc[3] = {0, 0, 0};
-  const int64_t a_plane = (int64_t)l2 * l3, a_line = l3;
-  int64_t e = (int64_t)l1 * l2 * l3 - 1;
+  const bool nib = m.flags & 1;   // e counts cells; a row holds dstride bytes
+  const int64_t a_line = nib ? 2 * (int64_t)m.dstride : m.dstride, a_plane = (int64_t)l2 * a_line;
+  int64_t e = ((int64_t)l1 * l2 - 1) * a_line + (l3 - 1);
   int i1 = l1 - 1, i2 = l2 - 1, i3 = l3 - 1, len = 0, bad = 0;
   int last[3] = {gap, gap, gap};
+  // per move index: which sequences advance (bits 0-2) -- P3, P1, P2, S3, S1, S2, SS
+  constexpr uint32_t kUse = 1u | (2u << 3) | (3u << 6) | (5u << 9) | (6u << 12) | (7u << 15) | (4u << 18);
   while (e > 0) {
-    const int d = dm[e];
-    int x1 = gap, x2 = gap, x3 = gap;
-    const bool m1 = d & (kS2 | kS3 | kP3 | kP2), m2 = d & (kS2 | kS1 | kP1 | kP2), m3 = d & (kS2 | kS3 | kS1 | kSS);
+    const int d = nib ? (dm[e >> 1] >> (4 * (e & 1))) & 15 : dm[e];
+    if (d > kSS) { bad = 1; break; }
+    const uint32_t u = (kUse >> (3 * d)) & 7u;
+    const bool m1 = u & 1u, m2 = u & 2u, m3 = u & 4u;
     if ((m1 && i1 < 0) || (m2 && i2 < 0) || (m3 && i3 < 0)) { bad = 1; break; }
-    if (d & kS2) { x1 = s1[i1--]; x2 = s2[i2--]; x3 = s3[i3--]; e -= a_plane + a_line + 1; }
-    else if (d & kS3) { x1 = s1[i1--]; x3 = s3[i3--]; e -= a_plane + 1; }
-    else if (d & kS1) { x2 = s2[i2--]; x3 = s3[i3--]; e -= a_line + 1; }
-    else if (d & kP3) { x1 = s1[i1--]; e -= a_plane; }
-    else if (d & kSS) { x3 = s3[i3--]; e -= 1; }
-    else if (d & kP1) { x2 = s2[i2--]; e -= a_line; }
-    else if (d & kP2) { x1 = s1[i1--]; x2 = s2[i2--]; e -= a_plane + a_line; }
-    else { bad = 1; break; }
+    const int x1 = m1 ? (int)s1[i1--] : gap, x2 = m2 ? (int)s2[i2--] : gap, x3 = m3 ? (int)s3[i3--] : gap;
+    e -= (m1 ? a_plane : 0) + (m2 ? a_line : 0) + (m3 ? 1 : 0);
     if (len == 0) { last[0] = x1; last[1] = x2; last[2] = x3; }
     rev_put(o, pos[0], acc[0], x1);
     rev_put(o + m.cap, pos[1], acc[1], x2);
@@ -361,6 +692,10 @@ using namespace poyg;
 
 // Triple plans are kept apart from pair plans; the C ABI shares poyg_plan
 // through a tagged wrapper defined here.
+//
+// Row classes: a triple with n3 <= 64*R (R = 4, 8, 12, 16) runs in the row
+// kernel of that R; longer s3 fall back to cube_fill_kernel (64-cell chunks).
+constexpr int kCubeClasses = 5;   // 0: chunked fallback, c >= 1: row kernel with R = 4c
 struct TriplePlan {
   poyg_ctx *ctx;
   int n;
@@ -369,8 +704,10 @@ struct TriplePlan {
   std::vector<std::pair<int, int>> batches;   // [begin, end) in `order`
   std::vector<int64_t> out_off;
   int64_t out_bytes = 0, cells = 0, dm_bytes = 0;
-  int64_t scratch_per_wave = 0;
+  int64_t scratch_bytes_per_wave = 0;
   int grid = 1;
+  bool narrow = false;   // int16 ring rows
+  std::vector<int> cls;  // row class per triple
   void *d_tcm = nullptr, *d_meta = nullptr, *d_order = nullptr, *d_dm = nullptr, *d_scratch = nullptr,
        *d_ctr = nullptr, *d_status = nullptr;
   hipStream_t stream = nullptr;
@@ -379,6 +716,29 @@ struct TriplePlan {
       if (p) (void)hipFree(p);
   }
 };
+
+static int cube_class(int l3) { return l3 > 1024 ? 0 : (l3 + 255) / 256; }
+
+template <typename T, int R>
+static hipError_t launch_rows(const TriplePlan *P, const int32_t *ord, int nb, int *ctr, const uint8_t *d_seq,
+                              int32_t *d_cost, hipStream_t s) {
+  const int grid = std::max(1, std::min(P->grid, nb));
+  hipLaunchKernelGGL((cube_rows_kernel<T, R>), dim3(grid), dim3(64), 0, s, (const TripleMeta *)P->d_meta, ord, nb,
+                     ctr, d_seq, (const DevTcm3 *)P->d_tcm, (uint8_t *)P->d_dm, (T *)P->d_scratch,
+                     (int64_t)(P->scratch_bytes_per_wave / (int64_t)sizeof(T)), d_cost);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_class(const TriplePlan *P, int c, const int32_t *ord, int nb, int *ctr,
+                               const uint8_t *d_seq, int32_t *d_cost, hipStream_t s) {
+  switch (c) {
+    case 1: return launch_rows<T, 4>(P, ord, nb, ctr, d_seq, d_cost, s);
+    case 2: return launch_rows<T, 8>(P, ord, nb, ctr, d_seq, d_cost, s);
+    case 3: return launch_rows<T, 12>(P, ord, nb, ctr, d_seq, d_cost, s);
+    default: return launch_rows<T, 16>(P, ord, nb, ctr, d_seq, d_cost, s);
+  }
+}
 
 extern "C" {
 
@@ -392,18 +752,31 @@ int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const
     return POYG_EUNSUPPORTED;
   }
   HIP_TRY3(hipSetDevice(device));
+  DevTcm3 *dt = new DevTcm3;
+  to_dev_tcm3(cm3, dt);
+  // cost bounds of the masks a sequence can hold (1..31)
+  int64_t maxc = 0, minc = 0;
+  for (int a = 1; a < 32; ++a)
+    for (int b = 1; b < 32; ++b)
+      for (int c = 1; c < 32; ++c) {
+        const int64_t v = dt->cost[(a * 32 + b) * 32 + c];
+        maxc = std::max(maxc, v);
+        minc = std::min(minc, v);
+      }
   TriplePlan *P = new TriplePlan;
   P->ctx = ctx;
   P->n = n;
   P->stream = (hipStream_t)stream;
   P->meta.resize(n);
   P->out_off.resize(n);
-  std::vector<int64_t> cells(n);
-  int maxw = 0, maxn3 = 0;
+  P->cls.resize(n);
+  std::vector<int64_t> cells(n), dmb(n);
+  int64_t max_sum = 0;
   for (int k = 0; k < n; ++k) {
     TripleMeta &m = P->meta[k];
     if (len_1[k] < 1 || len_2[k] < 1 || len_3[k] < 1) {
       delete P;
+      delete dt;
       set_error("poyg_plan_triples: empty sequence in triple %d", k);
       return POYG_EINVAL;
     }
@@ -415,38 +788,68 @@ int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const
     P->out_bytes += 4 * (int64_t)m.cap;
     cells[k] = (int64_t)m.l1 * m.l2 * m.l3;
     P->cells += cells[k];
-    maxw = std::max(maxw, m.l1 + m.l2);
-    maxn3 = std::max(maxn3, m.l3);
+    max_sum = std::max(max_sum, (int64_t)m.l1 + m.l2 + m.l3);
+  }
+  // every cell value is at most (l1 + l2 + l3) * maxc (a path of single
+  // moves), so the row kernel's keys (value * 8 + move) stay below 2^28 and,
+  // for non-negative costs below 2^15, the ring rows fit int16
+  const bool rows_ok = minc >= 0 && max_sum * std::max<int64_t>(maxc, 1) < (int64_t(1) << 24);
+  P->narrow = rows_ok && max_sum * maxc < 32767;
+  int64_t need = 0;
+  for (int k = 0; k < n; ++k) {
+    TripleMeta &m = P->meta[k];
+    const int c = rows_ok ? cube_class(m.l3) : 0;
+    P->cls[k] = c;
+    // row kernel: 4-bit codes, rows padded to whole lanes' cells; fallback: bytes
+    m.dstride = c ? (m.l3 + 4 * c - 1) / (4 * c) * (2 * c) : m.l3;
+    m.flags = c ? 1 : 0;
+    dmb[k] = (int64_t)m.l1 * m.l2 * m.dstride;
+    const int64_t w = (int64_t)m.l1 + m.l2;
+    const int64_t bytes = c ? w * 64 * 4 * c * (P->narrow ? 2 : 4)
+                            : 4 * (w * m.l3 + m.l3 + 64);
+    need = std::max(need, bytes);
+  }
+  P->scratch_bytes_per_wave = (need + 255) & ~(int64_t)255;
+  // wave priority: the triples with the most rows (each is one wave's serial work)
+  {
+    int64_t max_rows = 1;
+    for (int k = 0; k < n; ++k) max_rows = std::max(max_rows, (int64_t)P->meta[k].l1 * P->meta[k].l2);
+    for (int k = 0; k < n; ++k) {
+      const int64_t rows = (int64_t)P->meta[k].l1 * P->meta[k].l2;
+      const int prio = rows * 2 >= max_rows ? 2 : rows * 4 >= max_rows ? 1 : 0;
+      P->meta[k].flags |= prio << 1;
+    }
   }
   P->order.resize(n);
   std::iota(P->order.begin(), P->order.end(), 0);
-  std::stable_sort(P->order.begin(), P->order.end(), [&](int a, int b) { return cells[a] > cells[b]; });
+  std::stable_sort(P->order.begin(), P->order.end(), [&](int a, int b) {
+    return P->cls[a] != P->cls[b] ? P->cls[a] > P->cls[b] : cells[a] > cells[b];
+  });
   // split into batches whose direction cubes fit the budget
   int64_t cur = 0;
   int begin = 0;
   for (int q = 0; q < n; ++q) {
     const int k = P->order[q];
-    if (cur > 0 && cur + cells[k] > dm_budget) {
+    if (cur > 0 && cur + dmb[k] > dm_budget) {
       P->batches.push_back({begin, q});
       begin = q;
       cur = 0;
     }
     P->meta[k].dm_off = cur;
-    cur += (cells[k] + 15) & ~(int64_t)15;
+    cur += (dmb[k] + 15) & ~(int64_t)15;
     P->dm_bytes = std::max(P->dm_bytes, cur);
   }
   if (n) P->batches.push_back({begin, n});
-  P->scratch_per_wave = ((int64_t)maxw * maxn3 + maxn3 + 64 + 63) & ~(int64_t)63;   // 256-byte aligned
-  P->grid = std::max(1, std::min(n, cus * 4));
-  DevTcm3 *dt = new DevTcm3;
-  to_dev_tcm3(cm3, dt);
+  const char *wv = getenv("POYG_CUBE_WAVES_PER_CU");
+  const int waves = wv ? std::max(1, atoi(wv)) : 8;
+  P->grid = std::max(1, std::min(n, cus * waves));
   hipError_t e = hipSuccess;
   auto A = [&](void **p, size_t b) { if (e == hipSuccess) e = hipMalloc(p, b ? b : 16); };
   A(&P->d_tcm, sizeof(DevTcm3));
   A(&P->d_meta, sizeof(TripleMeta) * (size_t)n);
   A(&P->d_order, sizeof(int32_t) * (size_t)n);
   A(&P->d_dm, (size_t)P->dm_bytes);
-  A(&P->d_scratch, sizeof(int) * (size_t)P->scratch_per_wave * P->grid);
+  A(&P->d_scratch, (size_t)P->scratch_bytes_per_wave * P->grid);
   A(&P->d_ctr, 64);
   A(&P->d_status, 64);
   if (e == hipSuccess) e = hipMemcpy(P->d_tcm, dt, sizeof(DevTcm3), hipMemcpyHostToDevice);
@@ -485,13 +888,29 @@ int poyg_triple_plan_run(void *pp, const uint8_t *d_seq, int32_t *d_cost, uint8_
     const int nb = b.second - b.first;
     const int32_t *ord = (const int32_t *)P->d_order + b.first;
     if (timing) HIP_TRY3(hipEventRecord(ev[0], s));
-    HIP_TRY3(launch_reset((int *)P->d_ctr, 4, s));
+    HIP_TRY3(launch_reset((int *)P->d_ctr, 16, s));
     HIP_TRY3(launch_reset((int *)P->d_status, 4, s));
-    const int grid = std::max(1, std::min(P->grid, nb));
-    hipLaunchKernelGGL(cube_fill_kernel, dim3(grid), dim3(64), 0, s, (const TripleMeta *)P->d_meta, ord, nb,
-                       (int *)P->d_ctr, d_seq, (const DevTcm3 *)P->d_tcm, (uint8_t *)P->d_dm, (int *)P->d_scratch,
-                       P->scratch_per_wave, d_cost);
-    HIP_TRY3(hipGetLastError());
+    // one launch per row class present in the batch (the order is sorted by class)
+    int q = b.first, slot = 0;
+    while (q < b.second) {
+      const int c = P->cls[P->order[q]];
+      int q2 = q;
+      while (q2 < b.second && P->cls[P->order[q2]] == c) ++q2;
+      const int32_t *o2 = (const int32_t *)P->d_order + q;
+      int *ctr = (int *)P->d_ctr + slot++;
+      if (c == 0) {
+        const int grid = std::max(1, std::min(P->grid, q2 - q));
+        hipLaunchKernelGGL(cube_fill_kernel, dim3(grid), dim3(64), 0, s, (const TripleMeta *)P->d_meta, o2, q2 - q,
+                           ctr, d_seq, (const DevTcm3 *)P->d_tcm, (uint8_t *)P->d_dm, (int *)P->d_scratch,
+                           P->scratch_bytes_per_wave / 4, d_cost);
+        HIP_TRY3(hipGetLastError());
+      } else if (P->narrow) {
+        HIP_TRY3(launch_class<int16_t>(P, c, o2, q2 - q, ctr, d_seq, d_cost, s));
+      } else {
+        HIP_TRY3(launch_class<int32_t>(P, c, o2, q2 - q, ctr, d_seq, d_cost, s));
+      }
+      q = q2;
+    }
     if (timing) HIP_TRY3(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(cube_trace_kernel, dim3((nb + 63) / 64), dim3(64), 0, s, (const TripleMeta *)P->d_meta, ord,
                        nb, d_seq, (const DevTcm3 *)P->d_tcm, (const uint8_t *)P->d_dm, d_out, d_out_len,

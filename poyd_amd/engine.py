@@ -21,6 +21,7 @@ POYG_MODE_LINEAR_COST = 0
 POYG_MODE_LINEAR_ALIGN = 1
 POYG_MODE_AFFINE_COST = 2
 POYG_MODE_AFFINE_ALIGN = 3
+POYG_MODE_TRIPLE = 4
 
 _lib = None
 u8p = C.POINTER(C.c_uint8)
@@ -425,6 +426,34 @@ class PairPlan:
             self.close()
         except Exception:
             pass
+
+
+class TriplePlan(PairPlan):
+    """A batched three-sequence plan (align_3: cost, three aligned sequences
+    and the median) over device-resident sequences."""
+
+    def __init__(self, ctx: Context, cm3: CostMatrix3, off_1, len_1, off_2, len_2, off_3, len_3):
+        self.ctx = ctx
+        self.cm = cm3
+        self.mode = POYG_MODE_TRIPLE
+        self.n = len(len_1)
+        self._keep = [np.ascontiguousarray(x, dtype=t) for x, t in
+                      ((off_1, np.int64), (len_1, np.int32), (off_2, np.int64), (len_2, np.int32),
+                       (off_3, np.int64), (len_3, np.int32))]
+        o1, l1, o2, l2, o3, l3 = self._keep
+        h = vp()
+        _check(load().poyg_plan_triples(ctx.h, cm3.h, self.n, o1.ctypes.data_as(i64p), l1.ctypes.data_as(i32p),
+                                        o2.ctypes.data_as(i64p), l2.ctypes.data_as(i32p),
+                                        o3.ctypes.data_as(i64p), l3.ctypes.data_as(i32p), C.byref(h)),
+               "poyg_plan_triples")
+        self.h = h
+        self.slots = load().poyg_plan_slots(h)
+        self.out_bytes = load().poyg_plan_out_bytes(h)
+        self.cells = load().poyg_plan_cells(h)
+        self.out_off = np.zeros(self.n, np.int64)
+        if self.n:
+            load().poyg_plan_out_offsets(h, self.out_off.ctypes.data_as(i64p))
+        self.cap = ((l1.astype(np.int64) + l2 + l3 + 1 + 15) // 16) * 16
 
 
 def batch_deltaw(a_batch, b_batch) -> np.ndarray:

@@ -133,7 +133,21 @@ def pairs_loguniform(n: int, lmin: int, lmax: int, dmin: float, dmax: float, see
     return PairWorkload(_pack(BASES[root], lengths), _pack(BASES[mut], mlen), name)
 
 
-def triples_uniform(n: int, lmin: int, lmax: int, p_sub: float, p_indel: float, seed: int):
+@dataclass
+class TripleWorkload:
+    batches: list          # three SeqBatch, one per member of the triples
+    name: str
+
+    @property
+    def ln(self):
+        return [b.ln for b in self.batches]
+
+    def get(self, q: int, k: int) -> np.ndarray:
+        return self.batches[q].get(k)
+
+
+def triples_uniform(n: int, lmin: int, lmax: int, p_sub: float, p_indel: float, seed: int,
+                    name: str = "triples") -> TripleWorkload:
     """Ancestor length uniform on {lmin..lmax}; three independent descendants."""
     rng = np.random.Generator(np.random.PCG64(seed))
     lengths = rng.integers(lmin, lmax + 1, size=n).astype(np.int64)
@@ -142,7 +156,7 @@ def triples_uniform(n: int, lmin: int, lmax: int, p_sub: float, p_indel: float, 
     for _ in range(3):
         mut, mlen = mutate(rng, root, lengths, np.full(n, p_sub), np.full(n, p_indel))
         out.append(_pack(BASES[mut], mlen))
-    return out
+    return TripleWorkload(out, name)
 
 
 # The workloads measured by bench.py and replayed by the parity tests.
@@ -158,4 +172,4 @@ def workload_c2(n: int = 65536) -> PairWorkload:
 
 def workload_c3(n: int = 4096):
     """4,096 triples, L ~ U{100..500}, 15% substitutions, 3% indels."""
-    return triples_uniform(n, 100, 500, 0.15, 0.03, SEED0 + 2)
+    return triples_uniform(n, 100, 500, 0.15, 0.03, SEED0 + 2, "three-way-4096x100-500")

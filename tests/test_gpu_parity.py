@@ -258,56 +258,97 @@ def test_three_d_matches_oracle(ctx, oracle):
             assert np.array_equal(o[cap - ln.value:], w), k
 
 
-def test_three_d_batch_matches_oracle(ctx, oracle):
+def run_triples(ctx, cm2, T):
+    """One triple plan over T (lists of three sequences): costs, aligned
+    triples and medians as numpy arrays."""
     from poyd_amd import engine
-    note("three-dimensional batch")
-    rng = np.random.default_rng(12)
-    cm2 = engine.CostMatrix.of_sub_indel(1, 2)
     cm3 = engine.CostMatrix3(cm2)
-    ref = oracle.CM3(oracle.CM2.sub_indel(1, 2))
-    T = []
-    for k in range(40):
-        anc = rand_seq(rng, int(rng.integers(5, 60)))
-        T.append([mutate(rng, anc, 0.15, 0.06) for _ in range(3)])
     seqs = [x for t in T for x in t]
     bt = batch_of(seqs)
     off = bt.off.reshape(-1, 3)
     ln = bt.ln.reshape(-1, 3)
-    import ctypes as C
-    L = engine.load()
-    h = engine.vp()
-    cols = [np.ascontiguousarray(off[:, q]) for q in range(3)] + [np.ascontiguousarray(ln[:, q]) for q in range(3)]
-    rc = L.poyg_plan_triples(ctx.h, cm3.h, len(T), cols[0].ctypes.data_as(engine.i64p), cols[3].ctypes.data_as(engine.i32p),
-                             cols[1].ctypes.data_as(engine.i64p), cols[4].ctypes.data_as(engine.i32p),
-                             cols[2].ctypes.data_as(engine.i64p), cols[5].ctypes.data_as(engine.i32p), C.byref(h))
-    assert rc == 0
-    ob = L.poyg_plan_out_bytes(h)
-    oo = np.zeros(len(T), np.int64)
-    L.poyg_plan_out_offsets(h, oo.ctypes.data_as(engine.i64p))
+    plan = engine.TriplePlan(ctx, cm3, off[:, 0], ln[:, 0], off[:, 1], ln[:, 1], off[:, 2], ln[:, 2])
     d_seq = engine.DeviceBuffer.from_array(ctx, bt.seq)
     d_cost = engine.DeviceBuffer(ctx, 4 * len(T))
-    d_out = engine.DeviceBuffer(ctx, ob)
+    d_out = engine.DeviceBuffer(ctx, plan.out_bytes)
     d_len = engine.DeviceBuffer(ctx, 16 * len(T))
-    assert L.poyg_plan_run(h, d_seq.ptr, d_cost.ptr, d_out.ptr, d_len.ptr, None) == 0
+    plan.run(d_seq.ptr, d_cost.ptr, d_out.ptr, d_len.ptr)
     ctx.sync()
+    import ctypes as C
     bad = C.c_int32(0)
-    L.poyg_plan_status(h, C.byref(bad))
+    engine.load().poyg_plan_status(plan.h, C.byref(bad))
     assert bad.value == 0
     cost = d_cost.download(np.int32, len(T))
-    out = d_out.download(np.uint8, ob)
+    out = d_out.download(np.uint8, plan.out_bytes)
     lens = d_len.download(np.int32, 4 * len(T))
-    L.poyg_plan_destroy(h)
+    res = []
+    for k, t in enumerate(T):
+        cap = ((sum(len(x) for x in t) + 1 + 15) // 16) * 16
+        got = []
+        for q in range(4):
+            n = lens[4 * k + q]
+            base = plan.out_off[k] + q * cap
+            got.append(out[base + cap - n: base + cap].copy())
+        res.append((int(cost[k]), got))
+    plan.close()
+    return res
+
+
+def check_triples(ctx, oracle, sub, indel, T):
+    from poyd_amd import engine
+    cm2 = engine.CostMatrix.of_sub_indel(sub, indel)
+    ref = oracle.CM3(oracle.CM2.sub_indel(sub, indel))
+    got = run_triples(ctx, cm2, T)
     for k, t in enumerate(T):
         want = oracle.align_3(*t, ref)
-        assert cost[k] == want[0], k
-        cap = ((sum(len(x) for x in t) + 1 + 15) // 16) * 16
+        assert got[k][0] == want[0], (k, [len(x) for x in t])
         for q in range(3):
-            n = lens[4 * k + q]
-            base = oo[k] + q * cap
-            assert np.array_equal(out[base + cap - n: base + cap], want[1 + q]), (k, q)
-        n = lens[4 * k + 3]
-        base = oo[k] + 3 * cap
-        assert np.array_equal(out[base + cap - n: base + cap], oracle.median_3(*want[1:], ref)), k
+            assert np.array_equal(got[k][1][q], want[1 + q]), (k, q, [len(x) for x in t])
+        assert np.array_equal(got[k][1][3], oracle.median_3(*want[1:], ref)), k
+
+
+def test_three_d_batch_matches_oracle(ctx, oracle):
+    note("three-dimensional batch")
+    rng = np.random.default_rng(12)
+    T = []
+    for k in range(40):
+        anc = rand_seq(rng, int(rng.integers(5, 60)))
+        T.append([mutate(rng, anc, 0.15, 0.06) for _ in range(3)])
+    check_triples(ctx, oracle, 1, 2, T)
+
+
+def shaped_triples(rng, shapes):
+    T = []
+    for n1, n2, n3 in shapes:
+        anc = rand_seq(rng, max(n1, n2, n3) - 1)
+        t = []
+        for n in (n1, n2, n3):
+            x = mutate(rng, anc, 0.15, 0.04)[:n]
+            if len(x) < n:
+                x = np.concatenate([x, rand_seq(rng, n - len(x))[1:]]).astype(np.uint8)
+            t.append(x)
+        T.append(t)
+    return T
+
+
+# s3 lengths across every row width of the row kernel (64 * 4, 8, 12, 16
+# cells) and past it (the chunked kernel), with degenerate planes and lines
+CUBE_SHAPES = [(1, 1, 1), (1, 1, 9), (1, 7, 1), (6, 1, 1), (1, 5, 300), (4, 1, 520), (2, 2, 2),
+               (9, 11, 255), (9, 11, 256), (9, 11, 257), (12, 8, 300), (7, 13, 511), (7, 13, 512),
+               (7, 13, 513), (10, 9, 700), (5, 6, 768), (5, 6, 769), (6, 5, 1000), (4, 4, 1024),
+               (3, 5, 1025), (4, 3, 1300), (40, 35, 90), (30, 45, 130)]
+
+
+def test_three_d_row_classes(ctx, oracle):
+    note("three-dimensional row classes, int16 rows")
+    rng = np.random.default_rng(31)
+    check_triples(ctx, oracle, 1, 2, shaped_triples(rng, CUBE_SHAPES))
+
+
+def test_three_d_wide_costs(ctx, oracle):
+    note("three-dimensional row classes, int32 rows (costs past 2^15)")
+    rng = np.random.default_rng(32)
+    check_triples(ctx, oracle, 90, 200, shaped_triples(rng, CUBE_SHAPES[:18]))
 
 
 def test_fitch_matches_oracle(ctx, oracle):
