@@ -50,7 +50,7 @@ extern "C" {
 int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64_t *off_1,
                             const int32_t *len_1, const int64_t *off_2, const int32_t *len_2,
                             const int64_t *off_3, const int32_t *len_3, int device, int cus, void *stream,
-                            int64_t dm_budget, void **out);
+                            void *side, void *side2, int64_t dm_budget, void **out);
 void poyg_triple_plan_destroy(void *p);
 int64_t poyg_triple_plan_out_bytes(void *p);
 int64_t poyg_triple_plan_cells(void *p);
@@ -241,7 +241,7 @@ int poyg_plan_triples(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64
   }
   void *tp = nullptr;
   int rc = poyg_triple_plan_create(ctx, cm3, n, off_1, len_1, off_2, len_2, off_3, len_3, ctx->device, ctx->cus,
-                                   ctx->stream, budget, &tp);
+                                   ctx->stream, ctx->side, ctx->side2, budget, &tp);
   if (rc) return rc;
   poyg_plan *P = new poyg_plan;
   P->ctx = ctx;

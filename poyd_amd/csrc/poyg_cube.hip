@@ -710,33 +710,38 @@ struct TriplePlan {
   std::vector<int> cls;  // row class per triple
   void *d_tcm = nullptr, *d_meta = nullptr, *d_order = nullptr, *d_dm = nullptr, *d_scratch = nullptr,
        *d_ctr = nullptr, *d_status = nullptr;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr, side = nullptr, side2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;   // row classes run on three streams
   ~TriplePlan() {
     for (void *p : {d_tcm, d_meta, d_order, d_dm, d_scratch, d_ctr, d_status})
       if (p) (void)hipFree(p);
+    for (hipEvent_t e : {fork, join, join2})
+      if (e) (void)hipEventDestroy(e);
   }
 };
 
 static int cube_class(int l3) { return l3 > 1024 ? 0 : (l3 + 255) / 256; }
 
+// Launch ri of a batch uses scratch region ri (launches run side by side).
 template <typename T, int R>
-static hipError_t launch_rows(const TriplePlan *P, const int32_t *ord, int nb, int *ctr, const uint8_t *d_seq,
-                              int32_t *d_cost, hipStream_t s) {
+static hipError_t launch_rows(const TriplePlan *P, int ri, const int32_t *ord, int nb, int *ctr,
+                              const uint8_t *d_seq, int32_t *d_cost, hipStream_t s) {
   const int grid = std::max(1, std::min(P->grid, nb));
+  T *scratch = (T *)((uint8_t *)P->d_scratch + (size_t)ri * P->grid * P->scratch_bytes_per_wave);
   hipLaunchKernelGGL((cube_rows_kernel<T, R>), dim3(grid), dim3(64), 0, s, (const TripleMeta *)P->d_meta, ord, nb,
-                     ctr, d_seq, (const DevTcm3 *)P->d_tcm, (uint8_t *)P->d_dm, (T *)P->d_scratch,
+                     ctr, d_seq, (const DevTcm3 *)P->d_tcm, (uint8_t *)P->d_dm, scratch,
                      (int64_t)(P->scratch_bytes_per_wave / (int64_t)sizeof(T)), d_cost);
   return hipGetLastError();
 }
 
 template <typename T>
-static hipError_t launch_class(const TriplePlan *P, int c, const int32_t *ord, int nb, int *ctr,
+static hipError_t launch_class(const TriplePlan *P, int c, int ri, const int32_t *ord, int nb, int *ctr,
                                const uint8_t *d_seq, int32_t *d_cost, hipStream_t s) {
   switch (c) {
-    case 1: return launch_rows<T, 4>(P, ord, nb, ctr, d_seq, d_cost, s);
-    case 2: return launch_rows<T, 8>(P, ord, nb, ctr, d_seq, d_cost, s);
-    case 3: return launch_rows<T, 12>(P, ord, nb, ctr, d_seq, d_cost, s);
-    default: return launch_rows<T, 16>(P, ord, nb, ctr, d_seq, d_cost, s);
+    case 1: return launch_rows<T, 4>(P, ri, ord, nb, ctr, d_seq, d_cost, s);
+    case 2: return launch_rows<T, 8>(P, ri, ord, nb, ctr, d_seq, d_cost, s);
+    case 3: return launch_rows<T, 12>(P, ri, ord, nb, ctr, d_seq, d_cost, s);
+    default: return launch_rows<T, 16>(P, ri, ord, nb, ctr, d_seq, d_cost, s);
   }
 }
 
@@ -745,7 +750,7 @@ extern "C" {
 int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64_t *off_1,
                             const int32_t *len_1, const int64_t *off_2, const int32_t *len_2,
                             const int64_t *off_3, const int32_t *len_3, int device, int cus, void *stream,
-                            int64_t dm_budget, void **out) {
+                            void *side, void *side2, int64_t dm_budget, void **out) {
   if (!cm3 || n < 0 || !out) return POYG_EINVAL;
   if (cm3->lcm != 5 || cm3->gap != kGap) {
     set_error("poyg_plan_triples: 5-bit DNA matrices only");
@@ -767,6 +772,8 @@ int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const
   P->ctx = ctx;
   P->n = n;
   P->stream = (hipStream_t)stream;
+  P->side = (hipStream_t)side;
+  P->side2 = (hipStream_t)side2;
   P->meta.resize(n);
   P->out_off.resize(n);
   P->cls.resize(n);
@@ -849,9 +856,18 @@ int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const
   A(&P->d_meta, sizeof(TripleMeta) * (size_t)n);
   A(&P->d_order, sizeof(int32_t) * (size_t)n);
   A(&P->d_dm, (size_t)P->dm_bytes);
-  A(&P->d_scratch, (size_t)P->scratch_bytes_per_wave * P->grid);
+  // one scratch region per row class (a batch runs its classes side by side)
+  int n_cls = 0;
+  {
+    bool seen[kCubeClasses] = {};
+    for (int k = 0; k < n; ++k) seen[P->cls[k]] = true;
+    for (bool x : seen) n_cls += x;
+  }
+  A(&P->d_scratch, (size_t)P->scratch_bytes_per_wave * P->grid * std::max(1, n_cls));
   A(&P->d_ctr, 64);
   A(&P->d_status, 64);
+  for (hipEvent_t *ev : {&P->fork, &P->join, &P->join2})
+    if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMemcpy(P->d_tcm, dt, sizeof(DevTcm3), hipMemcpyHostToDevice);
   if (e == hipSuccess && n) e = hipMemcpy(P->d_meta, P->meta.data(), sizeof(TripleMeta) * n, hipMemcpyHostToDevice);
   if (e == hipSuccess && n) e = hipMemcpy(P->d_order, P->order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
@@ -890,26 +906,53 @@ int poyg_triple_plan_run(void *pp, const uint8_t *d_seq, int32_t *d_cost, uint8_
     if (timing) HIP_TRY3(hipEventRecord(ev[0], s));
     HIP_TRY3(launch_reset((int *)P->d_ctr, 16, s));
     HIP_TRY3(launch_reset((int *)P->d_status, 4, s));
-    // one launch per row class present in the batch (the order is sorted by class)
-    int q = b.first, slot = 0;
-    while (q < b.second) {
-      const int c = P->cls[P->order[q]];
+    // one launch per row class present in the batch (the order is sorted by
+    // class), the classes side by side: the first on the third stream, the
+    // last on the run stream, any others on the second
+    std::vector<std::pair<int, int>> runs;
+    for (int q = b.first; q < b.second;) {
       int q2 = q;
-      while (q2 < b.second && P->cls[P->order[q2]] == c) ++q2;
+      while (q2 < b.second && P->cls[P->order[q2]] == P->cls[P->order[q]]) ++q2;
+      runs.push_back({q, q2});
+      q = q2;
+    }
+    const bool fan = runs.size() > 1 && P->side && P->side2;
+    bool used_side = false, used_side2 = false;
+    if (fan) {
+      HIP_TRY3(hipEventRecord(P->fork, s));
+      HIP_TRY3(hipStreamWaitEvent(P->side, P->fork, 0));
+      HIP_TRY3(hipStreamWaitEvent(P->side2, P->fork, 0));
+    }
+    for (size_t ri = 0; ri < runs.size(); ++ri) {
+      const int q = runs[ri].first, q2 = runs[ri].second;
+      const int c = P->cls[P->order[q]];
+      hipStream_t ls = s;
+      if (fan && ri + 1 < runs.size()) {
+        ls = ri == 0 ? P->side2 : P->side;
+        (ri == 0 ? used_side2 : used_side) = true;
+      }
       const int32_t *o2 = (const int32_t *)P->d_order + q;
-      int *ctr = (int *)P->d_ctr + slot++;
+      int *ctr = (int *)P->d_ctr + ri;
       if (c == 0) {
         const int grid = std::max(1, std::min(P->grid, q2 - q));
-        hipLaunchKernelGGL(cube_fill_kernel, dim3(grid), dim3(64), 0, s, (const TripleMeta *)P->d_meta, o2, q2 - q,
-                           ctr, d_seq, (const DevTcm3 *)P->d_tcm, (uint8_t *)P->d_dm, (int *)P->d_scratch,
+        hipLaunchKernelGGL(cube_fill_kernel, dim3(grid), dim3(64), 0, ls, (const TripleMeta *)P->d_meta, o2,
+                           q2 - q, ctr, d_seq, (const DevTcm3 *)P->d_tcm, (uint8_t *)P->d_dm,
+                           (int *)P->d_scratch + (size_t)ri * P->grid * (P->scratch_bytes_per_wave / 4),
                            P->scratch_bytes_per_wave / 4, d_cost);
         HIP_TRY3(hipGetLastError());
       } else if (P->narrow) {
-        HIP_TRY3(launch_class<int16_t>(P, c, o2, q2 - q, ctr, d_seq, d_cost, s));
+        HIP_TRY3(launch_class<int16_t>(P, c, ri, o2, q2 - q, ctr, d_seq, d_cost, ls));
       } else {
-        HIP_TRY3(launch_class<int32_t>(P, c, o2, q2 - q, ctr, d_seq, d_cost, s));
+        HIP_TRY3(launch_class<int32_t>(P, c, ri, o2, q2 - q, ctr, d_seq, d_cost, ls));
       }
-      q = q2;
+    }
+    if (used_side) {
+      HIP_TRY3(hipEventRecord(P->join, P->side));
+      HIP_TRY3(hipStreamWaitEvent(s, P->join, 0));
+    }
+    if (used_side2) {
+      HIP_TRY3(hipEventRecord(P->join2, P->side2));
+      HIP_TRY3(hipStreamWaitEvent(s, P->join2, 0));
     }
     if (timing) HIP_TRY3(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(cube_trace_kernel, dim3((nb + 63) / 64), dim3(64), 0, s, (const TripleMeta *)P->d_meta, ord,
