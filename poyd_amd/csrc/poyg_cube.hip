@@ -311,6 +311,7 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
 // (checked on the host), else int32_t.
 constexpr int kBig3 = 1 << 28;
 constexpr int kTop3 = 0x7fffffff;
+constexpr int kAhead = 3;   // rows whose inputs are in flight
 
 __device__ __forceinline__ int c_shr1(int v, int fill) {
   return __builtin_amdgcn_update_dpp(fill, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
@@ -526,78 +527,111 @@ __global__ __launch_bounds__(64) void cube_rows_kernel(const TripleMeta *__restr
       last = __builtin_amdgcn_readlane(v, (n3 - 1) / R);
     }
     if (n1 > 1) {
-      // planes p >= 1: rows are prefetched one ahead
-      RingRow<T, R> xn, pn;
-      int bn = 0;   // s2 symbol of the next row (loaded with its rows)
-      c_wave_fence();
-      {
-        int xi = cur + 1 - (n2 + 1 - 1);
-        xi += xi < 0 ? W : 0;
-        if (act) xn.load(ring + (size_t)xi * RS + lane * R); else xn.zero();
-        pn.zero();
-      }
+      // planes p >= 1 as one stream of rows f = p*n2 + r.  The inputs of row
+      // g are loaded kAhead rows early (register slot g % kAhead) when every
+      // row they read precedes the row being computed then, i.e. g - (p+1)
+      // and g - (n2+p-1) are at most g - kAhead - 1; other rows (the first
+      // planes) load their inputs when they are computed.  A wave's own
+      // stores are seen by its later loads in program order.
+      RingRow<T, R> xs[kAhead], ps[kAhead];
+      int bs[kAhead];
+      bool pre[kAhead];
       int U[R], D[R];
 #pragma unroll
       for (int j = 0; j < R; ++j) U[j] = D[j] = 0;
-      for (int p = 1; p < n1; ++p) {
-        const int a = __builtin_amdgcn_readfirstlane((int)(s1[p] & 31));
-        // plane table: cost(a, b, y) and cost(a, gap, y); all loads first
-        {
-          int v[16];
 #pragma unroll
-          for (int q = 0; q < 16; ++q) v[q] = tcm->cost[a * 1024 + q * 64 + lane];
-          const int z = (tcm->cost[(a * 32 + gap) * 32 + (lane & 31)] << 3) | kS3;
-#pragma unroll
-          for (int q = 0; q < 16; ++q) {
-            s_tab[q * 64 + lane].y = (v[q] << 3) | kS2;
-            s_tab[q * 64 + lane].z = z;
+      for (int i = 0; i < kAhead; ++i) {
+        xs[i].zero();
+        ps[i].zero();
+        bs[i] = 0;
+        pre[i] = false;
+      }
+      const int total = n1 * n2;
+      // row g's ring index and (plane, line), kAhead rows ahead of f
+      int f = n2, p = 1, r = 0, cur_ring = n2;
+      int g = f, gp = 1, gr = 0, g_ring = n2;
+      auto issue = [&](int i) {
+        // loads of row g into slot i when allowed, then advance g
+        pre[i] = false;
+        bs[i] = s2[gr];   // gr < n2 on every call: loaded unconditionally, used kAhead rows later
+        if (g < total && gp >= kAhead && n2 + gp >= kAhead + 2) {
+          int xi = g_ring - (n2 + gp - 1);
+          xi += xi < 0 ? W : 0;
+          if (act) xs[i].load(ring + (size_t)xi * RS + lane * R);
+          if (gr >= 1) {
+            int pi = g_ring - (gp + 1);
+            pi += pi < 0 ? W : 0;
+            if (act) ps[i].load(ring + (size_t)pi * RS + lane * R);
           }
+          pre[i] = true;
         }
-        c_lds_sync();
-        const int cP3 = __builtin_amdgcn_readfirstlane(s_tab[gap * 32 + s30].z) - (kS3 - kP3);
-        for (int r = 0; r < n2; ++r) {
-          const int out_ring = cur + 1 == W ? 0 : cur + 1;
-          c_wave_fence();   // the prefetched rows have arrived; earlier stores are done
-          RingRow<T, R> xc = xn, pc = pn;
-          const int b = __builtin_amdgcn_readfirstlane(bn) & 31;
-          // prefetch the next row's inputs
-          {
-            int pn_p = p, rn = r + 1;
-            if (rn == n2) { pn_p = p + 1; rn = 0; }
-            bn = s2[rn];   // in range on every row; consumed one row later
-            if (pn_p < n1) {
-              const int nxt = out_ring + 1 == W ? 0 : out_ring + 1;
-              int xi = nxt - (n2 + pn_p - 1);
+        ++g;
+        g_ring = g_ring + 1 == W ? 0 : g_ring + 1;
+        if (++gr == n2) { gr = 0; ++gp; }
+      };
+#pragma unroll
+      for (int i = 0; i < kAhead; ++i) issue(i);
+      int cP3 = 0;
+      while (f < total) {
+#pragma unroll
+        for (int i = 0; i < kAhead; ++i) {
+          if (f < total) {
+            RingRow<T, R> xc = xs[i], pc = ps[i];
+            int b = bs[i];
+            const bool had = pre[i];
+            issue(i);
+            if (!had) {
+              int xi = cur_ring - (n2 + p - 1);
               xi += xi < 0 ? W : 0;
-              if (act) xn.load(ring + (size_t)xi * RS + lane * R);
-              if (rn >= 1) {
-                int pi = nxt - (pn_p + 1);
+              if (act) xc.load(ring + (size_t)xi * RS + lane * R);
+              if (r >= 1) {
+                int pi = cur_ring - (p + 1);
                 pi += pi < 0 ? W : 0;
-                if (act) pn.load(ring + (size_t)pi * RS + lane * R);
+                if (act) pc.load(ring + (size_t)pi * RS + lane * R);
               }
+              b = s2[r];
+              // complete these loads here, so that the rows computed from
+              // prefetched slots do not wait for the loads just issued
+              __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
             }
-          }
+            b = __builtin_amdgcn_readfirstlane(b) & 31;
 #pragma unroll
-          for (int j = 0; j < R; ++j) {
-            D[j] = U[j];
-            U[j] = xc.get(j);
-          }
-          if (r == 0) {
-            cube_row<R, 1>(U, U, U, G3s, yo, s_tab + gap * 32, cP3, 0, 0, mm, codes);
-          } else {
-            const int4 *tab = s_tab + b * 32;
-            const int4 c0 = tab[s30];
-            const int cP1 = __builtin_amdgcn_readfirstlane(c0.x) - (kS1 - kP1);
-            const int cP2 = __builtin_amdgcn_readfirstlane(c0.y) - (kS2 - kP2);
-            int P[R];
+            for (int j = 0; j < R; ++j) {
+              D[j] = U[j];
+              U[j] = xc.get(j);
+            }
+            if (r == 0) {
+              // plane table: cost(a, b, y) and cost(a, gap, y); all loads first
+              const int a = __builtin_amdgcn_readfirstlane((int)(s1[p] & 31));
+              int v[16];
 #pragma unroll
-            for (int j = 0; j < R; ++j) P[j] = pc.get(j);
-            cube_row<R, 2>(U, D, P, G3s, yo, tab, cP3, cP1, cP2, mm, codes);
-          }
-          cur = out_ring;
-          if (act) {
-            store_ring<T, R>(ring + (size_t)cur * RS + lane * R, mm);
-            store_codes<R>(dm + ((size_t)p * n2 + r) * dstride + lane * (R / 2), codes);
+              for (int q = 0; q < 16; ++q) v[q] = tcm->cost[a * 1024 + q * 64 + lane];
+              const int z = (tcm->cost[(a * 32 + gap) * 32 + (lane & 31)] << 3) | kS3;
+#pragma unroll
+              for (int q = 0; q < 16; ++q) {
+                s_tab[q * 64 + lane].y = (v[q] << 3) | kS2;
+                s_tab[q * 64 + lane].z = z;
+              }
+              c_lds_sync();
+              cP3 = __builtin_amdgcn_readfirstlane(s_tab[gap * 32 + s30].z) - (kS3 - kP3);
+              cube_row<R, 1>(U, U, U, G3s, yo, s_tab + gap * 32, cP3, 0, 0, mm, codes);
+            } else {
+              const int4 *tab = s_tab + b * 32;
+              const int4 c0 = tab[s30];
+              const int cP1 = __builtin_amdgcn_readfirstlane(c0.x) - (kS1 - kP1);
+              const int cP2 = __builtin_amdgcn_readfirstlane(c0.y) - (kS2 - kP2);
+              int P[R];
+#pragma unroll
+              for (int j = 0; j < R; ++j) P[j] = pc.get(j);
+              cube_row<R, 2>(U, D, P, G3s, yo, tab, cP3, cP1, cP2, mm, codes);
+            }
+            if (act) {
+              store_ring<T, R>(ring + (size_t)cur_ring * RS + lane * R, mm);
+              store_codes<R>(dm + (size_t)f * dstride + lane * (R / 2), codes);
+            }
+            ++f;
+            cur_ring = cur_ring + 1 == W ? 0 : cur_ring + 1;
+            if (++r == n2) { r = 0; ++p; }
           }
         }
       }
