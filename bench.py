@@ -114,9 +114,9 @@ def cpu_baseline(w, kind: str, budget_s: float):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="affine", choices=["affine", "linear", "cube"])
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 10; tree: 1)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 2; tree: 0)")
+    ap.add_argument("--workload", default="affine", choices=["affine", "linear", "cube", "tree"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -129,6 +129,13 @@ def main() -> int:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               f"(pairs are independent: N GPUs would each take 1/N of the batch)", file=sys.stderr)
         return 2
+    tree = args.workload == "tree"
+    if args.steps is None:
+        args.steps = 1 if tree else 10
+    if args.warmup is None:
+        args.warmup = 0 if tree else 2
+    if tree:
+        return bench_tree(args)
 
     from poyd_amd import engine
     w, kind = workload(args.workload)
@@ -251,6 +258,177 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     plan.close()
     return 0
+
+
+def bench_tree(args) -> int:
+    """Tree workload: 128-taxon Yule tree data (5 DNA loci of 300-1,500 bp,
+    10,000 nonadditive characters); one step = a Wagner build in taxon order
+    (every taxon tried on every edge: directional medians, edge medians and
+    join distances, each tree level one batch on the GPU) plus the final
+    tree's diagnosis (medians, single assignment, Adjusted cost, Fitch
+    length).  value = band-GCUPS over every alignment of the step."""
+    from poyd_amd import engine, synth, treebuild, treepass
+    w = synth.workload_c4()
+    ds = treebuild.dataset_of(w)
+    ctx = engine.Context(0)
+    be = treepass.EngineBackend(ctx, engine.CostMatrix.of_sub_indel(1, 2))
+
+    def step():
+        t0 = time.perf_counter()
+        def progress(x):
+            if x % 32 == 0:
+                print(f"[tree] {x} of {ds.n_taxa} taxa joined, {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
+
+        r = treebuild.wagner(ds, be, progress)
+        t1 = time.perf_counter()
+        d = treepass.TreeEval(ds, r.tree, be).diagnose()
+        return r, d, t1 - t0, time.perf_counter() - t1
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    c0, f0 = be.cells, be.fitch_chars
+    costs, b_s, d_s = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r, d, tb, td = step()
+        costs.append((d.cost, d.nonadd))
+        b_s.append(tb)
+        d_s.append(td)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    cells = (be.cells - c0) / args.steps
+    fitch = (be.fitch_chars - f0) / args.steps
+    gcups = cells / (ms / 1e3) / 1e9
+    ops = OPS_PER_CELL["linear"]
+    valu = cells * ops / (ms / 1e3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": w.name,
+            "taxa": ds.n_taxa,
+            "loci": len(ds.chars),
+            "nonadditive_chars": int(ds.nonadd.shape[1]),
+            "cost_model": "linear sub 1 indel 2; nonadditive Fitch, weight 1",
+            "step": "Wagner build in taxon order + diagnosis of the built tree",
+            "tree_cost": int(costs[-1][0]),
+            "nonadditive_cost": int(costs[-1][1]),
+            "band_cells_per_step": int(cells),
+            "fitch_char_medians_per_step": int(fitch),
+            "build_ms": round(1e3 * float(np.mean(b_s)), 1),
+            "diagnosis_ms": round(1e3 * float(np.mean(d_s)), 1),
+            "deterministic": len(set(costs)) == 1,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(valu, 4),
+            "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "TOP/s (int32)",
+            "frac": round(valu / VALU_PEAK_TOPS, 6),
+            "traffic": None,
+            "kernel": "whole step (one plan per tree level: small batches, host-bound)",
+        },
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline_tree(ds, r.tree, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["config"]["gpu_vs_cpu_1core"] = round(gcups / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline_tree(ds, tree, budget_s: float):
+    """The oracle on one pinned core: the alignments of the built tree's
+    diagnosis (directional and edge medians, single assignment, distances),
+    in the same order, until budget_s is spent; band cells per second."""
+    from oracle import pyoracle as po
+    from poyd_amd import treepass
+    cm = po.CM2.sub_indel(1, 2)
+
+    class Stop(Exception):
+        pass
+
+    class CpuBackend:
+        def __init__(self):
+            self.cells, self.t0 = 0, time.perf_counter()
+
+        def _tick(self):
+            if time.perf_counter() - self.t0 > budget_s:
+                raise Stop
+
+        def cost_table(self):
+            return np.array([[cm.cost(a, b) for b in range(32)] for a in range(32)], np.int64)
+
+        def _cells(self, a, b, dw=None):
+            l1, l2 = max(len(a), len(b)), min(len(a), len(b))
+            return po.algn_nw_cells(l1, l2, po.deltaw(a, b, cm, dw))
+
+        def median(self, pairs):
+            out = []
+            for a, b in pairs:
+                self._tick()
+                cost, oa, ob = po.align_2(a, b, cm)
+                self.cells += self._cells(a, b)
+                out.append((po.ancestor_2(oa, ob, cm), cost))
+            return out
+
+        def align(self, pairs):
+            out = []
+            for a, b in pairs:
+                self._tick()
+                cost, oa, ob = po.align_2(a, b, cm)
+                self.cells += self._cells(a, b)
+                out.append((oa, ob))
+            return out
+
+        def distance(self, pairs):
+            out = []
+            for a, b in pairs:
+                self._tick()
+                dw = max(8, abs(len(a) - len(b)))
+                out.append(po.cost_2(a, b, cm, dw))
+                self.cells += self._cells(a, b, dw)
+            return out
+
+        def fitch(self, a, b):
+            m, c = zip(*[po.nonadd_median(x, y) for x, y in zip(a, b)])
+            return np.array(m, np.uint8), np.array(c, np.int64)
+
+    try:
+        old_aff = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old_aff)})
+    except (AttributeError, OSError):
+        old_aff = None
+    be = CpuBackend()
+    done = "whole diagnosis"
+    try:
+        treepass.TreeEval(ds, tree, be).diagnose()
+    except Stop:
+        done = "part of the diagnosis"
+    dt = time.perf_counter() - be.t0
+    if old_aff is not None:
+        os.sched_setaffinity(0, old_aff)
+    return {
+        "value": be.cells / dt / 1e9,
+        "unit": "GCUPS",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} of the built {ds.n_taxa}-taxon tree on the oracle: {be.cells} band cells in "
+                  f"{dt:.1f} s, 1 pinned core of {cpu_model()}",
+    }
 
 
 def measured_traffic(kind: str):

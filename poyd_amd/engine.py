@@ -365,6 +365,7 @@ class PairResult:
     out_off: np.ndarray | None
     cap: np.ndarray | None
     slots: int
+    cells: int = 0          # DP cells the plan evaluated
 
     def seq(self, k: int, slot: int) -> np.ndarray:
         cap = int(self.cap[k])
@@ -456,6 +457,28 @@ class TriplePlan(PairPlan):
         self.cap = ((l1.astype(np.int64) + l2 + l3 + 1 + 15) // 16) * 16
 
 
+def fitch_pairs(ctx: Context, a: np.ndarray, b: np.ndarray):
+    """nonadd_median (src/nonaddCSc.c:422-462) of row pairs on the GPU: a and
+    b are (n, len) u8 state sets; returns (medians (n, len), changes (n,))."""
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    n, m = a.shape
+    if n == 0:
+        return np.zeros((0, m), np.uint8), np.zeros(0, np.int64)
+    d_chars = DeviceBuffer.from_array(ctx, np.concatenate([a, b, np.zeros_like(a)]))
+    idx = np.arange(3 * n, dtype=np.int32)
+    d_ia = DeviceBuffer.from_array(ctx, idx[:n])
+    d_ib = DeviceBuffer.from_array(ctx, idx[n:2 * n])
+    d_ir = DeviceBuffer.from_array(ctx, idx[2 * n:])
+    d_add = DeviceBuffer.from_array(ctx, np.zeros(n, np.int64))
+    _check(load().poyg_fitch_down(ctx.h, d_chars.ptr, m, n, d_ia.ptr, d_ib.ptr, d_ir.ptr, d_add.ptr, None),
+           "poyg_fitch_down")
+    res = np.empty((n, m), np.uint8)
+    _check(load().poyg_dev_download(ctx.h, res.ctypes.data, d_chars.ptr + 2 * n * m, res.nbytes),
+           "poyg_dev_download")
+    return res, d_add.download(np.int64, n)
+
+
 def batch_deltaw(a_batch, b_batch) -> np.ndarray:
     """deltaw of Sequence.Align.cost_2 for every pair, vectorised."""
     la = a_batch.ln.astype(np.int64)
@@ -493,6 +516,6 @@ def run_pairs(ctx: Context, cm: CostMatrix, mode: int, a_batch, b_batch, dws=Non
     res = PairResult(d_cost.download(np.int32, plan.n),
                      d_out.download(np.uint8, plan.out_bytes) if d_out else None,
                      d_len.download(np.int32, plan.n * plan.slots) if d_len else None,
-                     plan.out_off, plan.cap, plan.slots)
+                     plan.out_off, plan.cap, plan.slots, int(plan.cells))
     plan.close()
     return res

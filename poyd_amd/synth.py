@@ -173,3 +173,92 @@ def workload_c2(n: int = 65536) -> PairWorkload:
 def workload_c3(n: int = 4096):
     """4,096 triples, L ~ U{100..500}, 15% substitutions, 3% indels."""
     return triples_uniform(n, 100, 500, 0.15, 0.03, SEED0 + 2, "three-way-4096x100-500")
+
+
+# ---------------------------------------------------------------- trees
+@dataclass
+class TreeWorkload:
+    """A simulated dataset: taxon codes 1..N, `loci[c][t-1]` the DNA
+    sequence of taxon t for locus c (leading gap included), `nonadd[t-1]` its
+    nonadditive characters as one-bit state sets (u8), and the simulated
+    tree (for reference only: the build starts from the data)."""
+
+    names: list
+    loci: list
+    nonadd: np.ndarray
+    states: np.ndarray     # number of states of each nonadditive character
+    name: str
+
+
+def yule_tree(rng: np.random.Generator, n_taxa: int, mean_bl: float):
+    """A Yule (pure-birth) rooted binary tree: start from two lineages below
+    the root, split a uniformly chosen extant lineage until there are n_taxa.
+    Returns (parent, bl, leaf_nodes): node 0 is the root, parent[v] < v,
+    bl[v] the length of the edge above v (Exp with mean mean_bl)."""
+    parent = [-1, 0, 0]
+    extant = [1, 2]
+    while len(extant) < n_taxa:
+        k = int(rng.integers(len(extant)))
+        v = extant[k]
+        a, b = len(parent), len(parent) + 1
+        parent += [v, v]
+        extant[k] = a
+        extant.append(b)
+    parent = np.array(parent, np.int64)
+    bl = rng.exponential(mean_bl, size=len(parent))
+    bl[0] = 0.0
+    return parent, bl, np.array(extant, np.int64)
+
+
+def evolve_tree_seqs(rng, parent, bl, root_len):
+    """Evolve one locus down a rooted tree (nodes in parent-before-child
+    order) with the substitution/indel model, p_sub = bl, p_indel = bl / 5."""
+    n = len(parent)
+    seqs = [None] * n
+    seqs[0] = random_roots(rng, np.array([root_len]))
+    for v in range(1, n):
+        par = seqs[parent[v]]
+        mut, ml = mutate(rng, par, np.array([len(par)]), np.array([bl[v]]), np.array([bl[v] / 5.0]))
+        seqs[v] = mut
+    return seqs
+
+
+def evolve_tree_mk(rng, parent, bl, states):
+    """Mk evolution of characters with k states (k >= 2) down the tree: on an
+    edge of length t a character changes with probability
+    (k-1)/k * (1 - exp(-k t / (k-1))), to a uniform other state."""
+    n, m = len(parent), len(states)
+    out = np.zeros((n, m), np.int64)
+    out[0] = rng.integers(0, states)
+    k = states.astype(np.float64)
+    for v in range(1, n):
+        pc = (k - 1) / k * (1.0 - np.exp(-k * bl[v] / (k - 1)))
+        ch = rng.random(m) < pc
+        shift = rng.integers(1, states)
+        out[v] = np.where(ch, (out[parent[v]] + shift) % states, out[parent[v]])
+    return out
+
+
+def tree_workload(n_taxa=128, n_loci=5, lmin=300, lmax=1500, n_nonadd=10000, kmin=2, kmax=8,
+                  mean_bl=0.05, seed=SEED0 + 3, name=None) -> TreeWorkload:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    parent, bl, leaves = yule_tree(rng, n_taxa, mean_bl)
+    # taxon codes: a seeded permutation of the leaves (the addition order of the build)
+    leaves = leaves[rng.permutation(len(leaves))]
+    loci = []
+    for _ in range(n_loci):
+        L = int(rng.integers(lmin, lmax + 1))
+        seqs = evolve_tree_seqs(rng, parent, bl, L)
+        loci.append([np.concatenate([[GAP], BASES[seqs[v]]]).astype(np.uint8) for v in leaves])
+    states = rng.integers(kmin, kmax + 1, size=n_nonadd)
+    mk = evolve_tree_mk(rng, parent, bl, states)
+    nonadd = (np.left_shift(1, mk[leaves])).astype(np.uint8)
+    names = [f"t{k + 1}" for k in range(n_taxa)]
+    return TreeWorkload(names, loci, nonadd, states,
+                        name or f"tree-{n_taxa}taxa-{n_loci}loci-{lmin}-{lmax}-nonadd{n_nonadd}")
+
+
+def workload_c4() -> TreeWorkload:
+    """128-taxon Yule tree (branch lengths Exp, mean 0.05), 5 DNA loci of
+    300-1,500 bp, 10,000 nonadditive characters of 2-8 states."""
+    return tree_workload()

@@ -1,0 +1,111 @@
+"""Wagner build over dynamic and nonadditive characters: taxa join the tree
+in code order, each at the edge whose join cost is least.
+
+Reference behaviour followed:
+
+* the first two taxa form one edge, the third joins it, and every later
+  taxon is tried on every edge of the current tree, keeping the best tree
+  (Ptree.make_wagner_tree and its add_node_everywhere / seq_add,
+  src/ptree.ml:948-1060, with the ordered sequence of taxa);
+* a candidate is scored by the join cost (AllDirChar.cost_fn,
+  src/allDirChar.ml:1279-1332): the distance between the taxon and the
+  edge's lazy median -- the median of the two directional medians across
+  the edge (create_lazy_edge, src/allDirChar.ml:58-73) -- per dynamic
+  character DOS.distance (src/seqCS.ml:818-864) and per nonadditive character
+  the count of empty intersections (nonadd_median's changes,
+  src/nonaddCSc.c:422-462);
+* the least total wins; ties go to the first edge in pre-order from the
+  handle (the search manager keeps one best tree).
+
+The order among tied joins (the search manager's edge order and its
+tie-keeping) is not pinned by any reference fixture; the build is
+deterministic and identical between the GPU backend and the CPU oracle
+backend.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .treedata import Dataset, Tree
+from .treepass import TreeEval, is_empty
+
+
+@dataclass
+class BuildResult:
+    tree: Tree
+    join_costs: list          # least join cost at each addition
+    alignments: int           # pairwise alignments issued during the build
+
+
+def start_tree(n_taxa: int) -> Tree:
+    """Taxa 1, 2, 3 around the first interior vertex n_taxa + 1."""
+    c = n_taxa + 1
+    t = Tree()
+    t.nbrs = {1: [c], 2: [c], 3: [c], c: [1, 2, 3]}
+    t.leaves = {1, 2, 3}
+    t.handle = 1
+    return t
+
+
+def join(t: Tree, edge: tuple[int, int], x: int, v: int) -> None:
+    """Join leaf x to edge (a, b) (oriented away from the handle) through the
+    new interior vertex v."""
+    a, b = edge
+    t.nbrs[a][t.nbrs[a].index(b)] = v
+    t.nbrs[b][t.nbrs[b].index(a)] = v
+    t.nbrs[v] = [a, b, x]
+    t.nbrs[x] = [v]
+    t.leaves.add(x)
+
+
+def join_costs(ev: TreeEval, x: int, edges, emed) -> np.ndarray:
+    """Distance of leaf x to every edge median (dynamic plus nonadditive)."""
+    ds, be = ev.ds, ev.be
+    total = np.zeros(len(edges), np.int64)
+    pairs, where = [], []
+    for k, e in enumerate(edges):
+        for c in range(len(ds.chars)):
+            a, b = ds.chars[c][x - 1], emed[k].seqs[c]
+            if is_empty(a) or is_empty(b):
+                continue
+            pairs.append((a, b))
+            where.append(k)
+    if pairs:
+        d = be.distance(pairs)
+        ev.n_align += len(pairs)
+        np.add.at(total, np.array(where), np.array(d, np.int64))
+    if ds.nonadd is not None and edges:
+        leaf = np.broadcast_to(ds.nonadd[x - 1], (len(edges), ds.nonadd.shape[1]))
+        _, changes = be.fitch(np.ascontiguousarray(leaf), np.stack([m.na for m in emed]))
+        total += np.asarray(changes, np.int64)
+    return total
+
+
+def wagner(ds: Dataset, backend, progress=None) -> BuildResult:
+    n = ds.n_taxa
+    if n < 3:
+        raise ValueError("a Wagner build needs at least three taxa")
+    t = start_tree(n)
+    nxt = n + 2
+    costs, n_align = [], 0
+    for x in range(4, n + 1):
+        ev = TreeEval(ds, t, backend)
+        d = ev.directional()
+        edges = t.pre_order_edges()
+        emed = ev._medians([(d[(b, a)], d[(a, b)]) for (a, b) in edges])
+        jc = join_costs(ev, x, edges, emed)
+        k = int(np.argmin(jc))           # first least
+        join(t, edges[k], x, nxt)
+        nxt += 1
+        costs.append(int(jc[k]))
+        n_align += ev.n_align
+        if progress is not None:
+            progress(x)
+    return BuildResult(t, costs, n_align)
+
+
+def dataset_of(w) -> Dataset:
+    """A Dataset of a simulated tree workload (dynamic loci + nonadditive)."""
+    return Dataset(list(w.names), [list(l) for l in w.loci], w.nonadd)

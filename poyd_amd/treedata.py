@@ -58,6 +58,9 @@ class Dataset:
 
     names: list[str]
     chars: list[list[np.ndarray]]
+    # nonadditive characters (NonaddCS8): nonadd[t-1] holds taxon t's state
+    # sets, one bit per state, one byte per character
+    nonadd: np.ndarray | None = None
 
     @property
     def n_taxa(self) -> int:

@@ -28,6 +28,11 @@ What is computed (reference behaviour, restated):
   DOS.distance between the two singles (src/seqCS.ml:818-864: band
   deltaw = max(8, length difference) for linear models, cost_2_affine for
   affine ones; 0 when a side is empty).
+* Nonadditive characters (NonaddCS8): every directional median also takes
+  the Fitch median of its two sides (nonadd_median, src/nonaddCSc.c:422-462,
+  through Node.cs_median, src/node.ml:264-309), and its cost adds the
+  changes; the tree's nonadditive cost is the Fitch length, the cost of any
+  edge median (the same on every edge).
 
 The alignments go to a backend with three batched operations (median,
 align, distance).  EngineBackend runs them on the GPU through libpoyg.
@@ -86,15 +91,18 @@ class DirData:
     seqs: list
     cost: int
     mcc: int
+    na: np.ndarray | None = None   # nonadditive state sets (Fitch median)
+    na_cost: int = 0
 
 
 @dataclass
 class Diagnosis:
-    cost: int                 # Adjusted tree cost
+    cost: int                 # Adjusted tree cost (dynamic characters) plus the nonadditive cost
     downpass: int             # total downpass cost at the chosen root
     root: tuple[int, int]
     singles: dict
     alignments: int           # pairwise alignments issued
+    nonadd: int = 0           # Fitch length of the nonadditive characters
 
 
 class TreeEval:
@@ -130,12 +138,19 @@ class TreeEval:
             for (j, c), (m, cost) in zip(where, res):
                 out[j].seqs[c] = m
                 out[j].cost += int(cost)
+        if self.ds.nonadd is not None and jobs:
+            med, added = self.be.fitch(np.stack([x.na for x, _ in jobs]), np.stack([y.na for _, y in jobs]))
+            for j, (x, y) in enumerate(jobs):
+                out[j].na = med[j]
+                out[j].na_cost = x.na_cost + y.na_cost + int(added[j])
         return out
 
     def directional(self):
         """dir[(u, v)] = v not_with u, for every ordered adjacent pair."""
         t = self.t
-        leaf = {v: DirData([ch[v - 1] for ch in self.ds.chars], 0, v) for v in t.leaves}
+        na = self.ds.nonadd
+        leaf = {v: DirData([ch[v - 1] for ch in self.ds.chars], 0, v, None if na is None else na[v - 1])
+                for v in t.leaves}
         height = {}
 
         def h(u, v):
@@ -244,7 +259,7 @@ class TreeEval:
         if pairs:
             total = int(sum(int(x) for x in self.be.distance(pairs)))
             self.n_align += len(pairs)
-        return Diagnosis(total, int(best), root, singles, self.n_align)
+        return Diagnosis(total + rmed.na_cost, int(best), root, singles, self.n_align, rmed.na_cost)
 
 
 # ---------------------------------------------------------------- backends
@@ -257,6 +272,8 @@ class EngineBackend:
         self.engine = engine
         self.ctx, self.cm = ctx, cm
         self.affine = cm.affine
+        self.cells = 0          # DP cells of every alignment run so far
+        self.fitch_chars = 0    # nonadditive character medians computed so far
 
     def cost_table(self) -> np.ndarray:
         return np.array([[self.cm.cost(a, b) for b in range(32)] for a in range(32)], np.int64)
@@ -273,7 +290,9 @@ class EngineBackend:
     def _run(self, mode, pairs, dws=None):
         A = self._batch([p[0] for p in pairs])
         B = self._batch([p[1] for p in pairs])
-        return self.engine.run_pairs(self.ctx, self.cm, mode, A, B, dws)
+        r = self.engine.run_pairs(self.ctx, self.cm, mode, A, B, dws)
+        self.cells += r.cells
+        return r
 
     def median(self, pairs):
         e = self.engine
@@ -290,6 +309,10 @@ class EngineBackend:
             return [(r.seq(k, 2).copy(), r.seq(k, 3).copy()) for k in range(len(pairs))]
         r = self._run(e.POYG_MODE_LINEAR_ALIGN, pairs)
         return [(r.seq(k, 0).copy(), r.seq(k, 1).copy()) for k in range(len(pairs))]
+
+    def fitch(self, a, b):
+        self.fitch_chars += int(np.size(a))
+        return self.engine.fitch_pairs(self.ctx, a, b)
 
     def distance(self, pairs):
         e = self.engine

@@ -41,3 +41,24 @@ def test_gpu_tree_singles_match_oracle(ctx, script, name):
     for v in o.singles:
         for c in range(len(ds.chars)):
             assert np.array_equal(g.singles[v][c], o.singles[v][c]), (v, c)
+
+
+@pytest.mark.parametrize("n_taxa,n_loci,lmin,lmax,n_nonadd,seed", [
+    (10, 2, 40, 120, 64, 21),
+    (24, 3, 150, 400, 1000, 22),
+])
+def test_gpu_wagner_build_matches_oracle(ctx, n_taxa, n_loci, lmin, lmax, n_nonadd, seed):
+    from poyd_amd import engine, synth, treebuild
+    from oracle import pyoracle as po
+    print(f"wagner {n_taxa} taxa {n_loci} loci {n_nonadd} nonadditive", flush=True)
+    w = synth.tree_workload(n_taxa=n_taxa, n_loci=n_loci, lmin=lmin, lmax=lmax, n_nonadd=n_nonadd, seed=seed)
+    ds = treebuild.dataset_of(w)
+    gb = treepass.EngineBackend(ctx, engine.CostMatrix.of_sub_indel(1, 2))
+    ob = OracleBackend(po.CM2.sub_indel(1, 2), False)
+    g = treebuild.wagner(ds, gb)
+    o = treebuild.wagner(ds, ob)
+    assert g.join_costs == o.join_costs
+    assert g.tree.nbrs == o.tree.nbrs
+    gd = treepass.TreeEval(ds, g.tree, gb).diagnose()
+    od = treepass.TreeEval(ds, o.tree, ob).diagnose()
+    assert (gd.cost, gd.nonadd, gd.root) == (od.cost, od.nonadd, od.root)
