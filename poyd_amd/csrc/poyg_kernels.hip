@@ -93,6 +93,11 @@ __device__ __forceinline__ void wave_sync_mem() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// Ordering of a wave's global row stores before its loads of them on the
+// next row, for kernels with one wave per workgroup: workgroup scope needs
+// no L2 write-back or invalidate (an agent-scope fence would do both).
+__device__ __forceinline__ void wave_fence_global() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
 // Wave-uniform copies of loaded values.  A work item's descriptor is the
 // same in every lane; reading it through v_readfirstlane lets the compiler
 // keep it in scalar registers and turn every branch on it into a scalar
@@ -260,7 +265,7 @@ __device__ __forceinline__ void aff_fill_pair(const PairMeta &m, const uint8_t *
     cost_out[pidx] = lb == 0 ? 0 : go + G[lb];
     return;
   }
-  if (kGlobalRows) __threadfence();
+  if (kGlobalRows) wave_fence_global();
   else wave_sync_mem();
 
   int ic = A[0] & 31;
@@ -369,7 +374,7 @@ __device__ __forceinline__ void aff_fill_pair(const PairMeta &m, const uint8_t *
       R.at(cbuf, ST_CB, ep + 1) = kHigh;
       if (kBt) drow[ep + 1] = kAffRight;
     }
-    if (kGlobalRows) __threadfence();   // row buffers in global memory
+    if (kGlobalRows) wave_fence_global();   // row buffers in global memory
     else wave_sync_mem();
   }
 }
@@ -1564,6 +1569,8 @@ __global__ __launch_bounds__(kWave * 4) void aff_trace_wave_kernel(
 // ---------------------------------------------------------------------------
 constexpr int kLinRB = 512;
 constexpr int kLinWaves = 4;
+constexpr int kLinStage = 2048;   // longest s2 whose column data is staged in LDS
+constexpr int kLinWideRB = 4096;  // widest row the wide-band kernel keeps in LDS
 enum { LK_FIRST_TAIL = 0, LK_FIRST_COST = 1, LK_LEFT = 2 };
 enum { RK_RIGHT = 0, RK_LAST = 1 };
 
@@ -1626,11 +1633,15 @@ __global__ __launch_bounds__(kWave * 4) void lin_prep_kernel(const PairMeta *__r
   }
 }
 
-template <bool kDm, bool kGlobalRows>
+// kStaged: the columns' insert-chain prefix sums and s2 symbols are copied
+// to LDS (stage_gi, stage_x; n2 <= kLinStage) once per pair, so that a row
+// reads them from LDS rather than from global memory.
+template <bool kDm, bool kGlobalRows, bool kStaged = false>
 __device__ __forceinline__ void lin_fill_pair(const PairMeta &m, const uint8_t *__restrict__ seq, const int *s_cost,
                                               const int *s_prep, const int *s_tail, uint8_t *__restrict__ dm_base,
                                               int *aux, int32_t *__restrict__ cost_out, int pidx, int *rows,
-                                              int mask, int bstride) {
+                                              int mask, int bstride, int *stage_gi = nullptr,
+                                              uint8_t *stage_x = nullptr) {
   const int lane = lane_id();
   // the pair's geometry is wave-uniform (scalar registers, scalar branches)
   const uint8_t *s1 = seq + uniform64(m.off_r);   // the longer sequence (rows)
@@ -1641,6 +1652,17 @@ __device__ __forceinline__ void lin_fill_pair(const PairMeta &m, const uint8_t *
   int *GI = aux + uniform64(m.aux_off);
   int *GP = GI + N;
   uint8_t *dm = dm_base + uniform64(m.dm_off);
+  const int *gi_rd = GI;
+  const uint8_t *x_rd = s2;
+  if (kStaged) {
+    for (int j = lane; j < N; j += kWave) {
+      stage_gi[j] = GI[j];
+      stage_x[j] = s2[j];
+    }
+    wave_sync_mem();
+    gi_rd = stage_gi;
+    x_rd = stage_x;
+  }
   // row 0 (algn_fill_first_row, :584-607, or the first row of algn_fill_plane)
   const int hi0 = lin_row0_hi(N, cs, w);
   for (int j0 = 0; j0 <= hi0; j0 += kWave) {
@@ -1656,8 +1678,12 @@ __device__ __forceinline__ void lin_fill_pair(const PairMeta &m, const uint8_t *
     cost_out[pidx] = GP[N - 1];
     return;
   }
-  if (kGlobalRows) __threadfence();
+  if (kGlobalRows) wave_fence_global();
   else wave_sync_mem();
+  // the rows' symbols are loaded 64 at a time (lane k holds s1[i + k]) and
+  // read with readlane: a load consumed on every row would wait for the
+  // previous row's direction stores as well (one in-order counter)
+  int sblk = 0;
   for (int i = 1; i < M; ++i) {
     int lo, hi, lk, rk;
     lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
@@ -1667,7 +1693,12 @@ __device__ __forceinline__ void lin_fill_pair(const PairMeta &m, const uint8_t *
     rk = uniform32(rk);
     const int *prv = rows + ((i - 1) & 1) * bstride;
     int *cur = rows + (i & 1) * bstride;
-    const int si = s1[i] & 31;
+    if (((i - 1) & (kWave - 1)) == 0) {
+      const int ii = i + lane;
+      sblk = s1[min(ii, M - 1)];
+      __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0) here, once per 64 rows, not on every row
+    }
+    const int si = __builtin_amdgcn_readlane(sblk, (i - 1) & (kWave - 1)) & 31;
     const int c_del = s_cost[si * 32 + kGap];
     const int tail = s_tail[si];
     const int *crow = s_cost + si * 32;
@@ -1684,11 +1715,11 @@ __device__ __forceinline__ void lin_fill_pair(const PairMeta &m, const uint8_t *
       const bool has_ins = act && !(j == lo);
       const bool right = j == hi && rk == RK_RIGHT;
       const bool last = j == hi && rk == RK_LAST && j > 0;
-      const int GIraw = GI[jj];
+      const int GIraw = gi_rd[jj];
       const int GIj = act ? GIraw : 0;
-      const int x = s2[jj] & 31;
+      const int x = x_rd[jj] & 31;
       const int gj = s_cost[kGap * 32 + x];
-      const int p_left = prv[(jj - 1) & mask];   // unused (first) when jj == lo
+      const int p_left = prv[max(jj - 1, 0) & mask];   // unused (first) when jj == lo
       const int p_up = prv[jj & mask];
       const int p_first = prv[0 & mask];
       const int t3 = first ? kBig : p_left + crow[x];
@@ -1716,7 +1747,7 @@ __device__ __forceinline__ void lin_fill_pair(const PairMeta &m, const uint8_t *
       carry_mm = lane63(mm);
       carry_gi = lane63(GIj);
     }
-    if (kGlobalRows) __threadfence();
+    if (kGlobalRows) wave_fence_global();
     else wave_sync_mem();
   }
 }
@@ -1730,6 +1761,8 @@ __global__ __launch_bounds__(kWave * kLinWaves) void lin_fill_lds_kernel(
   __shared__ int s_prep[32];
   __shared__ int s_tail[32];
   __shared__ int s_rows[kLinWaves][2][kLinRB];
+  __shared__ int s_gi[kLinWaves][kLinStage];
+  __shared__ uint8_t s_x[kLinWaves][kLinStage];
   load_tables(tcm, s_cost, s_prep, nullptr);
   for (int k = threadIdx.x; k < 32; k += blockDim.x) s_tail[k] = tcm->tail[k];
   __syncthreads();
@@ -1739,11 +1772,18 @@ __global__ __launch_bounds__(kWave * kLinWaves) void lin_fill_lds_kernel(
     if (k >= n) break;
     const int p = uniform32(order[k]);
     const PairMeta m = uniform_meta(meta, p);
-    lin_fill_pair<kDm, false>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, &s_rows[wv][0][0],
-                       kLinRB - 1, kLinRB);
+    if (uniform32(m.lc) <= kLinStage)
+      lin_fill_pair<kDm, false, true>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, &s_rows[wv][0][0],
+                                      kLinRB - 1, kLinRB, &s_gi[wv][0], &s_x[wv][0]);
+    else
+      lin_fill_pair<kDm, false>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, &s_rows[wv][0][0],
+                                kLinRB - 1, kLinRB);
   }
 }
 
+// Bands wider than kLinRB: one wave per workgroup.  When a row fits
+// kLinWideRB columns, both rows and the staged column data live in dynamic
+// LDS (8 + 5 bytes per column); wider rows use global memory.
 template <bool kDm>
 __global__ __launch_bounds__(kWave) void lin_fill_glb_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
@@ -1752,16 +1792,22 @@ __global__ __launch_bounds__(kWave) void lin_fill_glb_kernel(
   __shared__ int s_cost[32 * 32];
   __shared__ int s_prep[32];
   __shared__ int s_tail[32];
+  extern __shared__ int s_dyn[];
   load_tables(tcm, s_cost, s_prep, nullptr);
   for (int k = threadIdx.x; k < 32; k += blockDim.x) s_tail[k] = tcm->tail[k];
   __syncthreads();
+  const bool lds_rows = row_width <= kLinWideRB;
   int *mine = rowscratch + (size_t)blockIdx.x * 2 * row_width;
   for (;;) {
     const int k = next_item(counter);
     if (k >= n) break;
     const int p = uniform32(order[k]);
     const PairMeta m = uniform_meta(meta, p);
-    lin_fill_pair<kDm, true>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, mine, -1, row_width);
+    if (lds_rows)
+      lin_fill_pair<kDm, false, true>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, s_dyn, -1, row_width,
+                                      s_dyn + 2 * row_width, (uint8_t *)(s_dyn + 3 * row_width));
+    else
+      lin_fill_pair<kDm, true>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, mine, -1, row_width);
   }
 }
 
@@ -2043,12 +2089,15 @@ hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *met
       hipLaunchKernelGGL(lin_fill_lds_kernel<false>, dim3(grid), dim3(kWave * kLinWaves), 0, s, meta, order, n,
                          counter, seq, tcm, dm, aux, cost);
   } else {
+    // rows (two of row_width ints), the prefix sums (row_width ints) and the
+    // symbols (row_width bytes); s2 is shorter than row_width
+    const size_t dyn = row_width <= kLinWideRB ? (size_t)row_width * 13 : 0;
     if (with_dm)
-      hipLaunchKernelGGL(lin_fill_glb_kernel<true>, dim3(grid), dim3(kWave), 0, s, meta, order, n, counter, seq,
+      hipLaunchKernelGGL(lin_fill_glb_kernel<true>, dim3(grid), dim3(kWave), dyn, s, meta, order, n, counter, seq,
                          tcm, dm, aux, cost, rowscratch, row_width);
     else
-      hipLaunchKernelGGL(lin_fill_glb_kernel<false>, dim3(grid), dim3(kWave), 0, s, meta, order, n, counter, seq,
-                         tcm, dm, aux, cost, rowscratch, row_width);
+      hipLaunchKernelGGL(lin_fill_glb_kernel<false>, dim3(grid), dim3(kWave), dyn, s, meta, order, n, counter,
+                         seq, tcm, dm, aux, cost, rowscratch, row_width);
   }
   return hipGetLastError();
 }
