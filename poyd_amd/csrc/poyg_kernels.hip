@@ -1570,6 +1570,11 @@ __global__ __launch_bounds__(kWave * 4) void aff_trace_wave_kernel(
 constexpr int kLinRB = 512;
 constexpr int kLinWaves = 4;
 constexpr int kLinStage = 2048;   // longest s2 whose column data is staged in LDS
+// LDS index of column j in the R-cells-per-lane rows: one pad word after every
+// 32, so that lanes 4 or 8 words apart fall in distinct banks
+__device__ __forceinline__ int lin_pad(int j) { return j + (j >> 5); }
+constexpr int kLinRBP = kLinRB + kLinRB / 32;         // padded ring row
+constexpr int kLinStageP = kLinStage + kLinStage / 32;
 constexpr int kLinWideRB = 4096;  // widest row the wide-band kernel keeps in LDS
 enum { LK_FIRST_TAIL = 0, LK_FIRST_COST = 1, LK_LEFT = 2 };
 enum { RK_RIGHT = 0, RK_LAST = 1 };
@@ -1752,6 +1757,144 @@ __device__ __forceinline__ void lin_fill_pair(const PairMeta &m, const uint8_t *
   }
 }
 
+// The row formulation with R consecutive cells per lane: cell j = lo + L*R + r
+// of lane L (bands of up to 64R columns), so a row is one pass with one
+// prefix minimum for the insert chain instead of one per 64 columns.  Rows
+// are LDS rings of kLinRB columns; the columns' prefix sums and symbols are
+// staged in LDS.  Same cell rules as lin_fill_pair (src/algn.c:375-620).
+template <bool kDm, int R>
+__device__ __forceinline__ void lin_fill_pair_r(const PairMeta &m, const uint8_t *__restrict__ seq,
+                                                const int *s_cost, const int *s_prep, const int *s_tail,
+                                                uint8_t *__restrict__ dm_base, int *aux,
+                                                int32_t *__restrict__ cost_out, int pidx, int *rows,
+                                                int *stage_gi, uint8_t *stage_x) {
+  static_assert(R % 4 == 0 && 64 * R <= kLinRB, "a row pass covers at most one ring");
+  constexpr int mask = kLinRB - 1;
+  const int lane = lane_id();
+  const uint8_t *s1 = seq + uniform64(m.off_r);
+  const uint8_t *s2 = seq + uniform64(m.off_c);
+  const int M = uniform32(m.lr), N = uniform32(m.lc), cs = uniform32(m.p0), w = uniform32(m.p1),
+            h = uniform32(m.p2), S = uniform32(m.stride);
+  pidx = uniform32(pidx);
+  int *GI = aux + uniform64(m.aux_off);
+  int *GP = GI + N;
+  uint8_t *dm = dm_base + uniform64(m.dm_off);
+  for (int j = lane; j < N; j += kWave) {
+    stage_gi[lin_pad(j)] = GI[j];
+    stage_x[j] = s2[j];
+  }
+  const int hi0 = lin_row0_hi(N, cs, w);
+  for (int j0 = 0; j0 <= hi0; j0 += kWave) {
+    const int j = j0 + lane;
+    if (j <= hi0) {
+      rows[lin_pad(j & mask)] = GP[j];
+      if (kDm) dm[j] = j == 0 ? 1 : 2;   // ALIGN, then INSERT
+    }
+  }
+  if (M == 1) {
+    cost_out[pidx] = GP[N - 1];
+    return;
+  }
+  wave_sync_mem();
+  int sblk = 0;
+  for (int i = 1; i < M; ++i) {
+    int lo, hi, lk, rk;
+    lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
+    lo = uniform32(lo);
+    hi = uniform32(hi);
+    lk = uniform32(lk);
+    rk = uniform32(rk);
+    const int *prv = rows + ((i - 1) & 1) * kLinRBP;
+    int *cur = rows + (i & 1) * kLinRBP;
+    if (((i - 1) & (kWave - 1)) == 0) {
+      sblk = s1[min(i + lane, M - 1)];
+      __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0) here, once per 64 rows
+    }
+    const int si = __builtin_amdgcn_readlane(sblk, (i - 1) & (kWave - 1)) & 31;
+    const int c_del = s_cost[si * 32 + kGap];
+    const int tail = s_tail[si];
+    const int *crow = s_cost + si * 32;
+    const int p_first = prv[0];
+    const int j0 = lo + lane * R;
+    // per cell: T (DELETE / ALIGN / last-column tail, or the first cell's
+    // value), t1, t3, cst and the chain key T - GI
+    int T[R], t1v[R], t3v[R], cstv[R], gjv[R], giv[R], wk[R];
+    int up_prev = prv[lin_pad(max(j0 - 1, 0) & mask)];
+    int run = kBig;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = j0 + r;
+      const bool act = j <= hi;
+      const int jj = min(j, hi);
+      const bool first = (j == lo) && lk != LK_LEFT;
+      const bool right = j == hi && rk == RK_RIGHT;
+      const bool last = j == hi && rk == RK_LAST && j > 0;
+      const int GIj = act ? stage_gi[lin_pad(jj)] : 0;
+      const int x = stage_x[jj] & 31;
+      gjv[r] = s_cost[kGap * 32 + x];
+      const int p_up = prv[lin_pad(jj & mask)];
+      const int p_left = up_prev;
+      up_prev = p_up;
+      const int t3 = first ? kBig : p_left + crow[x];
+      const int t1 = (first || right) ? kBig : p_up + c_del;
+      const int cst = (!first && last) ? tail + p_up : kBig;
+      int Tv = first ? p_first + (lk == LK_FIRST_TAIL ? tail : c_del) : min(min(t1, t3), cst);
+      Tv = act ? Tv : kBig;
+      T[r] = Tv;
+      t1v[r] = t1;
+      t3v[r] = t3;
+      cstv[r] = cst;
+      giv[r] = GIj;
+      wk[r] = act ? Tv - GIj : kBig;
+      run = min(run, wk[r]);
+    }
+    // insert chain: mm[j] = GI[j] + min over k <= j of (T[k] - GI[k])
+    const int incl = wave_prefix_min(run);
+    const int excl = shfl_up1(incl, kBig);
+    int pm = excl;
+    int mm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      pm = min(pm, wk[r]);
+      mm[r] = giv[r] + pm;
+    }
+    const int mm_in = shfl_up1(mm[R - 1], kBig);   // cell j0 - 1
+    uint32_t codes[R / 4];
+#pragma unroll
+    for (int q = 0; q < R / 4; ++q) codes[q] = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = j0 + r;
+      const bool act = j <= hi;
+      const bool first = (j == lo) && lk != LK_LEFT;
+      const bool has_ins = act && j != lo;
+      const int mm_left = r ? mm[r - 1] : mm_in;
+      const int t1 = t1v[r], t3 = t3v[r], cst = cstv[r];
+      const int t2 = has_ins ? mm_left + gjv[r] : kBig;
+      int best = min(min(t1, t2), t3);
+      int msk = (t3 == best ? 1 : 0) | (has_ins && t2 == best ? 2 : 0) | (t1 == best ? 4 : 0);
+      const bool cst_lt = cst < best, cst_eq = cst == best && cst != kBig;
+      best = cst_lt ? cst : best;
+      msk = cst_lt ? 4 : (cst_eq ? (msk | 4) : msk);
+      best = first ? T[r] : best;
+      msk = first ? 4 : msk;
+      codes[r >> 2] |= (uint32_t)(msk & 0xff) << (8 * (r & 3));
+      if (act) {
+        cur[lin_pad(j & mask)] = best;
+        if (i == M - 1 && j == N - 1) cost_out[pidx] = best;
+      }
+    }
+    if (kDm) {
+      // whole dwords of the row's stride (S covers hi - lo + 1 rounded to 4)
+      uint32_t *drow = (uint32_t *)(dm + (size_t)i * S);
+#pragma unroll
+      for (int q = 0; q < R / 4; ++q)
+        if (lane * R + 4 * q <= hi - lo) drow[(lane * R) / 4 + q] = codes[q];
+    }
+    wave_sync_mem();
+  }
+}
+
 template <bool kDm>
 __global__ __launch_bounds__(kWave * kLinWaves) void lin_fill_lds_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
@@ -1760,8 +1903,8 @@ __global__ __launch_bounds__(kWave * kLinWaves) void lin_fill_lds_kernel(
   __shared__ int s_cost[32 * 32];
   __shared__ int s_prep[32];
   __shared__ int s_tail[32];
-  __shared__ int s_rows[kLinWaves][2][kLinRB];
-  __shared__ int s_gi[kLinWaves][kLinStage];
+  __shared__ int s_rows[kLinWaves][2][kLinRBP];
+  __shared__ int s_gi[kLinWaves][kLinStageP];
   __shared__ uint8_t s_x[kLinWaves][kLinStage];
   load_tables(tcm, s_cost, s_prep, nullptr);
   for (int k = threadIdx.x; k < 32; k += blockDim.x) s_tail[k] = tcm->tail[k];
@@ -1772,12 +1915,21 @@ __global__ __launch_bounds__(kWave * kLinWaves) void lin_fill_lds_kernel(
     if (k >= n) break;
     const int p = uniform32(order[k]);
     const PairMeta m = uniform_meta(meta, p);
-    if (uniform32(m.lc) <= kLinStage)
+    // R cells per lane where they fill most lanes (the widest row in
+    // (48R, 64R]); 64-column chunks otherwise
+    const int wide = uniform32(m.stride);
+    if (uniform32(m.lc) <= kLinStage && wide > 192 && wide <= 256)
+      lin_fill_pair_r<kDm, 4>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, &s_rows[wv][0][0],
+                              &s_gi[wv][0], &s_x[wv][0]);
+    else if (uniform32(m.lc) <= kLinStage && wide > 384)
+      lin_fill_pair_r<kDm, 8>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, &s_rows[wv][0][0],
+                              &s_gi[wv][0], &s_x[wv][0]);
+    else if (uniform32(m.lc) <= kLinStage)
       lin_fill_pair<kDm, false, true>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, &s_rows[wv][0][0],
-                                      kLinRB - 1, kLinRB, &s_gi[wv][0], &s_x[wv][0]);
+                                      kLinRB - 1, kLinRBP, &s_gi[wv][0], &s_x[wv][0]);
     else
       lin_fill_pair<kDm, false>(m, seq, s_cost, s_prep, s_tail, dm, aux, cost_out, p, &s_rows[wv][0][0],
-                                kLinRB - 1, kLinRB);
+                                kLinRB - 1, kLinRBP);
   }
 }
 

@@ -37,6 +37,7 @@ class BuildResult:
     tree: Tree
     join_costs: list          # least join cost at each addition
     alignments: int           # pairwise alignments issued during the build
+    directional: dict         # the built tree's directional medians
 
 
 def start_tree(n_taxa: int) -> Tree:
@@ -83,6 +84,39 @@ def join_costs(ev: TreeEval, x: int, edges, emed) -> np.ndarray:
     return total
 
 
+def update_directional(ev: TreeEval, d: dict, v: int) -> None:
+    """Bring the directional medians d up to date after vertex v joined the
+    tree.  d[(u, w)] (w's side away from u) changes exactly when that side
+    holds v, i.e. for the neighbours u of w other than the one toward v; the
+    others describe unchanged subtrees and are kept.  The changed medians are
+    computed in order of distance from v, one batch per distance: each needs
+    the one toward v, computed one batch earlier."""
+    t = ev.t
+    toward = {v: None}
+    frontier = [v]
+    while frontier:
+        batch, nxt = [], []
+        for w in frontier:
+            if w in t.leaves:
+                continue
+            for u in t.nbrs[w]:
+                if u != toward[w]:
+                    batch.append((u, w))
+        if batch:
+            jobs = []
+            for (u, w) in batch:
+                y1, y2 = t.others(w, u)
+                jobs.append((d[(w, y1)], d[(w, y2)]))
+            for key, r in zip(batch, ev._medians(jobs)):
+                d[key] = r
+        for w in frontier:
+            for u in t.nbrs[w]:
+                if u not in toward:
+                    toward[u] = w
+                    nxt.append(u)
+        frontier = nxt
+
+
 def wagner(ds: Dataset, backend, progress=None) -> BuildResult:
     n = ds.n_taxa
     if n < 3:
@@ -90,20 +124,28 @@ def wagner(ds: Dataset, backend, progress=None) -> BuildResult:
     t = start_tree(n)
     nxt = n + 2
     costs, n_align = [], 0
+    ev = TreeEval(ds, t, backend)
+    d = ev.directional()
     for x in range(4, n + 1):
-        ev = TreeEval(ds, t, backend)
-        d = ev.directional()
+        ev.n_align = 0
         edges = t.pre_order_edges()
         emed = ev._medians([(d[(b, a)], d[(a, b)]) for (a, b) in edges])
         jc = join_costs(ev, x, edges, emed)
         k = int(np.argmin(jc))           # first least
-        join(t, edges[k], x, nxt)
+        a, b = edges[k]
+        v = nxt
+        join(t, edges[k], x, v)
         nxt += 1
+        # the joined edge's two directions now end at v; x is a leaf
+        d[(v, a)] = d.pop((b, a))
+        d[(v, b)] = d.pop((a, b))
+        d[(v, x)] = ev.leaf_data(x)
+        update_directional(ev, d, v)
         costs.append(int(jc[k]))
         n_align += ev.n_align
         if progress is not None:
             progress(x)
-    return BuildResult(t, costs, n_align)
+    return BuildResult(t, costs, n_align, d)
 
 
 def dataset_of(w) -> Dataset:

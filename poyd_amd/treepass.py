@@ -113,6 +113,10 @@ class TreeEval:
         self.closest = closest_table(backend.cost_table())
         self.n_align = 0
 
+    def leaf_data(self, v: int) -> DirData:
+        na = self.ds.nonadd
+        return DirData([ch[v - 1] for ch in self.ds.chars], 0, v, None if na is None else na[v - 1])
+
     # ---------------------------------------------------------- medians
     def _medians(self, jobs):
         """jobs: list of (x: DirData, y: DirData).  Returns DirData list."""
@@ -148,9 +152,7 @@ class TreeEval:
     def directional(self):
         """dir[(u, v)] = v not_with u, for every ordered adjacent pair."""
         t = self.t
-        na = self.ds.nonadd
-        leaf = {v: DirData([ch[v - 1] for ch in self.ds.chars], 0, v, None if na is None else na[v - 1])
-                for v in t.leaves}
+        leaf = {v: self.leaf_data(v) for v in t.leaves}
         height = {}
 
         def h(u, v):

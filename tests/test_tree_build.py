@@ -75,3 +75,20 @@ def test_simulator_shapes():
     # one state per taxon and character, within the character's states
     assert (np.bitwise_and(w.nonadd, w.nonadd - 1) == 0).all()
     assert (w.nonadd < (1 << w.states)[None, :]).all()
+
+
+def test_incremental_directional_medians_equal_a_full_pass():
+    # after every join only the directions whose side holds the new vertex
+    # are recomputed; the result must equal a full directional pass
+    w = synth.tree_workload(n_taxa=13, n_loci=2, lmin=40, lmax=110, n_nonadd=64, seed=15)
+    ds = treebuild.dataset_of(w)
+    be = OracleBackend(po.CM2.sub_indel(1, 2), False)
+    r = treebuild.wagner(ds, be)
+    full = treepass.TreeEval(ds, r.tree, be).directional()
+    assert set(full) == set(r.directional)
+    for k, x in full.items():
+        y = r.directional[k]
+        assert (x.cost, x.mcc, x.na_cost) == (y.cost, y.mcc, y.na_cost), k
+        assert np.array_equal(x.na, y.na), k
+        for c in range(len(ds.chars)):
+            assert np.array_equal(x.seqs[c], y.seqs[c]), (k, c)
