@@ -339,7 +339,8 @@ def bench_tree(args) -> int:
             "unit": "TOP/s (int32)",
             "frac": round(valu / VALU_PEAK_TOPS, 6),
             "traffic": None,
-            "kernel": "whole step (one plan per tree level: small batches, host-bound)",
+            "kernel": "whole step (one plan per tree level: a few pairs per launch, each launch as long as "
+                      "one wave takes for its longest pair)",
         },
     }
     if not args.no_cpu:
