@@ -97,6 +97,22 @@ void poyg_cm3_destroy(poyg_cm3 *cm3);
 int poyg_cm3_get_cost(const poyg_cm3 *cm3, int a, int b, int c);
 int poyg_cm3_get_median(const poyg_cm3 *cm3, int a, int b, int c);
 
+/* Custom-block wire format, so that a stub whose block holds a poyg_cm keeps
+ * poyd's Marshal bytes and checkpoints: the exact byte stream of
+ * cm_CAML_serialize / cm_CAML_deserialize (src/cm.c:860-932; identifier
+ * "http://www.amnh.org/poy/cost_matrix/two_dimensional.0.1") and of the
+ * three-dimensional pair (src/cm.c:890-952; ".../three_dimensional.0.1"):
+ * 4-byte big-endian header fields and tables, one byte per median.  The stub
+ * moves the buffer with caml_serialize_block_1 / caml_deserialize_block_1. */
+int64_t poyg_cm_serialized_size(const poyg_cm *cm);
+int poyg_cm_serialize(const poyg_cm *cm, uint8_t *buf, int64_t cap);
+int poyg_cm_deserialize(const uint8_t *buf, int64_t size, poyg_cm **out);
+int64_t poyg_cm3_serialized_size(const poyg_cm3 *cm3);
+int poyg_cm3_serialize(const poyg_cm3 *cm3, uint8_t *buf, int64_t cap);
+int poyg_cm3_deserialize(const uint8_t *buf, int64_t size, poyg_cm3 **out);
+/* cm_compare (src/cm.c:980-1010): cm_CAML_compare's order; 0 = same content. */
+int poyg_cm_compare(const poyg_cm *a, const poyg_cm *b);
+
 /* ---- per-call alignment entry points (host pointers) ------------------ */
 /* algn_CAML_simple_2 (src/algn.c:3408-3427): s1 must be at least as long
  * as s2; deltawh is the band parameter Sequence.Align.cost_2 computes. */

@@ -336,6 +336,138 @@ int poyg_cm3_of_two_dim(const poyg_cm *m, poyg_cm3 **out) {
 }
 
 void poyg_cm3_destroy(poyg_cm3 *c) { free(c); }
+
+// ---- custom-block wire format ---------------------------------------------
+// The byte stream the OCaml runtime produces for cm_CAML_serialize /
+// cm_CAML_serialize_3d (src/cm.c:909-952): caml_serialize_int_4 and
+// caml_serialize_block_4 store 4-byte big-endian values, block_1 raw bytes
+// (SEQT is unsigned char, src/seq.h:43-45).  A stub writes this buffer with
+// caml_serialize_block_1 and reads it back with caml_deserialize_block_1,
+// which gives the same bytes on the wire as the reference.
+static uint8_t *put_be32(uint8_t *p, int32_t v) {
+  const uint32_t u = (uint32_t)v;
+  p[0] = (uint8_t)(u >> 24); p[1] = (uint8_t)(u >> 16); p[2] = (uint8_t)(u >> 8); p[3] = (uint8_t)u;
+  return p + 4;
+}
+static const uint8_t *get_be32(const uint8_t *p, int32_t *v) {
+  *v = (int32_t)(((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | (uint32_t)p[3]);
+  return p + 4;
+}
+static int64_t cm2_len(int lcm) { return 2 * (int64_t(1) << lcm) * (int64_t(1) << lcm); }
+static int64_t cm3_len(int lcm) { return (int64_t(1) << (lcm + 1)) * (int64_t(1) << (lcm + 1)) * (int64_t(1) << (lcm + 1)); }
+
+int64_t poyg_cm_serialized_size(const poyg_cm *c) {
+  return c ? 8 * 4 + cm2_len(c->lcm) * (4 + 1 + 4 + 4 + 4) : -1;
+}
+
+int poyg_cm_serialize(const poyg_cm *c, uint8_t *buf, int64_t cap) {
+  if (!c || !buf || cap < poyg_cm_serialized_size(c)) {
+    set_error("poyg_cm_serialize: buffer too small");
+    return POYG_EINVAL;
+  }
+  const int64_t len = cm2_len(c->lcm);
+  uint8_t *p = buf;
+  for (int v : {c->a_sz, c->lcm, c->gap, c->cost_model, c->combinations, c->gap_open, c->is_metric,
+                c->all_elements})
+    p = put_be32(p, v);
+  for (int64_t k = 0; k < len; ++k) p = put_be32(p, c->cost[k]);
+  for (int64_t k = 0; k < len; ++k) *p++ = c->median[k];
+  for (int64_t k = 0; k < len; ++k) p = put_be32(p, c->worst[k]);
+  for (int64_t k = 0; k < len; ++k) p = put_be32(p, c->prepend[k]);
+  for (int64_t k = 0; k < len; ++k) p = put_be32(p, c->tail[k]);
+  return POYG_OK;
+}
+
+// cm_CAML_deserialize (src/cm.c:860-888).
+int poyg_cm_deserialize(const uint8_t *buf, int64_t size, poyg_cm **out) {
+  if (!buf || !out || size < 8 * 4) {
+    set_error("poyg_cm_deserialize: truncated header");
+    return POYG_EINVAL;
+  }
+  int32_t h[8];
+  const uint8_t *p = buf;
+  for (int k = 0; k < 8; ++k) p = get_be32(p, &h[k]);
+  if (h[1] < 1 || h[1] > 5) {
+    set_error("poyg_cm_deserialize: lcm %d is not a 5-bit alphabet", h[1]);
+    return POYG_EUNSUPPORTED;
+  }
+  const int64_t len = cm2_len(h[1]);
+  if (size < 8 * 4 + len * 17) {
+    set_error("poyg_cm_deserialize: truncated tables");
+    return POYG_EINVAL;
+  }
+  poyg_cm *c = (poyg_cm *)calloc(1, sizeof(poyg_cm));
+  if (!c) return POYG_ENOMEM;
+  c->a_sz = h[0]; c->lcm = h[1]; c->gap = h[2]; c->cost_model = h[3];
+  c->combinations = h[4]; c->gap_open = h[5]; c->is_metric = h[6]; c->all_elements = h[7];
+  for (int64_t k = 0; k < len; ++k) p = get_be32(p, &c->cost[k]);
+  for (int64_t k = 0; k < len; ++k) c->median[k] = *p++;
+  for (int64_t k = 0; k < len; ++k) p = get_be32(p, &c->worst[k]);
+  for (int64_t k = 0; k < len; ++k) p = get_be32(p, &c->prepend[k]);
+  for (int64_t k = 0; k < len; ++k) p = get_be32(p, &c->tail[k]);
+  *out = c;
+  return POYG_OK;
+}
+
+int64_t poyg_cm3_serialized_size(const poyg_cm3 *c) { return c ? 7 * 4 + cm3_len(c->lcm) * (4 + 1) : -1; }
+
+int poyg_cm3_serialize(const poyg_cm3 *c, uint8_t *buf, int64_t cap) {
+  if (!c || !buf || cap < poyg_cm3_serialized_size(c)) {
+    set_error("poyg_cm3_serialize: buffer too small");
+    return POYG_EINVAL;
+  }
+  const int64_t len = cm3_len(c->lcm);
+  uint8_t *p = buf;
+  for (int v : {c->a_sz, c->lcm, c->gap, c->cost_model, c->combinations, c->gap_open, c->all_elements})
+    p = put_be32(p, v);
+  for (int64_t k = 0; k < len; ++k) p = put_be32(p, c->cost[k]);
+  for (int64_t k = 0; k < len; ++k) *p++ = c->median[k];
+  return POYG_OK;
+}
+
+// cm_CAML_deserialize_3d (src/cm.c:890-907).
+int poyg_cm3_deserialize(const uint8_t *buf, int64_t size, poyg_cm3 **out) {
+  if (!buf || !out || size < 7 * 4) {
+    set_error("poyg_cm3_deserialize: truncated header");
+    return POYG_EINVAL;
+  }
+  int32_t h[7];
+  const uint8_t *p = buf;
+  for (int k = 0; k < 7; ++k) p = get_be32(p, &h[k]);
+  if (h[1] < 1 || h[1] > 5) {
+    set_error("poyg_cm3_deserialize: lcm %d is not a 5-bit alphabet", h[1]);
+    return POYG_EUNSUPPORTED;
+  }
+  const int64_t len = cm3_len(h[1]);
+  if (size < 7 * 4 + len * 5) {
+    set_error("poyg_cm3_deserialize: truncated tables");
+    return POYG_EINVAL;
+  }
+  poyg_cm3 *c = (poyg_cm3 *)calloc(1, sizeof(poyg_cm3));
+  if (!c) return POYG_ENOMEM;
+  c->a_sz = h[0]; c->lcm = h[1]; c->gap = h[2]; c->cost_model = h[3];
+  c->combinations = h[4]; c->gap_open = h[5]; c->all_elements = h[6];
+  for (int64_t k = 0; k < len; ++k) p = get_be32(p, &c->cost[k]);
+  for (int64_t k = 0; k < len; ++k) c->median[k] = *p++;
+  *out = c;
+  return POYG_OK;
+}
+
+// cm_compare (src/cm.c:980-1010): the order of the custom block's compare,
+// and the content equality under which device copies may be shared.
+int poyg_cm_compare(const poyg_cm *a, const poyg_cm *b) {
+  if (a->a_sz != b->a_sz) return a->a_sz - b->a_sz;
+  if (a->combinations != b->combinations) return a->combinations - b->combinations;
+  if (a->cost_model != b->cost_model) return a->cost_model - b->cost_model;
+  if (a->gap_open != b->gap_open) return a->gap_open - b->gap_open;
+  if (a->is_metric != b->is_metric) return a->is_metric - b->is_metric;
+  const size_t len = (size_t)cm2_len(a->lcm);
+  int cmp = memcmp(a->cost, b->cost, len * sizeof(int));
+  if (cmp) return cmp;
+  cmp = memcmp(a->median, b->median, len);
+  if (cmp) return cmp;
+  return memcmp(a->worst, b->worst, len * sizeof(int));
+}
 int poyg_cm3_get_cost(const poyg_cm3 *c, int a, int b, int d) { return c->cost[(((a << c->lcm) + b) << c->lcm) + d]; }
 int poyg_cm3_get_median(const poyg_cm3 *c, int a, int b, int d) { return c->median[(((a << c->lcm) + b) << c->lcm) + d]; }
 

@@ -70,6 +70,13 @@ SIGNATURES = {
     "poyg_cm3_destroy": (None, [vp]),
     "poyg_cm3_get_cost": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
     "poyg_cm3_get_median": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "poyg_cm_serialized_size": (C.c_int64, [vp]),
+    "poyg_cm_serialize": (C.c_int, [vp, u8p, C.c_int64]),
+    "poyg_cm_deserialize": (C.c_int, [u8p, C.c_int64, C.POINTER(vp)]),
+    "poyg_cm3_serialized_size": (C.c_int64, [vp]),
+    "poyg_cm3_serialize": (C.c_int, [vp, u8p, C.c_int64]),
+    "poyg_cm3_deserialize": (C.c_int, [u8p, C.c_int64, C.POINTER(vp)]),
+    "poyg_cm_compare": (C.c_int, [vp, vp]),
     "poyg_algn_simple_2": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, C.c_int, i32p]),
     "poyg_algn_align_2d": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int, u8p, u8p,
                                      C.c_int, i32p, i32p]),
@@ -217,6 +224,23 @@ class CostMatrix:
                "poyg_cm_of_list")
         return cls(h)
 
+    def serialize(self) -> bytes:
+        """cm_CAML_serialize's byte stream (src/cm.c:909-932)."""
+        n = load().poyg_cm_serialized_size(self.h)
+        buf = np.zeros(n, np.uint8)
+        _check(load().poyg_cm_serialize(self.h, _p8(buf), n), "poyg_cm_serialize")
+        return buf.tobytes()
+
+    @classmethod
+    def deserialize(cls, data: bytes) -> "CostMatrix":
+        buf = np.frombuffer(data, np.uint8).copy()
+        h = vp()
+        _check(load().poyg_cm_deserialize(_p8(buf), len(buf), C.byref(h)), "poyg_cm_deserialize")
+        return cls(h)
+
+    def compare(self, other: "CostMatrix") -> int:
+        return load().poyg_cm_compare(self.h, other.h)
+
     def set_affine(self, model: int, gap_open: int) -> None:
         _check(load().poyg_cm_set_affine(self.h, model, gap_open), "poyg_cm_set_affine")
 
@@ -262,6 +286,22 @@ class CostMatrix3:
 
     def median(self, a, b, c):
         return load().poyg_cm3_get_median(self.h, a, b, c)
+
+    def serialize(self) -> bytes:
+        """cm_CAML_serialize_3d's byte stream (src/cm.c:934-952)."""
+        n = load().poyg_cm3_serialized_size(self.h)
+        buf = np.zeros(n, np.uint8)
+        _check(load().poyg_cm3_serialize(self.h, _p8(buf), n), "poyg_cm3_serialize")
+        return buf.tobytes()
+
+    @classmethod
+    def deserialize(cls, data: bytes) -> "CostMatrix3":
+        buf = np.frombuffer(data, np.uint8).copy()
+        h = vp()
+        _check(load().poyg_cm3_deserialize(_p8(buf), len(buf), C.byref(h)), "poyg_cm3_deserialize")
+        obj = cls.__new__(cls)
+        obj.h = h
+        return obj
 
     def __del__(self):
         try:

@@ -94,3 +94,70 @@ def test_host_column_rules_equal_oracle(oracle):
             assert v.value == oracle.verify_2(a, b, ref)
             L.poyg_algn_worst_2(cm.h, engine._p8(a), engine._p8(b), n, C.byref(v))
             assert v.value == oracle.worst_2(a, b, ref)
+
+
+def be32(b, off):
+    return int.from_bytes(b[off:off + 4], "big", signed=True)
+
+
+@pytest.mark.parametrize("sub,indel,go", [(1, 2, None), (1, 1, 3), (3, 1, 5)])
+def test_cm_wire_format(sub, indel, go):
+    # the byte layout of cm_CAML_serialize (src/cm.c:909-932): eight 4-byte
+    # big-endian fields, then cost, median (bytes), worst, prepend, tail over
+    # 2 * 32 * 32 entries indexed (a << 5) + b
+    from poyd_amd import engine
+    cm = engine.CostMatrix.of_sub_indel(sub, indel, go)
+    b = cm.serialize()
+    L = 2 * 32 * 32
+    assert len(b) == 32 + L * 17
+    lib = engine.load()
+    assert [be32(b, 4 * k) for k in range(8)][:3] == [31, 5, 16]
+    assert be32(b, 12) == lib.poyg_cm_get_affine(cm.h)
+    assert be32(b, 20) == lib.poyg_cm_get_gap_opening(cm.h)
+    for a in range(1, 32):
+        for c in range(1, 32):
+            k = (a << 5) + c
+            assert be32(b, 32 + 4 * k) == cm.cost(a, c)
+            assert b[32 + 4 * L + k] == cm.median(a, c)
+            assert be32(b, 32 + 5 * L + 4 * k) == cm.worst(a, c)
+        assert be32(b, 32 + 9 * L + 4 * a) == cm.prepend(a)
+        assert be32(b, 32 + 13 * L + 4 * a) == cm.tail(a)
+    back = engine.CostMatrix.deserialize(b)
+    assert back.compare(cm) == 0 and cm.compare(back) == 0
+    assert back.serialize() == b
+
+
+def test_cm_compare_order_and_errors():
+    from poyd_amd import engine
+    a = engine.CostMatrix.of_sub_indel(1, 1, 3)
+    b = engine.CostMatrix.of_sub_indel(1, 1, 5)
+    c = engine.CostMatrix.of_sub_indel(1, 2)
+    # fields first (gap_open, then the model), then the tables
+    assert a.compare(b) < 0 < b.compare(a)
+    assert a.compare(c) != 0 and (a.compare(c) > 0) == (c.compare(a) < 0)
+    d = engine.CostMatrix.of_sub_indel(1, 3)
+    assert (c.compare(d) < 0) == (c.cost(1, 16) < d.cost(1, 16))
+    blob = a.serialize()
+    with pytest.raises(engine.PoygError):
+        engine.CostMatrix.deserialize(blob[:100])
+    bad = bytearray(blob)
+    bad[4:8] = (7).to_bytes(4, "big")
+    with pytest.raises(engine.PoygError):
+        engine.CostMatrix.deserialize(bytes(bad))
+
+
+def test_cm3_wire_format():
+    # cm_CAML_serialize_3d (src/cm.c:934-952): seven fields, then cost and
+    # median over 64 * 64 * 64 entries indexed ((a << 5) + b) << 5 + c
+    from poyd_amd import engine
+    cm3 = engine.CostMatrix3(engine.CostMatrix.of_sub_indel(1, 2))
+    b = cm3.serialize()
+    L = 64 ** 3
+    assert len(b) == 28 + 5 * L
+    assert [be32(b, 4 * k) for k in range(3)] == [31, 5, 16]
+    for (x, y, z) in [(1, 2, 4), (16, 1, 1), (3, 5, 16), (31, 31, 31), (8, 8, 8)]:
+        k = (((x << 5) + y) << 5) + z
+        assert be32(b, 28 + 4 * k) == cm3.cost(x, y, z)
+        assert b[28 + 4 * L + k] == cm3.median(x, y, z)
+    back = engine.CostMatrix3.deserialize(b)
+    assert back.serialize() == b
