@@ -311,7 +311,7 @@ __global__ __launch_bounds__(64) void cube_fill_kernel(const TripleMeta *__restr
 // (checked on the host), else int32_t.
 constexpr int kBig3 = 1 << 28;
 constexpr int kTop3 = 0x7fffffff;
-constexpr int kAhead = 3;   // rows whose inputs are in flight
+constexpr int kAhead = 4;   // rows whose inputs are in flight
 
 __device__ __forceinline__ int c_shr1(int v, int fill) {
   return __builtin_amdgcn_update_dpp(fill, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
@@ -435,6 +435,76 @@ __device__ __forceinline__ void cube_row(const int (&U)[R], const int (&D)[R], c
   }
 }
 
+// Inclusive prefix minimum over the wave without the EXEC guard (for use
+// inside a block whose EXEC is unchanged since its one guard).
+__device__ __forceinline__ int c_prefix_min_ng(int v) {
+  constexpr int I = 0x7fffffff;
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x111, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x112, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x114, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x118, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x142, 0xa, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x143, 0xc, 0xf, false));
+  return v;
+}
+
+// Two consecutive rows of one plane (lines r and r+1 >= 1): independent of
+// each other (the lag puts every input of either at least two rows back),
+// computed in one block with one EXEC guard so that their dependent chains
+// interleave.  The second row's diagonal row is the first row's upper row.
+template <int R>
+__device__ __forceinline__ void cube_row_pair(const int (&U0)[R], const int (&D0)[R], const int (&P0)[R],
+                                              const int4 *tab0, int cP1a, int cP2a, const int (&U1)[R],
+                                              const int (&P1)[R], const int4 *tab1, int cP1b, int cP2b,
+                                              const int (&G3s)[R], const int (&yo)[R], int cP3,
+                                              int (&mm0)[R], uint32_t (&codes0)[(R + 7) / 8], int (&mm1)[R],
+                                              uint32_t (&codes1)[(R + 7) / 8]) {
+  constexpr int kNone = kBig3 >> 3;
+  c_dpp_guard();
+  const int Ul0 = c_shr1(U0[R - 1], kNone), Dl0 = c_shr1(D0[R - 1], kNone), Pl0 = c_shr1(P0[R - 1], kNone);
+  const int Ul1 = c_shr1(U1[R - 1], kNone), Pl1 = c_shr1(P1[R - 1], kNone);
+  int key0[R], key1[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int4 ta = tab0[yo[j]], tb = tab1[yo[j]];
+    const int ul0 = j ? U0[j - 1] : Ul0, dl0 = j ? D0[j - 1] : Dl0, pl0 = j ? P0[j - 1] : Pl0;
+    const int ul1 = j ? U1[j - 1] : Ul1, dl1 = ul0, pl1 = j ? P1[j - 1] : Pl1;
+    int k0 = min(min((U0[j] << 3) + cP3, (P0[j] << 3) + cP1a), (D0[j] << 3) + cP2a);
+    k0 = min(min(k0, (ul0 << 3) + ta.z), min((pl0 << 3) + ta.x, (dl0 << 3) + ta.y));
+    int k1 = min(min((U1[j] << 3) + cP3, (P1[j] << 3) + cP1b), (U0[j] << 3) + cP2b);
+    k1 = min(min(k1, (ul1 << 3) + tb.z), min((pl1 << 3) + tb.x, (dl1 << 3) + tb.y));
+    key0[j] = k0;
+    key1[j] = k1;
+  }
+  int wv0[R], lp0[R], wv1[R], lp1[R];
+  int run0 = kTop3, run1 = kTop3;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    wv0[j] = (key0[j] & ~7) - G3s[j];
+    wv1[j] = (key1[j] & ~7) - G3s[j];
+    run0 = min(run0, wv0[j]);
+    run1 = min(run1, wv1[j]);
+    lp0[j] = run0;
+    lp1[j] = run1;
+  }
+  const int excl0 = c_shr1(c_prefix_min_ng(run0), kTop3);
+  const int excl1 = c_shr1(c_prefix_min_ng(run1), kTop3);
+  int prev0 = excl0, prev1 = excl1;
+#pragma unroll
+  for (int j = 0; j < (R + 7) / 8; ++j) codes0[j] = codes1[j] = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int pm0 = min(excl0, lp0[j]), pm1 = min(excl1, lp1[j]);
+    const bool ss0 = prev0 < wv0[j], ss1 = prev1 < wv1[j];
+    mm0[j] = (G3s[j] + pm0) >> 3;
+    mm1[j] = (G3s[j] + pm1) >> 3;
+    codes0[j >> 3] |= (ss0 ? (uint32_t)kSS : (uint32_t)(key0[j] & 7)) << (4 * (j & 7));
+    codes1[j >> 3] |= (ss1 ? (uint32_t)kSS : (uint32_t)(key1[j] & 7)) << (4 * (j & 7));
+    prev0 = pm0;
+    prev1 = pm1;
+  }
+}
+
 template <typename T, int R>
 __global__ __launch_bounds__(64) void cube_rows_kernel(const TripleMeta *__restrict__ meta,
                                                        const int32_t *__restrict__ order, int n, int *counter,
@@ -554,7 +624,7 @@ __global__ __launch_bounds__(64) void cube_rows_kernel(const TripleMeta *__restr
         // loads of row g into slot i when allowed, then advance g
         pre[i] = false;
         bs[i] = s2[gr];   // gr < n2 on every call: loaded unconditionally, used kAhead rows later
-        if (g < total && gp >= kAhead && n2 + gp >= kAhead + 2) {
+        if (g < total && gp >= kAhead + 1 && n2 + gp >= kAhead + 3) {
           int xi = g_ring - (n2 + gp - 1);
           xi += xi < 0 ? W : 0;
           if (act) xs[i].load(ring + (size_t)xi * RS + lane * R);
@@ -572,66 +642,131 @@ __global__ __launch_bounds__(64) void cube_rows_kernel(const TripleMeta *__restr
 #pragma unroll
       for (int i = 0; i < kAhead; ++i) issue(i);
       int cP3 = 0;
+      // inputs of row (pp, rr) at ring row rg: the prefetched slot, else loaded now
+      auto inputs = [&](int i, int pp, int rr, int rg, RingRow<T, R> &xc, RingRow<T, R> &pc, int &b) {
+        xc = xs[i];
+        pc = ps[i];
+        b = bs[i];
+        return pre[i];
+      };
+      auto load_now = [&](int pp, int rr, int rg, RingRow<T, R> &xc, RingRow<T, R> &pc, int &b) {
+        int xi = rg - (n2 + pp - 1);
+        xi += xi < 0 ? W : 0;
+        if (act) xc.load(ring + (size_t)xi * RS + lane * R);
+        if (rr >= 1) {
+          int pi = rg - (pp + 1);
+          pi += pi < 0 ? W : 0;
+          if (act) pc.load(ring + (size_t)pi * RS + lane * R);
+        }
+        b = s2[rr];
+      };
+      auto store_row = [&](int rg, int ff) {
+        if (act) {
+          store_ring<T, R>(ring + (size_t)rg * RS + lane * R, mm);
+          store_codes<R>(dm + (size_t)ff * dstride + lane * (R / 2), codes);
+        }
+      };
+      auto advance = [&]() {
+        ++f;
+        cur_ring = cur_ring + 1 == W ? 0 : cur_ring + 1;
+        if (++r == n2) { r = 0; ++p; }
+      };
+      auto single = [&](int i) {
+        RingRow<T, R> xc, pc;
+        int b;
+        const bool had = inputs(i, p, r, cur_ring, xc, pc, b);
+        issue(i);
+        if (!had) {
+          load_now(p, r, cur_ring, xc, pc, b);
+          // complete these loads here, so that the rows computed from
+          // prefetched slots do not wait for the loads just issued
+          __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        }
+        b = __builtin_amdgcn_readfirstlane(b) & 31;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          D[j] = U[j];
+          U[j] = xc.get(j);
+        }
+        if (r == 0) {
+          // plane table: cost(a, b, y) and cost(a, gap, y); all loads first
+          const int a = __builtin_amdgcn_readfirstlane((int)(s1[p] & 31));
+          int v[16];
+#pragma unroll
+          for (int q = 0; q < 16; ++q) v[q] = tcm->cost[a * 1024 + q * 64 + lane];
+          const int z = (tcm->cost[(a * 32 + gap) * 32 + (lane & 31)] << 3) | kS3;
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            s_tab[q * 64 + lane].y = (v[q] << 3) | kS2;
+            s_tab[q * 64 + lane].z = z;
+          }
+          c_lds_sync();
+          cP3 = __builtin_amdgcn_readfirstlane(s_tab[gap * 32 + s30].z) - (kS3 - kP3);
+          cube_row<R, 1>(U, U, U, G3s, yo, s_tab + gap * 32, cP3, 0, 0, mm, codes);
+        } else {
+          const int4 *tab = s_tab + b * 32;
+          const int4 c0 = tab[s30];
+          const int cP1 = __builtin_amdgcn_readfirstlane(c0.x) - (kS1 - kP1);
+          const int cP2 = __builtin_amdgcn_readfirstlane(c0.y) - (kS2 - kP2);
+          int P[R];
+#pragma unroll
+          for (int j = 0; j < R; ++j) P[j] = pc.get(j);
+          cube_row<R, 2>(U, D, P, G3s, yo, tab, cP3, cP1, cP2, mm, codes);
+        }
+        store_row(cur_ring, f);
+        advance();
+      };
       while (f < total) {
 #pragma unroll
-        for (int i = 0; i < kAhead; ++i) {
-          if (f < total) {
-            RingRow<T, R> xc = xs[i], pc = ps[i];
-            int b = bs[i];
-            const bool had = pre[i];
+        for (int i = 0; i < kAhead; i += 2) {
+          if (f + 1 < total && r >= 1 && r + 1 < n2) {
+            // rows f and f+1: lines r and r+1 of plane p
+            RingRow<T, R> xa, pa, xb, pb;
+            int ba, bb;
+            const int ring_b = cur_ring + 1 == W ? 0 : cur_ring + 1;
+            const bool ha = inputs(i, p, r, cur_ring, xa, pa, ba);
+            const bool hb = inputs(i + 1, p, r + 1, ring_b, xb, pb, bb);
             issue(i);
-            if (!had) {
-              int xi = cur_ring - (n2 + p - 1);
-              xi += xi < 0 ? W : 0;
-              if (act) xc.load(ring + (size_t)xi * RS + lane * R);
-              if (r >= 1) {
-                int pi = cur_ring - (p + 1);
-                pi += pi < 0 ? W : 0;
-                if (act) pc.load(ring + (size_t)pi * RS + lane * R);
-              }
-              b = s2[r];
-              // complete these loads here, so that the rows computed from
-              // prefetched slots do not wait for the loads just issued
-              __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
-            }
-            b = __builtin_amdgcn_readfirstlane(b) & 31;
+            issue(i + 1);
+            if (!ha) load_now(p, r, cur_ring, xa, pa, ba);
+            if (!hb) load_now(p, r + 1, ring_b, xb, pb, bb);
+            if (!ha || !hb) __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+            ba = __builtin_amdgcn_readfirstlane(ba) & 31;
+            bb = __builtin_amdgcn_readfirstlane(bb) & 31;
+            int U1[R], P0[R], P1[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) {
               D[j] = U[j];
-              U[j] = xc.get(j);
+              U[j] = xa.get(j);
+              P0[j] = pa.get(j);
+              U1[j] = xb.get(j);
+              P1[j] = pb.get(j);
             }
-            if (r == 0) {
-              // plane table: cost(a, b, y) and cost(a, gap, y); all loads first
-              const int a = __builtin_amdgcn_readfirstlane((int)(s1[p] & 31));
-              int v[16];
+            const int4 *tab0 = s_tab + ba * 32, *tab1 = s_tab + bb * 32;
+            const int4 c0 = tab0[s30], c1 = tab1[s30];
+            const int cP1a = __builtin_amdgcn_readfirstlane(c0.x) - (kS1 - kP1);
+            const int cP2a = __builtin_amdgcn_readfirstlane(c0.y) - (kS2 - kP2);
+            const int cP1b = __builtin_amdgcn_readfirstlane(c1.x) - (kS1 - kP1);
+            const int cP2b = __builtin_amdgcn_readfirstlane(c1.y) - (kS2 - kP2);
+            int mm1[R];
+            uint32_t codes1[(R + 7) / 8];
+            cube_row_pair<R>(U, D, P0, tab0, cP1a, cP2a, U1, P1, tab1, cP1b, cP2b, G3s, yo, cP3, mm, codes, mm1,
+                             codes1);
+            store_row(cur_ring, f);
+            advance();
 #pragma unroll
-              for (int q = 0; q < 16; ++q) v[q] = tcm->cost[a * 1024 + q * 64 + lane];
-              const int z = (tcm->cost[(a * 32 + gap) * 32 + (lane & 31)] << 3) | kS3;
-#pragma unroll
-              for (int q = 0; q < 16; ++q) {
-                s_tab[q * 64 + lane].y = (v[q] << 3) | kS2;
-                s_tab[q * 64 + lane].z = z;
-              }
-              c_lds_sync();
-              cP3 = __builtin_amdgcn_readfirstlane(s_tab[gap * 32 + s30].z) - (kS3 - kP3);
-              cube_row<R, 1>(U, U, U, G3s, yo, s_tab + gap * 32, cP3, 0, 0, mm, codes);
-            } else {
-              const int4 *tab = s_tab + b * 32;
-              const int4 c0 = tab[s30];
-              const int cP1 = __builtin_amdgcn_readfirstlane(c0.x) - (kS1 - kP1);
-              const int cP2 = __builtin_amdgcn_readfirstlane(c0.y) - (kS2 - kP2);
-              int P[R];
-#pragma unroll
-              for (int j = 0; j < R; ++j) P[j] = pc.get(j);
-              cube_row<R, 2>(U, D, P, G3s, yo, tab, cP3, cP1, cP2, mm, codes);
+            for (int j = 0; j < R; ++j) {
+              D[j] = U[j];
+              U[j] = U1[j];
+              mm[j] = mm1[j];
             }
-            if (act) {
-              store_ring<T, R>(ring + (size_t)cur_ring * RS + lane * R, mm);
-              store_codes<R>(dm + (size_t)f * dstride + lane * (R / 2), codes);
-            }
-            ++f;
-            cur_ring = cur_ring + 1 == W ? 0 : cur_ring + 1;
-            if (++r == n2) { r = 0; ++p; }
+#pragma unroll
+            for (int q = 0; q < (R + 7) / 8; ++q) codes[q] = codes1[q];
+            store_row(cur_ring, f);
+            advance();
+          } else {
+            if (f < total) single(i);
+            if (f < total) single(i + 1);
           }
         }
       }
