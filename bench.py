@@ -7,7 +7,8 @@ pairs: cost, both aligned sequences, median and median-with-gaps, with the
 sequences resident in HBM before timing starts.  The metric is band-GCUPS:
 DP cells the reference algorithm evaluates (its band) per second.
 
-Prints one JSON line (see README of the bench contract in DESIGN.md).
+Prints one JSON line: the metric, its value and unit, the run shape, the
+workload, the fill stage's roofline and the single-core CPU baseline.
 """
 from __future__ import annotations
 
@@ -46,11 +47,11 @@ def cpu_model() -> str:
 def workload(name: str):
     from poyd_amd import synth
     if name == "affine":
-        return synth.workload_c2(), "affine"
+        return synth.workload_affine_65536(), "affine"
     if name == "linear":
-        return synth.workload_c1(), "linear"
+        return synth.workload_linear_1000(), "linear"
     if name == "cube":
-        return synth.workload_c3(), "cube"
+        return synth.workload_three_way_4096(), "cube"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -268,7 +269,7 @@ def bench_tree(args) -> int:
     tree's diagnosis (medians, single assignment, Adjusted cost, Fitch
     length).  value = band-GCUPS over every alignment of the step."""
     from poyd_amd import engine, synth, treebuild, treepass
-    w = synth.workload_c4()
+    w = synth.workload_tree_128()
     ds = treebuild.dataset_of(w)
     ctx = engine.Context(0)
     be = treepass.EngineBackend(ctx, engine.CostMatrix.of_sub_indel(1, 2))

@@ -1,8 +1,8 @@
 /*
  * poy_oracle.h -- CPU restatement of POY's direct-optimization hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine
- * in poyd_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * TEST INFRASTRUCTURE ONLY.  The HIP engine in poyd_amd/ is compared with
+ * this library: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it, and only to check or to time the reference
  * algorithm on the CPU.  The product path never calls it.
  *

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker of the HIP engine.  Every
+cpu_baseline leg of bench.py; the HIP engine's results are compared with it.  Every
 function here forwards to the C restatement in poy_oracle.c, which cites the
 reference lines it follows.
 """

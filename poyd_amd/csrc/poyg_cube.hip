@@ -598,11 +598,12 @@ __global__ __launch_bounds__(64) void cube_rows_kernel(const TripleMeta *__restr
     }
     if (n1 > 1) {
       // planes p >= 1 as one stream of rows f = p*n2 + r.  The inputs of row
-      // g are loaded kAhead rows early (register slot g % kAhead) when every
-      // row they read precedes the row being computed then, i.e. g - (p+1)
-      // and g - (n2+p-1) are at most g - kAhead - 1; other rows (the first
-      // planes) load their inputs when they are computed.  A wave's own
-      // stores are seen by its later loads in program order.
+      // g are loaded kAhead rows early (register slot (g - n2) % kAhead) when
+      // every row they read is stored by then even if two rows are computed
+      // together, i.e. g - (p+1) and g - (n2+p-1) are at most g - kAhead - 2;
+      // other rows (the first planes) load their inputs when they are
+      // computed.  A wave's own stores are seen by its later loads in program
+      // order.
       RingRow<T, R> xs[kAhead], ps[kAhead];
       int bs[kAhead];
       bool pre[kAhead];
@@ -642,8 +643,8 @@ __global__ __launch_bounds__(64) void cube_rows_kernel(const TripleMeta *__restr
 #pragma unroll
       for (int i = 0; i < kAhead; ++i) issue(i);
       int cP3 = 0;
-      // inputs of row (pp, rr) at ring row rg: the prefetched slot, else loaded now
-      auto inputs = [&](int i, int pp, int rr, int rg, RingRow<T, R> &xc, RingRow<T, R> &pc, int &b) {
+      // the contents of slot i; true when they hold the row's inputs
+      auto inputs = [&](int i, RingRow<T, R> &xc, RingRow<T, R> &pc, int &b) {
         xc = xs[i];
         pc = ps[i];
         b = bs[i];
@@ -674,7 +675,7 @@ __global__ __launch_bounds__(64) void cube_rows_kernel(const TripleMeta *__restr
       auto single = [&](int i) {
         RingRow<T, R> xc, pc;
         int b;
-        const bool had = inputs(i, p, r, cur_ring, xc, pc, b);
+        const bool had = inputs(i, xc, pc, b);
         issue(i);
         if (!had) {
           load_now(p, r, cur_ring, xc, pc, b);
@@ -724,8 +725,8 @@ __global__ __launch_bounds__(64) void cube_rows_kernel(const TripleMeta *__restr
             RingRow<T, R> xa, pa, xb, pb;
             int ba, bb;
             const int ring_b = cur_ring + 1 == W ? 0 : cur_ring + 1;
-            const bool ha = inputs(i, p, r, cur_ring, xa, pa, ba);
-            const bool hb = inputs(i + 1, p, r + 1, ring_b, xb, pb, bb);
+            const bool ha = inputs(i, xa, pa, ba);
+            const bool hb = inputs(i + 1, xb, pb, bb);
             issue(i);
             issue(i + 1);
             if (!ha) load_now(p, r, cur_ring, xa, pa, ba);

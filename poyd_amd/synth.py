@@ -160,17 +160,17 @@ def triples_uniform(n: int, lmin: int, lmax: int, p_sub: float, p_indel: float, 
 
 
 # The workloads measured by bench.py and replayed by the parity tests.
-def workload_c1(n: int = 1000) -> PairWorkload:
+def workload_linear_1000(n: int = 1000) -> PairWorkload:
     """1,000 pairs, L ~ U{200..400}, 10% substitutions, 2% indels."""
     return pairs_uniform(n, 200, 400, 0.10, 0.02, SEED0 + 0, "pairwise-linear-1000x200-400")
 
 
-def workload_c2(n: int = 65536) -> PairWorkload:
+def workload_affine_65536(n: int = 65536) -> PairWorkload:
     """65,536 pairs, L log-uniform on [50, 5000], divergence U[5%, 30%]."""
     return pairs_loguniform(n, 50, 5000, 0.05, 0.30, SEED0 + 1, "pairwise-affine-65536x50-5000")
 
 
-def workload_c3(n: int = 4096):
+def workload_three_way_4096(n: int = 4096):
     """4,096 triples, L ~ U{100..500}, 15% substitutions, 3% indels."""
     return triples_uniform(n, 100, 500, 0.15, 0.03, SEED0 + 2, "three-way-4096x100-500")
 
@@ -258,7 +258,7 @@ def tree_workload(n_taxa=128, n_loci=5, lmin=300, lmax=1500, n_nonadd=10000, kmi
                         name or f"tree-{n_taxa}taxa-{n_loci}loci-{lmin}-{lmax}-nonadd{n_nonadd}")
 
 
-def workload_c4() -> TreeWorkload:
+def workload_tree_128() -> TreeWorkload:
     """128-taxon Yule tree (branch lengths Exp, mean 0.05), 5 DNA loci of
     300-1,500 bp, 10,000 nonadditive characters of 2-8 states."""
     return tree_workload()

@@ -1,5 +1,5 @@
 """Tree-pass backend that runs every alignment on the CPU oracle (test
-infrastructure: the checker for poyd_amd.treepass on the GPU)."""
+infrastructure: GPU tree passes are compared with it)."""
 from __future__ import annotations
 
 import numpy as np

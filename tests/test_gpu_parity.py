@@ -374,7 +374,7 @@ def test_workload_pairwise_linear_1000(ctx, oracle):
     every pair bit-exact: cost, both aligned sequences and the median."""
     from poyd_amd import engine
     note("workload pairwise-linear-1000")
-    w = synth.workload_c1()
+    w = synth.workload_linear_1000()
     cm = engine.CostMatrix.of_sub_indel(1, 2)
     ref = oracle.CM2.sub_indel(1, 2)
     res = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, w.a, w.b)
@@ -393,7 +393,7 @@ def test_workload_pairwise_affine_sample(ctx, oracle):
     stripping gaps returns the inputs."""
     from poyd_amd import engine
     note("workload pairwise-affine-65536")
-    w = synth.workload_c2()
+    w = synth.workload_affine_65536()
     cm = engine.CostMatrix.of_sub_indel(1, 1, 3)
     ref = oracle.CM2.sub_indel(1, 1, 3)
     res = engine.run_pairs(ctx, cm, engine.POYG_MODE_AFFINE_ALIGN, w.a, w.b)
