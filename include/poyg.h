@@ -135,6 +135,15 @@ int poyg_algn_cost_affine_3(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *si,
 int poyg_algn_align_3d(poyg_ctx *ctx, const poyg_cm3 *cm3, const uint8_t *s1, int l1,
                        const uint8_t *s2, int l2, const uint8_t *s3, int l3,
                        uint8_t *r1, uint8_t *r2, uint8_t *r3, int cap, int *rlen, int *cost);
+/* powell_3D_align (src/ukkCommon.c:110-151; the Iterative ThreeD
+ * readjustment, src/sequence.ml:1078-1139): the least cost of Powell's
+ * three-way affine model (mismatch mm, gap opening go, extension ge) and an
+ * alignment of that cost, right-aligned in ra/rb/rc (cap >= la+lb+lc) with
+ * the leading gap.  The cost is the reference's; the reference's search may
+ * pick another alignment of the same cost.  POYG_EINVAL, as the reference's
+ * failure, when a character past the leading gap has no base bit. */
+int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, int lb, const uint8_t *c, int lc,
+                   int mm, int go, int ge, uint8_t *ra, uint8_t *rb, uint8_t *rc, int cap, int *rlen, int *cost);
 /* algn_CAML_median_2_no_gaps / _with_gaps / ancestor_2 / union / median_3
  * (src/algn.c:4186-4235, 4287-4299) on already aligned sequences. */
 int poyg_algn_median_2_no_gaps(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len,

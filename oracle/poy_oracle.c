@@ -1113,3 +1113,171 @@ int64_t po_nonadd_dist_2(const uint8_t *a, const uint8_t *b, const uint8_t *c, i
 void po_nonadd_reroot_median(const uint8_t *a, const uint8_t *b, uint8_t *res, int len) {
     for (int i = 0; i < len; i++) res[i] = a[i] | b[i];
 }
+
+/* ---- R15: powell_3D_align (src/ukkCommon.c:110-151, 236-330, 380-445;
+ * src/ukk.checkp.c:113-212, 446-500) -------------------------------------
+ * Powell's three-way alignment with affine gaps.  The reference runs an
+ * Ukkonen-style search (cost-bounded furthest reaching with check points),
+ * which returns the least cost of any alignment under its model; this
+ * restatement finds the same least cost with an exhaustive DP over the model:
+ *   states: per sequence one of match / del / ins (enumerated as
+ *     s = t0 + 3 t1 + 9 t2, match 0, del 1, ins 2), kept when at least one
+ *     sequence matches and at most one inserts (setup, :259-269);
+ *   a state consumes the characters of its matching sequences when it has no
+ *     insert, else the inserting sequence's only (neighbours, :274-285);
+ *   entering state t from s costs go for every sequence whose state in t is
+ *     del or ins and differs from s (transCost, :310-320), then ge for an
+ *     insert column, else ge per del plus mm per mismatch among the matched
+ *     characters (0 / 1 / 2 for three: whichCharCost, :157-184; 0 / 1 for
+ *     two), as alignmentCost (:380-445) recomputes a finished alignment;
+ *   the alignment starts in the all-match state at cost 0.
+ * Characters are the first set bit of each 5-bit mask after the leading
+ * gap (copySequence, :88-108); outputs get the leading gap back
+ * (printTraceBack, :456-500).  The least cost equals the reference's; among
+ * alignments of that cost this one takes, at every cell, the first least
+ * previous state and, at the end, the first least state. */
+#define PW_NSTATES 16
+typedef struct {
+  int st[3], adv[3], n_ins, n_del;
+} pw_state;
+
+static int pw_setup(pw_state *S, int trans[PW_NSTATES][PW_NSTATES], int go) {
+  int ns = 0;
+  for (int s = 0; s < 27; ++s) {
+    int st[3] = {s % 3, (s / 3) % 3, (s / 9) % 3};
+    int nm = 0, nd = 0, ni = 0;
+    for (int q = 0; q < 3; ++q) {
+      nm += st[q] == 0;
+      nd += st[q] == 1;
+      ni += st[q] == 2;
+    }
+    if (nm == 0 || ni > 1) continue;
+    for (int q = 0; q < 3; ++q) {
+      S[ns].st[q] = st[q];
+      S[ns].adv[q] = ni == 0 ? st[q] == 0 : st[q] == 2;
+    }
+    S[ns].n_ins = ni;
+    S[ns].n_del = nd;
+    ++ns;
+  }
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b < ns; ++b) {
+      int c = 0;
+      for (int q = 0; q < 3; ++q)
+        if (S[b].st[q] != 0 && S[b].st[q] != S[a].st[q]) c += go;
+      trans[a][b] = c;
+    }
+  return ns;
+}
+
+static int pw_base(uint8_t m) {
+  return (m & 1) ? 'A' : (m & 2) ? 'C' : (m & 4) ? 'G' : (m & 8) ? 'T' : 0;
+}
+
+/* column cost of state t whose consumed characters end at (i, j, k) */
+static int pw_step(const pw_state *t, const char *x[3], const int pos[3], int mm, int ge) {
+  if (t->n_ins) return ge;
+  int c = ge * t->n_del, ch[3], n = 0;
+  for (int q = 0; q < 3; ++q)
+    if (t->st[q] == 0) ch[n++] = x[q][pos[q] - 1];
+  if (n == 3) c += mm * ((ch[0] == ch[1] && ch[0] == ch[2]) ? 0 : (ch[0] == ch[1] || ch[0] == ch[2] || ch[1] == ch[2]) ? 1 : 2);
+  else if (n == 2) c += mm * (ch[0] != ch[1]);
+  return c;
+}
+
+/* Inputs include POY's leading gap (la, lb, lc >= 1).  Returns the cost, or
+ * -1 when a character has no base bit (the reference fails there) or memory
+ * runs out.  When oa is given, writes the three aligned sequences (masks,
+ * gap 16, leading gap first) and their length to *olen; cap >= la+lb+lc. */
+int po_powell_3d(const uint8_t *A, int la, const uint8_t *B, int lb, const uint8_t *Cs, int lc, int mm,
+                 int go, int ge, uint8_t *oa, uint8_t *ob, uint8_t *oc, int cap, int *olen) {
+  pw_state S[PW_NSTATES];
+  int trans[PW_NSTATES][PW_NSTATES];
+  const int ns = pw_setup(S, trans, go);
+  const int n1 = la - 1, n2 = lb - 1, n3 = lc - 1;
+  if (n1 < 0 || n2 < 0 || n3 < 0) return -1;
+  char *x[3];
+  const uint8_t *src[3] = {A, B, Cs};
+  const int len[3] = {n1, n2, n3};
+  for (int q = 0; q < 3; ++q) {
+    x[q] = (char *)malloc((size_t)len[q] + 1);
+    for (int i = 0; i < len[q]; ++i) {
+      x[q][i] = (char)pw_base(src[q][i + 1]);
+      if (!x[q][i]) {
+        for (int r = 0; r <= q; ++r) free(x[r]);
+        return -1;
+      }
+    }
+  }
+  const int64_t d1 = n1 + 1, d2 = n2 + 1, d3 = n3 + 1, cells = d1 * d2 * d3;
+  int *V = (int *)malloc(sizeof(int) * cells * ns);
+  uint8_t *from = (uint8_t *)malloc((size_t)cells * ns);
+  if (!V || !from) {
+    free(V); free(from);
+    for (int q = 0; q < 3; ++q) free(x[q]);
+    return -1;
+  }
+  const int INF = 0x3fffffff;
+  const char *xc[3] = {x[0], x[1], x[2]};
+  for (int i = 0; i <= n1; ++i)
+    for (int j = 0; j <= n2; ++j)
+      for (int k = 0; k <= n3; ++k) {
+        const int64_t c = (i * d2 + j) * d3 + k;
+        for (int t = 0; t < ns; ++t) {
+          int best = INF, arg = 0;
+          if (i == 0 && j == 0 && k == 0) {
+            best = t == 0 ? 0 : INF;
+          } else {
+            const int pi = i - S[t].adv[0], pj = j - S[t].adv[1], pk = k - S[t].adv[2];
+            if (pi >= 0 && pj >= 0 && pk >= 0) {
+              const int64_t pc = (pi * d2 + pj) * d3 + pk;
+              for (int s = 0; s < ns; ++s) {
+                const int v = V[pc * ns + s];
+                if (v >= INF) continue;
+                const int w = v + trans[s][t];
+                if (w < best) { best = w; arg = s; }
+              }
+              if (best < INF) {
+                const int pos[3] = {i, j, k};
+                best += pw_step(&S[t], xc, pos, mm, ge);
+              }
+            }
+          }
+          V[c * ns + t] = best;
+          from[c * ns + t] = (uint8_t)arg;
+        }
+      }
+  const int64_t end = cells - 1;
+  int cost = INF, st = 0;
+  for (int t = 0; t < ns; ++t)
+    if (V[end * ns + t] < cost) { cost = V[end * ns + t]; st = t; }
+  if (oa) {
+    int i = n1, j = n2, k = n3, n = 0;
+    uint8_t *o[3] = {oa, ob, oc};
+    /* collect backwards, then reverse */
+    while (i > 0 || j > 0 || k > 0) {
+      const int pos[3] = {i, j, k};
+      for (int q = 0; q < 3; ++q) {
+        const int ch = S[st].adv[q] ? xc[q][pos[q] - 1] : '-';
+        o[q][n] = ch == 'A' ? 1 : ch == 'C' ? 2 : ch == 'G' ? 4 : ch == 'T' ? 8 : 16;
+      }
+      ++n;
+      const int64_t c = (i * d2 + j) * d3 + k;
+      const int prev = from[c * ns + st];
+      i -= S[st].adv[0]; j -= S[st].adv[1]; k -= S[st].adv[2];
+      st = prev;
+      if (n >= cap - 1) break;
+    }
+    for (int q = 0; q < 3; ++q) {
+      for (int a = 0, b = n - 1; a < b; ++a, --b) {
+        const uint8_t t = o[q][a]; o[q][a] = o[q][b]; o[q][b] = t;
+      }
+      memmove(o[q] + 1, o[q], (size_t)n);
+      o[q][0] = 16;
+    }
+    *olen = n + 1;
+  }
+  free(V); free(from);
+  for (int q = 0; q < 3; ++q) free(x[q]);
+  return cost;
+}

@@ -138,6 +138,13 @@ int64_t po_nonadd_distance(const uint8_t *a, const uint8_t *b, int len);
 int64_t po_nonadd_dist_2(const uint8_t *a, const uint8_t *b, const uint8_t *c, int len);
 void po_nonadd_reroot_median(const uint8_t *a, const uint8_t *b, uint8_t *res, int len);
 
+/* ---- R15: powell_3D_align (src/ukkCommon.c, src/ukk.checkp.c) ----------
+ * Least cost of Powell's three-way affine model (mismatch mm, gap open go,
+ * gap extension ge) and one alignment of that cost; inputs with the leading
+ * gap.  -1 when a character has no base bit. */
+int po_powell_3d(const uint8_t *a, int la, const uint8_t *b, int lb, const uint8_t *c, int lc, int mm,
+                 int go, int ge, uint8_t *oa, uint8_t *ob, uint8_t *oc, int cap, int *olen);
+
 #ifdef __cplusplus
 }
 #endif

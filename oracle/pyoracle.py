@@ -77,6 +77,8 @@ def lib():
             "po_nonadd_distance": (C.c_int64, [u8p, u8p, C.c_int]),
             "po_nonadd_dist_2": (C.c_int64, [u8p, u8p, u8p, C.c_int]),
             "po_nonadd_reroot_median": (None, [u8p, u8p, u8p, C.c_int]),
+            "po_powell_3d": (C.c_int, [u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, u8p, u8p, u8p, C.c_int, i32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -337,3 +339,17 @@ def nonadd_distance(a, b) -> int:
 def nonadd_dist_2(a, b, c) -> int:
     x, y, z = _u8(a), _u8(b), _u8(c)
     return int(lib().po_nonadd_dist_2(_p8(x), _p8(y), _p8(z), len(x)))
+
+
+def powell_3d(a, b, c, mm: int, go: int, ge: int):
+    """powell_3D_align's cost and one least-cost alignment (three aligned
+    sequences with the leading gap); None when a character has no base."""
+    x, y, z = _u8(a), _u8(b), _u8(c)
+    cap = len(x) + len(y) + len(z) + 2
+    o = [np.zeros(cap, np.uint8) for _ in range(3)]
+    n = C.c_int(0)
+    cost = lib().po_powell_3d(_p8(x), len(x), _p8(y), len(y), _p8(z), len(z), mm, go, ge,
+                              _p8(o[0]), _p8(o[1]), _p8(o[2]), cap, C.byref(n))
+    if cost < 0:
+        return None
+    return cost, o[0][:n.value].copy(), o[1][:n.value].copy(), o[2][:n.value].copy()

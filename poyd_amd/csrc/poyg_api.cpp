@@ -866,6 +866,104 @@ int poyg_algn_align_3d(poyg_ctx *ctx, const poyg_cm3 *cm3, const uint8_t *s1, in
   return rc;
 }
 
+// ---- powell_3D_align (src/ukkCommon.c:110-151) ---------------------------
+extern "C++" {
+namespace poyg {
+struct PwTables;
+void pw_tables(int go, PwTables *T);
+size_t pw_tables_size();
+hipError_t launch_powell(int n1, int n2, int n3, const uint8_t *x1, const uint8_t *x2, const uint8_t *x3, int *V,
+                         uint8_t *from, const PwTables *tab, int mm, int ge, uint8_t *out, int cap, int *res,
+                         hipStream_t s);
+}  // namespace poyg
+}
+
+int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, int lb, const uint8_t *c, int lc,
+                   int mm, int go, int ge, uint8_t *ra, uint8_t *rb, uint8_t *rc, int cap, int *rlen, int *cost) {
+  if (!ctx || !a || !b || !c || !ra || !rb || !rc || la < 1 || lb < 1 || lc < 1) return POYG_EINVAL;
+  // the reference's preconditions (src/ukkCommon.c:132)
+  if (mm == 0 || go < 0 || ge <= 0) {
+    set_error("powell_3d: needs mm != 0, go >= 0, ge > 0");
+    return POYG_EINVAL;
+  }
+  if (cap < la + lb + lc) {
+    set_error("powell_3d: output capacity %d below %d", cap, la + lb + lc);
+    return POYG_EINVAL;
+  }
+  // the first set base bit of every mask after the leading gap
+  // (copySequence, src/ukkCommon.c:88-108)
+  const int n[3] = {la - 1, lb - 1, lc - 1};
+  const uint8_t *src[3] = {a, b, c};
+  std::vector<uint8_t> x[3];
+  for (int q = 0; q < 3; ++q) {
+    x[q].resize(std::max(n[q], 1));
+    for (int i = 0; i < n[q]; ++i) {
+      const uint8_t m = src[q][i + 1];
+      const char ch = (m & 1) ? 'A' : (m & 2) ? 'C' : (m & 4) ? 'G' : (m & 8) ? 'T' : 0;
+      if (!ch) {
+        set_error("powell_3d: position %d of sequence %d holds no base (the reference fails: This is impossible!)",
+                  i + 1, q + 1);
+        return POYG_EINVAL;
+      }
+      x[q][i] = (uint8_t)ch;
+    }
+  }
+  const int64_t plane = (int64_t)(n[0] + 1) * (n[1] + 1);
+  const int64_t cells = plane * (n[2] + 1);
+  const int64_t from_bytes = cells * 16, v_bytes = 4 * plane * 16 * 4;
+  if (from_bytes > (int64_t(16) << 30)) {
+    set_error("powell_3d: %lld lattice cells exceed the 1 Gi-cell limit", (long long)cells);
+    return POYG_ENOMEM;
+  }
+  const int dcap = n[0] + n[1] + n[2] + 2;
+  std::vector<uint8_t> tab(pw_tables_size());
+  pw_tables(go, (PwTables *)tab.data());
+  DevBuf dx[3], dv, dfrom, dtab, dout, dres;
+  for (int q = 0; q < 3; ++q)
+    if (dx[q].alloc(x[q].size())) return POYG_ENOMEM;
+  if (dv.alloc((size_t)v_bytes) || dfrom.alloc((size_t)from_bytes) || dtab.alloc(tab.size()) ||
+      dout.alloc((size_t)3 * dcap) || dres.alloc(8)) {
+    for (auto &d : dx) d.release();
+    dv.release(); dfrom.release(); dtab.release(); dout.release(); dres.release();
+    set_error("powell_3d: device allocation failed");
+    return POYG_ENOMEM;
+  }
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipSuccess;
+  for (int q = 0; q < 3 && e == hipSuccess; ++q)
+    e = hipMemcpyAsync(dx[q].p, x[q].data(), x[q].size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dtab.p, tab.data(), tab.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = launch_powell(n[0], n[1], n[2], (const uint8_t *)dx[0].p, (const uint8_t *)dx[1].p,
+                      (const uint8_t *)dx[2].p, (int *)dv.p, (uint8_t *)dfrom.p, (const PwTables *)dtab.p, mm, ge,
+                      (uint8_t *)dout.p, dcap, (int *)dres.p, s);
+  std::vector<uint8_t> h_out((size_t)3 * dcap);
+  int h_res[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(h_out.data(), dout.p, h_out.size(), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(h_res, dres.p, 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  for (auto &d : dx) d.release();
+  dv.release(); dfrom.release(); dtab.release(); dout.release(); dres.release();
+  if (e != hipSuccess) {
+    set_error("powell_3d: %s", hipGetErrorString(e));
+    return POYG_EDEVICE;
+  }
+  const int len = h_res[0];
+  if (h_res[1] < 0) {
+    set_error("powell_3d: the traceback left the lattice");
+    return POYG_EBAND;
+  }
+  if (len < 1 || len > dcap || len > cap) {
+    set_error("powell_3d: alignment of %d columns does not fit", len);
+    return POYG_EINVAL;
+  }
+  uint8_t *dst[3] = {ra, rb, rc};
+  for (int q = 0; q < 3; ++q) memcpy(dst[q] + cap - len, h_out.data() + (size_t)q * dcap + (dcap - len), len);
+  if (rlen) *rlen = len;
+  if (cost) *cost = h_res[1];
+  return POYG_OK;
+}
+
 // ---- host-side column rules (O(L) per call) ------------------------------
 static int put_rev(uint8_t *out, int cap, int &n, int v) {
   if (n >= cap) return POYG_EINVAL;

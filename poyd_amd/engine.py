@@ -85,6 +85,8 @@ SIGNATURES = {
     "poyg_algn_cost_affine_3": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, i32p]),
     "poyg_algn_align_3d": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, u8p, u8p, u8p,
                                      C.c_int, i32p, i32p]),
+    "poyg_powell_3d": (C.c_int, [vp, u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 u8p, u8p, u8p, C.c_int, i32p, i32p]),
     "poyg_algn_median_2_no_gaps": (C.c_int, [vp, u8p, u8p, C.c_int, u8p, C.c_int, i32p]),
     "poyg_algn_median_2_with_gaps": (C.c_int, [vp, u8p, u8p, C.c_int, u8p, C.c_int, i32p]),
     "poyg_algn_ancestor_2": (C.c_int, [vp, u8p, u8p, C.c_int, u8p, C.c_int, i32p]),
@@ -495,6 +497,19 @@ class TriplePlan(PairPlan):
         if self.n:
             load().poyg_plan_out_offsets(h, self.out_off.ctypes.data_as(i64p))
         self.cap = ((l1.astype(np.int64) + l2 + l3 + 1 + 15) // 16) * 16
+
+
+def powell_3d(ctx: Context, a, b, c, mm: int, go: int, ge: int):
+    """powell_3D_align (src/ukkCommon.c:110-151): (cost, three aligned
+    sequences with the leading gap)."""
+    x, y, z = _u8(a), _u8(b), _u8(c)
+    cap = len(x) + len(y) + len(z)
+    out = [np.zeros(cap, np.uint8) for _ in range(3)]
+    n, cost = C.c_int32(0), C.c_int32(0)
+    _check(load().poyg_powell_3d(ctx.h, _p8(x), len(x), _p8(y), len(y), _p8(z), len(z), mm, go, ge,
+                                 _p8(out[0]), _p8(out[1]), _p8(out[2]), cap, C.byref(n), C.byref(cost)),
+           "poyg_powell_3d")
+    return int(cost.value), [o[cap - n.value:].copy() for o in out]
 
 
 def fitch_pairs(ctx: Context, a: np.ndarray, b: np.ndarray):

@@ -413,3 +413,32 @@ def test_workload_pairwise_affine_sample(ctx, oracle):
         elif not (np.array_equal(ri[ri != GAP], w.a.get(k)[1:]) and np.array_equal(rj[rj != GAP], w.b.get(k)[1:])):
             bad += 1
     assert bad == 0
+
+
+@pytest.mark.parametrize("mm,go,ge", [(1, 2, 1), (2, 0, 1), (1, 3, 2)])
+def test_powell_matches_oracle(ctx, oracle, mm, go, ge):
+    from poyd_amd import engine
+    note(f"powell_3d mm={mm} go={go} ge={ge}")
+    rng = np.random.default_rng(50 + mm + 7 * go + 13 * ge)
+    shapes = [(0, 0, 0), (0, 3, 5), (4, 0, 0), (1, 1, 1), (7, 7, 7)] + \
+             [tuple(int(v) for v in rng.integers(1, 60, size=3)) for _ in range(12)] + [(150, 140, 160)]
+    for n1, n2, n3 in shapes:
+        anc = rand_seq(rng, max(n1, n2, n3, 1))
+        t = []
+        for n in (n1, n2, n3):
+            x = mutate(rng, anc, 0.15, 0.05)[:n + 1]
+            if len(x) < n + 1:
+                x = np.concatenate([x, rand_seq(rng, n + 1 - len(x))[1:]]).astype(np.uint8)
+            t.append(x)
+        cost, al = engine.powell_3d(ctx, *t, mm, go, ge)
+        want = oracle.powell_3d(*t, mm, go, ge)
+        assert cost == want[0], (n1, n2, n3)
+        for q in range(3):
+            assert np.array_equal(al[q], want[1 + q]), (n1, n2, n3, q)
+
+
+def test_powell_rejects_gap_only_characters(ctx):
+    from poyd_amd import engine
+    x = np.array([16, 1, 16, 2], np.uint8)
+    with pytest.raises(engine.PoygError):
+        engine.powell_3d(ctx, x, x, x, 1, 2, 1)

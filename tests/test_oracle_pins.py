@@ -343,3 +343,110 @@ def test_fitch_distance_helpers(oracle):
     c = rng.integers(1, 256, size=1000).astype(np.uint8)
     assert oracle.nonadd_distance(a, b) == int(((a & b) == 0).sum())
     assert oracle.nonadd_dist_2(a, b, c) == int((((a & b) == 0) & ((a & c) == 0)).sum())
+
+
+# ---- R15: Powell's three-way affine model -------------------------------
+def powell_states():
+    """The model's states in the reference's enumeration order
+    (src/ukkCommon.c:252-330): per sequence match 0 / del 1 / ins 2."""
+    out = []
+    for s in range(27):
+        st = (s % 3, (s // 3) % 3, (s // 9) % 3)
+        if st.count(0) == 0 or st.count(2) > 1:
+            continue
+        adv = tuple(int(t == 0) for t in st) if st.count(2) == 0 else tuple(int(t == 2) for t in st)
+        out.append((st, adv))
+    return out
+
+
+def powell_column_cost(prev, st, chars, mm, go, ge):
+    """alignmentCost's cost of one column (src/ukkCommon.c:380-445)."""
+    cost = sum(go for q in range(3) if st[q] != 0 and st[q] != prev[q])
+    if st.count(2):
+        return cost + ge
+    cost += ge * st.count(1)
+    ch = [chars[q] for q in range(3) if st[q] == 0]
+    for a in range(len(ch) - 1, 0, -1):
+        if ch[a - 1] != ch[a]:
+            cost += mm
+    if len(ch) == 3 and ch[0] == ch[2] and ch[0] != ch[1]:
+        cost -= mm
+    return cost
+
+
+def powell_brute_force(x, mm, go, ge):
+    """Least cost over every sequence of states that consumes the three
+    strings (branch and bound over all paths, no shared subproblems)."""
+    states = powell_states()
+    n = [len(s) for s in x]
+    best = [None]
+
+    def walk(pos, prev, cost):
+        if best[0] is not None and cost >= best[0]:
+            return
+        if pos == n:
+            best[0] = cost
+            return
+        for st, adv in states:
+            nxt = [pos[q] + adv[q] for q in range(3)]
+            if any(nxt[q] > n[q] for q in range(3)):
+                continue
+            chars = [x[q][nxt[q] - 1] if adv[q] else None for q in range(3)]
+            walk(nxt, st, cost + powell_column_cost(prev, st, chars, mm, go, ge))
+
+    walk([0, 0, 0], (0, 0, 0), 0)
+    return best[0]
+
+
+def test_powell_model_has_sixteen_states():
+    # 7 without an insert (M/D with a match) and 3 x 3 with one
+    assert len(powell_states()) == 16
+
+
+@pytest.mark.parametrize("mm,go,ge", [(1, 2, 1), (2, 0, 1), (1, 3, 2), (3, 1, 1)])
+def test_powell_cost_equals_brute_force(oracle, mm, go, ge):
+    rng = np.random.default_rng(40 + mm + 10 * go + 100 * ge)
+    bases = "ACGT"
+    code = {"A": 1, "C": 2, "G": 4, "T": 8}
+    for _ in range(12):
+        lens = [int(v) for v in rng.integers(0, 4, size=3)]
+        if sum(lens) > 6:
+            lens = [min(v, 2) for v in lens]
+        x = ["".join(bases[int(v)] for v in rng.integers(0, 3, size=n)) for n in lens]
+        enc = [np.array([16] + [code[ch] for ch in s], np.uint8) for s in x]
+        got = oracle.powell_3d(*enc, mm, go, ge)
+        assert got is not None
+        assert got[0] == powell_brute_force(x, mm, go, ge), (x, mm, go, ge)
+
+
+def test_powell_alignment_recosts_to_its_cost(oracle):
+    # the returned alignment strips back to the inputs and its columns can be
+    # labelled with states whose alignmentCost total is the returned cost
+    rng = np.random.default_rng(44)
+    for _ in range(20):
+        anc = rng.choice([1, 2, 4, 8], size=int(rng.integers(3, 12)))
+        x = [np.concatenate([[16], anc[rng.random(len(anc)) > 0.2]]).astype(np.uint8) for _ in range(3)]
+        cost, a, b, c = oracle.powell_3d(*x, 1, 2, 1)
+        for out, src in zip((a, b, c), x):
+            assert out[0] == 16 and np.array_equal(out[1:][out[1:] != 16], src[1:])
+        states = powell_states()
+        inv = {1: "A", 2: "C", 4: "G", 8: "T"}
+        # least labelling of the fixed columns (insert columns may be I or DD)
+        best = {(0, 0, 0): 0}
+        for col in range(1, len(a)):
+            chars = [a[col], b[col], c[col]]
+            has = tuple(int(v != 16) for v in chars)
+            nxt = {}
+            for prev, cst in best.items():
+                for st, adv in states:
+                    if adv != has:
+                        continue
+                    v = cst + powell_column_cost(prev, st, [inv.get(int(ch)) for ch in chars], 1, 2, 1)
+                    nxt[st] = min(nxt.get(st, v), v)
+            best = nxt
+        assert min(best.values()) == cost
+
+
+def test_powell_rejects_gap_only_characters(oracle):
+    x = np.array([16, 1, 16, 2], np.uint8)
+    assert oracle.powell_3d(x, x, x, 1, 2, 1) is None
