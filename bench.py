@@ -117,7 +117,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 10; tree: 1)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 2; tree: 0)")
-    ap.add_argument("--workload", default="affine", choices=["affine", "linear", "cube", "tree"])
+    ap.add_argument("--workload", default="affine", choices=["affine", "linear", "cube", "tree", "fitch"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -137,6 +137,8 @@ def main() -> int:
         args.warmup = 0 if tree else 2
     if tree:
         return bench_tree(args)
+    if args.workload == "fitch":
+        return bench_fitch(args)
 
     from poyd_amd import engine
     w, kind = workload(args.workload)
@@ -348,6 +350,139 @@ def bench_tree(args) -> int:
         cb = cpu_baseline_tree(ds, r.tree, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["config"]["gpu_vs_cpu_1core"] = round(gcups / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_fitch(args) -> int:
+    """Nonadditive characters: Fitch down pass then up pass over a 1,024-taxon
+    Yule tree with 1,048,576 characters of 2-8 states (one byte each), every
+    node's sets resident in HBM.  One launch per tree level; value = character
+    medians (down and up) per second.  Algorithmic bytes: 3 per character per
+    down-pass node (two children read, one median written), 5 per up-pass
+    node (parent final, own preliminary, two children read, final written)."""
+    import ctypes as C
+    from poyd_amd import engine, synth
+    n_taxa, L = 1024, 1 << 20
+    rng = np.random.default_rng(synth.SEED0 + 5)
+    parent, _, leaves = synth.yule_tree(rng, n_taxa, 0.05)
+    nn = len(parent)
+    kids = [[] for _ in range(nn)]
+    for v in range(1, nn):
+        kids[parent[v]].append(v)
+    internal = [v for v in range(nn) if kids[v]]
+    height = np.zeros(nn, np.int64)
+    for v in range(nn - 1, -1, -1):
+        if kids[v]:
+            height[v] = 1 + max(height[c] for c in kids[v])
+    depth = np.zeros(nn, np.int64)
+    for v in range(1, nn):
+        depth[v] = depth[parent[v]] + 1
+    # rows: preliminary sets of every node (0..nn-1), final sets of internal nodes (nn + index)
+    fin = {v: nn + q for q, v in enumerate(internal)}
+    rows = nn + len(internal)
+    ctx = engine.Context(0)
+    d_chars = engine.DeviceBuffer(ctx, rows * L)
+    states = rng.integers(2, 9, size=L).astype(np.uint8)
+    lib = engine.load()
+    sample = []
+    for q, v in enumerate(leaves):
+        row = np.left_shift(np.uint8(1), rng.integers(0, 8, size=L, dtype=np.uint8) % states).astype(np.uint8)
+        lib.poyg_dev_upload(ctx.h, d_chars.ptr + int(v) * L, row.ctypes.data, L)
+        if q < 2:
+            sample.append(row)
+    down = []
+    for h in range(1, int(height.max()) + 1):
+        vs = [v for v in internal if height[v] == h]
+        down.append(np.array([[kids[v][0], kids[v][1], v] for v in vs], np.int32))
+    up = []
+    for dpt in range(1, int(depth.max()) + 1):
+        vs = [v for v in internal if depth[v] == dpt]
+        if vs:
+            up.append(np.array([[fin[parent[v]], v, kids[v][0], kids[v][1], fin[v]] for v in vs], np.int32))
+    dev_lv = [[engine.DeviceBuffer.from_array(ctx, np.ascontiguousarray(a[:, c])) for c in range(3)] for a in down]
+    dev_up = [[engine.DeviceBuffer.from_array(ctx, np.ascontiguousarray(a[:, c])) for c in range(5)] for a in up]
+    d_added = engine.DeviceBuffer(ctx, 8 * nn)
+    stream = load_stream(ctx)
+    root = 0
+    zeros = np.zeros(nn, np.int64)
+
+    def step():
+        d_added.upload(zeros)
+        for a, (ia, ib, ir) in zip(down, dev_lv):
+            engine._check(lib.poyg_fitch_down(ctx.h, d_chars.ptr, L, len(a), ia.ptr, ib.ptr, ir.ptr,
+                                              d_added.ptr + 8 * 0, stream), "poyg_fitch_down")
+        for a, (iA, iN, i1, i2, iF) in zip(up, dev_up):
+            engine._check(lib.poyg_fitch_up(ctx.h, d_chars.ptr, L, len(a), iA.ptr, iN.ptr, i1.ptr, i2.ptr,
+                                            iF.ptr, stream), "poyg_fitch_up")
+
+    # the root's final sets are its preliminary ones: copied once after a
+    # first down pass (the leaves never change)
+    for _ in range(max(1, args.warmup)):
+        step()
+    ctx.sync()
+    host_root = np.empty(L, np.uint8)
+    lib.poyg_dev_download(ctx.h, host_root.ctypes.data, d_chars.ptr + root * L, L)
+    lib.poyg_dev_upload(ctx.h, d_chars.ptr + fin[root] * L, host_root.ctypes.data, L)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    n_down = sum(len(a) for a in down)
+    n_up = sum(len(a) for a in up)
+    medians = (n_down + n_up) * L
+    alg_bytes = (3 * n_down + 5 * n_up) * L
+    gbs = alg_bytes / (ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(medians / (ms / 1e3) / 1e9, 3),
+        "unit": "G character medians/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": f"fitch-{n_taxa}taxa-{L}chars",
+            "down_pass_nodes": n_down,
+            "up_pass_nodes": n_up,
+            "levels": len(down) + len(up),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "fitch_down_kernel + fitch_up_kernel, one launch per tree level",
+        },
+    }
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        try:
+            old_aff = os.sched_getaffinity(0)
+            os.sched_setaffinity(0, {min(old_aff)})
+        except (AttributeError, OSError):
+            old_aff = None
+        a, b = sample[0], sample[1]
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            po.nonadd_median(a, b)
+            done += 1
+        dt = time.perf_counter() - t0
+        if old_aff is not None:
+            os.sched_setaffinity(0, old_aff)
+        out["cpu_baseline"] = {"value": done * L / dt / 1e9, "unit": "G character medians/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{done} down-pass medians of {L} characters in {dt:.1f} s, 1 pinned core "
+                                         f"of {cpu_model()}"}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -15,6 +15,8 @@
 // first filled column of the row; the backtraces run one thread per pair.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <type_traits>
 
 #include "poyg_internal.h"
@@ -2263,13 +2265,22 @@ hipError_t launch_lin_trace(const PairMeta *meta, const int32_t *order, int n, c
   return hipGetLastError();
 }
 
-hipError_t launch_fitch_down(uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
-                             const int32_t *ires, int64_t *added, hipStream_t s) {
+// Blocks per node along the characters: enough for the whole launch to
+// cover the chip (about 4,096 blocks) when there are few nodes, at most one
+// block per 256 vectors of 16 characters.
+static int fitch_blocks(int len, int n) {
   const int bs = 256;
   const int nvec = len / 16;
   int gx = (nvec + bs - 1) / bs;
-  if (gx < 1) gx = 1;
-  if (gx > 64) gx = 64;
+  const int want = std::max(64, 4096 / std::max(n, 1));
+  if (gx > want) gx = want;
+  return gx < 1 ? 1 : gx;
+}
+
+hipError_t launch_fitch_down(uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
+                             const int32_t *ires, int64_t *added, hipStream_t s) {
+  const int bs = 256;
+  const int gx = fitch_blocks(len, n);
   hipLaunchKernelGGL(fitch_down_kernel, dim3(gx, n), dim3(bs), 0, s, chars, len, n, ia, ib, ires, added);
   return hipGetLastError();
 }
@@ -2277,10 +2288,7 @@ hipError_t launch_fitch_down(uint8_t *chars, int len, int n, const int32_t *ia, 
 hipError_t launch_fitch_up(uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
                            const int32_t *iD1, const int32_t *iD2, const int32_t *iF, hipStream_t s) {
   const int bs = 256;
-  const int nvec = len / 16;
-  int gx = (nvec + bs - 1) / bs;
-  if (gx < 1) gx = 1;
-  if (gx > 64) gx = 64;
+  const int gx = fitch_blocks(len, n);
   hipLaunchKernelGGL(fitch_up_kernel, dim3(gx, n), dim3(bs), 0, s, chars, len, n, iA, iN, iD1, iD2, iF);
   return hipGetLastError();
 }
