@@ -47,11 +47,17 @@ const char *poyg_version(void);
 /* ---- context -------------------------------------------------------- */
 int poyg_ctx_create(int device, poyg_ctx **out);
 void poyg_ctx_destroy(poyg_ctx *ctx);
+/* Waits for all work queued on the context's device (every stream).  Device
+ * blocks released since the previous synchronization (poyg_dev_free, plan
+ * destruction) become reusable by later allocations of this library. */
 int poyg_ctx_sync(poyg_ctx *ctx);
 
 /* ---- device buffers (plumbing for callers without their own allocator;
  * no reference counterpart) ------------------------------------------- */
 int poyg_dev_alloc(poyg_ctx *ctx, int64_t bytes, void **out);
+/* Returns a block from poyg_dev_alloc.  Blocks of up to 64 MiB go to a
+ * per-device cache and are handed out again only after the next device-wide
+ * synchronization (poyg_ctx_sync); larger ones are freed at once. */
 int poyg_dev_free(poyg_ctx *ctx, void *p);
 int poyg_dev_upload(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes);
 int poyg_dev_download(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes);

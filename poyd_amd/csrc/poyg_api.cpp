@@ -5,7 +5,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
+#include <unordered_map>
 #include <vector>
 
 #include "poyg_internal.h"
@@ -34,6 +36,10 @@ hipError_t launch_aff_trace_wave(int grid, const PairMeta *meta, const int32_t *
 hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
                             const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
                             hipStream_t s);
+hipError_t launch_lin_sys(bool with_dm, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                          const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, const int *aux, int32_t *cost,
+                          hipStream_t s);
+int lin_sys_blocks_per_cu(bool with_dm);
 hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
                            int32_t *cost, int *rowscratch, int row_width, hipStream_t s);
@@ -79,13 +85,115 @@ struct poyg_ctx {
   hipStream_t side2 = nullptr;   // third stream: the widest bands (lane sets, row kernels)
 };
 
+// Device memory cache, one per device.  A tree pass builds and drops a plan
+// with a dozen buffers per tree level; hipMalloc and hipFree calls cost more
+// than the level's kernels.  Blocks of up to kPoolMax bytes are rounded to a
+// power of two and kept: a released block waits in `pending` until a
+// device-wide synchronization (poyg_ctx_sync, or the cache's own once the
+// pending bytes pass kPoolPendingMax), after which no queued work can still
+// use it, and is then handed out again.  Larger blocks are allocated and
+// freed directly.
+namespace {
+constexpr int kPoolMinLog = 8, kPoolMaxLog = 26;   // 256 B .. 64 MiB
+constexpr size_t kPoolMax = (size_t)1 << kPoolMaxLog;
+constexpr size_t kPoolPendingMax = (size_t)256 << 20;
+constexpr size_t kPoolCachedMax = (size_t)1 << 30;
+constexpr int kPoolDevices = 64;
+struct DevPool {
+  std::mutex mu;
+  std::vector<void *> free_[kPoolMaxLog + 1];
+  std::vector<std::pair<void *, int>> pending;
+  std::unordered_map<void *, int> cls;   // every block handed out: its class (-1 = not cached)
+  size_t pending_bytes = 0, free_bytes = 0;
+};
+DevPool g_pool[kPoolDevices];
+
+int pool_class(size_t n) {
+  if (n > kPoolMax) return -1;
+  int c = kPoolMinLog;
+  while (((size_t)1 << c) < n) ++c;
+  return c;
+}
+
+DevPool *pool_of_current() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kPoolDevices) return nullptr;
+  return &g_pool[dev];
+}
+
+// pending -> free once the device is idle; caller holds the lock
+void pool_promote(DevPool &P) {
+  for (auto &b : P.pending) {
+    const size_t sz = (size_t)1 << b.second;
+    if (P.free_bytes + sz > kPoolCachedMax) {
+      (void)hipFree(b.first);
+      continue;
+    }
+    P.free_[b.second].push_back(b.first);
+    P.free_bytes += sz;
+  }
+  P.pending.clear();
+  P.pending_bytes = 0;
+}
+
+hipError_t pool_alloc(void **p, size_t n) {
+  DevPool *P = pool_of_current();
+  const int c = pool_class(n);
+  if (P && c >= 0) {
+    std::lock_guard<std::mutex> g(P->mu);
+    if (!P->free_[c].empty()) {
+      *p = P->free_[c].back();
+      P->free_[c].pop_back();
+      P->free_bytes -= (size_t)1 << c;
+      P->cls[*p] = c;
+      return hipSuccess;
+    }
+  }
+  const hipError_t e = hipMalloc(p, c >= 0 ? (size_t)1 << c : n);
+  if (e != hipSuccess) return e;
+  if (P) {
+    std::lock_guard<std::mutex> g(P->mu);
+    P->cls[*p] = c;
+  }
+  return hipSuccess;
+}
+
+hipError_t pool_release(void *p) {
+  if (!p) return hipSuccess;
+  DevPool *P = pool_of_current();
+  if (!P) return hipFree(p);
+  std::lock_guard<std::mutex> g(P->mu);
+  auto it = P->cls.find(p);
+  if (it == P->cls.end()) return hipErrorInvalidValue;
+  const int c = it->second;
+  P->cls.erase(it);
+  if (c < 0) return hipFree(p);
+  P->pending.emplace_back(p, c);
+  P->pending_bytes += (size_t)1 << c;
+  if (P->pending_bytes > kPoolPendingMax) {
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return e;
+    pool_promote(*P);
+  }
+  return hipSuccess;
+}
+
+// after a device-wide synchronization of the current device
+void pool_synced() {
+  DevPool *P = pool_of_current();
+  if (!P) return;
+  std::lock_guard<std::mutex> g(P->mu);
+  pool_promote(*P);
+}
+}  // namespace
+
 // A device buffer owned by a plan.
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
   int alloc(size_t n) {
     if (n == 0) n = 16;
-    if (hipMalloc(&p, n) != hipSuccess) {
+    if (pool_alloc(&p, n) != hipSuccess) {
       p = nullptr;
       set_error("hipMalloc of %zu bytes failed", n);
       return POYG_ENOMEM;
@@ -94,7 +202,7 @@ struct DevBuf {
     return 0;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) (void)pool_release(p);
     p = nullptr;
     bytes = 0;
   }
@@ -114,10 +222,13 @@ struct poyg_plan {
   // four-pairs-per-wave kernel, the anti-diagonal kernel per lane-set count
   // (1, 2, 4), and the backtraces; pure: per pair, one sequence without
   // gap-bearing characters (written by a scan before the fills)
-  DevBuf tcm, meta, order_all, order_lds, order_glb, order_quad, order_diag[3], order_rows, order_tr, dm, aux,
-      rows, counters, status, sink, pure;
-  int n_lds = 0, n_glb = 0, n_quad = 0, n_diag[3] = {0, 0, 0}, n_rows = 0, n_tr = 0;
-  int grid_lds = 0, grid_glb = 0, grid_quad = 0, grid_diag[3] = {0, 0, 0}, grid_tr = 0, row_width = 0;
+  // (linear pairs: the systolic kernel takes every pair whose column
+  // sequence fits its LDS row, the row kernels the rest)
+  DevBuf tcm, meta, order_all, order_lds, order_glb, order_sys, order_quad, order_diag[3], order_rows, order_tr, dm,
+      aux, rows, counters, status, sink, pure;
+  int n_lds = 0, n_glb = 0, n_sys = 0, n_quad = 0, n_diag[3] = {0, 0, 0}, n_rows = 0, n_tr = 0;
+  int grid_lds = 0, grid_glb = 0, grid_sys = 0, grid_quad = 0, grid_diag[3] = {0, 0, 0}, grid_tr = 0,
+      row_width = 0;
   // the fill with the most cells runs on the main stream ("main": 1 the
   // four-pair kernel, 2 the one-set anti-diagonal kernel, 0 none); every
   // other fill runs on the side stream, followed by a second launch of the
@@ -129,8 +240,9 @@ struct poyg_plan {
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;   // hand-off to and from the side streams
   ~poyg_plan() {
     if (triple) poyg_triple_plan_destroy(triple);
-    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_quad, &order_diag[0], &order_diag[1],
-                      &order_diag[2], &order_rows, &order_tr, &dm, &aux, &rows, &counters, &status, &sink, &pure})
+    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_sys, &order_quad, &order_diag[0],
+                      &order_diag[1], &order_diag[2], &order_rows, &order_tr, &dm, &aux, &rows, &counters, &status,
+                      &sink, &pure})
       b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -180,19 +292,23 @@ void poyg_ctx_destroy(poyg_ctx *ctx) {
 }
 
 int poyg_ctx_sync(poyg_ctx *ctx) {
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (!ctx) return POYG_EINVAL;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipDeviceSynchronize());
+  pool_synced();
   return POYG_OK;
 }
 
 int poyg_dev_alloc(poyg_ctx *ctx, int64_t bytes, void **out) {
   if (!ctx || !out || bytes < 0) return POYG_EINVAL;
   HIP_TRY(hipSetDevice(ctx->device));
-  HIP_TRY(hipMalloc(out, (size_t)(bytes ? bytes : 16)));
+  HIP_TRY(pool_alloc(out, (size_t)(bytes ? bytes : 16)));
   return POYG_OK;
 }
 int poyg_dev_free(poyg_ctx *ctx, void *p) {
   if (!ctx) return POYG_EINVAL;
-  if (p) HIP_TRY(hipFree(p));
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (p) HIP_TRY(pool_release(p));
   return POYG_OK;
 }
 int poyg_dev_upload(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes) {
@@ -211,7 +327,11 @@ int poyg_dev_download(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes) 
 }
 void *poyg_ctx_stream(poyg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
-void poyg_plan_destroy(poyg_plan *plan) { delete plan; }
+void poyg_plan_destroy(poyg_plan *plan) {
+  if (!plan) return;
+  (void)hipSetDevice(plan->ctx->device);   // the plan's blocks return to its device's cache
+  delete plan;
+}
 int poyg_plan_slots(const poyg_plan *p) { return p->slots; }
 int64_t poyg_plan_out_bytes(const poyg_plan *p) {
   return p->triple ? poyg_triple_plan_out_bytes(p->triple) : p->out_bytes;
@@ -297,6 +417,7 @@ static void lin_row_host(int i, int m, int n, const LinGeom &g, int &lo, int &hi
 
 static constexpr int kAffRBHost = 256;   // must match kAffRB in the kernels
 static constexpr int kLinRBHost = 512;   // must match kLinRB
+static constexpr int kLinStageHost = 2048;   // must match kLinStage
 
 extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, const int64_t *off_a,
                                const int32_t *len_a, const int64_t *off_b, const int32_t *len_b,
@@ -328,7 +449,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   P->slots = align ? (affine ? 4 : 3) : 0;
   std::vector<PairMeta> meta(n);
   std::vector<int64_t> cells(n);
-  std::vector<char> glb(n, 0), diag(n, 0), quad(n, 0);
+  std::vector<char> glb(n, 0), diag(n, 0), quad(n, 0), sys(n, 0);
   int64_t dm_bytes = 0, aux_ints = 0, out_bytes = 0;
   int row_width = 0;
   P->out_off.resize(n);
@@ -417,7 +538,9 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
         m.dm_off = dm_bytes;
         dm_bytes += (int64_t)m.lr * m.stride;
       }
-      if (maxw > kLinRBHost) {
+      if (m.lc <= kLinStageHost) {
+        sys[k] = 1;
+      } else if (maxw > kLinRBHost) {
         glb[k] = 1;
         row_width = std::max(row_width, (m.lc + 1 + 63) & ~63);
       }
@@ -429,7 +552,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a] > cells[b]; });
-  std::vector<int32_t> o_lds, o_glb, o_quad, o_diag[3], o_rows, o_tr;
+  std::vector<int32_t> o_lds, o_glb, o_sys, o_quad, o_diag[3], o_rows, o_tr;
   int64_t c_quad3 = 0, c_diag1 = 0;
   for (int k : order) {
     if (quad[k]) {
@@ -441,12 +564,13 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       if (diag[k] == 1) c_diag1 += cells[k];
       o_tr.push_back(k);
     } else {
-      (glb[k] ? o_glb : o_lds).push_back(k);
+      (sys[k] ? o_sys : glb[k] ? o_glb : o_lds).push_back(k);
       o_rows.push_back(k);
     }
   }
   P->n_lds = (int)o_lds.size();
   P->n_glb = (int)o_glb.size();
+  P->n_sys = (int)o_sys.size();
   P->n_rows = (int)o_rows.size();
   P->n_tr = (int)o_tr.size();
   // persistent grids: exactly the workgroups that can be resident at once
@@ -464,6 +588,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   const int per_cu = affine ? 4 : 8;
   P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
   P->grid_glb = std::max(1, std::min(P->n_glb, 256));
+  if (P->n_sys) P->grid_sys = std::max(1, std::min((P->n_sys + 3) / 4, ctx->cus * lin_sys_blocks_per_cu(align)));
   if (affine) {
     // The fill with the most cells is the main kernel.  The side stream's
     // kernels are persistent too; the main kernel leaves them whole CUs, or
@@ -522,6 +647,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   rc |= P->order_all.alloc(sizeof(int32_t) * (size_t)n);
   rc |= P->order_lds.alloc(sizeof(int32_t) * (size_t)P->n_lds);
   rc |= P->order_glb.alloc(sizeof(int32_t) * (size_t)P->n_glb);
+  rc |= P->order_sys.alloc(sizeof(int32_t) * (size_t)P->n_sys);
   for (int q = 0; q < 3; ++q) rc |= P->order_diag[q].alloc(sizeof(int32_t) * (size_t)P->n_diag[q]);
   rc |= P->order_quad.alloc(sizeof(int32_t) * (size_t)P->n_quad);
   rc |= P->pure.alloc((size_t)n);
@@ -551,6 +677,8 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     e = hipMemcpyAsync(P->order_lds.p, o_lds.data(), sizeof(int32_t) * P->n_lds, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && P->n_glb)
     e = hipMemcpyAsync(P->order_glb.p, o_glb.data(), sizeof(int32_t) * P->n_glb, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && P->n_sys)
+    e = hipMemcpyAsync(P->order_sys.p, o_sys.data(), sizeof(int32_t) * P->n_sys, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && P->n_quad)
     e = hipMemcpyAsync(P->order_quad.p, o_quad.data(), sizeof(int32_t) * P->n_quad, hipMemcpyHostToDevice, s);
   for (int q = 0; q < 3; ++q)
@@ -655,6 +783,9 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
       HIP_TRY(hipStreamWaitEvent(s, P->join2, 0));
     }
   } else {
+    if (P->n_sys)
+      HIP_TRY(launch_lin_sys(align, P->grid_sys, meta, (const int32_t *)P->order_sys.p, P->n_sys, ctr + 2, d_seq,
+                             tcm, (uint8_t *)P->dm.p, (const int *)P->aux.p, d_cost, s));
     if (P->n_lds)
       HIP_TRY(launch_lin_fill(align, false, P->grid_lds, meta, (const int32_t *)P->order_lds.p, P->n_lds, ctr,
                               d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, nullptr, 0, s));

@@ -205,6 +205,32 @@ def test_linear_explicit_deltaw(ctx, oracle):
         assert res.cost[k] == oracle.cost_2(a, b, ref, dws[k])
 
 
+def test_linear_long_columns(ctx, oracle):
+    """Column sequences around 2,048: the systolic kernel takes up to 2,048
+    columns (a full-plane band of that width included), the row kernels
+    longer ones (narrow bands in LDS rings, a wide full-plane band)."""
+    from poyd_amd import engine
+    note("linear long columns")
+    rng = np.random.default_rng(11)
+    shapes = [(2047, 2047), (2048, 2048), (2049, 2049), (2100, 2200), (3072, 2048), (3100, 2049), (3600, 2300)]
+    A, B = [], []
+    for la, lb in shapes:
+        a = rand_seq(rng, la)
+        b = mutate(rng, a, 0.15, 0.05)[:lb] if lb <= la else rand_seq(rng, lb)
+        A.append(a)
+        B.append(b)
+    cm = engine.CostMatrix.of_sub_indel(1, 2)
+    ref = oracle.CM2.sub_indel(1, 2)
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
+    cres = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_COST, batch_of(A), batch_of(B))
+    for k, (a, b) in enumerate(zip(A, B)):
+        cost, oa, ob = oracle.align_2(a, b, ref)
+        assert res.cost[k] == cost, (k, len(a), len(b))
+        assert cres.cost[k] == cost, (k, len(a), len(b))
+        assert np.array_equal(res.seq(k, 0), oa), (k, len(a), len(b))
+        assert np.array_equal(res.seq(k, 1), ob), (k, len(a), len(b))
+
+
 def test_per_call_entry_points(ctx, oracle):
     from poyd_amd import engine
     note("per-call entry points")
