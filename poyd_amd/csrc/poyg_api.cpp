@@ -43,9 +43,10 @@ int lin_sys_blocks_per_cu(bool with_dm);
 hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
                            int32_t *cost, int *rowscratch, int row_width, hipStream_t s);
-hipError_t launch_lin_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
-                            const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
-                            hipStream_t s);
+int lin_trace_blocks_per_cu();
+hipError_t launch_lin_trace(int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                            const uint8_t *seq, const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len,
+                            int32_t *status, hipStream_t s);
 hipError_t launch_fitch_down(uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
                              const int32_t *ires, int64_t *added, hipStream_t s);
 hipError_t launch_fitch_up(uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
@@ -805,7 +806,8 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
                                  (const uint8_t *)P->dm.p, d_out, d_out_len, (int32_t *)P->status.p, s));
     }
     else
-      HIP_TRY(launch_lin_trace(meta, (const int32_t *)P->order_all.p, P->n, d_seq, tcm, (const uint8_t *)P->dm.p,
+      HIP_TRY(launch_lin_trace(std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * lin_trace_blocks_per_cu())), meta,
+                               (const int32_t *)P->order_all.p, P->n, ctr + 3, d_seq, tcm, (const uint8_t *)P->dm.p,
                                d_out, d_out_len, (int32_t *)P->status.p, s));
   }
   if (P->timing) {

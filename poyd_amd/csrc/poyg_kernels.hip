@@ -2112,55 +2112,133 @@ __global__ __launch_bounds__(kWave) void lin_fill_glb_kernel(
   }
 }
 
-// backtrack_2d (src/algn.c:3551-3666) + median_2 (:4042-4055), one thread per
+// backtrack_2d (src/algn.c:3551-3666) + median_2 (:4042-4055), one wave per
 // pair.  Slots: aligned s1 (caller's first), aligned s2, median_2.
-__global__ void lin_trace_kernel(const PairMeta *__restrict__ meta, const int32_t *__restrict__ order,
-                                 int n, const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm,
-                                 const uint8_t *__restrict__ dm_base, uint8_t *out, int32_t *out_len,
-                                 int32_t *status) {
+//
+// The walk itself is scalar.  It reads direction bytes from a 32 x 32 tile
+// whose bottom-right corner is the cell where the tile was loaded: lane L
+// holds 16 bytes of tile row L/2 (columns 16(L%2) .. +15) in four registers,
+// so one round of loads serves at least 32 steps (each step leaves the row
+// or the column of the previous one, never both tile edges at once).  One
+// more register carries the symbols of the tile's rows (lanes 0-31) and
+// columns (lanes 32-63).  A step's two output symbols go to lane (step % 64)
+// of two registers; every 64 steps the lanes store them (consecutive bytes,
+// right to left) and look up their medians, which a ballot compacts.
+constexpr int kTrTile = 32;
+constexpr int kTrDrop = 0x7ffffff0;   // buffer offset past any pair's direction band
+
+__device__ __forceinline__ int lin_row_lo(int i, int M, int N, int cs, int w, int h) {
+  int lo = 0, hi, lk, rk;
+  if (i > 0) lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
+  return lo;
+}
+
+__global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, const uint8_t *__restrict__ dm_base,
+    uint8_t *out, int32_t *out_len, int32_t *status) {
   __shared__ uint8_t s_med[32 * 32];
   for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_med[k] = tcm->median[k];
   __syncthreads();
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int p = order[t];
-  const PairMeta m = meta[p];
-  const uint8_t *s1 = seq + m.off_r;
-  const uint8_t *s2 = seq + m.off_c;
-  const int M = m.lr, N = m.lc, cs = m.p0, w = m.p1, h = m.p2, S = m.stride;
-  const uint8_t *dm = dm_base + m.dm_off;
-  uint8_t *o = out + m.out_off;
-  // swap == 0: the caller's first sequence is s1 (the longer one), and the
-  // backtrack runs with swaped = true (Sequence.Align.create_edited_2)
-  const bool swaped = m.swap == 0;
-  RevWriter w1, w2, wmed;
-  w1.init(o + (swaped ? 0 : 1) * m.cap, m.cap);   // aligned s1
-  w2.init(o + (swaped ? 1 : 0) * m.cap, m.cap);   // aligned s2
-  wmed.init(o + 2 * m.cap, m.cap);
-  int i = M - 1, j = N - 1, len = 0, bad = 0;
-  while (i >= 0 && j >= 0) {
-    int lo, hi, lk, rk;
-    if (i == 0) { lo = 0; hi = lin_row0_hi(N, cs, w); }
-    else lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
-    if (j < lo || j > hi) { bad = 1; break; }
-    const int d = dm[(size_t)i * S + (j - lo)];
-    int a, b;
-    if (d & 1) { a = s1[i]; b = s2[j]; --i; --j; }
-    else if (swaped ? (d & 2) : !(d & 4)) { a = kGap; b = s2[j]; --j; }
-    else { a = s1[i]; b = kGap; --i; }
-    w1.put(a); w2.put(b);
-    // median of the caller-ordered pair, gaps dropped
-    const int med = swaped ? s_med[(a & 31) * 32 + (b & 31)] : s_med[(b & 31) * 32 + (a & 31)];
-    if (med != kGap) wmed.put(med);
-    ++len;
-    if (j < 0 && i >= 0) { bad = 1; break; }
+  const int lane = lane_id();
+  for (;;) {
+    const int item = next_item(counter);
+    if (item >= n) break;
+    const int p = uniform32(order[item]);
+    const PairMeta m = uniform_meta(meta, p);
+    const uint8_t *s1 = seq + m.off_r;
+    const uint8_t *s2 = seq + m.off_c;
+    const int M = m.lr, N = m.lc, cs = m.p0, w = m.p1, h = m.p2, S = m.stride, cap = m.cap;
+    const __amdgpu_buffer_rsrc_t drsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(dm_base + m.dm_off), 0, M * S, 0x00020000);
+    uint8_t *o = out + m.out_off;
+    // swap == 0: the caller's first sequence is s1 (the longer one), and the
+    // backtrack runs with swaped = true (Sequence.Align.create_edited_2)
+    const bool swaped = m.swap == 0;
+    uint8_t *o1 = o + (swaped ? 0 : 1) * cap;   // aligned s1
+    uint8_t *o2 = o + (swaped ? 1 : 0) * cap;   // aligned s2
+    uint8_t *om = o + 2 * cap;
+    int i = M - 1, j = N - 1, len = 0, bad = 0;
+    int I = 0x3fffffff, J = 0x3fffffff;   // tile corner; the first step loads
+    uint32_t t0 = 0, t1 = 0, t2 = 0, t3 = 0, sym = 0, va = 0, vb = 0;
+    int outpos = cap, medpos = cap;
+    // stores the batch's lanes 0..cnt-1 and their non-gap medians
+    auto flush = [&](int cnt) {
+      const int a = (int)(va & 0xff), b = (int)(vb & 0xff);
+      const bool live = lane < cnt;
+      const int med = swaped ? s_med[(a & 31) * 32 + (b & 31)] : s_med[(b & 31) * 32 + (a & 31)];
+      const bool keep = live && med != kGap;
+      const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+      const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
+      if (live) {
+        o1[outpos - 1 - lane] = (uint8_t)a;
+        o2[outpos - 1 - lane] = (uint8_t)b;
+      }
+      if (keep) om[medpos - 1 - rank] = (uint8_t)med;
+      outpos -= cnt;
+      medpos -= __builtin_popcountll(km);
+    };
+    while (i >= 0 && j >= 0) {
+      if (i < I - (kTrTile - 1) || j < J - (kTrTile - 1)) {
+        I = i;
+        J = j;
+        const int r = I - (lane >> 1);
+        const int rlo = lin_row_lo(max(r, 0), M, N, cs, w, h);
+        const int c0 = J - (kTrTile - 1) + 16 * (lane & 1);
+        const int roff = r * S - rlo;
+        uint32_t wv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          wv[q] = 0;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int c = c0 + 4 * q + k;
+            const int off = r >= 0 && c >= rlo ? roff + c : kTrDrop;   // out of range reads 0
+            wv[q] |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(drsrc, off, 0, 0) << (8 * k);
+          }
+        }
+        t0 = wv[0];
+        t1 = wv[1];
+        t2 = wv[2];
+        t3 = wv[3];
+        sym = lane < kTrTile ? s1[max(I - lane, 0)] : s2[max(J - (kTrTile - 1) + (lane - kTrTile), 0)];
+      }
+      int lo = 0, hi, lk, rk;
+      if (i == 0) hi = lin_row0_hi(N, cs, w);
+      else lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
+      if (j < lo || j > hi) {
+        bad = 1;
+        break;
+      }
+      const int ri = I - i, cc = j - (J - (kTrTile - 1));
+      const int ln = ri * 2 + (cc >> 4), q = (cc >> 2) & 3;
+      const uint32_t word = q == 0 ? (uint32_t)__builtin_amdgcn_readlane((int)t0, ln)
+                            : q == 1 ? (uint32_t)__builtin_amdgcn_readlane((int)t1, ln)
+                            : q == 2 ? (uint32_t)__builtin_amdgcn_readlane((int)t2, ln)
+                                     : (uint32_t)__builtin_amdgcn_readlane((int)t3, ln);
+      const int d = (int)((word >> (8 * (cc & 3))) & 0xff);
+      const int sa = __builtin_amdgcn_readlane((int)sym, ri), sb = __builtin_amdgcn_readlane((int)sym, kTrTile + cc);
+      int a, b;
+      if (d & 1) { a = sa; b = sb; --i; --j; }
+      else if (swaped ? (d & 2) : !(d & 4)) { a = kGap; b = sb; --j; }
+      else { a = sa; b = kGap; --i; }
+      const bool mine = lane == (len & (kWave - 1));
+      va = mine ? (uint32_t)a : va;
+      vb = mine ? (uint32_t)b : vb;
+      ++len;
+      if ((len & (kWave - 1)) == 0) flush(kWave);
+      if (j < 0 && i >= 0) {
+        bad = 1;
+        break;
+      }
+    }
+    if (len & (kWave - 1)) flush(len & (kWave - 1));
+    if (lane == 0) om[medpos - 1] = kGap;   // the leading gap of median_2
+    out_len[3 * p + 0] = len;
+    out_len[3 * p + 1] = len;
+    out_len[3 * p + 2] = cap - (medpos - 1);
+    if (bad && lane == 0) atomicAdd(status, 1);
   }
-  wmed.put(kGap);
-  w1.flush(); w2.flush(); wmed.flush();
-  out_len[3 * p + 0] = len;
-  out_len[3 * p + 1] = len;
-  out_len[3 * p + 2] = m.cap - wmed.pos;
-  if (bad) atomicAdd(status, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -2424,12 +2502,17 @@ hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *met
   return hipGetLastError();
 }
 
-hipError_t launch_lin_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
-                            const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
-                            hipStream_t s) {
-  const int bs = 64;
-  hipLaunchKernelGGL(lin_trace_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, s, meta, order, n, seq, tcm, dm, out,
-                     out_len, status);
+int lin_trace_blocks_per_cu() {
+  int nb = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lin_trace_wave_kernel, kWave * 4, 0);
+  return (e == hipSuccess && nb > 0) ? nb : 2;
+}
+
+hipError_t launch_lin_trace(int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                            const uint8_t *seq, const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len,
+                            int32_t *status, hipStream_t s) {
+  hipLaunchKernelGGL(lin_trace_wave_kernel, dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq, tcm, dm,
+                     out, out_len, status);
   return hipGetLastError();
 }
 
