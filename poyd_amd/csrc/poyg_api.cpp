@@ -38,7 +38,7 @@ hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, c
                             hipStream_t s);
 hipError_t launch_lin_sys(bool with_dm, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, const int *aux, int32_t *cost,
-                          hipStream_t s);
+                          int32_t *status, hipStream_t s);
 int lin_sys_blocks_per_cu(bool with_dm);
 hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
@@ -589,7 +589,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   const int per_cu = affine ? 4 : 8;
   P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
   P->grid_glb = std::max(1, std::min(P->n_glb, 256));
-  if (P->n_sys) P->grid_sys = std::max(1, std::min((P->n_sys + 3) / 4, ctx->cus * lin_sys_blocks_per_cu(align)));
+  if (P->n_sys) P->grid_sys = std::max(1, std::min(P->n_sys, ctx->cus * lin_sys_blocks_per_cu(align)));
   if (affine) {
     // The fill with the most cells is the main kernel.  The side stream's
     // kernels are persistent too; the main kernel leaves them whole CUs, or
@@ -786,7 +786,7 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   } else {
     if (P->n_sys)
       HIP_TRY(launch_lin_sys(align, P->grid_sys, meta, (const int32_t *)P->order_sys.p, P->n_sys, ctr + 2, d_seq,
-                             tcm, (uint8_t *)P->dm.p, (const int *)P->aux.p, d_cost, s));
+                             tcm, (uint8_t *)P->dm.p, (const int *)P->aux.p, d_cost, (int32_t *)P->status.p, s));
     if (P->n_lds)
       HIP_TRY(launch_lin_fill(align, false, P->grid_lds, meta, (const int32_t *)P->order_lds.p, P->n_lds, ctr,
                               d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, nullptr, 0, s));

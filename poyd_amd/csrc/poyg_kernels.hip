@@ -1898,149 +1898,206 @@ __device__ __forceinline__ void lin_fill_pair_r(const PairMeta &m, const uint8_t
 }
 
 // Systolic form of the banded linear fill, for s2 of at most kLinStage
-// columns.  The wave sweeps strips of 64 rows: lane L holds row i0 + L and
-// reaches column j = c0 + t - L at step t (c0 = the strip's first column), so
-// the cells of one step lie on an anti-diagonal and depend only on the two
-// steps before it: (i-1, j) is lane L-1's result of step t-1 (one DPP shift),
-// (i-1, j-1) what this lane received from it a step earlier, (i, j-1) the
-// lane's own previous result.  Lane 0 reads row i0-1 from an LDS row that
-// lane 63 of the previous strip wrote, 63 columns behind lane 0's reads.
-// Each lane's band (lin_row) decides where it holds cells; the band's lower
-// and upper edges never decrease from row to row, so every lane's band lies
-// inside the strip's steps.  Out-of-band results are never read (a cell
-// reads (i-1, j) and (i-1, j-1) only where they lie in row i-1's band).
-// The loop changes no EXEC mask: stores that only some lanes make go to
-// per-lane LDS slots or through an out-of-range buffer offset, which the
-// hardware drops.  Same cell rules as lin_fill_pair (src/algn.c:375-620).
+// columns.  The rows are cut into strips of 64: lane L of a strip holds row
+// i0 + L and reaches column j = c0 + t - L at step t (c0 = the strip's first
+// column), so the cells of one step lie on an anti-diagonal and depend only
+// on the two steps before it: (i-1, j) is lane L-1's result of step t-1 (one
+// DPP shift), (i-1, j-1) what this lane received from it a step earlier,
+// (i, j-1) the lane's own previous result.  Lane 0 reads row i0-1 from the
+// LDS row that lane 63 of the previous strip wrote.  Each lane's band
+// (lin_row) decides where it holds cells; the band's lower and upper edges
+// never decrease from row to row, so every lane's band lies inside the
+// strip's steps.  Out-of-band results are never read (a cell reads (i-1, j)
+// and (i-1, j-1) only where they lie in row i-1's band).  The step loop
+// changes no EXEC mask: stores that only some lanes make go to per-lane LDS
+// slots or through an out-of-range buffer offset, which the hardware drops.
+// Same cell rules as lin_fill_pair (src/algn.c:375-620).
+//
+// The four waves of a workgroup share one pair: wave k takes strips k,
+// k+4, ...  A strip runs behind its predecessor: every kSysSync steps it
+// waits until the predecessor's progress counter (strip * kSysStripTag +
+// last column its lane 63 has written) covers the columns its lane 0 reads
+// next, and publishes its own.  Strip s writes hand-off row s % 4; strip
+// s+4 overwrites it only after strips s+1 .. s+3 have each run 63 columns
+// ahead of their successors, by which time strip s+1 has read it.
 constexpr int kSysRow = kWave + 1 + kLinStage;   // 64 dummy slots, column -1, columns 0..kLinStage-1
 constexpr int kSysWaves = 4;
 constexpr int kSysDrop = 0x7ffffff0;             // buffer offset past any pair's direction band
+constexpr int kSysSync = 8;                      // steps between hand-off checks
+constexpr int kSysStripTag = 1 << 16;            // progress = strip * kSysStripTag + column
+constexpr int kSysSpin = 1 << 22;                // polls before a hand-off wait gives up
+
+struct SysLds {
+  int cost[32 * 32];
+  int prep[32];
+  int tail[32];
+  int row[kSysWaves][kSysRow];   // hand-off rows; row[kSysWaves - 1] holds row 0 first
+  int prog[kSysWaves + kWave];   // progress per hand-off row, then one dummy slot per lane
+  uint8_t x[kLinStage];          // s2, masked to 5 bits
+  int item;
+};
 
 template <bool kDm>
-__device__ __forceinline__ void lin_sys_pair(const PairMeta &m, const uint8_t *__restrict__ seq, const int *s_cost,
-                                             const int *s_tail, uint8_t *__restrict__ dm_base, const int *aux,
-                                             int32_t *__restrict__ cost_out, int pidx, int *rowbuf, uint8_t *sx) {
+__device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, int cs, int w, int h, int S,
+                                              const uint8_t *__restrict__ s1, SysLds &L,
+                                              __amdgpu_buffer_rsrc_t drsrc, int32_t *__restrict__ cost_out,
+                                              int pidx) {
   const int lane = lane_id();
-  const uint8_t *s1 = seq + uniform64(m.off_r);   // the longer sequence (rows)
-  const uint8_t *s2 = seq + uniform64(m.off_c);
-  const int M = uniform32(m.lr), N = uniform32(m.lc), cs = uniform32(m.p0), w = uniform32(m.p1),
-            h = uniform32(m.p2), S = uniform32(m.stride);
-  pidx = uniform32(pidx);
-  const int *GP = aux + uniform64(m.aux_off) + N;
-  uint8_t *dm = dm_base + uniform64(m.dm_off);
-  int *row = rowbuf + kWave + 1;   // row[-1] is a slot; rowbuf[lane] is the lane's dummy slot
-  for (int j = lane; j < N; j += kWave) sx[j] = s2[j] & 31;
-  // row 0 (algn_fill_first_row, :584-607, or the first row of algn_fill_plane)
-  const int hi0 = lin_row0_hi(N, cs, w);
-  for (int j = lane; j <= hi0; j += kWave) {
-    row[j] = GP[j];
-    if (kDm) dm[j] = j == 0 ? 1 : 2;   // ALIGN, then INSERT
-  }
-  if (M == 1) {
-    cost_out[pidx] = GP[N - 1];   // every lane stores the same value
-    return;
-  }
-  wave_sync_mem();
-  const __amdgpu_buffer_rsrc_t drsrc =
-      __builtin_amdgcn_make_buffer_rsrc(dm, 0, kDm ? M * S : 0, 0x00020000);
-  const int *gcost = s_cost + kGap * 32;
-  int fin = 0, i0 = 1;
-  for (;; i0 += kWave) {
-    const int i = min(i0 + lane, M - 1);
-    const bool valid = i0 + lane < M;
-    int lo, hi, lk, rk;
-    lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
-    lo = valid ? lo : 0x3fffffff;   // rows past the last one hold no cell
-    hi = valid ? hi : -1;
-    const int c0 = uniform32(lo);
-    const int last_lane = min(kWave - 1, M - 1 - i0);
-    const int t_end = __builtin_amdgcn_readlane(hi, last_lane) - c0 + last_lane;
-    const int si = s1[i] & 31;
-    const int c_del = s_cost[si * 32 + kGap];
-    const int tail = s_tail[si];
-    const int *crow = s_cost + si * 32;
-    const int first_add = lk == LK_FIRST_TAIL ? tail : c_del;
-    const int lo_first = lk != LK_LEFT ? lo : -2;   // column of the first-column cell, if any
-    const int hi_right = rk == RK_RIGHT ? hi : -2;  // column without DELETE
-    const int hi_last = rk == RK_LAST && hi > 0 ? hi : -2;   // column re-checked with tail
-    const int drow = i * S;   // byte offset of the row in the pair's direction band
-    int res = kBig, acc = 0;
-    int up_prev = row[c0 - 1];
-    int j = c0 - lane;
-    // look-ahead: symbols two steps ahead, cost-table reads one step ahead
-    int x1 = sx[min(max(j + 1, 0), N - 1)];
-    int x0 = sx[min(max(j, 0), N - 1)];
-    int cr = crow[x0], gj = gcost[x0];
-    int fill = row[c0];
-    dpp_exec_guard();
-    for (int t = 0; t <= t_end; ++t, ++j) {
-      const int x2 = sx[min(max(j + 2, 0), N - 1)];
-      const int cr_n = crow[x1], gj_n = gcost[x1];
-      const int fill_n = row[min(c0 + t + 1, N - 1)];
-      const int up = __builtin_amdgcn_update_dpp(fill, res, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
-      const int diag = up_prev;
-      const int t3 = diag + cr;
-      const int t1 = j == hi_right ? kBig : up + c_del;
-      const bool has_ins = j != lo;
-      const int t2 = has_ins ? res + gj : kBig;
-      int best = min(min(t1, t2), t3);
-      int msk = (int)(t3 == best) | ((int)(has_ins && t2 == best) << 1) | ((int)(t1 == best) << 2);
-      // last column: DELETE with tail replaces a larger minimum and joins an
-      // equal one (bit arithmetic, so that no branch splits the step)
-      const bool is_last = j == hi_last;
-      const int cst = tail + up;
-      const int lt = (int)(is_last && cst < best), le = (int)(is_last && cst <= best);
-      best = min(best, is_last ? cst : kBig);
-      msk = (msk & (lt - 1)) | (le << 2);
-      const bool first = j == lo_first;
-      best = first ? up + first_add : best;
-      msk = first ? 4 : msk;
-      const bool act = j >= lo && j <= hi;
-      res = act ? best : kBig;
-      fin = act && j == N - 1 ? best : fin;
-      // lane 63 hands its row to the next strip
-      rowbuf[act && lane == kWave - 1 ? kWave + 1 + j : lane] = best;
-      if (kDm) {
-        const int b = j - lo;
-        acc |= act ? msk << (8 * (b & 3)) : 0;
-        const bool flush = act && ((b & 3) == 3 || j == hi);
-        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)acc, drsrc, flush ? drow + (b & ~3) : kSysDrop, 0,
-                                              0);
-        acc = flush ? 0 : acc;
+  const int i0 = 1 + kWave * s;
+  const int *prow = &L.row[(s + kSysWaves - 1) % kSysWaves][kWave + 1];   // prow[-1] is a slot
+  int *mine = &L.row[s % kSysWaves][0];   // mine[kWave + 1 + j] = column j, mine[lane] = dummy
+  const int pslot = (s + kSysWaves - 1) % kSysWaves, mslot = s % kSysWaves;
+  const uint8_t *sx = L.x;
+  const int *gcost = L.cost + kGap * 32;
+  const int i = min(i0 + lane, M - 1);
+  const bool valid = i0 + lane < M;
+  int lo, hi, lk, rk;
+  lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
+  lo = valid ? lo : 0x3fffffff;   // rows past the last one hold no cell
+  hi = valid ? hi : -1;
+  const int c0 = uniform32(lo);
+  const int last_lane = min(kWave - 1, M - 1 - i0);
+  const int t_end = __builtin_amdgcn_readlane(hi, last_lane) - c0 + last_lane;
+  // the predecessor's last row (row 0 for the first strip) ends at hi_prev
+  int plo, hi_prev, plk, prk;
+  if (i0 - 1 == 0) hi_prev = lin_row0_hi(N, cs, w);
+  else lin_row(i0 - 1, M, N, cs, w, h, plo, hi_prev, plk, prk);
+  hi_prev = uniform32(hi_prev);
+  bool ok = true;
+  // waits until the predecessor's last row holds every column up to col
+  auto wait_for = [&](int col) {
+    if (s == 0 || !ok) return;
+    const int want = (s - 1) * kSysStripTag + min(col, hi_prev);
+    for (int k = 0;; ++k) {
+      const int v = uniform32(__hip_atomic_load(&L.prog[pslot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (v >= want) break;
+      if (k == kSysSpin) {
+        ok = false;   // not reached: every strip runs to its end
+        break;
       }
-      up_prev = up;
-      fill = fill_n;
-      cr = cr_n;
-      gj = gj_n;
-      x1 = x2;
+      __builtin_amdgcn_s_sleep(1);
     }
-    wave_sync_mem();
-    if (i0 + kWave >= M) break;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  };
+  // lane 0 publishes (the other lanes store to their dummy slots), after
+  // this wave's earlier LDS writes have completed
+  auto publish = [&](int col) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+    __hip_atomic_store(&L.prog[lane == 0 ? mslot : kSysWaves + lane], s * kSysStripTag + col, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  const int si = s1[i] & 31;
+  const int c_del = L.cost[si * 32 + kGap];
+  const int tail = L.tail[si];
+  const int *crow = L.cost + si * 32;
+  const int first_add = lk == LK_FIRST_TAIL ? tail : c_del;
+  const int lo_first = lk != LK_LEFT ? lo : -2;   // column of the first-column cell, if any
+  const int hi_right = rk == RK_RIGHT ? hi : -2;  // column without DELETE
+  const int hi_last = rk == RK_LAST && hi > 0 ? hi : -2;   // column re-checked with tail
+  const int drow = i * S;   // byte offset of the row in the pair's direction band
+  wait_for(c0 + kSysSync);
+  int res = kBig, acc = 0, fin = 0;
+  int up_prev = prow[c0 - 1];
+  int j = c0 - lane;
+  // look-ahead: symbols two steps ahead, cost-table reads one step ahead
+  int x1 = sx[min(max(j + 1, 0), N - 1)];
+  const int x0 = sx[min(max(j, 0), N - 1)];
+  int cr = crow[x0], gj = gcost[x0];
+  int fill = prow[c0];
+  dpp_exec_guard();
+  for (int t = 0; t <= t_end; ++t, ++j) {
+    if (t != 0 && (t & (kSysSync - 1)) == 0) {
+      publish(c0 + t - kWave);   // lane 63 wrote columns up to c0 + t - 64
+      wait_for(c0 + t + kSysSync);
+    }
+    const int x2 = sx[min(max(j + 2, 0), N - 1)];
+    const int cr_n = crow[x1], gj_n = gcost[x1];
+    const int fill_n = prow[min(c0 + t + 1, N - 1)];
+    const int up = __builtin_amdgcn_update_dpp(fill, res, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    const int diag = up_prev;
+    const int t3 = diag + cr;
+    const int t1 = j == hi_right ? kBig : up + c_del;
+    const bool has_ins = j != lo;
+    const int t2 = has_ins ? res + gj : kBig;
+    int best = min(min(t1, t2), t3);
+    int msk = (int)(t3 == best) | ((int)(has_ins && t2 == best) << 1) | ((int)(t1 == best) << 2);
+    // last column: DELETE with tail replaces a larger minimum and joins an
+    // equal one (bit arithmetic, so that no branch splits the step)
+    const bool is_last = j == hi_last;
+    const int cst = tail + up;
+    const int lt = (int)(is_last && cst < best), le = (int)(is_last && cst <= best);
+    best = min(best, is_last ? cst : kBig);
+    msk = (msk & (lt - 1)) | (le << 2);
+    const bool first = j == lo_first;
+    best = first ? up + first_add : best;
+    msk = first ? 4 : msk;
+    const bool act = j >= lo && j <= hi;
+    res = act ? best : kBig;
+    fin = act && j == N - 1 ? best : fin;
+    // lane 63 hands its row to the next strip
+    mine[act && lane == kWave - 1 ? kWave + 1 + j : lane] = best;
+    if (kDm) {
+      const int b = j - lo;
+      acc |= act ? msk << (8 * (b & 3)) : 0;
+      const bool flush = act && ((b & 3) == 3 || j == hi);
+      __builtin_amdgcn_raw_buffer_store_b32((uint32_t)acc, drsrc, flush ? drow + (b & ~3) : kSysDrop, 0, 0);
+      acc = flush ? 0 : acc;
+    }
+    up_prev = up;
+    fill = fill_n;
+    cr = cr_n;
+    gj = gj_n;
+    x1 = x2;
   }
-  if (lane == M - 1 - i0) cost_out[pidx] = fin;
+  publish(kSysStripTag - 1);   // every column
+  if (s == nstrips - 1 && lane == M - 1 - i0) cost_out[pidx] = fin;
+  return ok;
 }
 
-// One pair per wave, kSysWaves waves per workgroup, pairs from the work list.
+// One pair per workgroup (kSysWaves waves), pairs from the work list.
 template <bool kDm>
 __global__ __launch_bounds__(kWave * kSysWaves) void lin_sys_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
-    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm, const int *aux,
-    int32_t *cost_out) {
-  __shared__ int s_cost[32 * 32];
-  __shared__ int s_prep[32];
-  __shared__ int s_tail[32];
-  __shared__ int s_row[kSysWaves][kSysRow];
-  __shared__ uint8_t s_x[kSysWaves][kLinStage];
-  load_tables(tcm, s_cost, s_prep, nullptr);
-  for (int k = threadIdx.x; k < 32; k += blockDim.x) s_tail[k] = tcm->tail[k];
-  __syncthreads();
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, const int *aux,
+    int32_t *cost_out, int32_t *status) {
+  __shared__ SysLds L;
+  load_tables(tcm, L.cost, L.prep, nullptr);
+  for (int k = threadIdx.x; k < 32; k += blockDim.x) L.tail[k] = tcm->tail[k];
   const int wv = wave_in_group();
   for (;;) {
-    const int k = next_item(counter);
+    __syncthreads();   // the previous pair is done in every wave
+    if (threadIdx.x == 0) L.item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int k = uniform32(L.item);
     if (k >= n) break;
     const int p = uniform32(order[k]);
     const PairMeta m = uniform_meta(meta, p);
-    lin_sys_pair<kDm>(m, seq, s_cost, s_tail, dm, aux, cost_out, p, &s_row[wv][0], &s_x[wv][0]);
+    const uint8_t *s1 = seq + m.off_r;   // the longer sequence (rows)
+    const uint8_t *s2 = seq + m.off_c;
+    const int M = m.lr, N = m.lc, cs = m.p0, w = m.p1, S = m.stride;
+    const int *GP = aux + m.aux_off + N;
+    uint8_t *dm = dm_base + m.dm_off;
+    // stage s2 and row 0 (algn_fill_first_row, :584-607, or the first row of
+    // algn_fill_plane), reset the progress counters
+    const int hi0 = lin_row0_hi(N, cs, w);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) L.x[j] = s2[j] & 31;
+    for (int j = threadIdx.x; j <= hi0; j += blockDim.x) {
+      L.row[kSysWaves - 1][kWave + 1 + j] = GP[j];
+      if (kDm) dm[j] = j == 0 ? 1 : 2;   // ALIGN, then INSERT
+    }
+    if (threadIdx.x < kSysWaves) L.prog[threadIdx.x] = -1;
+    __syncthreads();
+    if (M == 1) {
+      if (threadIdx.x == 0) cost_out[p] = GP[N - 1];
+      continue;
+    }
+    const __amdgpu_buffer_rsrc_t drsrc = __builtin_amdgcn_make_buffer_rsrc(dm, 0, kDm ? M * S : 0, 0x00020000);
+    const int nstrips = (M - 1 + kWave - 1) / kWave;
+    bool ok = true;
+    for (int s = wv; s < nstrips; s += kSysWaves)
+      ok = lin_sys_strip<kDm>(s, nstrips, M, N, cs, w, m.p2, S, s1, L, drsrc, cost_out, p) && ok;
+    if (!ok && lane_id() == 0) atomicAdd(status, 1);
   }
 }
 
@@ -2468,13 +2525,13 @@ int lin_sys_blocks_per_cu(bool with_dm) { return with_dm ? sys_occupancy<true>()
 
 hipError_t launch_lin_sys(bool with_dm, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, const int *aux, int32_t *cost,
-                          hipStream_t s) {
+                          int32_t *status, hipStream_t s) {
   if (with_dm)
     hipLaunchKernelGGL(lin_sys_kernel<true>, dim3(grid), dim3(kWave * kSysWaves), 0, s, meta, order, n, counter, seq,
-                       tcm, dm, aux, cost);
+                       tcm, dm, aux, cost, status);
   else
     hipLaunchKernelGGL(lin_sys_kernel<false>, dim3(grid), dim3(kWave * kSysWaves), 0, s, meta, order, n, counter,
-                       seq, tcm, dm, aux, cost);
+                       seq, tcm, dm, aux, cost, status);
   return hipGetLastError();
 }
 
