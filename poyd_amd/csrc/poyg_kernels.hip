@@ -2218,6 +2218,7 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
     int i = M - 1, j = N - 1, len = 0, bad = 0;
     int I = 0x3fffffff, J = 0x3fffffff;   // tile corner; the first step loads
     uint32_t t0 = 0, t1 = 0, t2 = 0, t3 = 0, sym = 0, va = 0, vb = 0;
+    int tlo = 0, thi = 0;   // lanes 0-31: band of tile row lane (row I - lane)
     int outpos = cap, medpos = cap;
     // stores the batch's lanes 0..cnt-1 and their non-gap medians
     auto flush = [&](int cnt) {
@@ -2241,6 +2242,13 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
         J = j;
         const int r = I - (lane >> 1);
         const int rlo = lin_row_lo(max(r, 0), M, N, cs, w, h);
+        {
+          const int rr = max(I - (lane & (kTrTile - 1)), 0);
+          int lo, hi, lk, rk;
+          lin_row(max(rr, 1), M, N, cs, w, h, lo, hi, lk, rk);
+          tlo = rr == 0 ? 0 : lo;
+          thi = rr == 0 ? lin_row0_hi(N, cs, w) : hi;
+        }
         const int c0 = J - (kTrTile - 1) + 16 * (lane & 1);
         const int roff = r * S - rlo;
         uint32_t wv[4];
@@ -2259,26 +2267,32 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
         t2 = wv[2];
         t3 = wv[3];
         sym = lane < kTrTile ? s1[max(I - lane, 0)] : s2[max(J - (kTrTile - 1) + (lane - kTrTile), 0)];
+        // wait here, once per tile, so that no step waits on the counter
+        // that the output stores also hold
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
       }
-      int lo = 0, hi, lk, rk;
-      if (i == 0) hi = lin_row0_hi(N, cs, w);
-      else lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
+      const int ri = I - i, cc = j - (J - (kTrTile - 1));
+      const int lo = __builtin_amdgcn_readlane(tlo, ri), hi = __builtin_amdgcn_readlane(thi, ri);
       if (j < lo || j > hi) {
         bad = 1;
         break;
       }
-      const int ri = I - i, cc = j - (J - (kTrTile - 1));
       const int ln = ri * 2 + (cc >> 4), q = (cc >> 2) & 3;
-      const uint32_t word = q == 0 ? (uint32_t)__builtin_amdgcn_readlane((int)t0, ln)
-                            : q == 1 ? (uint32_t)__builtin_amdgcn_readlane((int)t1, ln)
-                            : q == 2 ? (uint32_t)__builtin_amdgcn_readlane((int)t2, ln)
-                                     : (uint32_t)__builtin_amdgcn_readlane((int)t3, ln);
+      // all four words, then a select: no branch per step
+      const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)t0, ln);
+      const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)t1, ln);
+      const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)t2, ln);
+      const uint32_t w3 = (uint32_t)__builtin_amdgcn_readlane((int)t3, ln);
+      const uint32_t word = (w0 & (0u - (uint32_t)(q == 0))) | (w1 & (0u - (uint32_t)(q == 1))) |
+                            (w2 & (0u - (uint32_t)(q == 2))) | (w3 & (0u - (uint32_t)(q == 3)));
       const int d = (int)((word >> (8 * (cc & 3))) & 0xff);
       const int sa = __builtin_amdgcn_readlane((int)sym, ri), sb = __builtin_amdgcn_readlane((int)sym, kTrTile + cc);
-      int a, b;
-      if (d & 1) { a = sa; b = sb; --i; --j; }
-      else if (swaped ? (d & 2) : !(d & 4)) { a = kGap; b = sb; --j; }
-      else { a = sa; b = kGap; --i; }
+      // ALIGN first, then INSERT before DELETE when swaped, else DELETE first
+      const bool al = d & 1;
+      const bool ins = !al && (swaped ? (d & 2) != 0 : (d & 4) == 0);
+      const int a = ins ? kGap : sa, b = !al && !ins ? kGap : sb;
+      i -= !ins;
+      j -= al || ins;
       const bool mine = lane == (len & (kWave - 1));
       va = mine ? (uint32_t)a : va;
       vb = mine ? (uint32_t)b : vb;

@@ -534,17 +534,34 @@ def fitch_pairs(ctx: Context, a: np.ndarray, b: np.ndarray):
     return res, d_add.download(np.int64, n)
 
 
-def batch_deltaw(a_batch, b_batch) -> np.ndarray:
-    """deltaw of Sequence.Align.cost_2 for every pair, vectorised."""
+def batch_deltaw(a_batch, b_batch, dw=None) -> np.ndarray:
+    """deltaw of Sequence.Align.cost_2 for every pair, vectorised; dw (one
+    value per pair, or None) is the ?deltaw argument, as in deltaw()."""
     la = a_batch.ln.astype(np.int64)
     lb = b_batch.ln.astype(np.int64)
-    ga = np.add.reduceat((a_batch.seq & 16) != 0, a_batch.off) if len(la) else la
-    gb = np.add.reduceat((b_batch.seq & 16) != 0, b_batch.off) if len(lb) else lb
+    ga = _gaps_per_item(a_batch)
+    gb = _gaps_per_item(b_batch)
     longer = np.maximum(la, lb)
     dif = np.abs(la - lb)
     lower = np.floor(longer.astype(np.float64) * 0.10).astype(np.int64)
-    extra = np.where(dif < lower, lower // 2, 2)
+    if dw is None:
+        extra = np.where(dif < lower, lower // 2, 2)
+    else:
+        extra = np.where(dif < lower, lower, np.asarray(dw, np.int64))
     return (np.maximum(ga, gb) + extra).astype(np.int32)
+
+
+def _gaps_per_item(batch) -> np.ndarray:
+    """count_gaps of every sequence of a SeqBatch (empty ones count 0)."""
+    n = len(batch.ln)
+    if n == 0:
+        return np.zeros(0, np.int64)
+    gap = (batch.seq & 16) != 0
+    if (batch.ln > 0).all():   # reduceat sums [off[k], off[k+1]) when every item is non-empty
+        return np.add.reduceat(gap, batch.off).astype(np.int64)
+    pre = np.concatenate([[0], np.cumsum(gap, dtype=np.int64)])
+    off = batch.off.astype(np.int64)
+    return pre[off + batch.ln] - pre[off]
 
 
 def run_pairs(ctx: Context, cm: CostMatrix, mode: int, a_batch, b_batch, dws=None,

@@ -48,6 +48,8 @@ from .treedata import GAP, Dataset, Tree
 
 def is_empty(s: np.ndarray) -> bool:
     """Sequence.is_empty (src/sequence.ml:442-450): every element is a gap."""
+    if len(s) > 1 and s[1] != GAP:   # the common case: a base after the leading gap
+        return False
     return bool((s == GAP).all())
 
 
@@ -320,5 +322,9 @@ class EngineBackend:
         e = self.engine
         if self.affine:
             return self._run(e.POYG_MODE_AFFINE_COST, pairs).cost.tolist()
-        dws = np.array([e.deltaw(a, b, max(8, abs(len(a) - len(b)))) for a, b in pairs], np.int32)
-        return self._run(e.POYG_MODE_LINEAR_COST, pairs, dws).cost.tolist()
+        A = self._batch([p[0] for p in pairs])
+        B = self._batch([p[1] for p in pairs])
+        dws = e.batch_deltaw(A, B, np.maximum(8, np.abs(A.ln.astype(np.int64) - B.ln)))
+        r = e.run_pairs(self.ctx, self.cm, e.POYG_MODE_LINEAR_COST, A, B, dws)
+        self.cells += r.cells
+        return r.cost.tolist()

@@ -67,6 +67,29 @@ def test_engine_deltaw_equals_oracle(oracle):
         assert engine.deltaw(a, b, 7) == oracle.deltaw(a, b, cm, 7)
 
 
+def test_batch_deltaw_equals_oracle(oracle):
+    """The vectorised band parameter of a batch, with and without ?deltaw."""
+    from poyd_amd import engine, synth
+    cm = oracle.CM2.sub_indel(1, 2)
+    rng = np.random.default_rng(2)
+
+    def batch(seqs):
+        ln = np.array([len(s) for s in seqs], np.int32)
+        off = np.concatenate([[0], np.cumsum(ln[:-1])]).astype(np.int64)
+        return synth.SeqBatch(np.concatenate(seqs).astype(np.uint8), off, ln)
+
+    A, B = [], []
+    for _ in range(150):
+        A.append(np.concatenate([[16], rng.integers(1, 32, int(rng.integers(0, 300)))]).astype(np.uint8))
+        B.append(np.concatenate([[16], rng.integers(1, 32, int(rng.integers(0, 300)))]).astype(np.uint8))
+    dws = rng.integers(0, 40, len(A))
+    got = engine.batch_deltaw(batch(A), batch(B))
+    got_dw = engine.batch_deltaw(batch(A), batch(B), dws)
+    for k, (a, b) in enumerate(zip(A, B)):
+        assert got[k] == oracle.deltaw(a, b, cm), k
+        assert got_dw[k] == oracle.deltaw(a, b, cm, int(dws[k])), k
+
+
 def test_host_column_rules_equal_oracle(oracle):
     """median_2 / with_gaps / ancestor_2 / verify_2 / worst_2 are host code
     in the engine; they run without a GPU."""
