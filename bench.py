@@ -243,7 +243,8 @@ def main() -> int:
             "traffic": measured_traffic(kind),
             "kernel": ("fill stage: aff_quad_kernel (main and side-stream launches) with the one-set and "
                        "lane-set fills, one span of HIP events on the main stream") if kind == "affine"
-                      else "cube_fill_kernel" if kind == "cube" else "lin_fill_lds_kernel",
+                      else "cube_rows_kernel (every row class)" if kind == "cube"
+                      else "lin_sys_kernel (systolic strips, four waves per pair)",
         },
         "valu_roofline": {
             "bound": "valu",
@@ -343,7 +344,7 @@ def bench_tree(args) -> int:
             "frac": round(valu / VALU_PEAK_TOPS, 6),
             "traffic": None,
             "kernel": "whole step (one plan per tree level: a few pairs per launch, each launch as long as "
-                      "one wave takes for its longest pair)",
+                      "its longest pair takes on one workgroup)",
         },
     }
     if not args.no_cpu:
