@@ -201,8 +201,13 @@ def main() -> int:
     ops = OPS_PER_CELL[kind]
     valu_tops = cells * ops / (fill_ms / 1e3) / 1e12
     # algorithmic HBM bytes of the fill: one direction byte per band cell,
-    # both sequences read once, one int32 cost written per pair
-    alg_bytes = cells + int(sum(int(x.sum()) for x in seq_lens)) + 4 * plan.n
+    # both sequences read once, one int32 cost written per pair; the affine
+    # fills also run the backtrace of their pairs, which reads both
+    # sequences again and writes the aligned outputs
+    seq_bytes = int(sum(int(x.sum()) for x in seq_lens))
+    alg_bytes = cells + seq_bytes + 4 * plan.n
+    if kind == "affine":
+        alg_bytes += seq_bytes + int(d_len.download(np.int32, plan.n * plan.slots).astype(np.int64).sum())
     fill_gbs = alg_bytes / (fill_ms / 1e3) / 1e9
     if kind == "cube":
         matrix_cells = cells
@@ -241,8 +246,9 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(fill_gbs / HBM_PEAK_GBS, 5),
             "traffic": measured_traffic(kind),
-            "kernel": ("fill stage: aff_quad_kernel (main and side-stream launches) with the one-set and "
-                       "lane-set fills, one span of HIP events on the main stream") if kind == "affine"
+            "kernel": ("fill stage with the backtrace fused: aff_quad_kernel (main and side-stream launches) "
+                       "and the one-set and lane-set fills, each wave tracing its pairs after filling them; "
+                       "one span of HIP events on the main stream") if kind == "affine"
                       else "cube_rows_kernel (every row class)" if kind == "cube"
                       else "lin_sys_kernel (systolic strips, four waves per pair)",
         },

@@ -25,14 +25,11 @@ hipError_t launch_pair_pure(int grid, const PairMeta *meta, const int32_t *order
                             uint8_t *pure, hipStream_t s);
 hipError_t launch_aff_quad(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                            const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
-                           const uint8_t *pure, hipStream_t s);
+                           const uint8_t *pure, const TraceOut *tro, hipStream_t s);
 hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
-                           uint32_t *sink, hipStream_t s);
-int aff_trace_wave_blocks_per_cu();
-hipError_t launch_aff_trace_wave(int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
-                                 const uint8_t *seq, const DevTcm *tcm, const uint8_t *dm, uint8_t *out,
-                                 int32_t *out_len, int32_t *status, hipStream_t s);
+                           uint32_t *sink, const TraceOut *tro, hipStream_t s);
+hipError_t launch_set_trace_out(TraceOut *dst, uint8_t *out, int32_t *out_len, int32_t *status, hipStream_t s);
 hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
                             const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
                             hipStream_t s);
@@ -221,14 +218,15 @@ struct poyg_plan {
   // device
   // work lists (longest first): row kernels (LDS / global rows), the
   // four-pairs-per-wave kernel, the anti-diagonal kernel per lane-set count
-  // (1, 2, 4), and the backtraces; pure: per pair, one sequence without
-  // gap-bearing characters (written by a scan before the fills)
+  // (1, 2, 4; these two trace their own pairs), and the row kernels'
+  // backtraces; pure: per pair, one sequence without gap-bearing characters
+  // (written by a scan before the fills)
   // (linear pairs: the systolic kernel takes every pair whose column
   // sequence fits its LDS row, the row kernels the rest)
-  DevBuf tcm, meta, order_all, order_lds, order_glb, order_sys, order_quad, order_diag[3], order_rows, order_tr, dm,
-      aux, rows, counters, status, sink, pure;
-  int n_lds = 0, n_glb = 0, n_sys = 0, n_quad = 0, n_diag[3] = {0, 0, 0}, n_rows = 0, n_tr = 0;
-  int grid_lds = 0, grid_glb = 0, grid_sys = 0, grid_quad = 0, grid_diag[3] = {0, 0, 0}, grid_tr = 0,
+  DevBuf tcm, meta, order_all, order_lds, order_glb, order_sys, order_quad, order_diag[3], order_rows, dm,
+      aux, rows, counters, status, sink, pure, trout;
+  int n_lds = 0, n_glb = 0, n_sys = 0, n_quad = 0, n_diag[3] = {0, 0, 0}, n_rows = 0;
+  int grid_lds = 0, grid_glb = 0, grid_sys = 0, grid_quad = 0, grid_diag[3] = {0, 0, 0},
       row_width = 0;
   // the fill with the most cells runs on the main stream ("main": 1 the
   // four-pair kernel, 2 the one-set anti-diagonal kernel, 0 none); every
@@ -242,8 +240,8 @@ struct poyg_plan {
   ~poyg_plan() {
     if (triple) poyg_triple_plan_destroy(triple);
     for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_sys, &order_quad, &order_diag[0],
-                      &order_diag[1], &order_diag[2], &order_rows, &order_tr, &dm, &aux, &rows, &counters, &status,
-                      &sink, &pure})
+                      &order_diag[1], &order_diag[2], &order_rows, &dm, &aux, &rows, &counters, &status,
+                      &sink, &pure, &trout})
       b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -553,17 +551,15 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a] > cells[b]; });
-  std::vector<int32_t> o_lds, o_glb, o_sys, o_quad, o_diag[3], o_rows, o_tr;
+  std::vector<int32_t> o_lds, o_glb, o_sys, o_quad, o_diag[3], o_rows;
   int64_t c_quad3 = 0, c_diag1 = 0;
   for (int k : order) {
     if (quad[k]) {
       o_quad.push_back(k);
       c_quad3 += cells[k];
-      o_tr.push_back(k);
     } else if (diag[k]) {
       o_diag[diag[k] == 1 ? 0 : (diag[k] == 2 ? 1 : 2)].push_back(k);
       if (diag[k] == 1) c_diag1 += cells[k];
-      o_tr.push_back(k);
     } else {
       (sys[k] ? o_sys : glb[k] ? o_glb : o_lds).push_back(k);
       o_rows.push_back(k);
@@ -573,7 +569,6 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   P->n_glb = (int)o_glb.size();
   P->n_sys = (int)o_sys.size();
   P->n_rows = (int)o_rows.size();
-  P->n_tr = (int)o_tr.size();
   // persistent grids: exactly the workgroups that can be resident at once
   for (int q = 0; q < 3; ++q) {
     P->n_diag[q] = (int)o_diag[q].size();
@@ -584,7 +579,6 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   P->n_quad = (int)o_quad.size();
   if (P->n_quad)   // four pairs per wave, four waves per workgroup
     P->grid_quad = std::max(1, std::min((P->n_quad + 15) / 16, ctx->cus * aff_quad_blocks_per_cu(align)));
-  if (P->n_tr) P->grid_tr = std::max(1, std::min((P->n_tr + 3) / 4, ctx->cus * aff_trace_wave_blocks_per_cu()));
   P->row_width = row_width;
   const int per_cu = affine ? 4 : 8;
   P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
@@ -653,13 +647,13 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   rc |= P->order_quad.alloc(sizeof(int32_t) * (size_t)P->n_quad);
   rc |= P->pure.alloc((size_t)n);
   rc |= P->order_rows.alloc(sizeof(int32_t) * (size_t)P->n_rows);
-  rc |= P->order_tr.alloc(sizeof(int32_t) * (size_t)P->n_tr);
   rc |= P->dm.alloc((size_t)dm_bytes);
   rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
   rc |= P->rows.alloc(P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16);
   rc |= P->counters.alloc(sizeof(int) * 8);
   rc |= P->status.alloc(sizeof(int32_t) * 4);
   rc |= P->sink.alloc(sizeof(uint32_t) * kWave);
+  rc |= P->trout.alloc(sizeof(TraceOut));
   if (!rc && affine && P->grid_help &&
       (hipEventCreateWithFlags(&P->fork, hipEventDisableTiming) != hipSuccess ||
        hipEventCreateWithFlags(&P->join, hipEventDisableTiming) != hipSuccess ||
@@ -688,8 +682,6 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
                          hipMemcpyHostToDevice, s);
   if (e == hipSuccess && P->n_rows)
     e = hipMemcpyAsync(P->order_rows.p, o_rows.data(), sizeof(int32_t) * P->n_rows, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && P->n_tr)
-    e = hipMemcpyAsync(P->order_tr.p, o_tr.data(), sizeof(int32_t) * P->n_tr, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete P;
@@ -726,6 +718,8 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[0], s));
   HIP_TRY(launch_reset(ctr, 8, s));
   HIP_TRY(launch_reset((int *)P->status.p, 4, s));
+  if (affine && align)
+    HIP_TRY(launch_set_trace_out((TraceOut *)P->trout.p, d_out, d_out_len, (int32_t *)P->status.p, s));
   if (!affine || P->n_lds + P->n_glb)
     HIP_TRY(launch_prep(affine, std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * 8)), meta, P->n, d_seq, tcm,
                         (int *)P->aux.p, s));
@@ -751,11 +745,12 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
     uint32_t *sink = (uint32_t *)P->sink.p;
     auto quad = [&](int grid, hipStream_t st) {
       return launch_aff_quad(align, grid, meta, (const int32_t *)P->order_quad.p, P->n_quad, ctr + 6, d_seq, tcm, dm,
-                             d_cost, sink, (const uint8_t *)P->pure.p, st);
+                             d_cost, sink, (const uint8_t *)P->pure.p, (const TraceOut *)P->trout.p, st);
     };
     auto diag = [&](int q, int grid, hipStream_t st) {
       return launch_aff_diag(align, q == 0 ? 1 : 2 * q, grid, meta, (const int32_t *)P->order_diag[q].p,
-                             P->n_diag[q], ctr + 2 + q, d_seq, tcm, dm, d_cost, sink, st);
+                             P->n_diag[q], ctr + 2 + q, d_seq, tcm, dm, d_cost, sink, (const TraceOut *)P->trout.p,
+                             st);
     };
     for (int q = 2; q >= 1; --q)   // the widest bands: 4 and 2 lane sets
       if (P->n_diag[q]) HIP_TRY(diag(q, P->grid_diag[q], rs2));
@@ -798,9 +793,7 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[1], s));
   if (align) {
     if (affine) {
-      if (P->n_tr)
-        HIP_TRY(launch_aff_trace_wave(P->grid_tr, meta, (const int32_t *)P->order_tr.p, P->n_tr, ctr + 5, d_seq, tcm,
-                                      (const uint8_t *)P->dm.p, d_out, d_out_len, (int32_t *)P->status.p, s));
+      // the four-pair and anti-diagonal fills have traced their own pairs
       if (P->n_rows)
         HIP_TRY(launch_aff_trace(meta, (const int32_t *)P->order_rows.p, P->n_rows, d_seq, tcm,
                                  (const uint8_t *)P->dm.p, d_out, d_out_len, (int32_t *)P->status.p, s));

@@ -45,6 +45,16 @@ struct PairMeta {
   int32_t p0, p1, p2;  // mode specific band parameters
 };
 
+// Outputs of the backtrace that the affine fill kernels run on each pair
+// right after its fill (the same wave: its own band stores are visible to
+// its later loads, and the band is still in L2).  A plan keeps one in
+// device memory, written at the start of each run.
+struct TraceOut {
+  uint8_t *out;
+  int32_t *out_len;
+  int32_t *status;
+};
+
 // Linear band row descriptor kinds (src/algn.c:872-968).
 enum LinearCase : int32_t { kLinFull = 0, kLinCase2 = 1, kLinCase3 = 2 };
 

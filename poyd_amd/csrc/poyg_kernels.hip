@@ -617,14 +617,32 @@ struct AffDiagLanes {
   int fcost[R];
 };
 
+
+template <int K>
+__device__ __forceinline__ void aff_trace_pair(const PairMeta &m, int p, const uint8_t *__restrict__ seq,
+                                               const uint8_t *__restrict__ dm_base, const uint8_t *s_med,
+                                               uint32_t *tile, uint8_t *out, int32_t *out_len, int32_t *status);
+
+// The pair's backtrace by the calling wave.  K: tile groups, 16 for bands
+// of at most 64 diagonal pairs (the four-pair kernel's and the one-set
+// kernel's), 8 for wider ones (see aff_trace_pair).
+template <int K>
+__device__ __forceinline__ void aff_trace_own(const PairMeta &m, int p, const uint8_t *__restrict__ seq,
+                                              const uint8_t *__restrict__ dm_base, const uint8_t *s_med,
+                                              uint32_t *tile, const TraceOut &tro) {
+  aff_trace_pair<K>(m, p, seq, dm_base, s_med, tile, tro.out, tro.out_len, tro.status);
+}
+
 template <bool kBt, int R>
 __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
     const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, int32_t *cost_out,
-    uint32_t *sink) {
+    uint32_t *sink, const TraceOut *trop) {
   __shared__ int s_cost[32 * 32];
   __shared__ int s_prep[32];
-  load_tables(tcm, s_cost, s_prep, nullptr);
+  __shared__ uint8_t s_med[kBt ? 32 * 32 : 4];
+  __shared__ uint32_t s_tile[4][kBt ? kWave * 16 : 1];   // the backtrace's tile, per wave
+  load_tables(tcm, s_cost, s_prep, kBt ? s_med : nullptr);
   wave_priority_long_pairs();
   const int go = tcm->gap_open;
   const int lane = lane_id();
@@ -645,6 +663,7 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
       }
       sum = lane63(wave_prefix_sum(sum));
       cost_out[p] = lb == 0 ? 0 : go + sum;   // every lane stores the same value
+      if (kBt) aff_trace_own<R == 1 ? 16 : 8>(m, p, seq, dm_base, s_med, s_tile[wave_in_group()], *trop);
       continue;
     }
     uint8_t *dm = dm_base + m.dm_off;
@@ -788,6 +807,7 @@ __global__ __launch_bounds__(kWave * 4) void aff_diag_kernel(
 #pragma unroll
     for (int r = 1; r < R; ++r) fc = qr == r ? L.fcost[r] : fc;
     cost_out[p] = __builtin_amdgcn_readlane(fc, q & 63);
+    if (kBt) aff_trace_own<R == 1 ? 16 : 8>(m, p, seq, dm_base, s_med, s_tile[wave_in_group()], *trop);
   }
 }
 
@@ -1224,10 +1244,12 @@ template <bool kBt, int H>
 __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
     const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, int32_t *cost_out,
-    uint32_t *sink, const uint8_t *__restrict__ pure) {
+    uint32_t *sink, const uint8_t *__restrict__ pure, const TraceOut *trop) {
   __shared__ int s_cost[32 * 32];
   __shared__ int s_prep[32];
-  load_tables(tcm, s_cost, s_prep, nullptr);
+  __shared__ uint8_t s_med[kBt ? 32 * 32 : 4];
+  __shared__ uint32_t s_tile[4][kBt ? kWave * 16 : 1];   // the backtrace's tile, per wave
+  load_tables(tcm, s_cost, s_prep, kBt ? s_med : nullptr);
   const int go = tcm->gap_open;
   const int lane = lane_id();
   const int q = lane & 15;
@@ -1455,6 +1477,18 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
       const int pr = __builtin_amdgcn_readlane(p, 16 * r);
       if (__builtin_amdgcn_readlane((int)active, 16 * r)) cost_out[pr] = c;   // every lane stores the same value
     }
+    if (kBt) {
+      // read after the fill, so that nothing of the backtrace is held
+      // across the fill loop
+      const TraceOut tro = *trop;
+      uint32_t *tile = s_tile[wave_in_group()];
+      __builtin_amdgcn_s_setprio(0);
+      for (int r = 0; r < 4; ++r) {
+        if (!__builtin_amdgcn_readlane((int)active, 16 * r)) continue;
+        const int pr = __builtin_amdgcn_readlane(p, 16 * r);
+        aff_trace_own<16>(uniform_meta(meta, pr), pr, seq, dm_base, s_med, tile, tro);
+      }
+    }
   }
 }
 
@@ -1547,24 +1581,6 @@ __global__ void aff_trace_kernel(const PairMeta *__restrict__ meta, const int32_
   if (bad) atomicAdd(status, 1);
 }
 
-__global__ __launch_bounds__(kWave * 4) void aff_trace_wave_kernel(
-    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
-    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, const uint8_t *__restrict__ dm_base,
-    uint8_t *out, int32_t *out_len, int32_t *status) {
-  __shared__ uint8_t s_med[32 * 32];
-  __shared__ uint32_t s_tile[4][kWave * 16];   // lane-major: window lane k at [Kk, Kk + K)
-  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_med[k] = tcm->median[k];
-  __syncthreads();
-  uint32_t *tile = s_tile[wave_in_group()];
-  for (;;) {
-    const int item = next_item(counter);
-    if (item >= n) break;
-    const int p = uniform32(order[item]);
-    const PairMeta m = uniform_meta(meta, p);
-    if (m.p1 <= kWave) aff_trace_pair<16>(m, p, seq, dm_base, s_med, tile, out, out_len, status);
-    else aff_trace_pair<8>(m, p, seq, dm_base, s_med, tile, out, out_len, status);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Linear band (algn_fill_plane_2 + algn_fill_row family, src/algn.c:87-968)
@@ -2461,21 +2477,30 @@ int aff_diag_blocks_per_cu(bool bt, int sets) {
 template <int R>
 static void launch_diag_r(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
-                          hipStream_t s) {
+                          const TraceOut *tro, hipStream_t s) {
   if (bt)
     hipLaunchKernelGGL((aff_diag_kernel<true, R>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
-                       tcm, dm, cost, sink);
+                       tcm, dm, cost, sink, tro);
   else
     hipLaunchKernelGGL((aff_diag_kernel<false, R>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
-                       tcm, dm, cost, sink);
+                       tcm, dm, cost, sink, tro);
+}
+
+__global__ void set_trace_out_kernel(TraceOut *dst, uint8_t *out, int32_t *out_len, int32_t *status) {
+  if (threadIdx.x == 0) *dst = TraceOut{out, out_len, status};
+}
+
+hipError_t launch_set_trace_out(TraceOut *dst, uint8_t *out, int32_t *out_len, int32_t *status, hipStream_t s) {
+  hipLaunchKernelGGL(set_trace_out_kernel, dim3(1), dim3(kWave), 0, s, dst, out, out_len, status);
+  return hipGetLastError();
 }
 
 hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
-                           uint32_t *sink, hipStream_t s) {
-  if (sets == 1) launch_diag_r<1>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, s);
-  else if (sets == 2) launch_diag_r<2>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, s);
-  else launch_diag_r<4>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, s);
+                           uint32_t *sink, const TraceOut *tro, hipStream_t s) {
+  if (sets == 1) launch_diag_r<1>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, tro, s);
+  else if (sets == 2) launch_diag_r<2>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, tro, s);
+  else launch_diag_r<4>(bt, grid, meta, order, n, counter, seq, tcm, dm, cost, sink, tro, s);
   return hipGetLastError();
 }
 
@@ -2495,27 +2520,13 @@ hipError_t launch_pair_pure(int grid, const PairMeta *meta, const int32_t *order
 
 hipError_t launch_aff_quad(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                            const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
-                           const uint8_t *pure, hipStream_t s) {
+                           const uint8_t *pure, const TraceOut *tro, hipStream_t s) {
   if (bt)
     hipLaunchKernelGGL((aff_quad_kernel<true, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
-                       tcm, dm, cost, sink, pure);
+                       tcm, dm, cost, sink, pure, tro);
   else
     hipLaunchKernelGGL((aff_quad_kernel<false, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
-                       tcm, dm, cost, sink, pure);
-  return hipGetLastError();
-}
-
-int aff_trace_wave_blocks_per_cu() {
-  int nb = 0;
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_trace_wave_kernel, kWave * 4, 0);
-  return (e == hipSuccess && nb > 0) ? nb : 2;
-}
-
-hipError_t launch_aff_trace_wave(int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
-                                 const uint8_t *seq, const DevTcm *tcm, const uint8_t *dm, uint8_t *out,
-                                 int32_t *out_len, int32_t *status, hipStream_t s) {
-  hipLaunchKernelGGL(aff_trace_wave_kernel, dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq, tcm, dm,
-                     out, out_len, status);
+                       tcm, dm, cost, sink, pure, tro);
   return hipGetLastError();
 }
 
