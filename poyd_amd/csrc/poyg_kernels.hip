@@ -1949,7 +1949,7 @@ struct SysLds {
   int tail[32];
   int row[kSysWaves][kSysRow];   // hand-off rows; row[kSysWaves - 1] holds row 0 first
   int prog[kSysWaves + kWave];   // progress per hand-off row, then one dummy slot per lane
-  uint8_t x[kLinStage];          // s2, masked to 5 bits
+  uint8_t x[kWave + kLinStage + kWave + 2];   // s2, masked to 5 bits, at kWave; zero pads around it
   int item;
 };
 
@@ -1963,7 +1963,7 @@ __device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, 
   const int *prow = &L.row[(s + kSysWaves - 1) % kSysWaves][kWave + 1];   // prow[-1] is a slot
   int *mine = &L.row[s % kSysWaves][0];   // mine[kWave + 1 + j] = column j, mine[lane] = dummy
   const int pslot = (s + kSysWaves - 1) % kSysWaves, mslot = s % kSysWaves;
-  const uint8_t *sx = L.x;
+  const uint8_t *sx = L.x + kWave;   // columns -64 .. N + 65 are readable
   const int *gcost = L.cost + kGap * 32;
   const int i = min(i0 + lane, M - 1);
   const bool valid = i0 + lane < M;
@@ -1971,6 +1971,7 @@ __device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, 
   lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
   lo = valid ? lo : 0x3fffffff;   // rows past the last one hold no cell
   hi = valid ? hi : -1;
+  const int wid = valid ? hi - lo : 0;   // a cell where 0 <= j - lo <= wid
   const int c0 = uniform32(lo);
   const int last_lane = min(kWave - 1, M - 1 - i0);
   const int t_end = __builtin_amdgcn_readlane(hi, last_lane) - c0 + last_lane;
@@ -2017,8 +2018,8 @@ __device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, 
   int up_prev = prow[c0 - 1];
   int j = c0 - lane;
   // look-ahead: symbols two steps ahead, cost-table reads one step ahead
-  int x1 = sx[min(max(j + 1, 0), N - 1)];
-  const int x0 = sx[min(max(j, 0), N - 1)];
+  int x1 = sx[j + 1];
+  const int x0 = sx[j];
   int cr = crow[x0], gj = gcost[x0];
   int fill = prow[c0];
   dpp_exec_guard();
@@ -2027,14 +2028,15 @@ __device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, 
       publish(c0 + t - kWave);   // lane 63 wrote columns up to c0 + t - 64
       wait_for(c0 + t + kSysSync);
     }
-    const int x2 = sx[min(max(j + 2, 0), N - 1)];
+    const int x2 = sx[j + 2];   // lanes outside their band read the pads
     const int cr_n = crow[x1], gj_n = gcost[x1];
     const int fill_n = prow[min(c0 + t + 1, N - 1)];
     const int up = __builtin_amdgcn_update_dpp(fill, res, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
     const int diag = up_prev;
     const int t3 = diag + cr;
     const int t1 = j == hi_right ? kBig : up + c_del;
-    const bool has_ins = j != lo;
+    const int b = j - lo;
+    const bool has_ins = b != 0;
     const int t2 = has_ins ? res + gj : kBig;
     int best = min(min(t1, t2), t3);
     int msk = (int)(t3 == best) | ((int)(has_ins && t2 == best) << 1) | ((int)(t1 == best) << 2);
@@ -2048,13 +2050,12 @@ __device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, 
     const bool first = j == lo_first;
     best = first ? up + first_add : best;
     msk = first ? 4 : msk;
-    const bool act = j >= lo && j <= hi;
+    const bool act = (unsigned)b <= (unsigned)wid;
     res = act ? best : kBig;
     fin = act && j == N - 1 ? best : fin;
     // lane 63 hands its row to the next strip
     mine[act && lane == kWave - 1 ? kWave + 1 + j : lane] = best;
     if (kDm) {
-      const int b = j - lo;
       acc |= act ? msk << (8 * (b & 3)) : 0;
       const bool flush = act && ((b & 3) == 3 || j == hi);
       __builtin_amdgcn_raw_buffer_store_b32((uint32_t)acc, drsrc, flush ? drow + (b & ~3) : kSysDrop, 0, 0);
@@ -2097,7 +2098,9 @@ __global__ __launch_bounds__(kWave * kSysWaves) void lin_sys_kernel(
     // stage s2 and row 0 (algn_fill_first_row, :584-607, or the first row of
     // algn_fill_plane), reset the progress counters
     const int hi0 = lin_row0_hi(N, cs, w);
-    for (int j = threadIdx.x; j < N; j += blockDim.x) L.x[j] = s2[j] & 31;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) L.x[kWave + j] = s2[j] & 31;
+    for (int j = threadIdx.x; j < 2 * kWave + 2; j += blockDim.x)
+      L.x[j < kWave ? j : N + j] = 0;   // pads: columns -64..-1 and N..N+65
     for (int j = threadIdx.x; j <= hi0; j += blockDim.x) {
       L.row[kSysWaves - 1][kWave + 1 + j] = GP[j];
       if (kDm) dm[j] = j == 0 ? 1 : 2;   // ALIGN, then INSERT
