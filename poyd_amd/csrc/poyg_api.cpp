@@ -185,10 +185,16 @@ void pool_synced() {
 }
 }  // namespace
 
-// A device buffer owned by a plan.
+// A device buffer owned by a plan, or a view into another one.
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool own = true;
+  void view(const DevBuf &base, size_t off, size_t n) {
+    p = (char *)base.p + off;
+    bytes = n;
+    own = false;
+  }
   int alloc(size_t n) {
     if (n == 0) n = 16;
     if (pool_alloc(&p, n) != hipSuccess) {
@@ -200,9 +206,10 @@ struct DevBuf {
     return 0;
   }
   void release() {
-    if (p) (void)pool_release(p);
+    if (p && own) (void)pool_release(p);
     p = nullptr;
     bytes = 0;
+    own = true;
   }
 };
 
@@ -224,7 +231,7 @@ struct poyg_plan {
   // (linear pairs: the systolic kernel takes every pair whose column
   // sequence fits its LDS row, the row kernels the rest)
   DevBuf tcm, meta, order_all, order_lds, order_glb, order_sys, order_quad, order_diag[3], order_rows, dm,
-      aux, rows, counters, status, sink, pure, trout;
+      aux, rows, counters, status, sink, pure, trout, blob;
   int n_lds = 0, n_glb = 0, n_sys = 0, n_quad = 0, n_diag[3] = {0, 0, 0}, n_rows = 0;
   int grid_lds = 0, grid_glb = 0, grid_sys = 0, grid_quad = 0, grid_diag[3] = {0, 0, 0},
       row_width = 0;
@@ -241,7 +248,7 @@ struct poyg_plan {
     if (triple) poyg_triple_plan_destroy(triple);
     for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_sys, &order_quad, &order_diag[0],
                       &order_diag[1], &order_diag[2], &order_rows, &dm, &aux, &rows, &counters, &status,
-                      &sink, &pure, &trout})
+                      &sink, &pure, &trout, &blob})
       b->release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -414,6 +421,35 @@ static void lin_row_host(int i, int m, int n, const LinGeom &g, int &lo, int &hi
   else { lo = 1 + (i - (m - (n - w) + 1)); hi = n - 1; }
 }
 
+// Widest row and number of cells of a linear band.  Within each of the three
+// sections of lin_row_host the row width is affine in the row (growing by
+// one, constant, shrinking by one), so a section's cells are an arithmetic
+// series of its first and last widths and its widest row is one of the two.
+static void lin_band_extent(int m, int n, const LinGeom &g, int &maxw, int64_t &cells) {
+  auto width = [&](int i) {
+    int lo, hi;
+    lin_row_host(i, m, n, g, lo, hi);
+    return hi - lo + 1;
+  };
+  maxw = width(0);
+  cells = maxw;
+  if (g.cs == kLinFull) {
+    cells = (int64_t)m * n;
+    return;
+  }
+  const int t1 = g.cs == kLinCase2 ? g.h : n - g.w + 1;             // first row of the middle section
+  const int t2 = g.cs == kLinCase2 ? m - (g.h - 1) : m - (n - g.w) + 1;   // first row of the last one
+  const int sec[3][2] = {{1, std::min(t1, m)},
+                         {std::max(1, t1), std::min(m, t2)},
+                         {std::max(std::max(1, t1), t2), m}};
+  for (const auto &r : sec) {
+    if (r[0] >= r[1]) continue;
+    const int a = width(r[0]), b = width(r[1] - 1);
+    maxw = std::max(maxw, std::max(a, b));
+    cells += (int64_t)(a + b) * (r[1] - r[0]) / 2;
+  }
+}
+
 static constexpr int kAffRBHost = 256;   // must match kAffRB in the kernels
 static constexpr int kLinRBHost = 512;   // must match kLinRB
 static constexpr int kLinStageHost = 2048;   // must match kLinStage
@@ -523,12 +559,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       m.p2 = g.h;
       int maxw = 0;
       int64_t c = 0;
-      for (int i = 0; i < m.lr; ++i) {
-        int lo, hi;
-        lin_row_host(i, m.lr, m.lc, g, lo, hi);
-        maxw = std::max(maxw, hi - lo + 1);
-        c += hi - lo + 1;
-      }
+      lin_band_extent(m.lr, m.lc, g, maxw, c);
       cells[k] = c;
       m.stride = (maxw + 3) & ~3;
       m.aux_off = aux_ints;
@@ -636,17 +667,35 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       return POYG_EINVAL;
     }
   }
+  // the cost matrix, the pair descriptors and the work lists travel as one
+  // upload: sections of one block at 256-byte offsets
+  std::vector<std::pair<DevBuf *, std::pair<const void *, size_t>>> sections = {
+      {&P->tcm, {&dt, sizeof dt}},
+      {&P->meta, {meta.data(), sizeof(PairMeta) * (size_t)n}},
+      {&P->order_all, {order.data(), sizeof(int32_t) * (size_t)n}},
+      {&P->order_lds, {o_lds.data(), sizeof(int32_t) * o_lds.size()}},
+      {&P->order_glb, {o_glb.data(), sizeof(int32_t) * o_glb.size()}},
+      {&P->order_sys, {o_sys.data(), sizeof(int32_t) * o_sys.size()}},
+      {&P->order_quad, {o_quad.data(), sizeof(int32_t) * o_quad.size()}},
+      {&P->order_diag[0], {o_diag[0].data(), sizeof(int32_t) * o_diag[0].size()}},
+      {&P->order_diag[1], {o_diag[1].data(), sizeof(int32_t) * o_diag[1].size()}},
+      {&P->order_diag[2], {o_diag[2].data(), sizeof(int32_t) * o_diag[2].size()}},
+      {&P->order_rows, {o_rows.data(), sizeof(int32_t) * o_rows.size()}}};
+  std::vector<size_t> sec_off;
+  size_t blob_bytes = 0;
+  for (auto &sec : sections) {
+    sec_off.push_back(blob_bytes);
+    blob_bytes += (std::max<size_t>(sec.second.second, 16) + 255) & ~(size_t)255;
+  }
+  std::vector<uint8_t> staging(blob_bytes, 0);
+  for (size_t k = 0; k < sections.size(); ++k)
+    if (sections[k].second.second) memcpy(staging.data() + sec_off[k], sections[k].second.first, sections[k].second.second);
   int rc = 0;
-  rc |= P->tcm.alloc(sizeof(DevTcm));
-  rc |= P->meta.alloc(sizeof(PairMeta) * (size_t)n);
-  rc |= P->order_all.alloc(sizeof(int32_t) * (size_t)n);
-  rc |= P->order_lds.alloc(sizeof(int32_t) * (size_t)P->n_lds);
-  rc |= P->order_glb.alloc(sizeof(int32_t) * (size_t)P->n_glb);
-  rc |= P->order_sys.alloc(sizeof(int32_t) * (size_t)P->n_sys);
-  for (int q = 0; q < 3; ++q) rc |= P->order_diag[q].alloc(sizeof(int32_t) * (size_t)P->n_diag[q]);
-  rc |= P->order_quad.alloc(sizeof(int32_t) * (size_t)P->n_quad);
+  rc |= P->blob.alloc(blob_bytes);
+  if (!rc)
+    for (size_t k = 0; k < sections.size(); ++k)
+      sections[k].first->view(P->blob, sec_off[k], std::max<size_t>(sections[k].second.second, 16));
   rc |= P->pure.alloc((size_t)n);
-  rc |= P->order_rows.alloc(sizeof(int32_t) * (size_t)P->n_rows);
   rc |= P->dm.alloc((size_t)dm_bytes);
   rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
   rc |= P->rows.alloc(P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16);
@@ -665,23 +714,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   }
   hipStream_t s = ctx->stream;
   hipError_t e = hipSuccess;
-  e = hipMemcpyAsync(P->tcm.p, &dt, sizeof dt, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && n) e = hipMemcpyAsync(P->meta.p, meta.data(), sizeof(PairMeta) * n, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && n) e = hipMemcpyAsync(P->order_all.p, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && P->n_lds)
-    e = hipMemcpyAsync(P->order_lds.p, o_lds.data(), sizeof(int32_t) * P->n_lds, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && P->n_glb)
-    e = hipMemcpyAsync(P->order_glb.p, o_glb.data(), sizeof(int32_t) * P->n_glb, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && P->n_sys)
-    e = hipMemcpyAsync(P->order_sys.p, o_sys.data(), sizeof(int32_t) * P->n_sys, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && P->n_quad)
-    e = hipMemcpyAsync(P->order_quad.p, o_quad.data(), sizeof(int32_t) * P->n_quad, hipMemcpyHostToDevice, s);
-  for (int q = 0; q < 3; ++q)
-    if (e == hipSuccess && P->n_diag[q])
-      e = hipMemcpyAsync(P->order_diag[q].p, o_diag[q].data(), sizeof(int32_t) * P->n_diag[q],
-                         hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && P->n_rows)
-    e = hipMemcpyAsync(P->order_rows.p, o_rows.data(), sizeof(int32_t) * P->n_rows, hipMemcpyHostToDevice, s);
+  e = hipMemcpyAsync(P->blob.p, staging.data(), blob_bytes, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete P;

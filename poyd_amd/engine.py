@@ -520,18 +520,23 @@ def fitch_pairs(ctx: Context, a: np.ndarray, b: np.ndarray):
     n, m = a.shape
     if n == 0:
         return np.zeros((0, m), np.uint8), np.zeros(0, np.int64)
-    d_chars = DeviceBuffer.from_array(ctx, np.concatenate([a, b, np.zeros_like(a)]))
-    idx = np.arange(3 * n, dtype=np.int32)
-    d_ia = DeviceBuffer.from_array(ctx, idx[:n])
-    d_ib = DeviceBuffer.from_array(ctx, idx[n:2 * n])
-    d_ir = DeviceBuffer.from_array(ctx, idx[2 * n:])
-    d_add = DeviceBuffer.from_array(ctx, np.zeros(n, np.int64))
-    _check(load().poyg_fitch_down(ctx.h, d_chars.ptr, m, n, d_ia.ptr, d_ib.ptr, d_ir.ptr, d_add.ptr, None),
-           "poyg_fitch_down")
-    res = np.empty((n, m), np.uint8)
-    _check(load().poyg_dev_download(ctx.h, res.ctypes.data, d_chars.ptr + 2 * n * m, res.nbytes),
-           "poyg_dev_download")
-    return res, d_add.download(np.int64, n)
+    # one block, one upload: rows a, b, the result rows, the change counts
+    # (right after the results, so that one download returns both), then
+    # the row index lists
+    nm = n * m
+    add_off = (3 * nm + 255) // 256 * 256
+    idx_off = (add_off + 8 * n + 255) // 256 * 256
+    blob = np.zeros(idx_off + 12 * n, np.uint8)
+    blob[:nm] = a.reshape(-1)
+    blob[nm:2 * nm] = b.reshape(-1)
+    blob[idx_off:].view(np.int32)[:] = np.arange(3 * n, dtype=np.int32)
+    d = DeviceBuffer.from_array(ctx, blob)
+    _check(load().poyg_fitch_down(ctx.h, d.ptr, m, n, d.ptr + idx_off, d.ptr + idx_off + 4 * n,
+                                  d.ptr + idx_off + 8 * n, d.ptr + add_off, None), "poyg_fitch_down")
+    back = np.empty(add_off + 8 * n - 2 * nm, np.uint8)
+    _check(load().poyg_dev_download(ctx.h, back.ctypes.data, d.ptr + 2 * nm, back.nbytes), "poyg_dev_download")
+    res = back[:nm].reshape(n, m).copy()
+    return res, back[add_off - 2 * nm:].view(np.int64).copy()
 
 
 def batch_deltaw(a_batch, b_batch, dw=None) -> np.ndarray:
@@ -575,19 +580,23 @@ def run_pairs(ctx: Context, cm: CostMatrix, mode: int, a_batch, b_batch, dws=Non
         dws = batch_deltaw(a_batch, b_batch)
     plan = PairPlan(ctx, cm, mode, off_a, a_batch.ln, off_b, b_batch.ln, dws)
     d_seq = DeviceBuffer.from_array(ctx, seq)
-    d_cost = DeviceBuffer(ctx, 4 * max(plan.n, 1))
+    # costs and output lengths share one block (one download for both)
+    n_cost = (max(plan.n, 1) + 63) // 64 * 64
+    n_len = plan.n * plan.slots
+    d_small = DeviceBuffer(ctx, 4 * (n_cost + max(n_len, 1)))
     d_out = DeviceBuffer(ctx, max(plan.out_bytes, 16)) if plan.slots else None
-    d_len = DeviceBuffer(ctx, 4 * max(plan.n * plan.slots, 1)) if plan.slots else None
-    plan.run(d_seq.ptr, d_cost.ptr, d_out.ptr if d_out else None, d_len.ptr if d_len else None)
+    plan.run(d_seq.ptr, d_small.ptr, d_out.ptr if d_out else None,
+             d_small.ptr + 4 * n_cost if plan.slots else None)
     ctx.sync()
     if check_band:
         bad = C.c_int32(0)
         _check(load().poyg_plan_status(plan.h, C.byref(bad)), "poyg_plan_status")
         if bad.value:
             raise PoygError(f"{bad.value} backtraces left the filled band")
-    res = PairResult(d_cost.download(np.int32, plan.n),
+    small = d_small.download(np.int32, n_cost + n_len)
+    res = PairResult(small[:plan.n].copy(),
                      d_out.download(np.uint8, plan.out_bytes) if d_out else None,
-                     d_len.download(np.int32, plan.n * plan.slots) if d_len else None,
+                     small[n_cost:n_cost + n_len].copy() if plan.slots else None,
                      plan.out_off, plan.cap, plan.slots, int(plan.cells))
     plan.close()
     return res
