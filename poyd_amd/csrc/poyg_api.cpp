@@ -10,6 +10,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "poyg_band.h"
 #include "poyg_internal.h"
 
 namespace poyg {
@@ -392,63 +393,8 @@ int poyg_plan_last_times(const poyg_plan *p, float *fill_ms, float *trace_ms) {
 
 }  // extern "C"
 
-// Linear band geometry of algn_fill_plane_2 (src/algn.c:872-968).
-struct LinGeom {
-  int cs, w, h;
-};
-static LinGeom lin_geom(int m, int n, int dw) {
-  LinGeom g;
-  g.w = std::min(n, 50 + dw);
-  g.h = std::min(m, (m - n) + 50 + dw);
-  if ((float)m >= (float)((float)((float)3 / (float)2) * (float)n)) g.cs = kLinFull;
-  else if (2 * g.h < m) g.cs = kLinCase2;
-  else if (8 >= m - g.h) g.cs = kLinFull;
-  else g.cs = kLinCase3;
-  return g;
-}
-static void lin_row_host(int i, int m, int n, const LinGeom &g, int &lo, int &hi) {
-  if (g.cs == kLinFull) { lo = 0; hi = n - 1; return; }
-  if (i == 0) { lo = 0; hi = g.w - 1; return; }
-  const int w = g.w, h = g.h;
-  if (g.cs == kLinCase2) {
-    if (i < h) { lo = 0; hi = w + i - 1; }
-    else if (i < m - (h - 1)) { lo = 1 + (i - h); hi = lo + w + h - 2; }
-    else { lo = n - (w + h - 2) + (i - (m - (h - 1))); hi = n - 1; }
-    return;
-  }
-  if (i < n - w + 1) { lo = 0; hi = w + i - 1; }
-  else if (i < m - (n - w) + 1) { lo = 0; hi = n - 1; }
-  else { lo = 1 + (i - (m - (n - w) + 1)); hi = n - 1; }
-}
-
-// Widest row and number of cells of a linear band.  Within each of the three
-// sections of lin_row_host the row width is affine in the row (growing by
-// one, constant, shrinking by one), so a section's cells are an arithmetic
-// series of its first and last widths and its widest row is one of the two.
-static void lin_band_extent(int m, int n, const LinGeom &g, int &maxw, int64_t &cells) {
-  auto width = [&](int i) {
-    int lo, hi;
-    lin_row_host(i, m, n, g, lo, hi);
-    return hi - lo + 1;
-  };
-  maxw = width(0);
-  cells = maxw;
-  if (g.cs == kLinFull) {
-    cells = (int64_t)m * n;
-    return;
-  }
-  const int t1 = g.cs == kLinCase2 ? g.h : n - g.w + 1;             // first row of the middle section
-  const int t2 = g.cs == kLinCase2 ? m - (g.h - 1) : m - (n - g.w) + 1;   // first row of the last one
-  const int sec[3][2] = {{1, std::min(t1, m)},
-                         {std::max(1, t1), std::min(m, t2)},
-                         {std::max(std::max(1, t1), t2), m}};
-  for (const auto &r : sec) {
-    if (r[0] >= r[1]) continue;
-    const int a = width(r[0]), b = width(r[1] - 1);
-    maxw = std::max(maxw, std::max(a, b));
-    cells += (int64_t)(a + b) * (r[1] - r[0]) / 2;
-  }
-}
+static_assert(kBandFull == kLinFull && kBandCase2 == kLinCase2 && kBandCase3 == kLinCase3,
+              "the band cases of the host geometry are the kernels' PairMeta::p0 values");
 
 static constexpr int kAffRBHost = 256;   // must match kAffRB in the kernels
 static constexpr int kLinRBHost = 512;   // must match kLinRB

@@ -468,3 +468,32 @@ def test_powell_rejects_gap_only_characters(ctx):
     x = np.array([16, 1, 16, 2], np.uint8)
     with pytest.raises(engine.PoygError):
         engine.powell_3d(ctx, x, x, x, 1, 2, 1)
+
+
+def test_device_cache_reuse(ctx):
+    """Blocks released to the per-device cache come back only after a
+    synchronization, and a reused block holds what its new owner wrote."""
+    from poyd_amd import engine
+    note("device cache reuse")
+    rng = np.random.default_rng(5)
+    live = []
+    for cycle in range(40):
+        size = int(rng.choice([100, 256, 4000, 70000, 1 << 20]))
+        data = rng.integers(0, 256, size, dtype=np.uint8)
+        b = engine.DeviceBuffer.from_array(ctx, data)
+        live.append((b, data))
+        if cycle % 3 == 2:
+            # release one buffer without a sync: a block allocated now must not be it
+            old, _ = live.pop(0)
+            ptr = old.ptr
+            old.free()
+            fresh = engine.DeviceBuffer(ctx, 16)
+            assert fresh.ptr != ptr
+            fresh.free()
+        if cycle % 7 == 6:
+            ctx.sync()
+        for b2, d2 in live:
+            assert np.array_equal(b2.download(np.uint8, len(d2)), d2), cycle
+    for b2, _ in live:
+        b2.free()
+    ctx.sync()
