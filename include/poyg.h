@@ -203,8 +203,10 @@ int poyg_plan_run(poyg_plan *plan, const uint8_t *d_seq, int32_t *d_cost, uint8_
 int poyg_plan_last_times(const poyg_plan *plan, float *fill_ms, float *trace_ms);
 int poyg_plan_set_timing(poyg_plan *plan, int enable);
 
-/* Number of backtraces of the last run that left the filled band (0 when
- * every walk stayed inside it). */
+/* Number of pairs of the last run whose results are not to be trusted: a
+ * backtrace that left the filled band, or a linear fill whose wait for the
+ * previous strip of its pair gave up (a bounded poll; neither happens in a
+ * correct run).  0 when every pair completed normally. */
 int poyg_plan_status(poyg_plan *plan, int32_t *bad);
 
 /* ---- batched three-sequence alignment --------------------------------- */

@@ -592,7 +592,8 @@ def run_pairs(ctx: Context, cm: CostMatrix, mode: int, a_batch, b_batch, dws=Non
         bad = C.c_int32(0)
         _check(load().poyg_plan_status(plan.h, C.byref(bad)), "poyg_plan_status")
         if bad.value:
-            raise PoygError(f"{bad.value} backtraces left the filled band")
+            raise PoygError(f"{bad.value} pairs failed (a backtrace left the filled band, "
+                            "or a fill's wait for the previous strip gave up)")
     small = d_small.download(np.int32, n_cost + n_len)
     res = PairResult(small[:plan.n].copy(),
                      d_out.download(np.uint8, plan.out_bytes) if d_out else None,
