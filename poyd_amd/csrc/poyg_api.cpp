@@ -36,7 +36,7 @@ hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, c
                             hipStream_t s);
 hipError_t launch_lin_sys(bool with_dm, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, const int *aux, int32_t *cost,
-                          int32_t *status, hipStream_t s);
+                          int32_t *status, int spin_limit, hipStream_t s);
 int lin_sys_blocks_per_cu(bool with_dm);
 hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
@@ -395,6 +395,15 @@ int poyg_plan_last_times(const poyg_plan *p, float *fill_ms, float *trace_ms) {
 
 static_assert(kBandFull == kLinFull && kBandCase2 == kLinCase2 && kBandCase3 == kLinCase3,
               "the band cases of the host geometry are the kernels' PairMeta::p0 values");
+
+// Polls before a wait of the systolic linear fill for the previous strip of
+// its pair gives up (the kernel's default when negative).  POYG_SYS_SPIN_LIMIT
+// overrides it, so that a test can make the waits give up and check that the
+// plan reports the pairs as failed.
+static int sys_spin_limit() {
+  const char *e = getenv("POYG_SYS_SPIN_LIMIT");
+  return e && *e ? atoi(e) : -1;
+}
 
 static constexpr int kAffRBHost = 256;   // must match kAffRB in the kernels
 static constexpr int kLinRBHost = 512;   // must match kLinRB
@@ -760,7 +769,8 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   } else {
     if (P->n_sys)
       HIP_TRY(launch_lin_sys(align, P->grid_sys, meta, (const int32_t *)P->order_sys.p, P->n_sys, ctr + 2, d_seq,
-                             tcm, (uint8_t *)P->dm.p, (const int *)P->aux.p, d_cost, (int32_t *)P->status.p, s));
+                             tcm, (uint8_t *)P->dm.p, (const int *)P->aux.p, d_cost, (int32_t *)P->status.p,
+                             sys_spin_limit(), s));
     if (P->n_lds)
       HIP_TRY(launch_lin_fill(align, false, P->grid_lds, meta, (const int32_t *)P->order_lds.p, P->n_lds, ctr,
                               d_seq, tcm, (uint8_t *)P->dm.p, (int *)P->aux.p, d_cost, nullptr, 0, s));

@@ -1941,7 +1941,7 @@ constexpr int kSysWaves = 4;
 constexpr int kSysDrop = 0x7ffffff0;             // buffer offset past any pair's direction band
 constexpr int kSysSync = 8;                      // steps between hand-off checks
 constexpr int kSysStripTag = 1 << 16;            // progress = strip * kSysStripTag + column
-constexpr int kSysSpin = 1 << 22;                // polls before a hand-off wait gives up
+constexpr int kSysSpin = 1 << 22;                // polls before a hand-off wait gives up (default)
 
 struct SysLds {
   int cost[32 * 32];
@@ -1957,7 +1957,7 @@ template <bool kDm>
 __device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, int cs, int w, int h, int S,
                                               const uint8_t *__restrict__ s1, SysLds &L,
                                               __amdgpu_buffer_rsrc_t drsrc, int32_t *__restrict__ cost_out,
-                                              int pidx) {
+                                              int pidx, int spin_limit) {
   const int lane = lane_id();
   const int i0 = 1 + kWave * s;
   const int *prow = &L.row[(s + kSysWaves - 1) % kSysWaves][kWave + 1];   // prow[-1] is a slot
@@ -1988,8 +1988,8 @@ __device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, 
     for (int k = 0;; ++k) {
       const int v = uniform32(__hip_atomic_load(&L.prog[pslot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
       if (v >= want) break;
-      if (k == kSysSpin) {
-        ok = false;   // not reached: every strip runs to its end
+      if (k >= spin_limit) {
+        ok = false;   // not reached with the default limit: every strip runs to its end
         break;
       }
       __builtin_amdgcn_s_sleep(1);
@@ -2077,7 +2077,7 @@ template <bool kDm>
 __global__ __launch_bounds__(kWave * kSysWaves) void lin_sys_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
     const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, const int *aux,
-    int32_t *cost_out, int32_t *status) {
+    int32_t *cost_out, int32_t *status, int spin_limit) {
   __shared__ SysLds L;
   load_tables(tcm, L.cost, L.prep, nullptr);
   for (int k = threadIdx.x; k < 32; k += blockDim.x) L.tail[k] = tcm->tail[k];
@@ -2115,7 +2115,7 @@ __global__ __launch_bounds__(kWave * kSysWaves) void lin_sys_kernel(
     const int nstrips = (M - 1 + kWave - 1) / kWave;
     bool ok = true;
     for (int s = wv; s < nstrips; s += kSysWaves)
-      ok = lin_sys_strip<kDm>(s, nstrips, M, N, cs, w, m.p2, S, s1, L, drsrc, cost_out, p) && ok;
+      ok = lin_sys_strip<kDm>(s, nstrips, M, N, cs, w, m.p2, S, s1, L, drsrc, cost_out, p, spin_limit) && ok;
     if (!ok && lane_id() == 0) atomicAdd(status, 1);
   }
 }
@@ -2553,13 +2553,14 @@ int lin_sys_blocks_per_cu(bool with_dm) { return with_dm ? sys_occupancy<true>()
 
 hipError_t launch_lin_sys(bool with_dm, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, const int *aux, int32_t *cost,
-                          int32_t *status, hipStream_t s) {
+                          int32_t *status, int spin_limit, hipStream_t s) {
+  if (spin_limit < 0) spin_limit = kSysSpin;
   if (with_dm)
     hipLaunchKernelGGL(lin_sys_kernel<true>, dim3(grid), dim3(kWave * kSysWaves), 0, s, meta, order, n, counter, seq,
-                       tcm, dm, aux, cost, status);
+                       tcm, dm, aux, cost, status, spin_limit);
   else
     hipLaunchKernelGGL(lin_sys_kernel<false>, dim3(grid), dim3(kWave * kSysWaves), 0, s, meta, order, n, counter,
-                       seq, tcm, dm, aux, cost, status);
+                       seq, tcm, dm, aux, cost, status, spin_limit);
   return hipGetLastError();
 }
 

@@ -497,3 +497,26 @@ def test_device_cache_reuse(ctx):
     for b2, _ in live:
         b2.free()
     ctx.sync()
+
+
+def test_linear_fill_wait_gives_up_reported(ctx, oracle, monkeypatch):
+    """The systolic fill's wait for the previous strip of a pair is a bounded
+    poll.  With the poll limit forced to zero from the host, the waits give
+    up at once: the kernel still ends, the plan reports the pairs as failed
+    and the engine raises; the next run with the default limit is exact."""
+    from poyd_amd import engine
+    note("linear fill wait gives up")
+    rng = np.random.default_rng(17)
+    A = [rand_seq(rng, 1500) for _ in range(4)]
+    B = [mutate(rng, a, 0.1, 0.03) for a in A]
+    cm = engine.CostMatrix.of_sub_indel(1, 2)
+    ref = oracle.CM2.sub_indel(1, 2)
+    monkeypatch.setenv("POYG_SYS_SPIN_LIMIT", "0")
+    with pytest.raises(engine.PoygError, match="pairs failed"):
+        engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
+    monkeypatch.delenv("POYG_SYS_SPIN_LIMIT")
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
+    for k, (a, b) in enumerate(zip(A, B)):
+        cost, oa, ob = oracle.align_2(a, b, ref)
+        assert res.cost[k] == cost, k
+        assert np.array_equal(res.seq(k, 0), oa), k
