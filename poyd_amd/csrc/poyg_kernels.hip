@@ -2200,14 +2200,16 @@ __global__ __launch_bounds__(kWave) void lin_fill_glb_kernel(
 // columns (lanes 32-63).  A step's two output symbols go to lane (step % 64)
 // of two registers; every 64 steps the lanes store them (consecutive bytes,
 // right to left) and look up their medians, which a ballot compacts.
+//
+// Runs of ALIGN, most of an alignment, go in one step: per tile, lane L
+// also holds a 32-bit mask of the ALIGN bits of diagonal L (cells (t, L-t)
+// of the tile, built through an LDS copy of the tile), so the run from a
+// cell is the trailing ones of one readlane, and its output columns are
+// gathered from the symbol register by two ds_bpermutes.  Bytes of cells
+// outside their row's band are zeroed when the tile is loaded, so a run
+// never leaves the band.
 constexpr int kTrTile = 32;
 constexpr int kTrDrop = 0x7ffffff0;   // buffer offset past any pair's direction band
-
-__device__ __forceinline__ int lin_row_lo(int i, int M, int N, int cs, int w, int h) {
-  int lo = 0, hi, lk, rk;
-  if (i > 0) lin_row(i, M, N, cs, w, h, lo, hi, lk, rk);
-  return lo;
-}
 
 __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
@@ -2217,6 +2219,8 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
   for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_med[k] = tcm->median[k];
   __syncthreads();
   const int lane = lane_id();
+  __shared__ uint4 s_tt[4][kTrTile * kTrTile / 16];   // per wave: the tile, row-major
+  uint4 *tt = s_tt[wave_in_group()];
   for (;;) {
     const int item = next_item(counter);
     if (item >= n) break;
@@ -2236,7 +2240,7 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
     uint8_t *om = o + 2 * cap;
     int i = M - 1, j = N - 1, len = 0, bad = 0;
     int I = 0x3fffffff, J = 0x3fffffff;   // tile corner; the first step loads
-    uint32_t t0 = 0, t1 = 0, t2 = 0, t3 = 0, sym = 0, va = 0, vb = 0;
+    uint32_t t0 = 0, t1 = 0, t2 = 0, t3 = 0, sym = 0, va = 0, vb = 0, dmk = 0;
     int tlo = 0, thi = 0;   // lanes 0-31: band of tile row lane (row I - lane)
     int outpos = cap, medpos = cap;
     // stores the batch's lanes 0..cnt-1 and their non-gap medians
@@ -2260,7 +2264,13 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
         I = i;
         J = j;
         const int r = I - (lane >> 1);
-        const int rlo = lin_row_lo(max(r, 0), M, N, cs, w, h);
+        int rlo = 0, rhi = lin_row0_hi(N, cs, w);
+        {
+          int lk, rk, lo, hi;
+          lin_row(max(r, 1), M, N, cs, w, h, lo, hi, lk, rk);
+          rlo = r >= 1 ? lo : rlo;
+          rhi = r >= 1 ? hi : rhi;
+        }
         {
           const int rr = max(I - (lane & (kTrTile - 1)), 0);
           int lo, hi, lk, rk;
@@ -2277,7 +2287,8 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
             const int c = c0 + 4 * q + k;
-            const int off = r >= 0 && c >= rlo ? roff + c : kTrDrop;   // out of range reads 0
+            // outside the row's band (or the matrix): no byte, the load reads 0
+            const int off = r >= 0 && c >= rlo && c <= rhi ? roff + c : kTrDrop;
             wv[q] |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(drsrc, off, 0, 0) << (8 * k);
           }
         }
@@ -2285,12 +2296,56 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
         t1 = wv[1];
         t2 = wv[2];
         t3 = wv[3];
+        // the ALIGN bits of diagonal lane: cells (t, lane - t)
+        tt[lane] = make_uint4(t0, t1, t2, t3);
+        wave_sync_mem();
+        const uint8_t *tb = (const uint8_t *)tt;
+        dmk = 0;
+#pragma unroll
+        for (int t = 0; t < kTrTile; ++t) {
+          const int c = lane - t;
+          const int v = tb[t * kTrTile + min(max(c, 0), kTrTile - 1)];
+          dmk |= (uint32_t)((c >= 0 && c < kTrTile) ? (v & 1) : 0) << t;
+        }
+        wave_sync_mem();   // the next tile's copy overwrites this one
         sym = lane < kTrTile ? s1[max(I - lane, 0)] : s2[max(J - (kTrTile - 1) + (lane - kTrTile), 0)];
         // wait here, once per tile, so that no step waits on the counter
         // that the output stores also hold
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
       }
       const int ri = I - i, cc = j - (J - (kTrTile - 1));
+      {
+        // a run of ALIGN along the walk's diagonal (lane cc + ri), inside the tile
+        const uint32_t dm = (uint32_t)__builtin_amdgcn_readlane((int)dmk, cc + ri) >> ri;
+        const int ones = ~dm ? __builtin_ctz(~dm) : kTrTile;
+        const int run = min(ones, min(kTrTile - ri, cc + 1));
+        if (run >= 2) {
+          int left = run, r0 = ri, c0 = cc;
+          while (left > 0) {
+            const int x0 = len & (kWave - 1);
+            const int chunk = min(left, kWave - x0);
+            const int k = lane - x0;
+            const bool in = k >= 0 && k < chunk;
+            const int kk = in ? k : 0;
+            const int sa = __builtin_amdgcn_ds_bpermute(4 * (r0 + kk), (int)sym);
+            const int sb = __builtin_amdgcn_ds_bpermute(4 * (kTrTile + c0 - kk), (int)sym);
+            va = in ? (uint32_t)sa : va;
+            vb = in ? (uint32_t)sb : vb;
+            len += chunk;
+            r0 += chunk;
+            c0 -= chunk;
+            left -= chunk;
+            if ((len & (kWave - 1)) == 0) flush(kWave);
+          }
+          i -= run;
+          j -= run;
+          if (j < 0 && i >= 0) {
+            bad = 1;
+            break;
+          }
+          continue;
+        }
+      }
       const int lo = __builtin_amdgcn_readlane(tlo, ri), hi = __builtin_amdgcn_readlane(thi, ri);
       if (j < lo || j > hi) {
         bad = 1;
