@@ -516,7 +516,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       int64_t c = 0;
       lin_band_extent(m.lr, m.lc, g, maxw, c);
       cells[k] = c;
-      m.stride = (maxw + 3) & ~3;
+      m.stride = (maxw + 3 + 3) & ~3;   // room for a row start phase of up to 3 bytes (systolic fill)
       m.aux_off = aux_ints;
       aux_ints += (2 * (int64_t)m.lc + 63) & ~(int64_t)63;
       if (align) {

@@ -2012,7 +2012,10 @@ __device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, 
   const int lo_first = lk != LK_LEFT ? lo : -2;   // column of the first-column cell, if any
   const int hi_right = rk == RK_RIGHT ? hi : -2;  // column without DELETE
   const int hi_last = rk == RK_LAST && hi > 0 ? hi : -2;   // column re-checked with tail
-  const int drow = i * S;   // byte offset of the row in the pair's direction band
+  // row i's bytes start at phase (lane + lo - c0) mod 4 of its stride, so that
+  // byte (j - lo) of every lane lies at t mod 4 in its dword: the lanes fill
+  // their dwords in step and store them together every fourth step
+  const int drow = i * S + ((lane + lo - c0) & 3);
   wait_for(c0 + kSysSync);
   int res = kBig, acc = 0, fin = 0;
   int up_prev = prow[c0 - 1];
@@ -2056,10 +2059,14 @@ __device__ __forceinline__ bool lin_sys_strip(int s, int nstrips, int M, int N, 
     // lane 63 hands its row to the next strip
     mine[act && lane == kWave - 1 ? kWave + 1 + j : lane] = best;
     if (kDm) {
-      acc |= act ? msk << (8 * (b & 3)) : 0;
-      const bool flush = act && ((b & 3) == 3 || j == hi);
-      __builtin_amdgcn_raw_buffer_store_b32((uint32_t)acc, drsrc, flush ? drow + (b & ~3) : kSysDrop, 0, 0);
-      acc = flush ? 0 : acc;
+      acc |= act ? msk << (8 * (t & 3)) : 0;   // every cell's code is non-zero
+      if ((t & 3) == 3 || t == t_end) {          // uniform
+        // the dword holding this step's byte position; lanes without a cell
+        // in it store nothing
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)acc, drsrc,
+                                              acc ? ((drow + b) & ~3) : kSysDrop, 0, 0);
+        acc = 0;
+      }
     }
     up_prev = up;
     fill = fill_n;
@@ -2279,7 +2286,17 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
           thi = rr == 0 ? lin_row0_hi(N, cs, w) : hi;
         }
         const int c0 = J - (kTrTile - 1) + 16 * (lane & 1);
-        const int roff = r * S - rlo;
+        // rows the systolic fill wrote (s2 up to kLinStage) start at the
+        // phase (L + lo - c0) mod 4 of their stride, L being the row's lane in
+        // its strip and c0 the strip's first column
+        int phase = 0;
+        if (N <= kLinStage) {
+          const int L = (max(r, 1) - 1) & (kWave - 1);
+          int lk, rk, slo, shi;
+          lin_row(max(r, 1) - L, M, N, cs, w, h, slo, shi, lk, rk);
+          phase = r >= 1 ? (L + rlo - slo) & 3 : 0;
+        }
+        const int roff = r * S - rlo + phase;
         uint32_t wv[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
