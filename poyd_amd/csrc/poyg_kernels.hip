@@ -1925,8 +1925,9 @@ __device__ __forceinline__ void lin_fill_pair_r(const PairMeta &m, const uint8_t
 // never decrease from row to row, so every lane's band lies inside the
 // strip's steps.  Out-of-band results are never read (a cell reads (i-1, j)
 // and (i-1, j-1) only where they lie in row i-1's band).  The step loop
-// changes no EXEC mask: stores that only some lanes make go to per-lane LDS
-// slots or through an out-of-range buffer offset, which the hardware drops.
+// changes no EXEC mask (its branches are uniform): stores that only some
+// lanes make go to per-lane LDS slots or through an out-of-range buffer
+// offset, which the hardware drops.
 // Same cell rules as lin_fill_pair (src/algn.c:375-620).
 //
 // The four waves of a workgroup share one pair: wave k takes strips k,
