@@ -58,7 +58,9 @@ def workload(name: str):
 def cpu_baseline(w, kind: str, budget_s: float):
     """Single-core oracle (the C restatement of the reference algorithm,
     built -O2 without -march like the reference) over a seeded sample of
-    the same pairs, timed until budget_s of CPU work has been spent."""
+    the same pairs, timed until budget_s of CPU work has been spent.
+    Returns the baseline record and the oracle cost of every sampled item
+    (item index -> cost), which the caller compares with the GPU's."""
     from oracle import pyoracle as po
     cm = po.CM2.sub_indel(1, 1, 3) if kind == "affine" else po.CM2.sub_indel(1, 2)
     if kind == "cube":
@@ -76,24 +78,22 @@ def cpu_baseline(w, kind: str, budget_s: float):
         old_aff = None
     cells = 0
     done = 0
+    want = {}
     t0 = time.perf_counter()
     for k in order:
         if kind == "cube":
             x = [w.get(q, int(k)) for q in range(3)]
-            po.align_3(*x, cm3)
+            want[int(k)] = po.align_3(*x, cm3)[0]
             cells += len(x[0]) * len(x[1]) * len(x[2])
-            done += 1
-            if time.perf_counter() - t0 > budget_s:
-                break
-            continue
-        a, b = w.a.get(int(k)), w.b.get(int(k))
-        if kind == "affine":
-            po.align_affine_3(a, b, cm)
-            cells += po.affine_cells(len(a), len(b))
         else:
-            po.align_2(a, b, cm)
-            l1, l2 = max(len(a), len(b)), min(len(a), len(b))
-            cells += po.algn_nw_cells(l1, l2, po.deltaw(a, b, cm))
+            a, b = w.a.get(int(k)), w.b.get(int(k))
+            if kind == "affine":
+                want[int(k)] = po.align_affine_3(a, b, cm)[3]
+                cells += po.affine_cells(len(a), len(b))
+            else:
+                want[int(k)] = po.align_2(a, b, cm)[0]
+                l1, l2 = max(len(a), len(b)), min(len(a), len(b))
+                cells += po.algn_nw_cells(l1, l2, po.deltaw(a, b, cm))
         done += 1
         if time.perf_counter() - t0 > budget_s:
             break
@@ -109,6 +109,121 @@ def cpu_baseline(w, kind: str, budget_s: float):
                   f"{cells} {'cube' if kind == 'cube' else 'band'} cells in {dt:.1f} s, "
                   f"{'align_3 + median_3 per triple' if kind == 'cube' else 'align + median per pair'}, "
                   f"1 pinned core of {cpu_model()}",
+    }, want
+
+
+def oracle_sample_costs(w, kind: str, count: int) -> dict:
+    """Oracle costs of `count` seeded random items (item index -> cost)."""
+    from oracle import pyoracle as po
+    cm = po.CM2.sub_indel(1, 1, 3) if kind == "affine" else po.CM2.sub_indel(1, 2)
+    cm3 = po.CM3(cm) if kind == "cube" else None
+    n = len(w.ln[0]) if kind == "cube" else len(w.a)
+    want = {}
+    for k in np.random.default_rng(11).choice(n, size=min(count, n), replace=False):
+        k = int(k)
+        if kind == "cube":
+            want[k] = po.align_3(*[w.get(q, k) for q in range(3)], cm3)[0]
+        elif kind == "affine":
+            want[k] = po.align_affine_3(w.a.get(k), w.b.get(k), cm)[3]
+        else:
+            want[k] = po.align_2(w.a.get(k), w.b.get(k), cm)[0]
+    return want
+
+
+def run_workload(kind: str, steps: int, warmup: int):
+    """Plans, runs and times one pair / triple workload; returns the
+    workload, the plan's numbers and a verification of the run."""
+    from poyd_amd import engine
+    w, kind = workload(kind)
+    ctx = engine.Context(0)
+    t_plan = time.perf_counter()
+    if kind == "cube":
+        cm = engine.CostMatrix.of_sub_indel(1, 2)
+        cm3 = engine.CostMatrix3(cm)
+        seq = np.concatenate([b.seq for b in w.batches])
+        base = np.cumsum([0] + [len(b.seq) for b in w.batches])
+        plan = engine.TriplePlan(ctx, cm3, *[x for q, b in enumerate(w.batches) for x in (b.off + base[q], b.ln)])
+        seq_lens = [b.ln for b in w.batches]
+    else:
+        if kind == "affine":
+            cm = engine.CostMatrix.of_sub_indel(1, 1, 3)
+            mode = engine.POYG_MODE_AFFINE_ALIGN
+            dws = None
+        else:
+            cm = engine.CostMatrix.of_sub_indel(1, 2)
+            mode = engine.POYG_MODE_LINEAR_ALIGN
+            dws = engine.batch_deltaw(w.a, w.b)
+        seq = np.concatenate([w.a.seq, w.b.seq])
+        plan = engine.PairPlan(ctx, cm, mode, w.a.off, w.a.ln, w.b.off + len(w.a.seq), w.b.ln, dws)
+        seq_lens = [w.a.ln, w.b.ln]
+    plan_ms = (time.perf_counter() - t_plan) * 1e3
+    d_seq = engine.DeviceBuffer.from_array(ctx, seq)
+    d_cost = engine.DeviceBuffer(ctx, 4 * plan.n)
+    d_out = engine.DeviceBuffer(ctx, plan.out_bytes)
+    d_len = engine.DeviceBuffer(ctx, 4 * plan.n * plan.slots)
+    stream = load_stream(ctx)
+
+    def step():
+        plan.run(d_seq.ptr, d_cost.ptr, d_out.ptr, d_len.ptr, stream)
+
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    first_costs = d_cost.download(np.int32, plan.n)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    costs = d_cost.download(np.int32, plan.n)
+    failed = plan.status()
+    deterministic = bool(np.array_equal(costs, first_costs))
+
+    # kernel timings from HIP events recorded on the run stream
+    plan.set_timing(True)
+    fills, traces = [], []
+    for _ in range(max(3, min(steps, 10))):
+        step()
+        f, t = plan.last_times()
+        fills.append(f)
+        traces.append(t)
+    plan.set_timing(False)
+    failed += plan.status()
+    seq_bytes = int(sum(int(x.sum()) for x in seq_lens))
+    out_bytes = int(d_len.download(np.int32, plan.n * plan.slots).astype(np.int64).sum())
+    r = dict(w=w, kind=kind, n=plan.n, cells=int(plan.cells), ms=ms, plan_ms=plan_ms,
+             fill_ms=float(np.mean(fills)), trace_ms=float(np.mean(traces)), costs=costs, failed=int(failed),
+             deterministic=deterministic, seq_bytes=seq_bytes, out_bytes=out_bytes)
+    plan.close()
+    return r
+
+
+def verify(r, want: dict) -> dict:
+    """Plan status (no failed pair in any timed or timing run) and the GPU
+    cost of every oracle-sampled item."""
+    bad = [k for k, c in want.items() if int(r["costs"][k]) != int(c)]
+    return {"plan_status_failed": r["failed"], "oracle_checked": len(want), "oracle_mismatches": len(bad),
+            "verified": r["failed"] == 0 and not bad and r["deterministic"]}
+
+
+def sub_result(kind: str, steps: int, warmup: int, n_check: int) -> dict:
+    """A compact line for a second workload run in the same process."""
+    r = run_workload(kind, steps, warmup)
+    want = oracle_sample_costs(r["w"], r["kind"], n_check)
+    ops = OPS_PER_CELL[r["kind"]]
+    valu = r["cells"] * ops / (r["fill_ms"] / 1e3) / 1e12
+    return {
+        "workload": r["w"].name,
+        "value": round(r["cells"] / (r["ms"] / 1e3) / 1e9, 3),
+        "unit": "GCUPS",
+        "ms_per_step": round(r["ms"], 3),
+        "steps": steps,
+        ("cube_cells" if r["kind"] == "cube" else "band_cells"): r["cells"],
+        "plan_ms": round(r["plan_ms"], 1),
+        "fill_ms": round(r["fill_ms"], 3),
+        "valu_frac": round(valu / VALU_PEAK_TOPS, 4),
+        **verify(r, want),
     }
 
 
@@ -117,9 +232,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 10; tree: 1)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 2; tree: 0)")
-    ap.add_argument("--workload", default="affine", choices=["affine", "linear", "cube", "tree", "fitch"])
+    ap.add_argument("--workload", default="affine", choices=["affine", "linear", "cube", "tree", "fitch", "tbr"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sub", action="store_true", help="skip the linear and three-way sub-results")
+    ap.add_argument("--seeds", type=int, default=None, help="tbr: seeded datasets for the identical-cost rate")
+    ap.add_argument("--taxa", type=int, default=None, help="tbr: taxa per dataset (default 256)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (the reference dispatch that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -132,71 +250,20 @@ def main() -> int:
         return 2
     tree = args.workload == "tree"
     if args.steps is None:
-        args.steps = 1 if tree else 10
+        args.steps = 1 if tree or args.workload == "tbr" else 10
     if args.warmup is None:
-        args.warmup = 0 if tree else 2
+        args.warmup = 0 if tree or args.workload == "tbr" else 2
     if tree:
         return bench_tree(args)
     if args.workload == "fitch":
         return bench_fitch(args)
+    if args.workload == "tbr":
+        from poyd_amd import tbrbench
+        return tbrbench.bench_tbr(args)
 
-    from poyd_amd import engine
-    w, kind = workload(args.workload)
-    ctx = engine.Context(0)
-    if kind == "cube":
-        cm = engine.CostMatrix.of_sub_indel(1, 2)
-        cm3 = engine.CostMatrix3(cm)
-        seq = np.concatenate([b.seq for b in w.batches])
-        base = np.cumsum([0] + [len(b.seq) for b in w.batches])
-        plan = engine.TriplePlan(ctx, cm3, *[x for q, b in enumerate(w.batches) for x in (b.off + base[q], b.ln)])
-        seq_lens = [b.ln for b in w.batches]
-    elif kind == "affine":
-        cm = engine.CostMatrix.of_sub_indel(1, 1, 3)
-        mode = engine.POYG_MODE_AFFINE_ALIGN
-        dws = None
-    else:
-        cm = engine.CostMatrix.of_sub_indel(1, 2)
-        mode = engine.POYG_MODE_LINEAR_ALIGN
-        dws = engine.batch_deltaw(w.a, w.b)
-    if kind != "cube":
-        seq = np.concatenate([w.a.seq, w.b.seq])
-        plan = engine.PairPlan(ctx, cm, mode, w.a.off, w.a.ln, w.b.off + len(w.a.seq), w.b.ln, dws)
-        seq_lens = [w.a.ln, w.b.ln]
-    d_seq = engine.DeviceBuffer.from_array(ctx, seq)
-    d_cost = engine.DeviceBuffer(ctx, 4 * plan.n)
-    d_out = engine.DeviceBuffer(ctx, plan.out_bytes)
-    d_len = engine.DeviceBuffer(ctx, 4 * plan.n * plan.slots)
-    stream = load_stream(ctx)
-
-    def step():
-        plan.run(d_seq.ptr, d_cost.ptr, d_out.ptr, d_len.ptr, stream)
-
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
-    first_costs = d_cost.download(np.int32, plan.n)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.sync()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    costs = d_cost.download(np.int32, plan.n)
-    deterministic = bool(np.array_equal(costs, first_costs))
-
-    # kernel timings from HIP events recorded on the run stream
-    plan.set_timing(True)
-    fills, traces = [], []
-    for _ in range(max(3, min(args.steps, 10))):
-        step()
-        f, t = plan.last_times()
-        fills.append(f)
-        traces.append(t)
-    plan.set_timing(False)
-    fill_ms = float(np.mean(fills))
-    trace_ms = float(np.mean(traces))
-
-    cells = plan.cells
+    r = run_workload(args.workload, args.steps, args.warmup)
+    w, kind, ms, cells = r["w"], r["kind"], r["ms"], r["cells"]
+    fill_ms, trace_ms = r["fill_ms"], r["trace_ms"]
     gcups = cells / (ms / 1e3) / 1e9
     ops = OPS_PER_CELL[kind]
     valu_tops = cells * ops / (fill_ms / 1e3) / 1e12
@@ -204,16 +271,19 @@ def main() -> int:
     # both sequences read once, one int32 cost written per pair; the affine
     # fills also run the backtrace of their pairs, which reads both
     # sequences again and writes the aligned outputs
-    seq_bytes = int(sum(int(x.sum()) for x in seq_lens))
-    alg_bytes = cells + seq_bytes + 4 * plan.n
+    alg_bytes = cells + r["seq_bytes"] + 4 * r["n"]
     if kind == "affine":
-        alg_bytes += seq_bytes + int(d_len.download(np.int32, plan.n * plan.slots).astype(np.int64).sum())
+        alg_bytes += r["seq_bytes"] + r["out_bytes"]
     fill_gbs = alg_bytes / (fill_ms / 1e3) / 1e9
     if kind == "cube":
         matrix_cells = cells
     else:
         matrix_cells = int(((w.a.ln.astype(np.int64) - 1) * (w.b.ln.astype(np.int64) - 1)).sum())
-
+    kernel = ("fill stage with the backtrace fused: aff_quad_kernel (main and side-stream launches) "
+              "and the one-set and lane-set fills, each wave tracing its pairs after filling them; "
+              "one span of HIP events on the main stream") if kind == "affine" \
+        else "cube_rows_kernel (every row class)" if kind == "cube" \
+        else "lin_sys_kernel (systolic strips, four waves per pair)"
     out = {
         "metric": METRIC,
         "value": round(gcups, 3),
@@ -229,45 +299,49 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": w.name,
-            ("triples" if kind == "cube" else "pairs"): plan.n,
+            ("triples" if kind == "cube" else "pairs"): r["n"],
             ("cube_cells" if kind == "cube" else "band_cells"): int(cells),
             "matrix_equivalent_gcups": round(matrix_cells / (ms / 1e3) / 1e9, 3),
             "cost_model": "affine open 3 extend 1 sub 1" if kind == "affine" else "linear sub 1 indel 2",
             "outputs": ("cost, three aligned sequences, median" if kind == "cube" else
                         "cost, aligned pair, median" + (", median with gaps" if kind == "affine" else "")),
+            "plan_ms": round(r["plan_ms"], 1),
             "fill_ms": round(fill_ms, 3),
             "backtrace_ms": round(trace_ms, 3),
-            "deterministic": deterministic,
+            "deterministic": r["deterministic"],
         },
         "roofline": {
-            "bound": "hbm",
-            "achieved": round(fill_gbs, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(fill_gbs / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(kind),
-            "kernel": ("fill stage with the backtrace fused: aff_quad_kernel (main and side-stream launches) "
-                       "and the one-set and lane-set fills, each wave tracing its pairs after filling them; "
-                       "one span of HIP events on the main stream") if kind == "affine"
-                      else "cube_rows_kernel (every row class)" if kind == "cube"
-                      else "lin_sys_kernel (systolic strips, four waves per pair)",
-        },
-        "valu_roofline": {
             "bound": "valu",
             "achieved": round(valu_tops, 3),
             "peak": round(VALU_PEAK_TOPS, 2),
             "unit": "TOP/s (int32)",
             "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
+            "traffic": measured_traffic(kind),
             "ops_per_cell": ops,
+            "kernel": kernel,
+        },
+        "hbm_roofline": {
+            "bound": "hbm",
+            "achieved": round(fill_gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(fill_gbs / HBM_PEAK_GBS, 5),
+            "algorithmic_bytes": int(alg_bytes),
         },
     }
+    want = {}
     if not args.no_cpu:
-        cb = cpu_baseline(w, kind, args.cpu_seconds)
+        cb, want = cpu_baseline(w, kind, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["config"]["gpu_vs_cpu_1core"] = round(gcups / cb["value"], 1)
+    else:
+        want = oracle_sample_costs(w, kind, 16)
+    out["config"].update(verify(r, want))
+    if not args.no_sub and args.workload == "affine":
+        out["sub_results"] = {"linear_1000": sub_result("linear", 20, 3, 64),
+                              "three_way_4096": sub_result("cube", 1, 1, 3)}
     print(json.dumps(out), flush=True)
-    plan.close()
-    return 0
+    return 0 if out["config"]["verified"] else 1
 
 
 def bench_tree(args) -> int:

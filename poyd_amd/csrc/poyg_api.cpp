@@ -243,6 +243,7 @@ struct poyg_plan {
   int main_fill = 0, grid_help = 0, grid_help2 = 0;
   bool timing = false;
   float fill_ms = 0.f, trace_ms = 0.f;
+  hipStream_t last_stream = nullptr;   // the stream of the last run (its status is read there)
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;   // hand-off to and from the side streams
   ~poyg_plan() {
@@ -516,7 +517,13 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       int64_t c = 0;
       lin_band_extent(m.lr, m.lc, g, maxw, c);
       cells[k] = c;
-      m.stride = (maxw + 3 + 3) & ~3;   // room for a row start phase of up to 3 bytes (systolic fill)
+      m.stride = lin_dm_stride(maxw);   // room for a row start phase of up to 3 bytes (systolic fill)
+      if (align && !lin_dm_fits(m.lr, maxw)) {
+        delete P;
+        set_error("poyg_plan_pairs: pair %d (%d x %d) has a direction band of %lld bytes, beyond the linear "
+                  "kernels' 32-bit band offsets", k, m.lr, m.lc, (long long)m.lr * m.stride);
+        return POYG_EUNSUPPORTED;
+      }
       m.aux_off = aux_ints;
       aux_ints += (2 * (int64_t)m.lc + 63) & ~(int64_t)63;
       if (align) {
@@ -600,6 +607,29 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   }
   DevTcm dt;
   to_dev_tcm(cm, &dt);
+  if (!affine) {
+    // The linear kernels mark cells outside the band with 2^30 - 1 and add
+    // costs to it: every entry must be non-negative and the most costly
+    // alignment (every column at the largest entry) must stay below 2^30,
+    // so that no sum overflows and the marker stays above every cell value.
+    int64_t cmax = 0, longest = 0;
+    bool negative = false;
+    for (int x = 0; x < 32 * 32; ++x) {
+      negative |= dt.cost[x] < 0;
+      cmax = std::max<int64_t>(cmax, dt.cost[x]);
+    }
+    for (int x = 0; x < 32; ++x) {
+      negative |= dt.prepend[x] < 0 || dt.tail[x] < 0;
+      cmax = std::max<int64_t>(cmax, std::max(dt.prepend[x], dt.tail[x]));
+    }
+    for (int k = 0; k < n; ++k) longest = std::max<int64_t>(longest, (int64_t)len_a[k] + len_b[k] + 2);
+    if (negative || cmax * longest >= ((int64_t)1 << 30)) {
+      delete P;
+      set_error("poyg_plan_pairs: linear costs must be non-negative and small enough that an alignment costs "
+                "less than 2^30 (largest entry %lld, longest pair %lld)", (long long)cmax, (long long)longest);
+      return POYG_EINVAL;
+    }
+  }
   if (affine) {
     // The affine kernels rank equal minima by packing (value << 2 | rank);
     // every DP value is non-negative and at most 3 kHigh plus the cost of
@@ -655,7 +685,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
   rc |= P->rows.alloc(P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16);
   rc |= P->counters.alloc(sizeof(int) * 8);
-  rc |= P->status.alloc(sizeof(int32_t) * 4);
+  rc |= P->status.alloc(sizeof(int32_t) * ((size_t)kStatusFlags + n));
   rc |= P->sink.alloc(sizeof(uint32_t) * kWave);
   rc |= P->trout.alloc(sizeof(TraceOut));
   if (!rc && affine && P->grid_help &&
@@ -705,7 +735,8 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   const DevTcm *tcm = (const DevTcm *)P->tcm.p;
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[0], s));
   HIP_TRY(launch_reset(ctr, 8, s));
-  HIP_TRY(launch_reset((int *)P->status.p, 4, s));
+  HIP_TRY(launch_reset((int *)P->status.p, kStatusFlags + P->n, s));
+  P->last_stream = s;
   if (affine && align)
     HIP_TRY(launch_set_trace_out((TraceOut *)P->trout.p, d_out, d_out_len, (int32_t *)P->status.p, s));
   if (!affine || P->n_lds + P->n_glb)
@@ -801,11 +832,15 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   return POYG_OK;
 }
 
-// Number of backtraces of the last run that left the band (0 when sane).
+// Number of pairs of the last run that failed (0 when sane), read on the
+// stream that run was queued on, after its kernels.
 extern "C" int poyg_plan_status(poyg_plan *P, int32_t *bad) {
+  if (!P || !bad) return POYG_EINVAL;
   if (P->triple) return poyg_triple_plan_status(P->triple, bad);
-  HIP_TRY(hipMemcpyAsync(bad, P->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, P->ctx->stream));
-  HIP_TRY(hipStreamSynchronize(P->ctx->stream));
+  *bad = 0;
+  if (!P->last_stream) return POYG_OK;   // never run (or nothing to run)
+  HIP_TRY(hipMemcpyAsync(bad, P->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, P->last_stream));
+  HIP_TRY(hipStreamSynchronize(P->last_stream));
   return POYG_OK;
 }
 

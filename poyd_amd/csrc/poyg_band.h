@@ -41,6 +41,15 @@ inline void lin_row_host(int i, int m, int n, const LinGeom &g, int &lo, int &hi
   else { lo = 1 + (i - (m - (n - w) + 1)); hi = n - 1; }
 }
 
+// The linear kernels address a pair's direction band with 32-bit offsets
+// and use 0x7ffffff0 as their "out of range" offset: a pair whose band
+// (rows x stride bytes) reaches it cannot be aligned by them.
+constexpr int64_t kLinDmLimit = 0x7ffffff0;
+// Bytes per direction row of a linear band whose widest row is maxw: room
+// for a start phase of up to 3 bytes, rounded to 4.
+inline int lin_dm_stride(int maxw) { return (maxw + 3 + 3) & ~3; }
+inline bool lin_dm_fits(int m, int maxw) { return (int64_t)m * lin_dm_stride(maxw) < kLinDmLimit; }
+
 // Widest row and number of cells of a linear band.  Within each of the three
 // sections of lin_row_host the row width is affine in the row (growing by
 // one, constant, shrinking by one), so a section's cells are an arithmetic

@@ -881,6 +881,7 @@ struct TriplePlan {
   void *d_tcm = nullptr, *d_meta = nullptr, *d_order = nullptr, *d_dm = nullptr, *d_scratch = nullptr,
        *d_ctr = nullptr, *d_status = nullptr;
   hipStream_t stream = nullptr, side = nullptr, side2 = nullptr;
+  hipStream_t last_stream = nullptr;   // the stream of the last run (its status is read there)
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;   // row classes run on three streams
   ~TriplePlan() {
     for (void *p : {d_tcm, d_meta, d_order, d_dm, d_scratch, d_ctr, d_status})
@@ -1070,12 +1071,14 @@ int poyg_triple_plan_run(void *pp, const uint8_t *d_seq, int32_t *d_cost, uint8_
   if (timing)
     for (auto &x : ev) HIP_TRY3(hipEventCreate(&x));
   float f_tot = 0, t_tot = 0;
+  // the status counts the failed triples of the whole run (every batch)
+  HIP_TRY3(launch_reset((int *)P->d_status, 4, s));
+  P->last_stream = s;
   for (auto &b : P->batches) {
     const int nb = b.second - b.first;
     const int32_t *ord = (const int32_t *)P->d_order + b.first;
     if (timing) HIP_TRY3(hipEventRecord(ev[0], s));
     HIP_TRY3(launch_reset((int *)P->d_ctr, 16, s));
-    HIP_TRY3(launch_reset((int *)P->d_status, 4, s));
     // one launch per row class present in the batch (the order is sorted by
     // class), the classes side by side: the first on the third stream, the
     // last on the run stream, any others on the second
@@ -1147,9 +1150,13 @@ int poyg_triple_plan_run(void *pp, const uint8_t *d_seq, int32_t *d_cost, uint8_
   return POYG_OK;
 }
 
+// Failed triples of the last run, read on the stream that run was queued on.
 int poyg_triple_plan_status(void *pp, int32_t *bad) {
   TriplePlan *P = (TriplePlan *)pp;
-  HIP_TRY3(hipMemcpy(bad, P->d_status, sizeof(int32_t), hipMemcpyDeviceToHost));
+  *bad = 0;
+  if (!P->last_stream) return POYG_OK;
+  HIP_TRY3(hipMemcpyAsync(bad, P->d_status, sizeof(int32_t), hipMemcpyDeviceToHost, P->last_stream));
+  HIP_TRY3(hipStreamSynchronize(P->last_stream));
   return POYG_OK;
 }
 

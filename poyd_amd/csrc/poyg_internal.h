@@ -55,6 +55,11 @@ struct TraceOut {
   int32_t *status;
 };
 
+// A plan's status block: word 0 counts the pairs of the last run that
+// failed, words kStatusFlags + p hold one flag per pair (so that a pair that
+// fails in several waves or kernels counts once).
+constexpr int kStatusFlags = 4;
+
 // Linear band row descriptor kinds (src/algn.c:872-968).
 enum LinearCase : int32_t { kLinFull = 0, kLinCase2 = 1, kLinCase3 = 2 };
 

@@ -146,7 +146,16 @@ __device__ __forceinline__ int next_item(int *counter) {
 }
 
 __global__ void reset_counters_kernel(int *c, int n) {
-  if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) c[k] = 0;
+}
+
+// Marks pair p of the run as failed: its flag (word kStatusFlags + p of the
+// plan's status block) is set once, and the count in word 0 grows only for
+// the first mark, so that a pair that fails in several waves or kernels
+// counts once.  Safe to call from every lane of a wave.
+__device__ __forceinline__ void mark_failed(int32_t *status, int p) {
+  const int old = atomicOr(&status[kStatusFlags + p], 1);
+  atomicAdd(&status[0], old == 0 ? 1 : 0);
 }
 
 // Right-to-left byte writer that stores whole dwords (slot bases and
@@ -1180,7 +1189,7 @@ __device__ __forceinline__ void aff_trace_pair(const PairMeta &m, int p, const u
   out_len[4 * p + 1] = S0;
   out_len[4 * p + 2] = S0;
   out_len[4 * p + 3] = S0;
-  if (bad) atomicAdd(status, lane == 0 ? 1 : 0);
+  if (bad) mark_failed(status, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1578,7 +1587,7 @@ __global__ void aff_trace_kernel(const PairMeta *__restrict__ meta, const int32_
   out_len[4 * p + 1] = m.cap - ww.pos;
   out_len[4 * p + (m.swap ? 3 : 2)] = m.cap - wa.pos;
   out_len[4 * p + (m.swap ? 2 : 3)] = m.cap - wb.pos;
-  if (bad) atomicAdd(status, 1);
+  if (bad) mark_failed(status, p);
 }
 
 
@@ -2124,7 +2133,7 @@ __global__ __launch_bounds__(kWave * kSysWaves) void lin_sys_kernel(
     bool ok = true;
     for (int s = wv; s < nstrips; s += kSysWaves)
       ok = lin_sys_strip<kDm>(s, nstrips, M, N, cs, w, m.p2, S, s1, L, drsrc, cost_out, p, spin_limit) && ok;
-    if (!ok && lane_id() == 0) atomicAdd(status, 1);
+    if (!ok) mark_failed(status, p);
   }
 }
 
@@ -2400,7 +2409,7 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
     out_len[3 * p + 0] = len;
     out_len[3 * p + 1] = len;
     out_len[3 * p + 2] = cap - (medpos - 1);
-    if (bad && lane == 0) atomicAdd(status, 1);
+    if (bad) mark_failed(status, p);
   }
 }
 
@@ -2502,7 +2511,8 @@ __global__ void fitch_up_kernel(uint8_t *chars, int len, int n, const int32_t *i
 namespace poyg {
 
 hipError_t launch_reset(int *counters, int n, hipStream_t s) {
-  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters, n);
+  const int grid = std::max(1, std::min((n + 255) / 256, 1024));
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(grid), dim3(256), 0, s, counters, n);
   return hipGetLastError();
 }
 

@@ -454,6 +454,12 @@ class PairPlan:
         load().poyg_plan_last_times(self.h, C.byref(f), C.byref(t))
         return f.value, t.value
 
+    def status(self) -> int:
+        """Pairs of the last run that failed (poyg_plan_status)."""
+        bad = C.c_int32(0)
+        _check(load().poyg_plan_status(self.h, C.byref(bad)), "poyg_plan_status")
+        return int(bad.value)
+
     def run(self, d_seq, d_cost, d_out=None, d_out_len=None, stream=None) -> None:
         """d_* are device pointers (ints) of caller-owned buffers."""
         _check(load().poyg_plan_run(self.h, d_seq, d_cost, d_out or None, d_out_len or None, stream),

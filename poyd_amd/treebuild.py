@@ -117,7 +117,11 @@ def update_directional(ev: TreeEval, d: dict, v: int) -> None:
         frontier = nxt
 
 
-def wagner(ds: Dataset, backend, progress=None) -> BuildResult:
+def wagner(ds: Dataset, backend, progress=None, on_step=None) -> BuildResult:
+    """Wagner build in taxon order.  on_step(x, tree, edges, costs), if
+    given, sees each addition before it is made: the tree so far (mutated
+    afterwards, so a caller that keeps it copies it), its pre-order edges
+    and taxon x's join cost to each."""
     n = ds.n_taxa
     if n < 3:
         raise ValueError("a Wagner build needs at least three taxa")
@@ -131,6 +135,8 @@ def wagner(ds: Dataset, backend, progress=None) -> BuildResult:
         edges = t.pre_order_edges()
         emed = ev._medians([(d[(b, a)], d[(a, b)]) for (a, b) in edges])
         jc = join_costs(ev, x, edges, emed)
+        if on_step is not None:
+            on_step(x, t, edges, jc)
         k = int(np.argmin(jc))           # first least
         a, b = edges[k]
         v = nxt

@@ -55,3 +55,42 @@ def test_band_extent_closed_form_equals_row_walk(tmp_path):
     assert bad == 0
     assert tot > 100000
     assert full > 0 and case2 > 0 and case3 > 0   # every band case is exercised
+
+
+LIMIT_CHECKER = r'''
+#include <cstdio>
+#include "poyg_band.h"
+using namespace poyg;
+int main(int argc, char **argv) {
+  int m, n, dw;
+  while (scanf("%d %d %d", &m, &n, &dw) == 3) {
+    const LinGeom g = lin_geom(m, n, dw);
+    int mx;
+    int64_t c;
+    lin_band_extent(m, n, g, mx, c);
+    printf("%d %lld\n", lin_dm_fits(m, mx) ? 1 : 0, (long long)m * lin_dm_stride(mx));
+  }
+  return 0;
+}
+'''
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_linear_direction_band_limit(tmp_path):
+    """The linear plans refuse (POYG_EUNSUPPORTED) a pair whose direction
+    band reaches the kernels' 32-bit "out of range" offset 0x7ffffff0: the
+    header's test agrees with the byte count for pairs on both sides of it."""
+    src = tmp_path / "limit.cpp"
+    exe = tmp_path / "limit"
+    src.write_text(LIMIT_CHECKER)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.dirname(HEADER), "-o", str(exe), str(src)],
+                   check=True)
+    cases = [(16385, 16385, 2000), (5001, 5001, 600), (60000, 40000, 100), (40000, 39000, 5000),
+             (70000, 40000, 10)]
+    inp = "".join(f"{m} {n} {dw}\n" for m, n, dw in cases)
+    out = subprocess.run([str(exe)], input=inp, check=True, capture_output=True, text=True).stdout.split("\n")
+    got = [tuple(map(int, line.split())) for line in out if line.strip()]
+    for (m, n, dw), (fits, nbytes) in zip(cases, got):
+        assert fits == (nbytes < 0x7ffffff0), (m, n, dw)
+    assert got[0][0] == 1 and got[1][0] == 1            # reference-sized sequences (<= 16384) fit
+    assert got[2][0] == 0 and got[4][0] == 0            # a full plane of 60k x 40k does not

@@ -512,7 +512,8 @@ def test_linear_fill_wait_gives_up_reported(ctx, oracle, monkeypatch):
     cm = engine.CostMatrix.of_sub_indel(1, 2)
     ref = oracle.CM2.sub_indel(1, 2)
     monkeypatch.setenv("POYG_SYS_SPIN_LIMIT", "0")
-    with pytest.raises(engine.PoygError, match="pairs failed"):
+    # each failed pair counts once, however many of its waves gave up
+    with pytest.raises(engine.PoygError, match=r"\b[1-4] pairs failed"):
         engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
     monkeypatch.delenv("POYG_SYS_SPIN_LIMIT")
     res = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
@@ -520,3 +521,30 @@ def test_linear_fill_wait_gives_up_reported(ctx, oracle, monkeypatch):
         cost, oa, ob = oracle.align_2(a, b, ref)
         assert res.cost[k] == cost, k
         assert np.array_equal(res.seq(k, 0), oa), k
+
+
+def test_plan_rejects_what_the_kernels_cannot_hold(ctx):
+    """Planning refuses, before any allocation or launch: a linear matrix
+    with a negative entry (POYG_EINVAL: the kernels' out-of-band marker must
+    stay above every cell value), and a linear alignment whose direction band
+    reaches the kernels' 32-bit offsets (POYG_EUNSUPPORTED)."""
+    import ctypes as C
+    from poyd_amd import engine
+    note("plan guards")
+    L = engine.load()
+    cm = engine.CostMatrix.of_sub_indel(1, 2)
+    neg = engine.CostMatrix.of_list([[0, -1, 1, 1, 2], [-1, 0, 1, 1, 2], [1, 1, 0, 1, 2], [1, 1, 1, 0, 2],
+                                     [2, 2, 2, 2, 0]])
+
+    def plan(m, mode, la, lb, dw):
+        h = C.c_void_p()
+        rc = L.poyg_plan_pairs(ctx.h, m.h, mode, 1, (C.c_int64 * 1)(0), (C.c_int32 * 1)(la),
+                               (C.c_int64 * 1)(0), (C.c_int32 * 1)(lb), (C.c_int32 * 1)(dw), C.byref(h))
+        if rc == 0:
+            L.poyg_plan_destroy(h)
+        return rc
+
+    assert plan(cm, engine.POYG_MODE_LINEAR_ALIGN, 300, 280, 40) == 0
+    assert plan(neg, engine.POYG_MODE_LINEAR_COST, 300, 280, 40) == -1
+    assert plan(cm, engine.POYG_MODE_LINEAR_ALIGN, 60000, 40000, 100) == -4
+    assert plan(cm, engine.POYG_MODE_LINEAR_COST, 60000, 40000, 100) == 0   # no direction band

@@ -266,6 +266,116 @@ def test_cube_alignment_is_valid(oracle):
         assert len(med) == len(a1) and len(set(med.tolist())) <= 1
 
 
+def _plane0(s2, s3, c):
+    """Plane 0 of algn_fill_cube (src/algn.c:2940-2985): the textbook 2-D DP
+    over s2, s3 with the costs of aligning against the gap of s1."""
+    m, n = len(s2), len(s3)
+    P = np.zeros((m, n), np.int64)
+    for k in range(1, n):
+        P[0, k] = P[0, k - 1] + c(GAP, GAP, int(s3[k]))
+    for r in range(1, m):
+        b = int(s2[r])
+        P[r, 0] = P[r - 1, 0] + c(GAP, b, GAP)
+        for k in range(1, n):
+            P[r, k] = min(P[r - 1, k] + c(GAP, b, GAP), P[r - 1, k - 1] + c(GAP, b, int(s3[k])),
+                          P[r, k - 1] + c(GAP, GAP, int(s3[k])))
+    return P
+
+
+def _line0(up, a, s3, c):
+    """Line 0 of a plane p >= 1 (src/algn.c:2997-3016): moves from the row
+    the "diagonal" pointer names (`up`) and along the line itself."""
+    n = len(s3)
+    X = np.zeros(n, np.int64)
+    X[0] = up[0] + c(a, GAP, GAP)
+    for k in range(1, n):
+        x3 = int(s3[k])
+        X[k] = min(up[k] + c(a, GAP, GAP), up[k - 1] + c(a, GAP, x3), X[k - 1] + c(GAP, GAP, x3))
+    return X
+
+
+def test_cube_one_line_planes_read_plane_zero(oracle):
+    """With s2 = [gap] the line loop of algn_fill_cube (src/algn.c:3023-3053)
+    never runs, so the pointer that line 0 of every plane reads (diag_m,
+    src/algn.c:2991, 3002) never leaves plane 0: the cost depends on s1's
+    last symbol only.  A textbook three-way recurrence fails both checks."""
+    cm3 = oracle.CM3(oracle.CM2.sub_indel(1, 2))
+    c = cm3.cost
+    rng = np.random.default_rng(41)
+    s2 = np.array([GAP], np.uint8)
+    differs = 0
+    for t in range(40):
+        s1 = rand_dna(rng, int(rng.integers(1, 12)))
+        s3 = rand_dna(rng, int(rng.integers(0, 12)))
+        got = oracle.cost_3(s1, s2, s3, cm3)
+        want = _line0(_plane0(s2, s3, c)[0], int(s1[-1]), s3, c)[-1]
+        assert got == want, (t, s1, s3)
+        assert got == oracle.cost_3(np.array([GAP, s1[-1]], np.uint8), s2, s3, cm3)
+        # the textbook cost (every plane reads its predecessor) differs once
+        # s1 has two or more symbols and s3 is long enough
+        X = _plane0(s2, s3, c)[0]
+        for a in s1[1:]:
+            X = _line0(X, int(a), s3, c)
+        differs += int(X[-1] != got)
+    assert differs > 0
+
+
+def test_cube_two_planes_lagged_previous_line(oracle):
+    """Two planes (s1 = [gap, a]): in plane 1, lines j >= 1 read plane 0's
+    lines j - 1 ("diagonal") and j ("upper") and, as their "previous" row,
+    the flat row j - 2 rows after plane 0's end: plane 0's last line for
+    j = 1, plane 1's line j - 2 after that (pointer walk of
+    src/algn.c:2987-3053: diag_m, upper_m and prev_m advance n2 - 1 rows per
+    plane while mm advances n2).  Moves in the reference's order: P3, P1,
+    P2, S3, S1, S2, then the self pass."""
+    cm3 = oracle.CM3(oracle.CM2.sub_indel(1, 2))
+    c = cm3.cost
+    rng = np.random.default_rng(43)
+    for t in range(40):
+        a = int(rng.choice([1, 2, 4, 8]))
+        s1 = np.array([GAP, a], np.uint8)
+        s2 = rand_dna(rng, int(rng.integers(1, 10)))
+        s3 = rand_dna(rng, int(rng.integers(0, 10)))
+        P0 = _plane0(s2, s3, c)
+        n2, n3 = len(s2), len(s3)
+        X = [_line0(P0[0], a, s3, c)]
+        for j in range(1, n2):
+            b = int(s2[j])
+            prev = P0[n2 - 1] if j == 1 else X[j - 2]
+            up, dg = P0[j], P0[j - 1]
+            row = np.zeros(n3, np.int64)
+            for k in range(n3):
+                v = min(up[k] + c(a, GAP, GAP), prev[k] + c(GAP, b, GAP), dg[k] + c(a, b, GAP))
+                if k:
+                    x3 = int(s3[k])
+                    v = min(v, up[k - 1] + c(a, GAP, x3), prev[k - 1] + c(GAP, b, x3), dg[k - 1] + c(a, b, x3),
+                            row[k - 1] + c(GAP, GAP, x3))
+                row[k] = v
+            X.append(row)
+        assert oracle.cost_3(s1, s2, s3, cm3) == X[-1][-1], (t, s1, s2, s3)
+
+
+def test_median_3_is_the_last_columns_median(oracle):
+    """algn_get_median_3d (src/algn.c:4160-4173) reads *end1, *end2, *end3
+    without moving them: the median has one symbol per aligned column (the
+    loop runs len(s1) times), each the three-way median of the last
+    column."""
+    cm3 = oracle.CM3(oracle.CM2.sub_indel(1, 2))
+    rng = np.random.default_rng(47)
+    for _ in range(30):
+        anc = rand_dna(rng, int(rng.integers(3, 25)))
+        s = [mutate(rng, anc) for _ in range(3)]
+        cost, a1, a2, a3 = oracle.align_3(*s, cm3)
+        med = oracle.median_3(a1, a2, a3, cm3)
+        want = cm3.median(int(a1[-1]), int(a2[-1]), int(a3[-1]))
+        assert len(med) == len(a1) and all(int(x) == want for x in med)
+    # columns other than the last do not matter
+    x = np.array([GAP, A, Cb, G], np.uint8)
+    y = np.array([GAP, T, T, G], np.uint8)
+    z = np.array([GAP, GAP, A, G], np.uint8)
+    assert oracle.median_3(x, y, z, cm3).tolist() == [cm3.median(G, G, G)] * 4
+
+
 # ---------------------------------------------------------------- R12/R13 --
 def random_tree(rng, nleaves):
     """Random rooted binary tree: list of (node, left, right), root last."""
