@@ -238,6 +238,7 @@ def main() -> int:
     ap.add_argument("--no-sub", action="store_true", help="skip the linear and three-way sub-results")
     ap.add_argument("--seeds", type=int, default=None, help="tbr: seeded datasets for the identical-cost rate")
     ap.add_argument("--taxa", type=int, default=None, help="tbr: taxa per dataset (default 256)")
+    ap.add_argument("--epochs", type=int, default=8, help="tbr: training epochs of the join scorer")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (the reference dispatch that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -258,8 +259,7 @@ def main() -> int:
     if args.workload == "fitch":
         return bench_fitch(args)
     if args.workload == "tbr":
-        from poyd_amd import tbrbench
-        return tbrbench.bench_tbr(args)
+        return bench_tbr(args)
 
     r = run_workload(args.workload, args.steps, args.warmup)
     w, kind, ms, cells = r["w"], r["kind"], r["ms"], r["cells"]
@@ -566,6 +566,265 @@ def bench_fitch(args) -> int:
                                          f"of {cpu_model()}"}
     print(json.dumps(out), flush=True)
     return 0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def tbr_dataset(n_taxa: int, seed: int):
+    """256 taxa (or n_taxa) on a seeded Yule tree, 3 DNA loci of 500-2,000
+    bp, no nonadditive characters; the same simulator as the tree workload."""
+    from poyd_amd import synth
+    from poyd_amd.treedata import Dataset
+    w = synth.tree_workload(n_taxa=n_taxa, n_loci=3, lmin=500, lmax=2000, n_nonadd=0, seed=seed,
+                            name=f"tbr-{n_taxa}taxa-3loci-500-2000-seed{seed}")
+    return Dataset(list(w.names), [list(l) for l in w.loci], None), w.name
+
+
+class TbrCase:
+    """One dataset: its Wagner tree (built on the GPU), the unbroken tree's
+    directional medians in a device store, and the TBR candidates."""
+
+    def __init__(self, ctx, cm, n_taxa, seed, arena_gb):
+        from poyd_amd import seqstore, tbr, treebuild, treepass
+        self.ds, self.name = tbr_dataset(n_taxa, seed)
+        t0 = time.perf_counter()
+        self.tree = treebuild.wagner(self.ds, treepass.EngineBackend(ctx, cm)).tree
+        self.build_s = time.perf_counter() - t0
+        self.store = seqstore.DeviceStore(ctx, cm, int(arena_gb * (1 << 30)))
+        L = len(self.ds.chars)
+        leaf_ids = {v: self.store.add([self.ds.chars[c][v - 1] for c in range(L)]) for v in self.tree.leaves}
+        t0 = time.perf_counter()
+        self.dirs, self.index = tbr.directional(self.store, self.tree, leaf_ids)
+        self.dir_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        self.sp = tbr.neighbourhood(self.tree, self.index)
+        self.cand = tbr.candidates(self.sp, self.dirs, len(self.dirs))
+        self.enum_s = time.perf_counter() - t0
+
+    def close(self):
+        self.store.arena.free()
+
+
+def tbr_training_rows(case, res, n_rows, rng):
+    """(clade id, edge id, exact distance) rows of the model's training
+    set: the model reads the unbroken tree's medians, the label is the
+    exact per-character join cost of the exhaustive round."""
+    from poyd_amd import surrogate
+    te, g_te = surrogate.unbroken_edge_medians(case.store, case.sp, case.dirs, case.index)
+    K = len(case.dirs)
+    ca = surrogate.approx_ids(case.cand.clade, case.dirs, te, g_te, K)
+    ea = surrogate.approx_ids(case.cand.edge, case.dirs, te, g_te, K)
+    flat = rng.choice(ca.size, size=min(n_rows, ca.size), replace=False)
+    return ca.ravel()[flat], ea.ravel()[flat], res.dist.ravel()[flat], te
+
+
+def bench_tbr(args) -> int:
+    """Surrogate-guided TBR against exhaustive exact TBR.  One step = one
+    TBR neighbourhood round of a Wagner tree: every break x clade root x
+    join edge candidate.  Exhaustive: every candidate scored by the exact
+    DP (the changed component medians, every edge median, every join
+    distance).  Guided: the learned model scores every candidate from 4-mer
+    profiles of the unbroken tree's medians; the best 8% are scored exactly.
+    value = candidate joins per second of the guided round; the exhaustive
+    round's rate, the best-join recall and the identical-best-cost rate over
+    the seeds are in config.  The model is trained first, on the exact
+    costs of the exhaustive round of a separate training dataset."""
+    from poyd_amd import engine, surrogate, tbr, treepass
+    n_taxa = args.taxa or 256
+    seeds = args.seeds or 1
+    ctx = engine.Context(0)
+    cm = engine.CostMatrix.of_sub_indel(1, 2)
+    arena_gb = 64 if n_taxa >= 200 else 16
+    rng = np.random.default_rng(2026)
+    # ---- training: the exhaustive round of a training dataset labels 1M pairs
+    t0 = time.perf_counter()
+    tc = TbrCase(ctx, cm, n_taxa, synth_seed(100), arena_gb)
+    log(f"training dataset {tc.name}: Wagner build {tc.build_s:.1f} s, {len(tc.cand.brk)} candidates")
+    tres = tbr.exhaustive_round(tc.store, tc.sp, tc.dirs, tc.cand)
+    ia, ib, lab, _ = tbr_training_rows(tc, tres, 1 << 20, rng)
+    prof = surrogate.Profiles(tc.store, np.concatenate([ia, ib]))
+    x = prof.features(ctx, ia, ib)
+    tgt = engine.DeviceBuffer.from_array(ctx, (lab / surrogate.Y_SCALE).astype(np.float32))
+    n_train = len(lab)
+    scorer = surrogate.Scorer(ctx, batch_max=4096, seed=7)
+    t1 = time.perf_counter()
+    losses = scorer.train(x, tgt, n_train, epochs=args.epochs, lr=1e-3, seed=3, log=log)
+    ctx.sync()
+    train_s = time.perf_counter() - t1
+    data_s = t1 - t0
+    x.free()
+    tgt.free()
+    prof.counts.free()
+    tc.close()
+    del tc
+    ctx.sync()
+    # ---- evaluation seeds
+    rows = []
+    for s in range(seeds):
+        case = TbrCase(ctx, cm, n_taxa, synth_seed(4) + s, arena_gb)
+        J = len(case.cand.brk)
+        log(f"seed {s} {case.name}: Wagner build {case.build_s:.1f} s, {J} candidates, "
+            f"{len(case.sp.breaks)} breaks")
+        ctx.sync()
+        ex = tbr.exhaustive_round(case.store, case.sp, case.dirs, case.cand)
+        ctx.sync()
+        gr, info = surrogate.guided_round(case.store, scorer, case.sp, case.dirs, case.index, case.cand, 0.08)
+        ctx.sync()
+        # the same budget without the model: a seeded random 8%, and the plain
+        # L1 distance of the 4-mer profiles in place of the network
+        n_pick = len(gr.scored)
+        rnd = np.sort(np.random.default_rng(s).choice(J, size=n_pick, replace=False))
+        l1 = profile_l1_pick(case, n_pick)
+        be = treepass.EngineBackend(ctx, cm)
+
+        def exact_cost(k):
+            return treepass.TreeEval(case.ds, tbr.apply_join(case.sp, case.cand, int(k)), be).diagnose().cost
+
+        start_cost = treepass.TreeEval(case.ds, case.tree, be).diagnose().cost
+        ex_cost = exact_cost(ex.best)
+        gr_cost = ex_cost if gr.best == ex.best else exact_cost(gr.best)
+        best_set = np.nonzero(ex.estimates == ex.best_estimate)[0]
+        row = {
+            "seed": s, "candidates": J, "start_cost": int(start_cost),
+            "exhaustive_s": ex.seconds + case.enum_s, "guided_s": info["total_s"] + case.enum_s,
+            "guided_model_s": info["model_s"], "guided_exact_s": info["exact_s"],
+            "guided_edge_medians_s": info["edge_medians_s"], "enum_s": case.enum_s,
+            "exhaustive_alignments": ex.alignments, "guided_alignments": gr.alignments,
+            "exhaustive_cells": ex.cells, "guided_cells": gr.cells,
+            "best_estimate": ex.best_estimate, "guided_best_estimate": gr.best_estimate,
+            "best_cost": int(ex_cost), "guided_cost": int(gr_cost),
+            "recall": bool(np.isin(ex.best, gr.scored)),
+            "recall_any_best": bool(np.isin(best_set, gr.scored).any()),
+            "random_recall": bool(np.isin(ex.best, rnd)), "l1_recall": bool(np.isin(ex.best, l1)),
+            "random_best_estimate": int(ex.estimates[rnd].min()), "l1_best_estimate": int(ex.estimates[l1].min()),
+            "model_rank_of_best": int((info["predicted"].sum(1) + case.cand.base <
+                                       info["predicted"][ex.best].sum() + case.cand.base[ex.best]).sum()),
+        }
+        rows.append(row)
+        log(json.dumps(row))
+        case.close()
+        del case
+        ctx.sync()
+    r0 = rows[0]
+    g_rate = r0["candidates"] / r0["guided_s"]
+    e_rate = r0["candidates"] / r0["exhaustive_s"]
+    same = [r["guided_cost"] == r["best_cost"] for r in rows]
+    out = {
+        "metric": METRIC,
+        "value": round(g_rate, 1),
+        "unit": "joins/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * r0["guided_s"], 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 DP; bf16 model (f32 accumulate)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"tbr-{n_taxa}taxa-3loci-500-2000",
+            "taxa": n_taxa,
+            "loci": 3,
+            "cost_model": "linear sub 1 indel 2",
+            "start_tree": "Wagner build in taxon order (GPU)",
+            "round": "one TBR neighbourhood: every break x clade root x join edge",
+            "candidates": r0["candidates"],
+            "exhaustive_joins_per_s": round(e_rate, 1),
+            "guided_joins_per_s": round(g_rate, 1),
+            "speedup": round(g_rate / e_rate, 2),
+            "rescored_fraction": 0.08,
+            "seeds": seeds,
+            "best_join_recall": round(float(np.mean([r["recall"] for r in rows])), 4),
+            "identical_best_cost_rate": round(float(np.mean(same)), 4),
+            "random_8pct_recall": round(float(np.mean([r["random_recall"] for r in rows])), 4),
+            "profile_l1_8pct_recall": round(float(np.mean([r["l1_recall"] for r in rows])), 4),
+            "model": "MLP 256-256-256-1, ReLU, bf16 weights, f32 accumulate, Adam, MSE",
+            "training_pairs": int(n_train),
+            "training_loss": [round(float(l), 5) for l in losses],
+            "training_s": round(train_s, 2),
+            "training_data_s": round(data_s, 1),
+            "per_seed": rows,
+        },
+    }
+    ex_cells, ex_s = r0["exhaustive_cells"], r0["exhaustive_s"]
+    valu = ex_cells * OPS_PER_CELL["linear"] / ex_s / 1e12
+    out["roofline"] = {"bound": "valu", "achieved": round(valu, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                       "unit": "TOP/s (int32)", "frac": round(valu / VALU_PEAK_TOPS, 4), "traffic": None,
+                       "kernel": "exhaustive round: every linear DP of the round (lin_sys_kernel fills, "
+                                 "lin_trace_wave_kernel), 8 ops per band cell over the round's wall time"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_tbr(n_taxa, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def synth_seed(k: int) -> int:
+    from poyd_amd import synth
+    return synth.SEED0 + k
+
+
+def profile_l1_pick(case, n_pick):
+    """The n_pick candidates of least broken cost plus summed L1 distance of
+    4-mer profiles (the unbroken tree's medians), without the network."""
+    from poyd_amd import surrogate
+    te, g_te = surrogate.unbroken_edge_medians(case.store, case.sp, case.dirs, case.index)
+    K = len(case.dirs)
+    ca = surrogate.approx_ids(case.cand.clade, case.dirs, te, g_te, K)
+    ea = surrogate.approx_ids(case.cand.edge, case.dirs, te, g_te, K)
+    prof = surrogate.Profiles(case.store, np.concatenate([ca.ravel(), ea.ravel()]))
+    counts = prof.counts.download(np.float32, 256 * len(prof.ids)).reshape(len(prof.ids), 256)
+    prof.counts.free()
+    ra, rb = prof.rows(ca.ravel()), prof.rows(ea.ravel())
+    l1 = np.empty(len(ra), np.float32)
+    for c0 in range(0, len(ra), 1 << 18):
+        l1[c0:c0 + (1 << 18)] = np.abs(counts[ra[c0:c0 + (1 << 18)]] - counts[rb[c0:c0 + (1 << 18)]]).sum(1)
+    # scale the profile distance to the join cost's units by a least-squares
+    # fit would need exact costs; rank within the broken-cost ties instead
+    score = case.cand.base + l1.reshape(ca.shape).sum(1) * 0.5
+    return np.sort(np.argpartition(score, n_pick - 1)[:n_pick])
+
+
+def cpu_baseline_tbr(n_taxa: int, budget_s: float):
+    """The oracle scoring whole breaks of a smaller round exactly (the same
+    simulator, 32 taxa: the oracle cannot finish a 256-taxon round in the
+    bench's time) on one pinned core, breaks in a seeded random order until
+    the budget is spent: candidate joins per second, the component medians
+    of each break included."""
+    from oracle import pyoracle as po
+    from poyd_amd import tbr, treebuild
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_backend import OracleBackend, OracleStore
+    ds, name = tbr_dataset(32, synth_seed(4))
+    cm = po.CM2.sub_indel(1, 2)
+    tree = treebuild.wagner(ds, OracleBackend(cm, False)).tree
+    store = OracleStore(cm, False)
+    leaf_ids = {v: store.add([ds.chars[c][v - 1] for c in range(3)]) for v in tree.leaves}
+    dirs, index = tbr.directional(store, tree, leaf_ids)
+    sp = tbr.neighbourhood(tree, index)
+    cand = tbr.candidates(sp, dirs, len(dirs))
+    try:
+        old_aff = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old_aff)})
+    except (AttributeError, OSError):
+        old_aff = None
+    done, nb, t0 = 0, 0, time.perf_counter()
+    for b in np.random.default_rng(5).permutation(len(sp.breaks)):
+        if time.perf_counter() - t0 >= budget_s:
+            break
+        pick = np.nonzero(cand.brk == b)[0]
+        tbr.rescore_round(store, sp, dirs, cand, pick)
+        done += len(pick)
+        nb += 1
+    dt = time.perf_counter() - t0
+    if old_aff is not None:
+        os.sched_setaffinity(0, old_aff)
+    return {"value": done / dt, "unit": "joins/s", "cores": 1, "kind": "port",
+            "sample": f"{nb} of {len(sp.breaks)} breaks ({done} of {len(cand.brk)} candidate joins) of the TBR "
+                      f"round of {name}, each break scored exactly with its component medians, in {dt:.1f} s "
+                      f"on 1 pinned core of {cpu_model()}"}
 
 
 def cpu_baseline_tree(ds, tree, budget_s: float):

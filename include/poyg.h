@@ -61,8 +61,9 @@ int poyg_dev_alloc(poyg_ctx *ctx, int64_t bytes, void **out);
 int poyg_dev_free(poyg_ctx *ctx, void *p);
 int poyg_dev_upload(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes);
 int poyg_dev_download(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes);
-/* The context's stream as a hipStream_t. */
+/* The context's stream as a hipStream_t, and its device. */
 void *poyg_ctx_stream(poyg_ctx *ctx);
+int poyg_ctx_device(poyg_ctx *ctx);
 
 /* ---- two-dimensional cost matrix (src/cm.c, src/cost_matrix.ml) ------- */
 /* cm_CAML_create (src/cm.c:1389-1398); only 5-bit combination alphabets. */
@@ -235,6 +236,59 @@ int poyg_nonadd_median(poyg_ctx *ctx, const uint8_t *a, const uint8_t *b, uint8_
                        int64_t *added);
 int poyg_nonadd_median_3(poyg_ctx *ctx, const uint8_t *A, const uint8_t *N, const uint8_t *D1,
                          const uint8_t *D2, uint8_t *F, int len);
+
+/* ---- tree-pass support (device-resident sequences) ------------------- */
+/* Per sequence: stats[2k] = positions carrying the gap bit (seq_CAML_count,
+ * src/seq.c:571-581, the count in Sequence.Align.cost_2's band parameter,
+ * src/sequence.ml:691-713) and stats[2k + 1] = 1 when every position is
+ * the gap (Sequence.is_empty, src/sequence.ml:442-450, which DOS.median and
+ * DOS.distance test, src/seqCS.ml:747-776, 818-864).  Device pointers. */
+int poyg_seq_stats(poyg_ctx *ctx, const uint8_t *d_seq, const int64_t *d_off, const int32_t *d_len,
+                   int32_t n, int32_t *d_stats, void *stream);
+
+/* ---- learned join scorer ----------------------------------------------
+ * No counterpart in the reference: POY scores every candidate join exactly
+ * (AllDirChar.cost_fn, src/allDirChar.ml:1279-1332).  These entry points
+ * let a TBR round order candidates by a predicted join cost and score only
+ * the best of them exactly; every reported cost still comes from the exact
+ * DP.  The network is 256 -> 256 -> 256 -> 1 with ReLU, bf16 weights and
+ * activations, f32 accumulation (MFMA products), trained by mean squared
+ * error with Adam.  Parameter vector (f32, poyg_mlp_param_count() entries):
+ * W1[256 in][256 out], b1[256], W2[256][256], b2[256], w3[256], b3[1]. */
+typedef struct poyg_mlp poyg_mlp;
+int64_t poyg_mlp_param_count(void);
+/* He-uniform weights from `seed`, zero biases; batches of up to batch_max
+ * rows. */
+int poyg_mlp_create(poyg_ctx *ctx, int batch_max, uint64_t seed, poyg_mlp **out);
+void poyg_mlp_destroy(poyg_mlp *m);
+int poyg_mlp_set_params(poyg_mlp *m, const float *host, int64_t n);
+int poyg_mlp_get_params(poyg_mlp *m, float *host, int64_t n);
+/* Predictions y[r] of n rows of bf16 features x[r][256] (device). */
+int poyg_mlp_forward(poyg_mlp *m, const uint16_t *d_x, int64_t n, float *d_y, void *stream);
+/* Gradient of the mean squared error of one batch (n <= batch_max rows,
+ * device features and f32 targets), kept for poyg_mlp_adam and readable
+ * with poyg_mlp_get_grads; *loss (if not NULL) receives the batch loss. */
+int poyg_mlp_grad(poyg_mlp *m, const uint16_t *d_x, const float *d_t, int n, float *loss);
+int poyg_mlp_get_grads(poyg_mlp *m, float *host, int64_t n);
+/* One Adam step on the gradient of the last poyg_mlp_grad. */
+int poyg_mlp_adam(poyg_mlp *m, float lr, float beta1, float beta2, float eps);
+/* 4-mer profiles: counts[k][b0*64 + b1*16 + b2*4 + b3] over positions 1..
+ * of sequence k, a position's base being the lowest base bit of its mask;
+ * windows holding a position without a base bit are not counted. */
+int poyg_kmer_counts(poyg_ctx *ctx, const uint8_t *d_seq, const int64_t *d_off, const int32_t *d_len, int32_t n,
+                     float *d_counts, void *stream);
+/* Pair features x[p][k] = bf16(|counts[ia[p]][k] - counts[ib[p]][k]| * scale). */
+int poyg_pair_features(poyg_ctx *ctx, const float *d_counts, const int32_t *d_ia, const int32_t *d_ib, int64_t n,
+                       float scale, uint16_t *d_x, void *stream);
+/* dst row r = src row idx[r], rows of row_bytes (a multiple of 4) bytes:
+ * the training loop's batch shuffle. */
+int poyg_gather_rows(poyg_ctx *ctx, const void *d_src, int64_t row_bytes, const int32_t *d_idx, int64_t n,
+                     void *d_dst, void *stream);
+/* C (f32, row stride ldc) = A B over bf16 operands, A(i, k) = a[i*a_rs +
+ * k*a_cs], B(k, j) = b[k*b_rs + j*b_cs]: the MFMA tiles the network runs
+ * on, exposed for tests. */
+int poyg_gemm_bf16_f32(poyg_ctx *ctx, int M, int N, int K, const uint16_t *d_a, int64_t a_rs, int64_t a_cs,
+                       const uint16_t *d_b, int64_t b_rs, int64_t b_cs, float *d_c, int64_t ldc, void *stream);
 
 #ifdef __cplusplus
 }

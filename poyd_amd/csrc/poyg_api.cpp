@@ -84,6 +84,10 @@ struct poyg_ctx {
   hipStream_t side2 = nullptr;   // third stream: the widest bands (lane sets, row kernels)
 };
 
+namespace poyg {
+int ctx_cus(poyg_ctx *c) { return c ? c->cus : 1; }
+}  // namespace poyg
+
 // Device memory cache, one per device.  A tree pass builds and drops a plan
 // with a dozen buffers per tree level; hipMalloc and hipFree calls cost more
 // than the level's kernels.  Blocks of up to kPoolMax bytes are rounded to a
@@ -334,6 +338,7 @@ int poyg_dev_download(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes) 
   return POYG_OK;
 }
 void *poyg_ctx_stream(poyg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+int poyg_ctx_device(poyg_ctx *ctx) { return ctx ? ctx->device : -1; }
 
 void poyg_plan_destroy(poyg_plan *plan) {
   if (!plan) return;

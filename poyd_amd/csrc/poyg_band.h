@@ -8,6 +8,12 @@
 
 #include <algorithm>
 
+#if defined(__HIPCC__)
+#define POYG_HD __host__ __device__
+#else
+#define POYG_HD
+#endif
+
 namespace poyg {
 
 enum LinBandCase : int { kBandFull = 0, kBandCase2 = 1, kBandCase3 = 2 };
@@ -16,17 +22,17 @@ enum LinBandCase : int { kBandFull = 0, kBandCase2 = 1, kBandCase3 = 2 };
 struct LinGeom {
   int cs, w, h;
 };
-inline LinGeom lin_geom(int m, int n, int dw) {
+POYG_HD inline LinGeom lin_geom(int m, int n, int dw) {
   LinGeom g;
-  g.w = std::min(n, 50 + dw);
-  g.h = std::min(m, (m - n) + 50 + dw);
+  g.w = n < 50 + dw ? n : 50 + dw;
+  g.h = m < (m - n) + 50 + dw ? m : (m - n) + 50 + dw;
   if ((float)m >= (float)((float)((float)3 / (float)2) * (float)n)) g.cs = kBandFull;
   else if (2 * g.h < m) g.cs = kBandCase2;
   else if (8 >= m - g.h) g.cs = kBandFull;
   else g.cs = kBandCase3;
   return g;
 }
-inline void lin_row_host(int i, int m, int n, const LinGeom &g, int &lo, int &hi) {
+POYG_HD inline void lin_row_host(int i, int m, int n, const LinGeom &g, int &lo, int &hi) {
   if (g.cs == kBandFull) { lo = 0; hi = n - 1; return; }
   if (i == 0) { lo = 0; hi = g.w - 1; return; }
   const int w = g.w, h = g.h;
@@ -47,14 +53,14 @@ inline void lin_row_host(int i, int m, int n, const LinGeom &g, int &lo, int &hi
 constexpr int64_t kLinDmLimit = 0x7ffffff0;
 // Bytes per direction row of a linear band whose widest row is maxw: room
 // for a start phase of up to 3 bytes, rounded to 4.
-inline int lin_dm_stride(int maxw) { return (maxw + 3 + 3) & ~3; }
-inline bool lin_dm_fits(int m, int maxw) { return (int64_t)m * lin_dm_stride(maxw) < kLinDmLimit; }
+POYG_HD inline int lin_dm_stride(int maxw) { return (maxw + 3 + 3) & ~3; }
+POYG_HD inline bool lin_dm_fits(int m, int maxw) { return (int64_t)m * lin_dm_stride(maxw) < kLinDmLimit; }
 
 // Widest row and number of cells of a linear band.  Within each of the three
 // sections of lin_row_host the row width is affine in the row (growing by
 // one, constant, shrinking by one), so a section's cells are an arithmetic
 // series of its first and last widths and its widest row is one of the two.
-inline void lin_band_extent(int m, int n, const LinGeom &g, int &maxw, int64_t &cells) {
+POYG_HD inline void lin_band_extent(int m, int n, const LinGeom &g, int &maxw, int64_t &cells) {
   auto width = [&](int i) {
     int lo, hi;
     lin_row_host(i, m, n, g, lo, hi);

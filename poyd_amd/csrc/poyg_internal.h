@@ -81,6 +81,7 @@ struct poyg_cm3 {
 };
 
 namespace poyg {
+int ctx_cus(poyg_ctx *ctx);   // compute units of the context's device
 void set_error(const char *fmt, ...);
 void to_dev_tcm(const poyg_cm *cm, DevTcm *out);
 void to_dev_tcm3(const poyg_cm3 *cm, DevTcm3 *out);

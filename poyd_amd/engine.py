@@ -110,6 +110,22 @@ SIGNATURES = {
     "poyg_fitch_up": (C.c_int, [vp, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp]),
     "poyg_nonadd_median": (C.c_int, [vp, u8p, u8p, u8p, C.c_int, i64p]),
     "poyg_nonadd_median_3": (C.c_int, [vp, u8p, u8p, u8p, u8p, u8p, C.c_int]),
+    "poyg_ctx_device": (C.c_int, [vp]),
+    "poyg_seq_stats": (C.c_int, [vp, vp, vp, vp, C.c_int32, vp, vp]),
+    "poyg_mlp_param_count": (C.c_int64, []),
+    "poyg_mlp_create": (C.c_int, [vp, C.c_int, C.c_uint64, C.POINTER(vp)]),
+    "poyg_mlp_destroy": (None, [vp]),
+    "poyg_mlp_set_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int64]),
+    "poyg_mlp_get_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int64]),
+    "poyg_mlp_forward": (C.c_int, [vp, vp, C.c_int64, vp, vp]),
+    "poyg_mlp_grad": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_float)]),
+    "poyg_mlp_get_grads": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int64]),
+    "poyg_mlp_adam": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "poyg_kmer_counts": (C.c_int, [vp, vp, vp, vp, C.c_int32, vp, vp]),
+    "poyg_pair_features": (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_float, vp, vp]),
+    "poyg_gather_rows": (C.c_int, [vp, vp, C.c_int64, vp, C.c_int64, vp, vp]),
+    "poyg_gemm_bf16_f32": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int64, C.c_int64, vp, C.c_int64,
+                                     C.c_int64, vp, C.c_int64, vp]),
 }
 
 

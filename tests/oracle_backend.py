@@ -64,3 +64,71 @@ TREE_SUFFIX = {"cc": ".tree", "cc1": ".1.tree", "cc2": ".2.tree", "cc3": ".3.tre
 def model_cm(name: str) -> po.CM2:
     sub, indel, go = MODELS[name]
     return po.CM2.sub_indel(sub, indel, go)
+
+
+def is_empty(s) -> bool:
+    return bool((np.asarray(s) == 16).all())
+
+
+class OracleStore:
+    """The sequence-store interface of poyd_amd.seqstore on the CPU oracle:
+    sequences in a host list, DOS.median / DOS.distance per pair."""
+
+    def __init__(self, cm: po.CM2, affine: bool):
+        self.cm, self.affine = cm, affine
+        self.seqs = []
+        self.alignments = 0
+        self.cells = 0
+
+    def __len__(self):
+        return len(self.seqs)
+
+    def add(self, seqs):
+        first = len(self.seqs)
+        self.seqs.extend(np.asarray(s, np.uint8) for s in seqs)
+        return np.arange(first, len(self.seqs), dtype=np.int64)
+
+    def get(self, i):
+        return self.seqs[int(i)]
+
+    def mark(self):
+        return len(self.seqs)
+
+    def release(self, m):
+        del self.seqs[m:]
+
+    def median(self, a, b):
+        ids, costs = [], []
+        for x, y in zip(np.asarray(a), np.asarray(b)):
+            sx, sy = self.seqs[int(x)], self.seqs[int(y)]
+            if is_empty(sx):
+                ids.append(int(y))
+                costs.append(0)
+                continue
+            if is_empty(sy):
+                ids.append(int(x))
+                costs.append(0)
+                continue
+            if self.affine:
+                med, ri, rj, cost, mwg = po.align_affine_3(sx, sy, self.cm)
+            else:
+                cost, oa, ob = po.align_2(sx, sy, self.cm)
+                med = po.ancestor_2(oa, ob, self.cm)
+            self.seqs.append(med)
+            ids.append(len(self.seqs) - 1)
+            costs.append(cost)
+            self.alignments += 1
+        return np.array(ids, np.int64), np.array(costs, np.int64)
+
+    def distance(self, a, b):
+        out = []
+        for x, y in zip(np.asarray(a), np.asarray(b)):
+            sx, sy = self.seqs[int(x)], self.seqs[int(y)]
+            if is_empty(sx) or is_empty(sy):
+                out.append(0)
+            elif self.affine:
+                out.append(po.cost_affine_3(sx, sy, self.cm))
+            else:
+                out.append(po.cost_2(sx, sy, self.cm, max(8, abs(len(sx) - len(sy)))))
+            self.alignments += 1
+        return np.array(out, np.int64)
