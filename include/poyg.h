@@ -236,6 +236,20 @@ int poyg_nonadd_median(poyg_ctx *ctx, const uint8_t *a, const uint8_t *b, uint8_
                        int64_t *added);
 int poyg_nonadd_median_3(poyg_ctx *ctx, const uint8_t *A, const uint8_t *N, const uint8_t *D1,
                          const uint8_t *D2, uint8_t *F, int len);
+/* The same passes for characters of `width` bytes: 1, 2 or 4, i.e.
+ * char_nonadd8/16/32_CAML_basic_median and _median_3 (nonaddCSc.c compiled
+ * with CHARSIZE 8, 16, 32: src/myocamlbuild.ml.in:255-269; Node picks the
+ * width from the number of states, src/node.ml:1179-1234).  `len` counts
+ * characters; rows of `chars` are len * width bytes. */
+int poyg_fitch_down_w(poyg_ctx *ctx, int width, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_ia,
+                      const int32_t *d_ib, const int32_t *d_ires, int64_t *d_added, void *stream);
+int poyg_fitch_up_w(poyg_ctx *ctx, int width, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_iA,
+                    const int32_t *d_iN, const int32_t *d_iD1, const int32_t *d_iD2, const int32_t *d_iF,
+                    void *stream);
+int poyg_nonadd_median_w(poyg_ctx *ctx, int width, const void *a, const void *b, void *res, int len,
+                         int64_t *added);
+int poyg_nonadd_median_3_w(poyg_ctx *ctx, int width, const void *A, const void *N, const void *D1, const void *D2,
+                           void *F, int len);
 
 /* ---- tree-pass support (device-resident sequences) ------------------- */
 /* Per sequence: stats[2k] = positions carrying the gap bit (seq_CAML_count,

@@ -1095,6 +1095,47 @@ void po_nonadd_median_3(const uint8_t *A, const uint8_t *N, const uint8_t *D1, c
     }
 }
 
+/* The same two passes for characters of `width` bytes (1, 2 or 4): the
+ * reference compiles nonaddCSc.c with CHARSIZE 8, 16 and 32
+ * (src/myocamlbuild.ml.in:255-269) and picks the width from the number of
+ * states (src/node.ml:1179-1234). */
+static uint32_t po_get_w(const void *p, int width, int i) {
+    if (width == 1) return ((const uint8_t *)p)[i];
+    if (width == 2) return ((const uint16_t *)p)[i];
+    return ((const uint32_t *)p)[i];
+}
+static void po_put_w(void *p, int width, int i, uint32_t v) {
+    if (width == 1) ((uint8_t *)p)[i] = (uint8_t)v;
+    else if (width == 2) ((uint16_t *)p)[i] = (uint16_t)v;
+    else ((uint32_t *)p)[i] = v;
+}
+
+int64_t po_nonadd_median_w(int width, const void *a, const void *b, void *res, int len) {
+    int64_t added = 0;
+    for (int i = 0; i < len; i++) {
+        uint32_t x = po_get_w(a, width, i), y = po_get_w(b, width, i);
+        uint32_t in = x & y;
+        if (in == 0) added++;
+        po_put_w(res, width, i, in ? in : (x | y));
+    }
+    return added;
+}
+
+void po_nonadd_median_3_w(int width, const void *A, const void *N, const void *D1, const void *D2, void *F,
+                          int len) {
+    for (int i = 0; i < len; i++) {
+        uint32_t a = po_get_w(A, width, i), n = po_get_w(N, width, i);
+        uint32_t d1 = po_get_w(D1, width, i), d2 = po_get_w(D2, width, i);
+        uint32_t inter = n | (a & (d1 | d2));
+        uint32_t temp = inter;
+        uint32_t was_union = (d1 & d2) == 0 ? 0xFFFFFFFFu : 0;
+        inter = (inter | a) & was_union;
+        temp |= inter;
+        uint32_t f = (a & n) == a ? a : temp;
+        po_put_w(F, width, i, f);
+    }
+}
+
 /* char_nonadd_CAML_distance, src/nonaddCSc.c:1076-1105. */
 int64_t po_nonadd_distance(const uint8_t *a, const uint8_t *b, int len) {
     int64_t r = 0;

@@ -134,6 +134,9 @@ int po_median_3(const uint8_t *a, const uint8_t *b, const uint8_t *d, int len, c
 int64_t po_nonadd_median(const uint8_t *a, const uint8_t *b, uint8_t *res, int len);
 void po_nonadd_median_3(const uint8_t *A, const uint8_t *N, const uint8_t *D1,
                         const uint8_t *D2, uint8_t *F, int len);
+int64_t po_nonadd_median_w(int width, const void *a, const void *b, void *res, int len);
+void po_nonadd_median_3_w(int width, const void *A, const void *N, const void *D1, const void *D2, void *F,
+                          int len);
 int64_t po_nonadd_distance(const uint8_t *a, const uint8_t *b, int len);
 int64_t po_nonadd_dist_2(const uint8_t *a, const uint8_t *b, const uint8_t *c, int len);
 void po_nonadd_reroot_median(const uint8_t *a, const uint8_t *b, uint8_t *res, int len);

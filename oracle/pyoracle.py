@@ -74,6 +74,8 @@ def lib():
             "po_median_3": (C.c_int, [u8p, u8p, u8p, C.c_int, vp, u8p, C.c_int]),
             "po_nonadd_median": (C.c_int64, [u8p, u8p, u8p, C.c_int]),
             "po_nonadd_median_3": (None, [u8p, u8p, u8p, u8p, u8p, C.c_int]),
+            "po_nonadd_median_w": (C.c_int64, [C.c_int, vp, vp, vp, C.c_int]),
+            "po_nonadd_median_3_w": (None, [C.c_int, vp, vp, vp, vp, vp, C.c_int]),
             "po_nonadd_distance": (C.c_int64, [u8p, u8p, C.c_int]),
             "po_nonadd_dist_2": (C.c_int64, [u8p, u8p, u8p, C.c_int]),
             "po_nonadd_reroot_median": (None, [u8p, u8p, u8p, C.c_int]),
@@ -328,6 +330,26 @@ def nonadd_median_3(A, N, D1, D2):
     arrs = [_u8(v) for v in (A, N, D1, D2)]
     out = np.zeros(len(arrs[0]), np.uint8)
     lib().po_nonadd_median_3(*[_p8(v) for v in arrs], _p8(out), len(out))
+    return out
+
+
+_W_DTYPE = {1: np.uint8, 2: np.uint16, 4: np.uint32}
+
+
+def nonadd_median_w(a, b, width: int):
+    """nonadd_median at 1, 2 or 4 bytes per character (NonaddCS8/16/32)."""
+    t = _W_DTYPE[width]
+    x, y = np.ascontiguousarray(a, t), np.ascontiguousarray(b, t)
+    out = np.zeros(len(x), t)
+    added = lib().po_nonadd_median_w(width, x.ctypes.data, y.ctypes.data, out.ctypes.data, len(x))
+    return out, int(added)
+
+
+def nonadd_median_3_w(A, N, D1, D2, width: int):
+    t = _W_DTYPE[width]
+    arrs = [np.ascontiguousarray(v, t) for v in (A, N, D1, D2)]
+    out = np.zeros(len(arrs[0]), t)
+    lib().po_nonadd_median_3_w(width, *[v.ctypes.data for v in arrs], out.ctypes.data, len(out))
     return out
 
 

@@ -45,9 +45,9 @@ int lin_trace_blocks_per_cu();
 hipError_t launch_lin_trace(int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                             const uint8_t *seq, const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len,
                             int32_t *status, hipStream_t s);
-hipError_t launch_fitch_down(uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
+hipError_t launch_fitch_down(int width, uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
                              const int32_t *ires, int64_t *added, hipStream_t s);
-hipError_t launch_fitch_up(uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
+hipError_t launch_fitch_up(int width, uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
                            const int32_t *iD1, const int32_t *iD2, const int32_t *iF, hipStream_t s);
 }  // namespace poyg
 
@@ -1250,29 +1250,55 @@ int poyg_algn_verify_2(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, in
 }
 
 // ---- Fitch ---------------------------------------------------------------
-int poyg_fitch_down(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_ia,
-                    const int32_t *d_ib, const int32_t *d_ires, int64_t *d_added, void *stream) {
-  if (!ctx || n < 0 || len < 0) return POYG_EINVAL;
+static bool fitch_width_ok(int w) { return w == 1 || w == 2 || w == 4; }
+
+int poyg_fitch_down_w(poyg_ctx *ctx, int width, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_ia,
+                      const int32_t *d_ib, const int32_t *d_ires, int64_t *d_added, void *stream) {
+  if (!ctx || n < 0 || len < 0 || !fitch_width_ok(width)) {
+    set_error("poyg_fitch_down_w: bad arguments (width %d)", width);
+    return POYG_EINVAL;
+  }
   if (n == 0) return POYG_OK;
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-  HIP_TRY(launch_fitch_down(d_chars, len, n, d_ia, d_ib, d_ires, d_added, s));
+  HIP_TRY(launch_fitch_down(width, d_chars, len, n, d_ia, d_ib, d_ires, d_added, s));
   return POYG_OK;
+}
+
+int poyg_fitch_up_w(poyg_ctx *ctx, int width, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_iA,
+                    const int32_t *d_iN, const int32_t *d_iD1, const int32_t *d_iD2, const int32_t *d_iF,
+                    void *stream) {
+  if (!ctx || n < 0 || len < 0 || !fitch_width_ok(width)) {
+    set_error("poyg_fitch_up_w: bad arguments (width %d)", width);
+    return POYG_EINVAL;
+  }
+  if (n == 0) return POYG_OK;
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  HIP_TRY(launch_fitch_up(width, d_chars, len, n, d_iA, d_iN, d_iD1, d_iD2, d_iF, s));
+  return POYG_OK;
+}
+
+int poyg_fitch_down(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_ia,
+                    const int32_t *d_ib, const int32_t *d_ires, int64_t *d_added, void *stream) {
+  return poyg_fitch_down_w(ctx, 1, d_chars, len, n, d_ia, d_ib, d_ires, d_added, stream);
 }
 
 int poyg_fitch_up(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_iA,
                   const int32_t *d_iN, const int32_t *d_iD1, const int32_t *d_iD2, const int32_t *d_iF,
                   void *stream) {
-  if (!ctx || n < 0 || len < 0) return POYG_EINVAL;
-  if (n == 0) return POYG_OK;
-  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-  HIP_TRY(launch_fitch_up(d_chars, len, n, d_iA, d_iN, d_iD1, d_iD2, d_iF, s));
-  return POYG_OK;
+  return poyg_fitch_up_w(ctx, 1, d_chars, len, n, d_iA, d_iN, d_iD1, d_iD2, d_iF, stream);
 }
 
-static int fitch_host(poyg_ctx *ctx, const uint8_t *const *in, int nin, uint8_t *res, int len, int64_t *added) {
+// One node on host arrays of `len` characters of `width` bytes: two inputs
+// (down pass) or four (up pass).
+static int fitch_host(poyg_ctx *ctx, int width, const void *const *in, int nin, void *res, int len, int64_t *added) {
+  if (!ctx || len < 0 || !fitch_width_ok(width)) {
+    set_error("nonadd median: bad arguments (width %d)", width);
+    return POYG_EINVAL;
+  }
   DevBuf chars, idx, acc;
   const int rows = nin + 1;
-  if (chars.alloc((size_t)rows * len) || idx.alloc(sizeof(int32_t) * 8) || acc.alloc(sizeof(int64_t))) {
+  const size_t row = (size_t)len * width;
+  if (chars.alloc((size_t)rows * row) || idx.alloc(sizeof(int32_t) * 8) || acc.alloc(sizeof(int64_t))) {
     chars.release(); idx.release(); acc.release();
     return POYG_ENOMEM;
   }
@@ -1282,16 +1308,16 @@ static int fitch_host(poyg_ctx *ctx, const uint8_t *const *in, int nin, uint8_t 
   int64_t zero = 0;
   hipError_t e = hipSuccess;
   for (int k = 0; k < nin && e == hipSuccess; ++k)
-    e = hipMemcpyAsync((uint8_t *)chars.p + (size_t)k * len, in[k], len, hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync((uint8_t *)chars.p + (size_t)k * row, in[k], row, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(idx.p, h_idx, sizeof h_idx, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(acc.p, &zero, sizeof zero, hipMemcpyHostToDevice, s);
   const int32_t *I = (const int32_t *)idx.p;
   if (e == hipSuccess) {
-    if (nin == 2) e = launch_fitch_down((uint8_t *)chars.p, len, 1, I, I + 1, I + 2, (int64_t *)acc.p, s);
-    else e = launch_fitch_up((uint8_t *)chars.p, len, 1, I, I + 1, I + 2, I + 3, I + 4, s);
+    if (nin == 2) e = launch_fitch_down(width, (uint8_t *)chars.p, len, 1, I, I + 1, I + 2, (int64_t *)acc.p, s);
+    else e = launch_fitch_up(width, (uint8_t *)chars.p, len, 1, I, I + 1, I + 2, I + 3, I + 4, s);
   }
   if (e == hipSuccess)
-    e = hipMemcpyAsync(res, (uint8_t *)chars.p + (size_t)nin * len, len, hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(res, (uint8_t *)chars.p + (size_t)nin * row, row, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess && added) e = hipMemcpyAsync(added, acc.p, sizeof(int64_t), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
@@ -1302,15 +1328,25 @@ static int fitch_host(poyg_ctx *ctx, const uint8_t *const *in, int nin, uint8_t 
   return rc;
 }
 
+int poyg_nonadd_median_w(poyg_ctx *ctx, int width, const void *a, const void *b, void *res, int len,
+                         int64_t *added) {
+  const void *in[2] = {a, b};
+  return fitch_host(ctx, width, in, 2, res, len, added);
+}
+
+int poyg_nonadd_median_3_w(poyg_ctx *ctx, int width, const void *A, const void *N, const void *D1, const void *D2,
+                           void *F, int len) {
+  const void *in[4] = {A, N, D1, D2};
+  return fitch_host(ctx, width, in, 4, F, len, nullptr);
+}
+
 int poyg_nonadd_median(poyg_ctx *ctx, const uint8_t *a, const uint8_t *b, uint8_t *res, int len, int64_t *added) {
-  const uint8_t *in[2] = {a, b};
-  return fitch_host(ctx, in, 2, res, len, added);
+  return poyg_nonadd_median_w(ctx, 1, a, b, res, len, added);
 }
 
 int poyg_nonadd_median_3(poyg_ctx *ctx, const uint8_t *A, const uint8_t *N, const uint8_t *D1, const uint8_t *D2,
                          uint8_t *F, int len) {
-  const uint8_t *in[4] = {A, N, D1, D2};
-  return fitch_host(ctx, in, 4, F, len, nullptr);
+  return poyg_nonadd_median_3_w(ctx, 1, A, N, D1, D2, F, len);
 }
 
 }  // extern "C"

@@ -2414,25 +2414,40 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Fitch over packed nonadditive characters (src/nonaddCSc.c:422-544):
-// HBM-bound bit operations, 16 characters per lane per step.
+// Fitch over packed nonadditive characters (src/nonaddCSc.c:422-544), for
+// the reference's three character widths (nonaddCSc.c compiled with CHARSIZE
+// 8, 16 and 32, src/myocamlbuild.ml.in:255-269): W bytes per character.
+// HBM-bound bit operations, 16 bytes per lane per step.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t empty_bytes(uint32_t v) {
-  // 0xFF in every byte of v that is zero
-  uint32_t t = ((v & 0x7f7f7f7fu) + 0x7f7f7f7fu) | v;
-  t = ~(t | 0x7f7f7f7fu);   // 0x80 where the byte is zero
-  return (t >> 7) * 0xffu;
+// All ones in every W-byte lane of v that is zero.
+template <int W>
+__device__ __forceinline__ uint32_t empty_lanes(uint32_t v) {
+  if (W == 1) {
+    uint32_t t = ((v & 0x7f7f7f7fu) + 0x7f7f7f7fu) | v;
+    t = ~(t | 0x7f7f7f7fu);   // 0x80 where the byte is zero
+    return (t >> 7) * 0xffu;
+  } else if (W == 2) {
+    uint32_t t = ((v & 0x7fff7fffu) + 0x7fff7fffu) | v;
+    t = ~(t | 0x7fff7fffu);   // 0x8000 where the half is zero
+    return (t >> 15) * 0xffffu;
+  }
+  return v ? 0u : 0xffffffffu;
 }
+template <int W>
+using fitch_elt_t = typename std::conditional<W == 1, uint8_t, typename std::conditional<W == 2, uint16_t, uint32_t>::type>::type;
 
+template <int W>
 __global__ void fitch_down_kernel(uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
                                   const int32_t *ires, int64_t *added) {
+  using T = fitch_elt_t<W>;
   const int node = blockIdx.y;
   if (node >= n) return;
-  const uint8_t *a = chars + (size_t)ia[node] * len;
-  const uint8_t *b = chars + (size_t)ib[node] * len;
-  uint8_t *r = chars + (size_t)ires[node] * len;
+  const size_t row = (size_t)len * W;
+  const uint8_t *a = chars + (size_t)ia[node] * row;
+  const uint8_t *b = chars + (size_t)ib[node] * row;
+  uint8_t *r = chars + (size_t)ires[node] * row;
   long long cnt = 0;
-  const int nvec = (len % 16 == 0) ? len / 16 : 0;
+  const int nvec = (row % 16 == 0) ? (int)(row / 16) : 0;
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nvec; v += gridDim.x * blockDim.x) {
     const uint4 x = ((const uint4 *)a)[v];
     const uint4 y = ((const uint4 *)b)[v];
@@ -2440,17 +2455,19 @@ __global__ void fitch_down_kernel(uint8_t *chars, int len, int n, const int32_t 
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const uint32_t in = xs[q] & ys[q], un = xs[q] | ys[q];
-      const uint32_t emp = empty_bytes(in);
-      cnt += __popc(emp) >> 3;
+      const uint32_t emp = empty_lanes<W>(in);
+      cnt += __popc(emp) / (8 * W);
       rs[q] = (emp & un) | in;
     }
     ((uint4 *)r)[v] = make_uint4(rs[0], rs[1], rs[2], rs[3]);
   }
-  if (blockIdx.x == 0) {
-    for (int c = nvec * 16 + threadIdx.x; c < len; c += blockDim.x) {
-      const uint8_t in = a[c] & b[c];
+  if (blockIdx.x == 0 && nvec == 0) {
+    const T *ta = (const T *)a, *tb = (const T *)b;
+    T *tr = (T *)r;
+    for (int c = threadIdx.x; c < len; c += blockDim.x) {
+      const T in = ta[c] & tb[c];
       if (!in) ++cnt;
-      r[c] = in ? in : (uint8_t)(a[c] | b[c]);
+      tr[c] = in ? in : (T)(ta[c] | tb[c]);
     }
   }
   // block reduction through LDS, one atomic per block
@@ -2464,16 +2481,19 @@ __global__ void fitch_down_kernel(uint8_t *chars, int len, int n, const int32_t 
   if (threadIdx.x == 0 && s_red[0]) atomicAdd((unsigned long long *)&added[node], s_red[0]);
 }
 
+template <int W>
 __global__ void fitch_up_kernel(uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
                                 const int32_t *iD1, const int32_t *iD2, const int32_t *iF) {
+  using T = fitch_elt_t<W>;
   const int node = blockIdx.y;
   if (node >= n) return;
-  const uint8_t *A = chars + (size_t)iA[node] * len;
-  const uint8_t *N = chars + (size_t)iN[node] * len;
-  const uint8_t *D1 = chars + (size_t)iD1[node] * len;
-  const uint8_t *D2 = chars + (size_t)iD2[node] * len;
-  uint8_t *F = chars + (size_t)iF[node] * len;
-  const int nvec = (len % 16 == 0) ? len / 16 : 0;
+  const size_t row = (size_t)len * W;
+  const uint8_t *A = chars + (size_t)iA[node] * row;
+  const uint8_t *N = chars + (size_t)iN[node] * row;
+  const uint8_t *D1 = chars + (size_t)iD1[node] * row;
+  const uint8_t *D2 = chars + (size_t)iD2[node] * row;
+  uint8_t *F = chars + (size_t)iF[node] * row;
+  const int nvec = (row % 16 == 0) ? (int)(row / 16) : 0;
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nvec; v += gridDim.x * blockDim.x) {
     const uint4 a4 = ((const uint4 *)A)[v], n4 = ((const uint4 *)N)[v];
     const uint4 d14 = ((const uint4 *)D1)[v], d24 = ((const uint4 *)D2)[v];
@@ -2483,22 +2503,24 @@ __global__ void fitch_up_kernel(uint8_t *chars, int len, int n, const int32_t *i
     for (int q = 0; q < 4; ++q) {
       uint32_t inter = ns[q] | (as[q] & (d1[q] | d2[q]));
       uint32_t temp = inter;
-      const uint32_t was_union = empty_bytes(d1[q] & d2[q]);
+      const uint32_t was_union = empty_lanes<W>(d1[q] & d2[q]);
       inter = (inter | as[q]) & was_union;
       temp |= inter;
-      const uint32_t not_contained = ~empty_bytes((as[q] & ns[q]) ^ as[q]);   // 0xFF where A is not within N
+      const uint32_t not_contained = ~empty_lanes<W>((as[q] & ns[q]) ^ as[q]);   // ones where A is not within N
       fs[q] = (~not_contained & as[q]) | (not_contained & temp);
     }
     ((uint4 *)F)[v] = make_uint4(fs[0], fs[1], fs[2], fs[3]);
   }
-  if (blockIdx.x == 0) {
-    for (int c = nvec * 16 + threadIdx.x; c < len; c += blockDim.x) {
-      const uint8_t a = A[c], nn = N[c], x1 = D1[c], x2 = D2[c];
-      uint8_t f;
-      if ((a & nn) == a) f = a;
-      else if ((x1 & x2) == 0) f = nn | a;
-      else f = nn | (a & (x1 | x2));
-      F[c] = f;
+  if (blockIdx.x == 0 && nvec == 0) {
+    const T *tA = (const T *)A, *tN = (const T *)N, *t1 = (const T *)D1, *t2 = (const T *)D2;
+    T *tF = (T *)F;
+    for (int c = threadIdx.x; c < len; c += blockDim.x) {
+      const T a = tA[c], nn = tN[c], x1 = t1[c], x2 = t2[c];
+      T f;
+      if ((T)(a & nn) == a) f = a;
+      else if ((T)(x1 & x2) == 0) f = (T)(nn | a);
+      else f = (T)(nn | (a & (x1 | x2)));
+      tF[c] = f;
     }
   }
 }
@@ -2688,28 +2710,38 @@ hipError_t launch_lin_trace(int grid, const PairMeta *meta, const int32_t *order
 // Blocks per node along the characters: enough for the whole launch to
 // cover the chip (about 4,096 blocks) when there are few nodes, at most one
 // block per 256 vectors of 16 characters.
-static int fitch_blocks(int len, int n) {
+static int fitch_blocks(int64_t row_bytes, int n) {
   const int bs = 256;
-  const int nvec = len / 16;
-  int gx = (nvec + bs - 1) / bs;
+  const int64_t nvec = row_bytes / 16;
+  int64_t gx = (nvec + bs - 1) / bs;
   const int want = std::max(64, 4096 / std::max(n, 1));
   if (gx > want) gx = want;
-  return gx < 1 ? 1 : gx;
+  return gx < 1 ? 1 : (int)gx;
 }
 
-hipError_t launch_fitch_down(uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
+hipError_t launch_fitch_down(int width, uint8_t *chars, int len, int n, const int32_t *ia, const int32_t *ib,
                              const int32_t *ires, int64_t *added, hipStream_t s) {
   const int bs = 256;
-  const int gx = fitch_blocks(len, n);
-  hipLaunchKernelGGL(fitch_down_kernel, dim3(gx, n), dim3(bs), 0, s, chars, len, n, ia, ib, ires, added);
+  const int gx = fitch_blocks((int64_t)len * width, n);
+  if (width == 1)
+    hipLaunchKernelGGL(fitch_down_kernel<1>, dim3(gx, n), dim3(bs), 0, s, chars, len, n, ia, ib, ires, added);
+  else if (width == 2)
+    hipLaunchKernelGGL(fitch_down_kernel<2>, dim3(gx, n), dim3(bs), 0, s, chars, len, n, ia, ib, ires, added);
+  else
+    hipLaunchKernelGGL(fitch_down_kernel<4>, dim3(gx, n), dim3(bs), 0, s, chars, len, n, ia, ib, ires, added);
   return hipGetLastError();
 }
 
-hipError_t launch_fitch_up(uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
+hipError_t launch_fitch_up(int width, uint8_t *chars, int len, int n, const int32_t *iA, const int32_t *iN,
                            const int32_t *iD1, const int32_t *iD2, const int32_t *iF, hipStream_t s) {
   const int bs = 256;
-  const int gx = fitch_blocks(len, n);
-  hipLaunchKernelGGL(fitch_up_kernel, dim3(gx, n), dim3(bs), 0, s, chars, len, n, iA, iN, iD1, iD2, iF);
+  const int gx = fitch_blocks((int64_t)len * width, n);
+  if (width == 1)
+    hipLaunchKernelGGL(fitch_up_kernel<1>, dim3(gx, n), dim3(bs), 0, s, chars, len, n, iA, iN, iD1, iD2, iF);
+  else if (width == 2)
+    hipLaunchKernelGGL(fitch_up_kernel<2>, dim3(gx, n), dim3(bs), 0, s, chars, len, n, iA, iN, iD1, iD2, iF);
+  else
+    hipLaunchKernelGGL(fitch_up_kernel<4>, dim3(gx, n), dim3(bs), 0, s, chars, len, n, iA, iN, iD1, iD2, iF);
   return hipGetLastError();
 }
 

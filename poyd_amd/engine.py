@@ -111,6 +111,10 @@ SIGNATURES = {
     "poyg_nonadd_median": (C.c_int, [vp, u8p, u8p, u8p, C.c_int, i64p]),
     "poyg_nonadd_median_3": (C.c_int, [vp, u8p, u8p, u8p, u8p, u8p, C.c_int]),
     "poyg_ctx_device": (C.c_int, [vp]),
+    "poyg_fitch_down_w": (C.c_int, [vp, C.c_int, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp]),
+    "poyg_fitch_up_w": (C.c_int, [vp, C.c_int, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp]),
+    "poyg_nonadd_median_w": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, i64p]),
+    "poyg_nonadd_median_3_w": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
     "poyg_seq_stats": (C.c_int, [vp, vp, vp, vp, C.c_int32, vp, vp]),
     "poyg_mlp_param_count": (C.c_int64, []),
     "poyg_mlp_create": (C.c_int, [vp, C.c_int, C.c_uint64, C.POINTER(vp)]),

@@ -548,3 +548,30 @@ def test_plan_rejects_what_the_kernels_cannot_hold(ctx):
     assert plan(neg, engine.POYG_MODE_LINEAR_COST, 300, 280, 40) == -1
     assert plan(cm, engine.POYG_MODE_LINEAR_ALIGN, 60000, 40000, 100) == -4
     assert plan(cm, engine.POYG_MODE_LINEAR_COST, 60000, 40000, 100) == 0   # no direction band
+
+
+@pytest.mark.parametrize("width", [1, 2, 4])
+def test_fitch_widths_match_oracle(ctx, oracle, width):
+    """NonaddCS8/16/32: the down and up passes at 1, 2 and 4 bytes per
+    character, with state sets over every bit of the width, row lengths
+    that are and are not multiples of 16 bytes."""
+    import ctypes as C
+    from poyd_amd import engine
+    note(f"fitch width {width}")
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[width]
+    rng = np.random.default_rng(40 + width)
+    L = engine.load()
+    for length in (1, 7, 1000, 4096, 10003):
+        a, b, c, d = (rng.integers(1, 1 << (8 * width), size=length, dtype=np.uint64).astype(dt) for _ in range(4))
+        # sparse sets too, so that empty intersections occur at every width
+        a[::3] = (np.uint64(1) << rng.integers(0, 8 * width, size=len(a[::3])).astype(np.uint64)).astype(dt)
+        b[::3] = (np.uint64(1) << rng.integers(0, 8 * width, size=len(b[::3])).astype(np.uint64)).astype(dt)
+        res = np.zeros(length, dt)
+        added = C.c_int64(0)
+        assert L.poyg_nonadd_median_w(ctx.h, width, a.ctypes.data, b.ctypes.data, res.ctypes.data, length,
+                                      C.byref(added)) == 0
+        want, wadd = oracle.nonadd_median_w(a, b, width)
+        assert np.array_equal(res, want) and added.value == wadd, length
+        assert L.poyg_nonadd_median_3_w(ctx.h, width, a.ctypes.data, b.ctypes.data, c.ctypes.data, d.ctypes.data,
+                                        res.ctypes.data, length) == 0
+        assert np.array_equal(res, oracle.nonadd_median_3_w(a, b, c, d, width)), length

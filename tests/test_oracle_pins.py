@@ -391,34 +391,40 @@ def random_tree(rng, nleaves):
     return internal
 
 
-def test_fitch_matches_brute_force_parsimony(oracle):
+@pytest.mark.parametrize("width,shift", [(1, 0), (1, 5), (2, 11), (4, 27)])
+def test_fitch_matches_brute_force_parsimony(oracle, width, shift):
     """Down pass: the summed added costs equal the minimum number of changes
     over all internal state assignments.  Up pass: the final sets equal the
-    states a node takes in at least one most parsimonious reconstruction."""
-    rng = np.random.default_rng(17)
+    states a node takes in at least one most parsimonious reconstruction.
+    Characters of 1, 2 and 4 bytes (NonaddCS8/16/32), the states on bits
+    shift .. shift + k - 1 so that the wider characters use their high bytes."""
+    rng = np.random.default_rng(17 + shift)
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[width]
+    down = (lambda x, y: oracle.nonadd_median(x, y)) if width == 1 and shift == 0 else \
+        (lambda x, y: oracle.nonadd_median_w(x, y, width))
+    up = (lambda *v: oracle.nonadd_median_3(*v)) if width == 1 and shift == 0 else \
+        (lambda *v: oracle.nonadd_median_3_w(*v, width))
     for t in range(12):
         nleaves = int(rng.integers(3, 7))
         k = int(rng.integers(2, 4))
         nchar = 40
         tree = random_tree(rng, nleaves)
-        leaf = {v: rng.integers(1, 1 << k, size=nchar).astype(np.uint8) for v in range(nleaves)}
+        leaf = {v: (rng.integers(1, 1 << k, size=nchar).astype(np.uint64) << shift).astype(dt)
+                for v in range(nleaves)}
         prelim = dict(leaf)
         total = 0
         for v, a, b in tree:
-            prelim[v], added = oracle.nonadd_median(prelim[a], prelim[b])
+            prelim[v], added = down(prelim[a], prelim[b])
             total += added
         root = tree[-1][0]
         final = {root: prelim[root]}
-        for v, a, b in reversed(tree):
-            pass
         parent = {}
         for v, a, b in tree:
             parent[a] = v
             parent[b] = v
-        children = {v: (a, b) for v, a, b in tree}
         for v, a, b in reversed(tree[:-1]):
             p = parent[v]
-            final[v] = oracle.nonadd_median_3(final[p], prelim[v], prelim[a], prelim[b])
+            final[v] = up(final[p], prelim[v], prelim[a], prelim[b])
         internal = [v for v, _, _ in tree]
         bf_total = 0
         for ch in range(nchar):
@@ -432,7 +438,7 @@ def test_fitch_matches_brute_force_parsimony(oracle):
                         if c in st:
                             cost += st[c] != st[v]
                         else:
-                            cost += 0 if (leaf[c][ch] >> st[v]) & 1 else 1
+                            cost += 0 if (int(leaf[c][ch]) >> (shift + st[v])) & 1 else 1
                 if best is None or cost < best:
                     best = cost
                     mpr_states = {v: {st[v]} for v in internal}
@@ -441,8 +447,9 @@ def test_fitch_matches_brute_force_parsimony(oracle):
                         mpr_states[v].add(st[v])
             bf_total += best
             for v in internal:
-                got = {s for s in range(k) if (final[v][ch] >> s) & 1}
+                got = {s for s in range(k) if (int(final[v][ch]) >> (shift + s)) & 1}
                 assert got == mpr_states[v], (t, ch, v)
+                assert int(final[v][ch]) >> (shift + k) == 0 and int(final[v][ch]) & ((1 << shift) - 1) == 0
         assert total == bf_total
 
 
