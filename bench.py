@@ -130,32 +130,82 @@ def oracle_sample_costs(w, kind: str, count: int) -> dict:
     return want
 
 
-def run_workload(kind: str, steps: int, warmup: int):
-    """Plans, runs and times one pair / triple workload; returns the
-    workload, the plan's numbers and a verification of the run."""
+def make_plan(ctx, kind: str, w):
+    """The plan of a workload on ctx (its knobs as currently set), the
+    concatenated sequences and the per-sequence length arrays."""
     from poyd_amd import engine
-    w, kind = workload(kind)
-    ctx = engine.Context(0)
-    t_plan = time.perf_counter()
     if kind == "cube":
         cm = engine.CostMatrix.of_sub_indel(1, 2)
         cm3 = engine.CostMatrix3(cm)
         seq = np.concatenate([b.seq for b in w.batches])
         base = np.cumsum([0] + [len(b.seq) for b in w.batches])
         plan = engine.TriplePlan(ctx, cm3, *[x for q, b in enumerate(w.batches) for x in (b.off + base[q], b.ln)])
-        seq_lens = [b.ln for b in w.batches]
+        return plan, seq, [b.ln for b in w.batches]
+    if kind == "affine":
+        cm = engine.CostMatrix.of_sub_indel(1, 1, 3)
+        mode, dws = engine.POYG_MODE_AFFINE_ALIGN, None
     else:
-        if kind == "affine":
-            cm = engine.CostMatrix.of_sub_indel(1, 1, 3)
-            mode = engine.POYG_MODE_AFFINE_ALIGN
-            dws = None
-        else:
-            cm = engine.CostMatrix.of_sub_indel(1, 2)
-            mode = engine.POYG_MODE_LINEAR_ALIGN
-            dws = engine.batch_deltaw(w.a, w.b)
-        seq = np.concatenate([w.a.seq, w.b.seq])
-        plan = engine.PairPlan(ctx, cm, mode, w.a.off, w.a.ln, w.b.off + len(w.a.seq), w.b.ln, dws)
-        seq_lens = [w.a.ln, w.b.ln]
+        cm = engine.CostMatrix.of_sub_indel(1, 2)
+        mode, dws = engine.POYG_MODE_LINEAR_ALIGN, engine.batch_deltaw(w.a, w.b)
+    seq = np.concatenate([w.a.seq, w.b.seq])
+    plan = engine.PairPlan(ctx, cm, mode, w.a.off, w.a.ln, w.b.off + len(w.a.seq), w.b.ln, dws)
+    return plan, seq, [w.a.ln, w.b.ln]
+
+
+def tune_workload(ctx, kind: str, w):
+    """Times launch-shape trials of the workload's own plan (poyd_amd.autotune)
+    and leaves the best knobs set on ctx."""
+    from poyd_amd import autotune, engine
+    lengths = np.concatenate([np.asarray(x) for x in (w.ln if kind == "cube" else (w.a.ln, w.b.ln))])
+    seq = None
+    d = {}
+
+    def trial():
+        nonlocal seq
+        plan, s, _ = make_plan(ctx, kind, w)
+        if seq is None:
+            seq = s
+            d["seq"] = engine.DeviceBuffer.from_array(ctx, seq)
+        d_cost = engine.DeviceBuffer(ctx, 4 * plan.n)
+        d_out = engine.DeviceBuffer(ctx, plan.out_bytes)
+        d_len = engine.DeviceBuffer(ctx, 4 * plan.n * plan.slots)
+        plan.run(d["seq"].ptr, d_cost.ptr, d_out.ptr, d_len.ptr)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(2):
+            plan.run(d["seq"].ptr, d_cost.ptr, d_out.ptr, d_len.ptr)
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / 2
+        plan.close()
+        for b in (d_cost, d_out, d_len):
+            b.free()
+        ctx.sync()
+        return ms
+
+    res = autotune.tune(ctx, kind, lengths, trial, repeats=2, log=log)
+    d["seq"].free()
+    return res
+
+
+def run_workload(kind: str, steps: int, warmup: int, tune: str = "table"):
+    """Plans, runs and times one pair / triple workload; returns the
+    workload, the plan's numbers and a verification of the run.  tune:
+    "table" applies the tuned launch shapes for this device and batch,
+    "default" the built-in ones, "trials" tunes them on this batch first."""
+    from poyd_amd import autotune, engine
+    w, kind = workload(kind)
+    ctx = engine.Context(0)
+    lengths = np.concatenate([np.asarray(x) for x in (w.ln if kind == "cube" else (w.a.ln, w.b.ln))])
+    tuning = None
+    if tune == "trials":
+        tuning = tune_workload(ctx, kind, w)
+    elif tune == "table":
+        autotune.apply(ctx, kind, lengths)
+    else:
+        autotune.reset(ctx, kind)
+    knobs = {k: ctx.tuning(k) for k, _ in autotune.KNOBS[kind]}
+    t_plan = time.perf_counter()
+    plan, seq, seq_lens = make_plan(ctx, kind, w)
     plan_ms = (time.perf_counter() - t_plan) * 1e3
     d_seq = engine.DeviceBuffer.from_array(ctx, seq)
     d_cost = engine.DeviceBuffer(ctx, 4 * plan.n)
@@ -194,7 +244,7 @@ def run_workload(kind: str, steps: int, warmup: int):
     out_bytes = int(d_len.download(np.int32, plan.n * plan.slots).astype(np.int64).sum())
     r = dict(w=w, kind=kind, n=plan.n, cells=int(plan.cells), ms=ms, plan_ms=plan_ms,
              fill_ms=float(np.mean(fills)), trace_ms=float(np.mean(traces)), costs=costs, failed=int(failed),
-             deterministic=deterministic, seq_bytes=seq_bytes, out_bytes=out_bytes)
+             deterministic=deterministic, seq_bytes=seq_bytes, out_bytes=out_bytes, knobs=knobs, tuning=tuning)
     plan.close()
     return r
 
@@ -232,10 +282,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 10; tree: 1)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 2; tree: 0)")
-    ap.add_argument("--workload", default="affine", choices=["affine", "linear", "cube", "tree", "fitch", "tbr"])
+    ap.add_argument("--workload", default="affine",
+                    choices=["affine", "linear", "cube", "tree", "fitch", "tbr", "percall"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sub", action="store_true", help="skip the linear and three-way sub-results")
+    ap.add_argument("--tune", choices=["table", "default", "trials"], default="table",
+                    help="launch shapes: the tuned table (default), the built-in ones, or fresh trials")
     ap.add_argument("--seeds", type=int, default=None, help="tbr: seeded datasets for the identical-cost rate")
     ap.add_argument("--taxa", type=int, default=None, help="tbr: taxa per dataset (default 256)")
     ap.add_argument("--epochs", type=int, default=8, help="tbr: training epochs of the join scorer")
@@ -260,8 +313,10 @@ def main() -> int:
         return bench_fitch(args)
     if args.workload == "tbr":
         return bench_tbr(args)
+    if args.workload == "percall":
+        return bench_percall(args)
 
-    r = run_workload(args.workload, args.steps, args.warmup)
+    r = run_workload(args.workload, args.steps, args.warmup, args.tune)
     w, kind, ms, cells = r["w"], r["kind"], r["ms"], r["cells"]
     fill_ms, trace_ms = r["fill_ms"], r["trace_ms"]
     gcups = cells / (ms / 1e3) / 1e9
@@ -306,6 +361,8 @@ def main() -> int:
             "outputs": ("cost, three aligned sequences, median" if kind == "cube" else
                         "cost, aligned pair, median" + (", median with gaps" if kind == "affine" else "")),
             "plan_ms": round(r["plan_ms"], 1),
+            "launch_shapes": r["knobs"],
+            "tuning_trials": r["tuning"],
             "fill_ms": round(fill_ms, 3),
             "backtrace_ms": round(trace_ms, 3),
             "deterministic": r["deterministic"],
@@ -825,6 +882,54 @@ def cpu_baseline_tbr(n_taxa: int, budget_s: float):
             "sample": f"{nb} of {len(sp.breaks)} breaks ({done} of {len(cand.brk)} candidate joins) of the TBR "
                       f"round of {name}, each break scored exactly with its component medians, in {dt:.1f} s "
                       f"on 1 pinned core of {cpu_model()}"}
+
+
+def bench_percall(args) -> int:
+    """Latency of the per-call entry points the OCaml stubs use (one pair
+    per call, host buffers, synchronous): algn_CAML_simple_2 ->
+    poyg_algn_simple_2 and algn_CAML_align_affine_3 ->
+    poyg_algn_align_affine_3 on 300 bp and 1 kb pairs (10% substitutions,
+    2% indels), against the oracle doing the same call on one core."""
+    import ctypes as C
+    from oracle import pyoracle as po
+    from poyd_amd import engine, synth
+    ctx = engine.Context(0)
+    L = engine.load()
+    lin, aff = engine.CostMatrix.of_sub_indel(1, 2), engine.CostMatrix.of_sub_indel(1, 1, 3)
+    plin, paff = po.CM2.sub_indel(1, 2), po.CM2.sub_indel(1, 1, 3)
+    rows = []
+    for length in (300, 1000):
+        w = synth.pairs_uniform(64, length, length, 0.10, 0.02, 77 + length, f"percall-{length}")
+        pairs = [(w.a.get(k), w.b.get(k)) for k in range(len(w.a))]
+        for name in ("simple_2", "align_affine_3"):
+            def gpu_call(a, b):
+                if name == "simple_2":
+                    x, y = (a, b) if len(a) >= len(b) else (b, a)
+                    out = C.c_int32(0)
+                    engine._check(L.poyg_algn_simple_2(ctx.h, lin.h, engine._p8(x), len(x), engine._p8(y), len(y),
+                                                       engine.deltaw(x, y), C.byref(out)), "simple_2")
+                    return out.value
+                return engine.Align(ctx).align_affine_3(a, b, aff)[3]
+
+            def cpu_call(a, b):
+                if name == "simple_2":
+                    return po.cost_2(a, b, plin)
+                return po.align_affine_3(a, b, paff)[3]
+
+            for a, b in pairs[:4]:
+                gpu_call(a, b)
+            t0 = time.perf_counter()
+            got = [gpu_call(a, b) for a, b in pairs]
+            g_us = (time.perf_counter() - t0) / len(pairs) * 1e6
+            t0 = time.perf_counter()
+            want = [cpu_call(a, b) for a, b in pairs]
+            c_us = (time.perf_counter() - t0) / len(pairs) * 1e6
+            rows.append({"call": name, "length": length, "gpu_us": round(g_us, 1), "cpu_oracle_us": round(c_us, 1),
+                         "same_costs": got == want})
+            log(json.dumps(rows[-1]))
+    print(json.dumps({"metric": "per-call latency of the stub entry points", "unit": "us per call",
+                      "rows": rows, "cpu": cpu_model()}), flush=True)
+    return 0
 
 
 def cpu_baseline_tree(ds, tree, budget_s: float):

@@ -52,6 +52,21 @@ void poyg_ctx_destroy(poyg_ctx *ctx);
  * destruction) become reusable by later allocations of this library. */
 int poyg_ctx_sync(poyg_ctx *ctx);
 
+/* Launch-shape knobs of the plans made on this context afterwards (no
+ * reference counterpart; results do not depend on them): "aff_quad_max_lanes"
+ * (41..48, default 48: affine bands of at most this many diagonal pairs go
+ * to the four-pairs-per-wave fill, wider ones to the one-pair fill),
+ * "aff_side_div" (2..64, default 4: the side streams get at most
+ * CUs / value compute units), "lin_sys_blocks_per_cu" and
+ * "lin_trace_blocks_per_cu" (1..occupancy: resident workgroups per CU of the
+ * linear fill and backtrace), "cube_waves_per_cu" (1..64, default 8).  A
+ * negative value restores the default.  poyd_amd/autotune.py picks them by
+ * timed trials. */
+int poyg_ctx_set_tuning(poyg_ctx *ctx, const char *knob, int value);
+int poyg_ctx_get_tuning(poyg_ctx *ctx, const char *knob);
+/* "<device name>:<gfx arch>:<CUs>cu", the key of tuned settings. */
+int poyg_ctx_device_name(poyg_ctx *ctx, char *buf, int cap);
+
 /* ---- device buffers (plumbing for callers without their own allocator;
  * no reference counterpart) ------------------------------------------- */
 int poyg_dev_alloc(poyg_ctx *ctx, int64_t bytes, void **out);

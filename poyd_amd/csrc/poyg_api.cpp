@@ -55,7 +55,7 @@ extern "C" {
 int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64_t *off_1,
                             const int32_t *len_1, const int64_t *off_2, const int32_t *len_2,
                             const int64_t *off_3, const int32_t *len_3, int device, int cus, void *stream,
-                            void *side, void *side2, int64_t dm_budget, void **out);
+                            void *side, void *side2, int64_t dm_budget, int waves_per_cu, void **out);
 void poyg_triple_plan_destroy(void *p);
 int64_t poyg_triple_plan_out_bytes(void *p);
 int64_t poyg_triple_plan_cells(void *p);
@@ -76,9 +76,15 @@ using namespace poyg;
     }                                                                              \
   } while (0)
 
+// Launch-shape knobs a tuner may set per context (-1: the built-in choice).
+enum TuneKnob : int { kTuneQuadMaxLanes, kTuneSideDiv, kTuneSysBlocks, kTuneTraceBlocks, kTuneCubeWaves, kTuneKnobs };
+static const char *const kTuneNames[kTuneKnobs] = {"aff_quad_max_lanes", "aff_side_div", "lin_sys_blocks_per_cu",
+                                                   "lin_trace_blocks_per_cu", "cube_waves_per_cu"};
+
 struct poyg_ctx {
   int device = 0;
   int cus = 256;
+  int tune[kTuneKnobs] = {-1, -1, -1, -1, -1};
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;    // second stream: fills beside the main one
   hipStream_t side2 = nullptr;   // third stream: the widest bands (lane sets, row kernels)
@@ -220,6 +226,7 @@ struct DevBuf {
 
 struct poyg_plan {
   poyg_ctx *ctx = nullptr;
+  int device = 0;           // the context's device (the plan may outlive the context object)
   void *triple = nullptr;   // three-sequence plan (mode POYG_MODE_TRIPLE)
   int mode = 0;
   int n = 0;
@@ -245,6 +252,7 @@ struct poyg_plan {
   // other fill runs on the side stream, followed by a second launch of the
   // main kernel (grid_help workgroups) that pulls from the same work list
   int main_fill = 0, grid_help = 0, grid_help2 = 0;
+  int trace_bpc = 1;   // resident workgroups per CU of the linear backtrace
   bool timing = false;
   float fill_ms = 0.f, trace_ms = 0.f;
   hipStream_t last_stream = nullptr;   // the stream of the last run (its status is read there)
@@ -340,9 +348,48 @@ int poyg_dev_download(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes) 
 void *poyg_ctx_stream(poyg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 int poyg_ctx_device(poyg_ctx *ctx) { return ctx ? ctx->device : -1; }
 
+static int tune_index(const char *knob) {
+  if (!knob) return -1;
+  for (int k = 0; k < kTuneKnobs; ++k)
+    if (!strcmp(knob, kTuneNames[k])) return k;
+  return -1;
+}
+
+int poyg_ctx_set_tuning(poyg_ctx *ctx, const char *knob, int value) {
+  const int k = tune_index(knob);
+  if (!ctx || k < 0) {
+    set_error("poyg_ctx_set_tuning: unknown knob %s", knob ? knob : "(null)");
+    return POYG_EINVAL;
+  }
+  static const int lo[kTuneKnobs] = {41, 2, 1, 1, 1}, hi[kTuneKnobs] = {48, 64, 64, 64, 64};
+  if (value >= 0 && (value < lo[k] || value > hi[k])) {
+    set_error("poyg_ctx_set_tuning: %s = %d outside %d..%d", knob, value, lo[k], hi[k]);
+    return POYG_EINVAL;
+  }
+  ctx->tune[k] = value < 0 ? -1 : value;
+  return POYG_OK;
+}
+
+int poyg_ctx_device_name(poyg_ctx *ctx, char *buf, int cap) {
+  if (!ctx || !buf || cap < 1) return POYG_EINVAL;
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
+  snprintf(buf, (size_t)cap, "%s:%s:%dcu", prop.name, prop.gcnArchName, ctx->cus);
+  return POYG_OK;
+}
+
+int poyg_ctx_get_tuning(poyg_ctx *ctx, const char *knob) {
+  const int k = tune_index(knob);
+  if (!ctx || k < 0) return POYG_EINVAL;
+  return ctx->tune[k];
+}
+
 void poyg_plan_destroy(poyg_plan *plan) {
   if (!plan) return;
-  (void)hipSetDevice(plan->ctx->device);   // the plan's blocks return to its device's cache
+  // the plan's blocks return to its device's cache; the device was recorded
+  // when the plan was made, so a plan released after its context is still
+  // released correctly
+  if (hipSetDevice(plan->device) != hipSuccess) (void)hipGetLastError();
   delete plan;
 }
 int poyg_plan_slots(const poyg_plan *p) { return p->slots; }
@@ -374,10 +421,11 @@ int poyg_plan_triples(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64
   }
   void *tp = nullptr;
   int rc = poyg_triple_plan_create(ctx, cm3, n, off_1, len_1, off_2, len_2, off_3, len_3, ctx->device, ctx->cus,
-                                   ctx->stream, ctx->side, ctx->side2, budget, &tp);
+                                   ctx->stream, ctx->side, ctx->side2, budget, ctx->tune[kTuneCubeWaves], &tp);
   if (rc) return rc;
   poyg_plan *P = new poyg_plan;
   P->ctx = ctx;
+  P->device = ctx->device;
   P->triple = tp;
   P->mode = POYG_MODE_TRIPLE;
   P->n = n;
@@ -440,12 +488,20 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   HIP_TRY(hipSetDevice(ctx->device));
   poyg_plan *P = new poyg_plan;
   P->ctx = ctx;
+  P->device = ctx->device;
   P->mode = mode;
   P->n = n;
   P->slots = align ? (affine ? 4 : 3) : 0;
+  P->trace_bpc = ctx->tune[kTuneTraceBlocks] > 0 ? std::min(ctx->tune[kTuneTraceBlocks], lin_trace_blocks_per_cu())
+                                                 : lin_trace_blocks_per_cu();
   std::vector<PairMeta> meta(n);
   std::vector<int64_t> cells(n);
   std::vector<char> glb(n, 0), diag(n, 0), quad(n, 0), sys(n, 0);
+  // tuned launch shapes (see poyg_ctx_set_tuning)
+  const int quad_max = ctx->tune[kTuneQuadMaxLanes] > 0 ? ctx->tune[kTuneQuadMaxLanes] : 16 * kQuadH;
+  const int side_div = ctx->tune[kTuneSideDiv] > 0 ? ctx->tune[kTuneSideDiv] : 4;
+  const int sys_bpc = ctx->tune[kTuneSysBlocks] > 0 ? std::min(ctx->tune[kTuneSysBlocks], lin_sys_blocks_per_cu(align))
+                                                    : lin_sys_blocks_per_cu(align);
   int64_t dm_bytes = 0, aux_ints = 0, out_bytes = 0;
   int row_width = 0;
   P->out_off.resize(n);
@@ -481,7 +537,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
         // alone, ra == 0, stays with the one-set kernel).  Wider bands belong
         // to long pairs, whose latency the one-pair-per-wave kernel keeps
         // lower (a lane of it steps one diagonal pair instead of three).
-        if (ra > 0 && lanes <= 16 * kQuadH) quad[k] = 3;
+        if (ra > 0 && lanes <= quad_max) quad[k] = 3;
         m.p1 = lanes;
         m.p2 = 1;
         if (align && ra > 0) {
@@ -581,7 +637,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   const int per_cu = affine ? 4 : 8;
   P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
   P->grid_glb = std::max(1, std::min(P->n_glb, 256));
-  if (P->n_sys) P->grid_sys = std::max(1, std::min(P->n_sys, ctx->cus * lin_sys_blocks_per_cu(align)));
+  if (P->n_sys) P->grid_sys = std::max(1, std::min(P->n_sys, ctx->cus * sys_bpc));
   if (affine) {
     // The fill with the most cells is the main kernel.  The side stream's
     // kernels are persistent too; the main kernel leaves them whole CUs, or
@@ -599,8 +655,8 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     if (P->n_quad && P->main_fill != 1) side += cus_of(P->grid_quad, aff_quad_blocks_per_cu(align));
     if (P->n_diag[0] && P->main_fill != 2) side += cus_of(P->grid_diag[0], aff_diag_blocks_per_cu(align, 1));
     if ((side || side2) && P->main_fill) {
-      side = std::min(side, ctx->cus / 4);
-      side2 = std::min(side2, ctx->cus / 4);
+      side = std::min(side, ctx->cus / side_div);
+      side2 = std::min(side2, ctx->cus / side_div);
       const int bpc = P->main_fill == 1 ? aff_quad_blocks_per_cu(align) : aff_diag_blocks_per_cu(align, 1);
       int &grid_main = P->main_fill == 1 ? P->grid_quad : P->grid_diag[0];
       grid_main = std::max(1, std::min(grid_main, (ctx->cus - side - side2) * bpc));
@@ -735,6 +791,9 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
     return POYG_EINVAL;
   }
   hipStream_t s = stream ? (hipStream_t)stream : P->ctx->stream;
+  // the launches below report hipGetLastError; an error an earlier,
+  // unrelated call left behind is not this run's
+  (void)hipGetLastError();
   int *ctr = (int *)P->counters.p;
   const PairMeta *meta = (const PairMeta *)P->meta.p;
   const DevTcm *tcm = (const DevTcm *)P->tcm.p;
@@ -824,7 +883,7 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
                                  (const uint8_t *)P->dm.p, d_out, d_out_len, (int32_t *)P->status.p, s));
     }
     else
-      HIP_TRY(launch_lin_trace(std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * lin_trace_blocks_per_cu())), meta,
+      HIP_TRY(launch_lin_trace(std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * P->trace_bpc)), meta,
                                (const int32_t *)P->order_all.p, P->n, ctr + 3, d_seq, tcm, (const uint8_t *)P->dm.p,
                                d_out, d_out_len, (int32_t *)P->status.p, s));
   }

@@ -921,7 +921,7 @@ extern "C" {
 int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64_t *off_1,
                             const int32_t *len_1, const int64_t *off_2, const int32_t *len_2,
                             const int64_t *off_3, const int32_t *len_3, int device, int cus, void *stream,
-                            void *side, void *side2, int64_t dm_budget, void **out) {
+                            void *side, void *side2, int64_t dm_budget, int waves_per_cu, void **out) {
   if (!cm3 || n < 0 || !out) return POYG_EINVAL;
   if (cm3->lcm != 5 || cm3->gap != kGap) {
     set_error("poyg_plan_triples: 5-bit DNA matrices only");
@@ -1019,7 +1019,7 @@ int poyg_triple_plan_create(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const
   }
   if (n) P->batches.push_back({begin, n});
   const char *wv = getenv("POYG_CUBE_WAVES_PER_CU");
-  const int waves = wv ? std::max(1, atoi(wv)) : 8;
+  const int waves = waves_per_cu > 0 ? waves_per_cu : (wv ? std::max(1, atoi(wv)) : 8);
   P->grid = std::max(1, std::min(n, cus * waves));
   hipError_t e = hipSuccess;
   auto A = [&](void **p, size_t b) { if (e == hipSuccess) e = hipMalloc(p, b ? b : 16); };

@@ -111,6 +111,9 @@ SIGNATURES = {
     "poyg_nonadd_median": (C.c_int, [vp, u8p, u8p, u8p, C.c_int, i64p]),
     "poyg_nonadd_median_3": (C.c_int, [vp, u8p, u8p, u8p, u8p, u8p, C.c_int]),
     "poyg_ctx_device": (C.c_int, [vp]),
+    "poyg_ctx_set_tuning": (C.c_int, [vp, C.c_char_p, C.c_int]),
+    "poyg_ctx_get_tuning": (C.c_int, [vp, C.c_char_p]),
+    "poyg_ctx_device_name": (C.c_int, [vp, C.c_char_p, C.c_int]),
     "poyg_fitch_down_w": (C.c_int, [vp, C.c_int, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp]),
     "poyg_fitch_up_w": (C.c_int, [vp, C.c_int, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp]),
     "poyg_nonadd_median_w": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, i64p]),
@@ -172,6 +175,19 @@ class Context:
 
     def sync(self):
         _check(load().poyg_ctx_sync(self.h), "poyg_ctx_sync")
+
+    def set_tuning(self, knob: str, value: int) -> None:
+        """A launch-shape knob of later plans (poyg_ctx_set_tuning); -1
+        restores the default."""
+        _check(load().poyg_ctx_set_tuning(self.h, knob.encode(), int(value)), "poyg_ctx_set_tuning")
+
+    def tuning(self, knob: str) -> int:
+        return int(load().poyg_ctx_get_tuning(self.h, knob.encode()))
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        _check(load().poyg_ctx_device_name(self.h, buf, 256), "poyg_ctx_device_name")
+        return buf.value.decode()
 
     def close(self):
         if self.h:

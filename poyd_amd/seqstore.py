@@ -140,25 +140,28 @@ class DeviceStore:
     # ------------------------------------------------------- alignments
     def _run(self, mode: int, a: np.ndarray, b: np.ndarray, dws):
         plan = engine.PairPlan(self.ctx, self.cm, mode, self.off[a], self.ln[a], self.off[b], self.ln[b], dws)
-        n = plan.n
-        n_cost = (max(n, 1) + 63) // 64 * 64
-        n_len = n * plan.slots
-        d_small = engine.DeviceBuffer(self.ctx, 4 * (n_cost + max(n_len, 1)))
-        out_at = self._reserve(plan.out_bytes) if plan.slots else 0
-        plan.run(self.arena.ptr, d_small.ptr, self.arena.ptr + out_at if plan.slots else None,
-                 d_small.ptr + 4 * n_cost if plan.slots else None)
-        small = d_small.download(np.int32, n_cost + n_len)
-        bad = plan.status()
-        if bad:
-            raise engine.PoygError(f"{bad} pairs failed (a backtrace left the filled band, "
-                                   "or a fill's wait for the previous strip gave up)")
-        self.cells += int(plan.cells)
-        self.alignments += n
-        res = (small[:n].astype(np.int64), small[n_cost:n_cost + n_len].reshape(n, max(plan.slots, 1)),
-               out_at + plan.out_off, plan.cap)
-        plan.close()
-        d_small.free()
-        return res
+        d_small = None
+        try:
+            n = plan.n
+            n_cost = (max(n, 1) + 63) // 64 * 64
+            n_len = n * plan.slots
+            d_small = engine.DeviceBuffer(self.ctx, 4 * (n_cost + max(n_len, 1)))
+            out_at = self._reserve(plan.out_bytes) if plan.slots else 0
+            plan.run(self.arena.ptr, d_small.ptr, self.arena.ptr + out_at if plan.slots else None,
+                     d_small.ptr + 4 * n_cost if plan.slots else None)
+            small = d_small.download(np.int32, n_cost + n_len)
+            bad = plan.status()
+            if bad:
+                raise engine.PoygError(f"{bad} pairs failed (a backtrace left the filled band, "
+                                       "or a fill's wait for the previous strip gave up)")
+            self.cells += int(plan.cells)
+            self.alignments += n
+            lens = small[n_cost:n_cost + n_len].reshape(n, plan.slots) if plan.slots else None
+            return small[:n].astype(np.int64), lens, out_at + plan.out_off, plan.cap
+        finally:
+            plan.close()
+            if d_small is not None:
+                d_small.free()
 
     def median(self, a, b):
         """DOS.median of id pairs (a[k] first, the smaller taxon code side):
