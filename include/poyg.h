@@ -266,6 +266,26 @@ int poyg_nonadd_median_w(poyg_ctx *ctx, int width, const void *a, const void *b,
 int poyg_nonadd_median_3_w(poyg_ctx *ctx, int width, const void *A, const void *N, const void *D1, const void *D2,
                            void *F, int len);
 
+/* ---- batched linear costs without device allocation -------------------
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the costs of poyg_plan_pairs +
+ * poyg_plan_run in POYG_MODE_LINEAR_COST (Sequence.Align.cost_2 ->
+ * algn_CAML_simple_2, src/algn.c:3408-3427), with every buffer the plan
+ * would allocate taken from one static device array of
+ * poyg_noalloc_scratch_bytes() (32 MiB) per device instead.  Host arrays
+ * describe the pairs as for poyg_plan_pairs (deltaw required); d_seq and
+ * d_cost are the caller's device buffers.  Pairs whose shorter sequence
+ * has more than 2,048 positions, or batches whose descriptors and prefix
+ * sums (2 ints per position of each pair's shorter sequence) do not fit the
+ * array, are refused with POYG_EUNSUPPORTED.  Every call on a device shares
+ * that device's array: calls that overlap on one device race.  The call
+ * returns after its work is done. */
+int64_t poyg_noalloc_scratch_bytes(void);
+int poyg_pairs_cost_linear_noalloc(poyg_ctx *ctx, const poyg_cm *cm, int32_t n, const int64_t *off_a,
+                                   const int32_t *len_a, const int64_t *off_b, const int32_t *len_b,
+                                   const int32_t *deltaw, const uint8_t *d_seq, int32_t *d_cost, void *stream);
+
 /* ---- tree-pass support (device-resident sequences) ------------------- */
 /* Per sequence: stats[2k] = positions carrying the gap bit (seq_CAML_count,
  * src/seq.c:571-581, the count in Sequence.Align.cost_2's band parameter,

@@ -130,6 +130,8 @@ SIGNATURES = {
     "poyg_mlp_adam": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, C.c_float]),
     "poyg_kmer_counts": (C.c_int, [vp, vp, vp, vp, C.c_int32, vp, vp]),
     "poyg_pair_features": (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_float, vp, vp]),
+    "poyg_noalloc_scratch_bytes": (C.c_int64, []),
+    "poyg_pairs_cost_linear_noalloc": (C.c_int, [vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]),
     "poyg_gather_rows": (C.c_int, [vp, vp, C.c_int64, vp, C.c_int64, vp, vp]),
     "poyg_gemm_bf16_f32": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int64, C.c_int64, vp, C.c_int64,
                                      C.c_int64, vp, C.c_int64, vp]),
