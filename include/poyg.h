@@ -295,6 +295,44 @@ int poyg_pairs_cost_linear_noalloc(poyg_ctx *ctx, const poyg_cm *cm, int32_t n, 
 int poyg_seq_stats(poyg_ctx *ctx, const uint8_t *d_seq, const int64_t *d_off, const int32_t *d_len,
                    int32_t n, int32_t *d_stats, void *stream);
 
+/* ---- sequence unions (Sequence.Unions, src/sequence.ml:1810-1895) ------
+ * A union of n items is four arrays: seq (the items), end (the reference's
+ * `offset` array: 0 on a homologous item, else the item's 1-based place in
+ * its run of non-homologous items), ca / cb (`union_c1` / `union_c2`: the
+ * child-union positions a homologous item came from).  The reference keeps
+ * them as int16 (int32 with USE_LONG_SEQUENCES) bigarrays, right-aligned
+ * in their capacity; here they are int32.  A leaf's union is its sequence
+ * with end = 0 and ca = cb = 0 .. n-1 (Unions.leaf).
+ *
+ * union_CAML_make (src/union.c:209-226, union_merge :102-164): the union of
+ * two children's unions over their aligned pair (a, b: len columns) and its
+ * median with gaps (median == NULL: the matrix's median of each column, as
+ * median_2_with_gaps gives for linear matrices).  The children are read
+ * left-aligned (na / nb items; their ca / cb are not read); the result is
+ * written right-aligned into capacity-cap arrays (cap >= na + nb + 2, the
+ * capacity Unions.union allocates) and *out_n is its length.  POYG_EINVAL
+ * when the children do not match the alignment.  Host pointers. */
+int poyg_union_make(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, const uint8_t *median, int len,
+                    const uint8_t *ua_seq, const int32_t *ua_end, int na, const uint8_t *ub_seq,
+                    const int32_t *ub_end, int nb, uint8_t *out_seq, int32_t *out_end, int32_t *out_ca,
+                    int32_t *out_cb, int cap, int *out_n);
+/* One merge of a batch; every pointer is a device pointer. */
+typedef struct poyg_union_job {
+  const uint8_t *a, *b, *median;      /* aligned pair and median with gaps (NULL: column medians) */
+  const uint8_t *ua_seq, *ub_seq;     /* children's unions */
+  const int32_t *ua_end, *ub_end;
+  uint8_t *out_seq;                   /* result, right-aligned in cap */
+  int32_t *out_end, *out_ca, *out_cb;
+  int32_t len, na, nb, cap;
+} poyg_union_job;
+/* The merges of d_jobs[0..n) (a device array) on the device, one thread
+ * each; d_out_n[k] = length of union k, or < 0 when its children do not
+ * match its alignment or cap is too small.  No counterpart batch call in
+ * the reference, which makes one union per node (Node.Union.union,
+ * src/node.ml:2355-2408). */
+int poyg_union_merge(poyg_ctx *ctx, const poyg_cm *cm, int32_t n, const poyg_union_job *d_jobs,
+                     int32_t *d_out_n, void *stream);
+
 /* ---- learned join scorer ----------------------------------------------
  * No counterpart in the reference: POY scores every candidate join exactly
  * (AllDirChar.cost_fn, src/allDirChar.ml:1279-1332).  These entry points

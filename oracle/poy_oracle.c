@@ -618,6 +618,153 @@ int po_union(const uint8_t *a, const uint8_t *b, int len, uint8_t *out, int cap)
     return len;
 }
 
+/* Sequence unions: union_merge, src/union.c:102-164, with the helpers
+ * union_prepend_pair :29-38, union_prepend_item :40-46, union_copy_single
+ * :48-59, union_copy_non_homologous :61-69, union_prepend_and_fill_items
+ * :71-91 and union_move_left :93-100.  The unions are the structure of
+ * src/union.h (a sequence plus the arrays offset / union_c1 / union_c2 of
+ * Sequence.Unions.u, src/sequence.ml:1833-1837), restated here with
+ * pointers that walk right-aligned buffers exactly as the reference's do:
+ * `begin` is the first item, `end` the current entry of the offset array,
+ * `position` the current item.  The children are given left-aligned (n
+ * items); the result is written right-aligned into capacity-`cap` buffers.
+ * Returns its length, or -1 when a child runs out of items or a buffer is
+ * too small (the reference would read or write out of bounds there). */
+typedef struct {
+    const uint8_t *s;          /* items */
+    const int32_t *off;        /* offset array, same indexing as s */
+    int position;              /* current item */
+    int end;                   /* current entry of off */
+    int begin;
+} po_uin;
+
+typedef struct {
+    uint8_t *s;
+    int32_t *off, *ca, *cb;
+    int sbegin;                /* first item of s (sequences grow leftwards) */
+    int end, ca_at, cb_at;     /* next entry of each offset array */
+    int cap;
+    int fail;
+} po_uout;
+
+static void uo_prepend(po_uout *c, int v) {
+    if (c->sbegin == 0) { c->fail = 1; return; }
+    c->s[--c->sbegin] = (uint8_t)v;
+}
+
+static int ui_get(po_uin *u, po_uout *c) {
+    if (u->position < 0) { c->fail = 1; return 0; }
+    return u->s[u->position];
+}
+
+static void u_prepend_pair(po_uin *a, po_uin *b, po_uout *c, int or_) {
+    uo_prepend(c, or_ | ui_get(a, c) | ui_get(b, c));
+    a->position--; b->position--; a->end--; b->end--;
+}
+
+static void u_prepend_item(po_uin *src, po_uout *c, int or_) {
+    uo_prepend(c, or_ | ui_get(src, c));
+    src->position--; src->end--;
+}
+
+static int u_copy_single(po_uin *a, po_uout *c, int it, int or_) {
+    int i = a->off[a->end];
+    it += i;
+    while (i > 0 && !c->fail) { u_prepend_item(a, c, or_); i--; }
+    return it;
+}
+
+static int u_copy_non_homologous(po_uin *au, po_uin *bu, po_uout *c, int items, int gap) {
+    if (au->position != -1) items = u_copy_single(au, c, items, gap);
+    if (bu->position != -1 && !c->fail) items = u_copy_single(bu, c, items, gap);
+    return items;
+}
+
+static void u_slot(po_uout *c, int *at) {
+    if (*at < 0) c->fail = 1;
+}
+
+static void u_prepend_and_fill(int interm, po_uout *c, int prep, int apos, int bpos) {
+    for (int i = prep; i > 0 && !c->fail; i--) {
+        u_slot(c, &c->end); u_slot(c, &c->ca_at); u_slot(c, &c->cb_at);
+        if (c->fail) return;
+        c->off[c->end--] = i;
+        c->ca[c->ca_at--] = 0;
+        c->cb[c->cb_at--] = 0;
+    }
+    u_slot(c, &c->end); u_slot(c, &c->ca_at); u_slot(c, &c->cb_at);
+    if (c->fail) return;
+    c->ca[c->ca_at--] = apos;
+    c->cb[c->cb_at--] = bpos;
+    uo_prepend(c, interm);
+    c->off[c->end--] = 0;
+}
+
+static int u_move_left(po_uin *a, po_uout *c) {
+    int r = ui_get(a, c);
+    a->position--; a->end--;
+    return r;
+}
+
+int po_union_make(const uint8_t *a, const uint8_t *b, const uint8_t *median, int lena, int gap,
+                  const uint8_t *ua, const int32_t *ua_off, int na,
+                  const uint8_t *ub, const int32_t *ub_off, int nb,
+                  uint8_t *out, int32_t *out_off, int32_t *out_ca, int32_t *out_cb, int cap) {
+    po_uin au = {ua, ua_off, na - 1, na - 1, 0}, bu = {ub, ub_off, nb - 1, nb - 1, 0};
+    po_uout c = {out, out_off, out_ca, out_cb, cap, cap - 1, cap - 1, cap - 1, cap, 0};
+    int items_prepended = 0, i = lena - 1, interm, apos, bpos;
+    long guard = 0;
+    while ((au.begin <= au.end) || (bu.begin <= bu.end)) {
+        if (++guard > 4L * (na + nb + lena + 4)) return -1;   /* the reference would not end */
+        items_prepended = u_copy_non_homologous(&au, &bu, &c, items_prepended, gap);
+        if (c.fail) return -1;
+        if (i >= 0) {
+            interm = median[i];
+            if (gap == interm) {
+                if (i != 0) {
+                    if (gap == a[i]) u_prepend_item(&bu, &c, gap);
+                    else if (gap == b[i]) u_prepend_item(&au, &c, gap);
+                    else u_prepend_pair(&au, &bu, &c, gap);
+                } else {
+                    u_prepend_and_fill(gap, &c, items_prepended, 0, 0);
+                    if (c.fail || c.end + 1 != c.sbegin) return -1;
+                    return cap - c.sbegin;
+                }
+                items_prepended += 1;
+            } else {
+                if (gap == a[i]) {
+                    apos = 0;
+                    bpos = bu.position;
+                    interm = interm | u_move_left(&bu, &c);
+                } else if (gap == b[i]) {
+                    bpos = 0;
+                    apos = au.position;
+                    interm = interm | u_move_left(&au, &c);
+                } else {
+                    bpos = bu.position;
+                    apos = au.position;
+                    interm = interm | u_move_left(&au, &c);
+                    interm = interm | u_move_left(&bu, &c);
+                }
+                u_prepend_and_fill(interm, &c, items_prepended, apos, bpos);
+                items_prepended = 0;
+            }
+            if (c.fail) return -1;
+            i--;
+        }
+    }
+    if (c.sbegin == cap || gap != c.s[c.sbegin]) {
+        if (c.sbegin == 0 || c.end < 0) return -1;
+        uo_prepend(&c, gap);
+        c.ca[c.ca_at] = 0;
+        c.cb[c.cb_at] = 0;
+        c.off[c.end] = 0;
+        c.end--;
+    }
+    if (c.end + 1 != c.sbegin) return -1;
+    return cap - c.sbegin;
+}
+
 /* algn_calculate_from_2_aligned, src/algn.c:3316-3370. */
 static int calc_from_2_aligned(const uint8_t *s1, const uint8_t *s2, int len, const po_cm2 *c, const int *tab) {
     int res = 0, gap_row = 0, gap = c->gap, go = c->gap_open, comb = c->combinations, i;

@@ -103,6 +103,14 @@ int po_median_2_with_gaps(const uint8_t *a, const uint8_t *b, int len, const po_
 int po_ancestor_2(const uint8_t *a, const uint8_t *b, int len, const po_cm2 *c,
                   uint8_t *out, int cap);
 int po_union(const uint8_t *a, const uint8_t *b, int len, uint8_t *out, int cap);
+/* union_merge (src/union.c:102-164): the union of two children's unions
+ * over their aligned pair (a, b, lena columns) and its median with gaps.
+ * Children left-aligned (items, offset array); output right-aligned in cap
+ * (items, offset, union_c1, union_c2).  Returns the length or -1. */
+int po_union_make(const uint8_t *a, const uint8_t *b, const uint8_t *median, int lena, int gap,
+                  const uint8_t *ua, const int32_t *ua_off, int na,
+                  const uint8_t *ub, const int32_t *ub_off, int nb,
+                  uint8_t *out, int32_t *out_off, int32_t *out_ca, int32_t *out_cb, int cap);
 /* algn_calculate_from_2_aligned with the cost table (src/algn.c:3316-3370). */
 int po_verify_2(const uint8_t *a, const uint8_t *b, int len, const po_cm2 *c);
 int po_worst_2(const uint8_t *a, const uint8_t *b, int len, const po_cm2 *c);

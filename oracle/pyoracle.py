@@ -62,6 +62,8 @@ def lib():
             "po_median_2_with_gaps": (C.c_int, [u8p, u8p, C.c_int, vp, u8p, C.c_int]),
             "po_ancestor_2": (C.c_int, [u8p, u8p, C.c_int, vp, u8p, C.c_int]),
             "po_union": (C.c_int, [u8p, u8p, C.c_int, u8p, C.c_int]),
+            "po_union_make": (C.c_int, [u8p, u8p, u8p, C.c_int, C.c_int, u8p, i32p, C.c_int, u8p, i32p,
+                                        C.c_int, u8p, i32p, i32p, i32p, C.c_int]),
             "po_verify_2": (C.c_int, [u8p, u8p, C.c_int, vp]),
             "po_worst_2": (C.c_int, [u8p, u8p, C.c_int, vp]),
             "po_align_affine_3": (C.c_int, [u8p, C.c_int, u8p, C.c_int, vp, u8p, u8p, u8p, u8p,
@@ -261,6 +263,35 @@ def union(a, b):
     out = np.zeros(len(x), np.uint8)
     n = lib().po_union(_p8(x), _p8(y), len(x), _p8(out), len(x))
     return _tail(out, n)
+
+
+def _p32(a: np.ndarray):
+    return a.ctypes.data_as(i32p)
+
+
+def union_leaf(seq):
+    """Unions.leaf (src/sequence.ml:1870-1873): (items, offset, c1, c2)."""
+    x = _u8(seq)
+    pos = np.arange(len(x), dtype=np.int32)
+    return x.copy(), np.zeros(len(x), np.int32), pos, pos.copy()
+
+
+def union_make(a, b, median, gap, ua, ub):
+    """union_merge (src/union.c:102-164) of the children's unions ua, ub
+    (tuples whose first two entries are items and offsets) over the aligned
+    pair (a, b) and its median with gaps.  Returns (items, offset, c1, c2)."""
+    x, y, m = _u8(a), _u8(b), _u8(median)
+    assert len(x) == len(y) == len(m)
+    sa, oa = _u8(ua[0]), np.ascontiguousarray(ua[1], np.int32)
+    sb, ob = _u8(ub[0]), np.ascontiguousarray(ub[1], np.int32)
+    cap = len(sa) + len(sb) + 2
+    out = np.zeros(cap, np.uint8)
+    off, ca, cb = (np.zeros(cap, np.int32) for _ in range(3))
+    n = lib().po_union_make(_p8(x), _p8(y), _p8(m), len(x), int(gap), _p8(sa), _p32(oa), len(sa), _p8(sb), _p32(ob),
+                            len(sb), _p8(out), _p32(off), _p32(ca), _p32(cb), cap)
+    if n < 0:
+        raise ValueError("union_make: the children's unions do not match the alignment")
+    return out[cap - n:].copy(), off[cap - n:].copy(), ca[cap - n:].copy(), cb[cap - n:].copy()
 
 
 def verify_2(a, b, cm: CM2) -> int:

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpoyg.so")
-SOURCES = ["poyg_kernels.hip", "poyg_cube.hip", "poyg_powell.hip", "poyg_mlp.hip", "poyg_store.hip", "poyg_noalloc.hip", "poyg_api.cpp", "poyg_tcm.cpp"]
+SOURCES = ["poyg_kernels.hip", "poyg_cube.hip", "poyg_powell.hip", "poyg_mlp.hip", "poyg_store.hip", "poyg_noalloc.hip", "poyg_union.hip", "poyg_api.cpp", "poyg_tcm.cpp"]
 
 
 def hipcc() -> str:

@@ -119,6 +119,9 @@ SIGNATURES = {
     "poyg_nonadd_median_w": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, i64p]),
     "poyg_nonadd_median_3_w": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
     "poyg_seq_stats": (C.c_int, [vp, vp, vp, vp, C.c_int32, vp, vp]),
+    "poyg_union_make": (C.c_int, [vp, u8p, u8p, u8p, C.c_int, u8p, i32p, C.c_int, u8p, i32p, C.c_int, u8p, i32p,
+                                  i32p, i32p, C.c_int, i32p]),
+    "poyg_union_merge": (C.c_int, [vp, vp, C.c_int32, vp, vp, vp]),
     "poyg_mlp_param_count": (C.c_int64, []),
     "poyg_mlp_create": (C.c_int, [vp, C.c_int, C.c_uint64, C.POINTER(vp)]),
     "poyg_mlp_destroy": (None, [vp]),
@@ -541,6 +544,61 @@ class TriplePlan(PairPlan):
         if self.n:
             load().poyg_plan_out_offsets(h, self.out_off.ctypes.data_as(i64p))
         self.cap = ((l1.astype(np.int64) + l2 + l3 + 1 + 15) // 16) * 16
+
+
+class UnionJob(C.Structure):
+    """poyg_union_job: one merge of a batch (device pointers)."""
+    _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("median", C.c_void_p),
+                ("ua_seq", C.c_void_p), ("ub_seq", C.c_void_p), ("ua_end", C.c_void_p), ("ub_end", C.c_void_p),
+                ("out_seq", C.c_void_p), ("out_end", C.c_void_p), ("out_ca", C.c_void_p), ("out_cb", C.c_void_p),
+                ("len", C.c_int32), ("na", C.c_int32), ("nb", C.c_int32), ("cap", C.c_int32)]
+
+
+def union_leaf(seq):
+    """Unions.leaf (src/sequence.ml:1870-1873): (items, end, ca, cb)."""
+    x = _u8(seq).copy()
+    pos = np.arange(len(x), dtype=np.int32)
+    return x, np.zeros(len(x), np.int32), pos, pos.copy()
+
+
+def union_make(cm: "CostMatrix", a, b, median, ua, ub):
+    """union_CAML_make (src/union.c:209-226) on the host: the union of the
+    children's unions ua, ub ((items, end, ...) tuples) over the aligned pair
+    (a, b) and its median with gaps (None: the matrix's column medians).
+    Returns (items, end, ca, cb)."""
+    x, y = _u8(a), _u8(b)
+    m = None if median is None else _u8(median)
+    sa, ea = _u8(ua[0]), np.ascontiguousarray(ua[1], np.int32)
+    sb, eb = _u8(ub[0]), np.ascontiguousarray(ub[1], np.int32)
+    cap = len(sa) + len(sb) + 2
+    out = np.zeros(cap, np.uint8)
+    end, ca, cb = (np.zeros(cap, np.int32) for _ in range(3))
+    n = C.c_int32(0)
+    _check(load().poyg_union_make(cm.h, _p8(x), _p8(y), None if m is None else _p8(m), len(x), _p8(sa),
+                                  ea.ctypes.data_as(i32p), len(sa), _p8(sb), eb.ctypes.data_as(i32p), len(sb),
+                                  _p8(out), end.ctypes.data_as(i32p), ca.ctypes.data_as(i32p),
+                                  cb.ctypes.data_as(i32p), cap, C.byref(n)), "poyg_union_make")
+    k = cap - n.value
+    return out[k:].copy(), end[k:].copy(), ca[k:].copy(), cb[k:].copy()
+
+
+def union_merge(ctx: "Context", cm: "CostMatrix", jobs, stream=None) -> np.ndarray:
+    """poyg_union_merge over a list of UnionJob (device pointers); returns
+    the union lengths (negative: the children did not match)."""
+    n = len(jobs)
+    if n == 0:
+        return np.zeros(0, np.int32)
+    arr = (UnionJob * n)(*jobs)
+    d_jobs = DeviceBuffer(ctx, C.sizeof(arr))
+    d_n = DeviceBuffer(ctx, 4 * n)
+    try:
+        _check(load().poyg_dev_upload(ctx.h, d_jobs.ptr, C.addressof(arr), C.sizeof(arr)), "poyg_dev_upload")
+        _check(load().poyg_union_merge(ctx.h, cm.h, n, d_jobs.ptr, d_n.ptr, stream), "poyg_union_merge")
+        ctx.sync()
+        return d_n.download(np.int32, n)
+    finally:
+        d_jobs.free()
+        d_n.free()
 
 
 def powell_3d(ctx: Context, a, b, c, mm: int, go: int, ge: int):

@@ -47,6 +47,21 @@ def band_deltaw(la, lb, gaps_a, gaps_b, dw=None) -> np.ndarray:
     return (np.maximum(np.asarray(gaps_a, np.int64), np.asarray(gaps_b, np.int64)) + extra).astype(np.int32)
 
 
+class Aligned:
+    """Arena offsets of the alignments behind a batch of medians: per pair
+    the aligned children (a_off, b_off, `length` columns) and the median
+    with gaps (mwg_off, -1 where the kernel takes the matrix's column
+    medians); length -1 where no alignment ran (an empty side)."""
+
+    def __init__(self, a_off, b_off, mwg_off, length):
+        self.a_off, self.b_off, self.mwg_off, self.length = a_off, b_off, mwg_off, length
+
+    @classmethod
+    def none(cls, n: int) -> "Aligned":
+        return cls(np.full(n, -1, np.int64), np.full(n, -1, np.int64), np.full(n, -1, np.int64),
+                   np.full(n, -1, np.int32))
+
+
 class DeviceStore:
     """Sequences in one device arena, aligned by libpoyg plans."""
 
@@ -163,30 +178,44 @@ class DeviceStore:
             if d_small is not None:
                 d_small.free()
 
-    def median(self, a, b):
+    def median(self, a, b, aligned: bool = False):
         """DOS.median of id pairs (a[k] first, the smaller taxon code side):
-        returns (median ids, costs)."""
+        returns (median ids, costs), and with aligned=True also an Aligned
+        record of where each pair's alignment lies in the arena (the aligned
+        children and the median with gaps that Unions.union reads)."""
         a = np.asarray(a, np.int64)
         b = np.asarray(b, np.int64)
         out = np.where(self.empty[a], b, a).astype(np.int64)
         cost = np.zeros(len(a), np.int64)
+        al = Aligned.none(len(a)) if aligned else None
         work = np.nonzero(~self.empty[a] & ~self.empty[b])[0]
         for c0 in range(0, len(work), self.MEDIAN_CHUNK):
             idx = work[c0:c0 + self.MEDIAN_CHUNK]
             x, y = a[idx], b[idx]
             if self.affine:
+                # slots: median, median with gaps, aligned a, aligned b
                 costs, lens, base, cap = self._run(engine.POYG_MODE_AFFINE_ALIGN, x, y, None)
-                slot = 0
+                slot, sa, sb, smwg = 0, 2, 3, 1
             else:
+                # slots: aligned a, aligned b, median
                 dws = band_deltaw(self.ln[x], self.ln[y], self.gaps[x], self.gaps[y])
                 costs, lens, base, cap = self._run(engine.POYG_MODE_LINEAR_ALIGN, x, y, dws)
-                slot = 2
+                slot, sa, sb, smwg = 2, 0, 1, None
             mlen = lens[:, slot].astype(np.int32)
             moff = base + slot * cap + cap - mlen
             gaps, empty = self._stats(moff, mlen)
             out[idx] = self._append(moff, mlen, gaps, empty)
             cost[idx] = costs
-        return out, cost
+            if aligned:
+                alen = lens[:, sa].astype(np.int32)
+                al.length[idx] = alen
+                al.a_off[idx] = base + sa * cap + cap - alen
+                al.b_off[idx] = base + sb * cap + cap - alen
+                if smwg is not None:
+                    if not np.array_equal(lens[:, smwg], lens[:, sa]):
+                        raise engine.PoygError("median with gaps and aligned pair differ in length")
+                    al.mwg_off[idx] = base + smwg * cap + cap - alen
+        return (out, cost, al) if aligned else (out, cost)
 
     def distance(self, a, b) -> np.ndarray:
         """DOS.distance of id pairs."""
