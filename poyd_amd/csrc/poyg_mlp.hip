@@ -123,20 +123,25 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
                                                         int64_t a_rs, int64_t a_cs, const uint16_t *__restrict__ B,
                                                         int64_t b_rs, int64_t b_cs, const float *__restrict__ bias,
                                                         const uint16_t *__restrict__ mask, int64_t ldm,
-                                                        void *__restrict__ out, int64_t ldc) {
+                                                        void *__restrict__ out, int64_t ldc, int kchunk,
+                                                        int64_t split_stride) {
   __shared__ __attribute__((aligned(16))) uint16_t As[kTM][kLdsK];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[kTN][kLdsK];
   const int i0 = blockIdx.x * kTM, j0 = blockIdx.y * kTN;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int qi = (wv >> 1) * 32, qj = (wv & 1) * 32;
+  // split K: workgroup z takes k in [z kchunk, (z + 1) kchunk) and writes
+  // its partial product split_stride elements further (f32 output only)
+  const int kz0 = blockIdx.z * kchunk, kz1 = min(K, kz0 + kchunk);
+  if (kEpi == kEpiF32) out = (float *)out + blockIdx.z * split_stride;
   f32x4 acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += kTK) {
+  for (int k0 = kz0; k0 < kz1; k0 += kTK) {
     __syncthreads();
-    load_tile(As, A, a_rs, a_cs, i0, M, k0, K);
+    load_tile(As, A, a_rs, a_cs, i0, M, k0, kz1);
     // B(k, j): rows of the LDS tile are columns j, k contiguous
-    load_tile(Bs, B, b_cs, b_rs, j0, N, k0, K);
+    load_tile(Bs, B, b_cs, b_rs, j0, N, k0, kz1);
     __syncthreads();
     const int kk = 8 * (lane >> 4);
     bf16x8 av[2], bv[2];
@@ -196,15 +201,32 @@ __global__ __launch_bounds__(256) void head_kernel(int n, const uint16_t *__rest
   }
 }
 
-// Column sums over the batch: out[j] = sum_m X[m][j] (X bf16, or f32 dy
-// weighting: out[j] = sum_m X[m][j] * w[m] when w is given).  One thread
-// per column; rows in order, so the sum is deterministic.
-__global__ __launch_bounds__(256) void colsum_kernel(int n, const uint16_t *__restrict__ X,
-                                                     const float *__restrict__ w, float *__restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= kH) return;
+// Column sums over the batch: out[j] = sum_m X[m][j] (X bf16), or
+// sum_m X[m][j] * w[m] when w is given.  Two passes with a fixed order, so
+// that the sums are deterministic: workgroup b sums rows
+// [b * rows, (b + 1) * rows) of every column into part[b][j] (one thread
+// per column, four row streams interleaved to keep loads in flight); then
+// one thread per column adds the kColParts partials in order.
+constexpr int kColParts = 64;
+__global__ __launch_bounds__(256) void colsum_part_kernel(int n, const uint16_t *__restrict__ X,
+                                                          const float *__restrict__ w, float *__restrict__ part) {
+  const int j = threadIdx.x;   // kH == 256 columns
+  const int rows = (n + kColParts - 1) / kColParts;
+  const int m0 = blockIdx.x * rows, m1 = min(n, m0 + rows);
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  int m = m0;
+  for (; m + 3 < m1; m += 4) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s[r] += bf2f(X[(int64_t)(m + r) * kH + j]) * (w ? w[m + r] : 1.f);
+  }
+  for (; m < m1; ++m) s[0] += bf2f(X[(int64_t)m * kH + j]) * (w ? w[m] : 1.f);
+  part[blockIdx.x * kH + j] = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+__global__ __launch_bounds__(256) void colsum_fin_kernel(const float *__restrict__ part, float *__restrict__ out) {
+  const int j = threadIdx.x;
   float s = 0.f;
-  for (int m = 0; m < n; ++m) s += bf2f(X[(int64_t)m * kH + j]) * (w ? w[m] : 1.f);
+  for (int b = 0; b < kColParts; ++b) s += part[b * kH + j];
   out[j] = s;
 }
 
@@ -292,10 +314,12 @@ struct poyg_mlp {
   uint16_t *w1t = nullptr, *w2t = nullptr, *w2b = nullptr, *w3b = nullptr;
   uint16_t *h1 = nullptr, *h2 = nullptr, *dz2 = nullptr, *dz1 = nullptr;
   float *y = nullptr, *dy = nullptr, *loss = nullptr, *lsum = nullptr;
+  float *cpart = nullptr;   // column-sum partials, kColParts x kH per sum
+  float *wpart = nullptr;   // split-K partials of a weight gradient
   ~poyg_mlp() {
     for (void *q : {(void *)p, (void *)g, (void *)m1, (void *)m2, (void *)w1t, (void *)w2t, (void *)w2b,
                     (void *)w3b, (void *)h1, (void *)h2, (void *)dz2, (void *)dz1, (void *)y, (void *)dy,
-                    (void *)loss, (void *)lsum})
+                    (void *)loss, (void *)lsum, (void *)cpart, (void *)wpart})
       if (q) (void)hipFree(q);
   }
 };
@@ -308,15 +332,46 @@ static hipError_t gemm(int epi, int M, int N, int K, const uint16_t *A, int64_t 
                        int64_t ldm, void *C, int64_t ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return hipSuccess;
   dim3 grid((M + kTM - 1) / kTM, (N + kTN - 1) / kTN);
+  const int kall = std::max(K, 1);
   if (epi == kEpiBiasReluBf16)
     hipLaunchKernelGGL(gemm_bf16_kernel<kEpiBiasReluBf16>, grid, dim3(256), 0, s, M, N, K, A, a_rs, a_cs, B, b_rs,
-                       b_cs, bias, mask, ldm, C, ldc);
+                       b_cs, bias, mask, ldm, C, ldc, kall, (int64_t)0);
   else if (epi == kEpiMaskBf16)
     hipLaunchKernelGGL(gemm_bf16_kernel<kEpiMaskBf16>, grid, dim3(256), 0, s, M, N, K, A, a_rs, a_cs, B, b_rs,
-                       b_cs, bias, mask, ldm, C, ldc);
+                       b_cs, bias, mask, ldm, C, ldc, kall, (int64_t)0);
   else
     hipLaunchKernelGGL(gemm_bf16_kernel<kEpiF32>, grid, dim3(256), 0, s, M, N, K, A, a_rs, a_cs, B, b_rs, b_cs,
-                       bias, mask, ldm, C, ldc);
+                       bias, mask, ldm, C, ldc, kall, (int64_t)0);
+  return hipGetLastError();
+}
+
+// C (f32, M x N, ldc) = sum over z of part[z] (M x N each), in order.
+__global__ __launch_bounds__(256) void split_sum_kernel(int M, int N, int splits, const float *__restrict__ part,
+                                                        float *__restrict__ C, int64_t ldc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)M * N) return;
+  float s = 0.f;
+  for (int z = 0; z < splits; ++z) s += part[z * (int64_t)M * N + e];
+  C[(e / N) * ldc + e % N] = s;
+}
+
+// Weight gradients (M = N = kH, K = the batch): K split into kWgSplits
+// chunks so that 16 output tiles become 16 * kWgSplits workgroups; the
+// partial products go to `part` (kWgSplits x M x N floats) and are added
+// in a fixed order, so the result does not depend on scheduling.
+constexpr int kWgSplits = 32;
+static hipError_t gemm_wgrad(int M, int N, int K, const uint16_t *A, int64_t a_rs, int64_t a_cs, const uint16_t *B,
+                             int64_t b_rs, int64_t b_cs, float *part, float *C, int64_t ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  const int chunk = std::max(kTK, (K + kWgSplits - 1) / kWgSplits + kTK - 1) / kTK * kTK;
+  const int splits = std::max(1, (K + chunk - 1) / chunk);
+  dim3 grid((M + kTM - 1) / kTM, (N + kTN - 1) / kTN, splits);
+  hipLaunchKernelGGL(gemm_bf16_kernel<kEpiF32>, grid, dim3(256), 0, s, M, N, K, A, a_rs, a_cs, B, b_rs, b_cs,
+                     (const float *)nullptr, (const uint16_t *)nullptr, (int64_t)0, (void *)part, (int64_t)N, chunk,
+                     (int64_t)M * N);
+  const int64_t tot = (int64_t)M * N;
+  hipLaunchKernelGGL(split_sum_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, M, N, splits, part, C,
+                     ldc);
   return hipGetLastError();
 }
 
@@ -368,6 +423,8 @@ int poyg_mlp_create(poyg_ctx *ctx, int batch_max, uint64_t seed, poyg_mlp **out)
   A((void **)&m->dy, 4 * nb);
   A((void **)&m->loss, 4 * nb);
   A((void **)&m->lsum, 16);
+  A((void **)&m->cpart, 4 * 3 * kColParts * kH);
+  A((void **)&m->wpart, 4 * (size_t)kWgSplits * kH * kH);
   if (e != hipSuccess) {
     delete m;
     set_error("poyg_mlp_create: hipMalloc failed: %s", hipGetErrorString(e));
@@ -455,11 +512,15 @@ int poyg_mlp_grad(poyg_mlp *m, const uint16_t *d_x, const float *d_t, int n, flo
   // dZ1 = (dZ2 W2^T) where H1 > 0: B(k', j) = W2[j][k'] (W2b is [in][out])
   HIP_TRYM(gemm(kEpiMaskBf16, n, kH, kH, m->dz2, kH, 1, m->w2b, 1, kH, nullptr, m->h1, kH, m->dz1, kH, s));
   // dW2[k][j] = sum_m H1[m][k] dZ2[m][j]; dW1[k][j] = sum_m X[m][k] dZ1[m][j]
-  HIP_TRYM(gemm(kEpiF32, kH, kH, n, m->h1, 1, kH, m->dz2, kH, 1, nullptr, nullptr, 0, g + MlpLayout::W2, kH, s));
-  HIP_TRYM(gemm(kEpiF32, kH, kH, n, d_x, 1, kH, m->dz1, kH, 1, nullptr, nullptr, 0, g + MlpLayout::W1, kH, s));
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, s, n, m->dz2, nullptr, g + MlpLayout::B2);
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, s, n, m->dz1, nullptr, g + MlpLayout::B1);
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, s, n, m->h2, m->dy, g + MlpLayout::W3);
+  HIP_TRYM(gemm_wgrad(kH, kH, n, m->h1, 1, kH, m->dz2, kH, 1, m->wpart, g + MlpLayout::W2, kH, s));
+  HIP_TRYM(gemm_wgrad(kH, kH, n, d_x, 1, kH, m->dz1, kH, 1, m->wpart, g + MlpLayout::W1, kH, s));
+  float *cp = m->cpart;
+  hipLaunchKernelGGL(colsum_part_kernel, dim3(kColParts), dim3(kH), 0, s, n, m->dz2, nullptr, cp);
+  hipLaunchKernelGGL(colsum_part_kernel, dim3(kColParts), dim3(kH), 0, s, n, m->dz1, nullptr, cp + kColParts * kH);
+  hipLaunchKernelGGL(colsum_part_kernel, dim3(kColParts), dim3(kH), 0, s, n, m->h2, m->dy, cp + 2 * kColParts * kH);
+  hipLaunchKernelGGL(colsum_fin_kernel, dim3(1), dim3(kH), 0, s, cp, g + MlpLayout::B2);
+  hipLaunchKernelGGL(colsum_fin_kernel, dim3(1), dim3(kH), 0, s, cp + kColParts * kH, g + MlpLayout::B1);
+  hipLaunchKernelGGL(colsum_fin_kernel, dim3(1), dim3(kH), 0, s, cp + 2 * kColParts * kH, g + MlpLayout::W3);
   hipLaunchKernelGGL(vsum_kernel, dim3(1), dim3(256), 0, s, n, m->dy, g + MlpLayout::B3);
   HIP_TRYM(hipGetLastError());
   if (loss) {
