@@ -166,6 +166,16 @@ int poyg_algn_align_3d(poyg_ctx *ctx, const poyg_cm3 *cm3, const uint8_t *s1, in
  * failure, when a character past the leading gap has no base bit. */
 int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, int lb, const uint8_t *c, int lc,
                    int mm, int go, int ge, uint8_t *ra, uint8_t *rb, uint8_t *rc, int cap, int *rlen, int *cost);
+/* The direction matrix that the readers of Matrix.default see after
+ * cost_2 / create_edited_2 on (s1, s2) (mat_CAML_get_value,
+ * src/matrices.c:236-249; algn_CAML_create_backtrack, src/algn.c:4301-4328;
+ * their callers src/charTransform.ml:446-447, src/splitting.ml:225): dm
+ * (host, cap >= l1 * l2 entries) receives row i, column j at i * l2 + j --
+ * ALIGN 1 | INSERT 2 | DELETE 4 inside the band algn_fill_plane_2 fills, 0
+ * outside it (where the reference's matrix holds whatever the previous fill
+ * left).  s1 at least as long as s2, linear matrices; *cost as simple_2. */
+int poyg_algn_direction_matrix(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int l1, const uint8_t *s2,
+                               int l2, int deltawh, uint16_t *dm, int64_t cap, int *cost);
 /* algn_CAML_median_2_no_gaps / _with_gaps / ancestor_2 / union / median_3
  * (src/algn.c:4186-4235, 4287-4299) on already aligned sequences. */
 int poyg_algn_median_2_no_gaps(const poyg_cm *cm, const uint8_t *a, const uint8_t *b, int len,

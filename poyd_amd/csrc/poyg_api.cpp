@@ -38,6 +38,10 @@ hipError_t launch_lin_sys(bool with_dm, int grid, const PairMeta *meta, const in
                           const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, const int *aux, int32_t *cost,
                           int32_t *status, int spin_limit, hipStream_t s);
 int lin_sys_blocks_per_cu(bool with_dm);
+int lin_cost16_blocks_per_cu(bool with_dm, int n_max);   // resident waves per CU
+hipError_t launch_lin_cost16(bool with_dm, int grid_waves, const PairMeta *meta, const int32_t *order, int n,
+                             int *counter, const uint8_t *seq, const DevTcm *tcm, const int *aux, int32_t *cost,
+                             uint8_t *dm, int n_max, hipStream_t s);
 hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int *aux,
                            int32_t *cost, int *rowscratch, int row_width, hipStream_t s);
@@ -77,14 +81,19 @@ using namespace poyg;
   } while (0)
 
 // Launch-shape knobs a tuner may set per context (-1: the built-in choice).
-enum TuneKnob : int { kTuneQuadMaxLanes, kTuneSideDiv, kTuneSysBlocks, kTuneTraceBlocks, kTuneCubeWaves, kTuneKnobs };
+// lin_cost_kernel: 1 = the 32-bit systolic fill for every linear pair that
+// fits it, 2 = the packed 16-bit fill wherever a pair allows it (built-in).
+enum TuneKnob : int {
+  kTuneQuadMaxLanes, kTuneSideDiv, kTuneSysBlocks, kTuneTraceBlocks, kTuneCubeWaves, kTuneLinCost, kTuneKnobs
+};
 static const char *const kTuneNames[kTuneKnobs] = {"aff_quad_max_lanes", "aff_side_div", "lin_sys_blocks_per_cu",
-                                                   "lin_trace_blocks_per_cu", "cube_waves_per_cu"};
+                                                   "lin_trace_blocks_per_cu", "cube_waves_per_cu",
+                                                   "lin_cost_kernel"};
 
 struct poyg_ctx {
   int device = 0;
   int cus = 256;
-  int tune[kTuneKnobs] = {-1, -1, -1, -1, -1};
+  int tune[kTuneKnobs] = {-1, -1, -1, -1, -1, -1};
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;    // second stream: fills beside the main one
   hipStream_t side2 = nullptr;   // third stream: the widest bands (lane sets, row kernels)
@@ -242,9 +251,10 @@ struct poyg_plan {
   // (written by a scan before the fills)
   // (linear pairs: the systolic kernel takes every pair whose column
   // sequence fits its LDS row, the row kernels the rest)
-  DevBuf tcm, meta, order_all, order_lds, order_glb, order_sys, order_quad, order_diag[3], order_rows, dm,
-      aux, rows, counters, status, sink, pure, trout, blob;
+  DevBuf tcm, meta, order_all, order_lds, order_glb, order_sys, order_c16, order_quad, order_diag[3], order_rows,
+      dm, aux, rows, counters, status, sink, pure, trout, blob;
   int n_lds = 0, n_glb = 0, n_sys = 0, n_quad = 0, n_diag[3] = {0, 0, 0}, n_rows = 0;
+  int n_c16 = 0, grid_c16 = 0, c16_nmax = 0;   // cost-only linear pairs of the packed 16-bit fill
   int grid_lds = 0, grid_glb = 0, grid_sys = 0, grid_quad = 0, grid_diag[3] = {0, 0, 0},
       row_width = 0;
   // the fill with the most cells runs on the main stream ("main": 1 the
@@ -260,7 +270,7 @@ struct poyg_plan {
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;   // hand-off to and from the side streams
   ~poyg_plan() {
     if (triple) poyg_triple_plan_destroy(triple);
-    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_sys, &order_quad, &order_diag[0],
+    for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_sys, &order_c16, &order_quad, &order_diag[0],
                       &order_diag[1], &order_diag[2], &order_rows, &dm, &aux, &rows, &counters, &status,
                       &sink, &pure, &trout, &blob})
       b->release();
@@ -361,7 +371,7 @@ int poyg_ctx_set_tuning(poyg_ctx *ctx, const char *knob, int value) {
     set_error("poyg_ctx_set_tuning: unknown knob %s", knob ? knob : "(null)");
     return POYG_EINVAL;
   }
-  static const int lo[kTuneKnobs] = {41, 2, 1, 1, 1}, hi[kTuneKnobs] = {48, 64, 64, 64, 64};
+  static const int lo[kTuneKnobs] = {41, 2, 1, 1, 1, 1}, hi[kTuneKnobs] = {48, 64, 64, 64, 64, 2};
   if (value >= 0 && (value < lo[k] || value > hi[k])) {
     set_error("poyg_ctx_set_tuning: %s = %d outside %d..%d", knob, value, lo[k], hi[k]);
     return POYG_EINVAL;
@@ -496,7 +506,22 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
                                                  : lin_trace_blocks_per_cu();
   std::vector<PairMeta> meta(n);
   std::vector<int64_t> cells(n);
-  std::vector<char> glb(n, 0), diag(n, 0), quad(n, 0), sys(n, 0);
+  std::vector<char> glb(n, 0), diag(n, 0), quad(n, 0), sys(n, 0), c16(n, 0);
+  DevTcm dt;
+  to_dev_tcm(cm, &dt);
+  // the packed 16-bit fill: linear plans whose tail costs equal the
+  // deletion costs (its last and first columns need no rule of their own
+  // then); per pair, every cost below 0xffff
+  int64_t c16_cmax = 0;
+  bool c16_matrix = !affine && ctx->tune[kTuneLinCost] != 1;
+  for (int x = 0; x < 32; ++x) {
+    c16_matrix = c16_matrix && dt.tail[x] == dt.cost[x * 32 + kGap] && dt.prepend[x] >= 0 && dt.tail[x] >= 0;
+    c16_cmax = std::max<int64_t>(c16_cmax, std::max(dt.prepend[x], dt.tail[x]));
+  }
+  for (int x = 0; x < 32 * 32; ++x) {
+    c16_matrix = c16_matrix && dt.cost[x] >= 0;
+    c16_cmax = std::max<int64_t>(c16_cmax, dt.cost[x]);
+  }
   // tuned launch shapes (see poyg_ctx_set_tuning)
   const int quad_max = ctx->tune[kTuneQuadMaxLanes] > 0 ? ctx->tune[kTuneQuadMaxLanes] : 16 * kQuadH;
   const int side_div = ctx->tune[kTuneSideDiv] > 0 ? ctx->tune[kTuneSideDiv] : 4;
@@ -591,7 +616,11 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
         m.dm_off = dm_bytes;
         dm_bytes += (int64_t)m.lr * m.stride;
       }
-      if (m.lc <= kLinStageHost) {
+      if (c16_matrix && m.lc >= 2 && m.lc <= kLinStageHost &&
+          ((int64_t)m.lr + m.lc + 2) * std::max<int64_t>(c16_cmax, 1) < 0xfff0) {
+        c16[k] = 1;
+        m.p3 = 1;
+      } else if (m.lc <= kLinStageHost) {
         sys[k] = 1;
       } else if (maxw > kLinRBHost) {
         glb[k] = 1;
@@ -605,7 +634,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells[a] > cells[b]; });
-  std::vector<int32_t> o_lds, o_glb, o_sys, o_quad, o_diag[3], o_rows;
+  std::vector<int32_t> o_lds, o_glb, o_sys, o_c16, o_quad, o_diag[3], o_rows;
   int64_t c_quad3 = 0, c_diag1 = 0;
   for (int k : order) {
     if (quad[k]) {
@@ -614,6 +643,9 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     } else if (diag[k]) {
       o_diag[diag[k] == 1 ? 0 : (diag[k] == 2 ? 1 : 2)].push_back(k);
       if (diag[k] == 1) c_diag1 += cells[k];
+    } else if (c16[k]) {
+      o_c16.push_back(k);
+      P->c16_nmax = std::max(P->c16_nmax, (int)meta[k].lc);
     } else {
       (sys[k] ? o_sys : glb[k] ? o_glb : o_lds).push_back(k);
       o_rows.push_back(k);
@@ -638,6 +670,9 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
   P->grid_glb = std::max(1, std::min(P->n_glb, 256));
   if (P->n_sys) P->grid_sys = std::max(1, std::min(P->n_sys, ctx->cus * sys_bpc));
+  P->n_c16 = (int)o_c16.size();
+  if (P->n_c16)   // one pair per wave; grid_c16 counts waves
+    P->grid_c16 = std::max(1, std::min(P->n_c16, ctx->cus * lin_cost16_blocks_per_cu(align, P->c16_nmax)));
   if (affine) {
     // The fill with the most cells is the main kernel.  The side stream's
     // kernels are persistent too; the main kernel leaves them whole CUs, or
@@ -666,8 +701,6 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       P->grid_help2 = side2 ? side2 * bpc : 0;
     }
   }
-  DevTcm dt;
-  to_dev_tcm(cm, &dt);
   if (!affine) {
     // The linear kernels mark cells outside the band with 2^30 - 1 and add
     // costs to it: every entry must be non-negative and the most costly
@@ -722,6 +755,7 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
       {&P->order_lds, {o_lds.data(), sizeof(int32_t) * o_lds.size()}},
       {&P->order_glb, {o_glb.data(), sizeof(int32_t) * o_glb.size()}},
       {&P->order_sys, {o_sys.data(), sizeof(int32_t) * o_sys.size()}},
+      {&P->order_c16, {o_c16.data(), sizeof(int32_t) * o_c16.size()}},
       {&P->order_quad, {o_quad.data(), sizeof(int32_t) * o_quad.size()}},
       {&P->order_diag[0], {o_diag[0].data(), sizeof(int32_t) * o_diag[0].size()}},
       {&P->order_diag[1], {o_diag[1].data(), sizeof(int32_t) * o_diag[1].size()}},
@@ -862,6 +896,9 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
       HIP_TRY(hipStreamWaitEvent(s, P->join2, 0));
     }
   } else {
+    if (P->n_c16)
+      HIP_TRY(launch_lin_cost16(align, P->grid_c16, meta, (const int32_t *)P->order_c16.p, P->n_c16, ctr + 4, d_seq,
+                                tcm, (const int *)P->aux.p, d_cost, (uint8_t *)P->dm.p, P->c16_nmax, s));
     if (P->n_sys)
       HIP_TRY(launch_lin_sys(align, P->grid_sys, meta, (const int32_t *)P->order_sys.p, P->n_sys, ctr + 2, d_seq,
                              tcm, (uint8_t *)P->dm.p, (const int *)P->aux.p, d_cost, (int32_t *)P->status.p,
@@ -984,6 +1021,50 @@ int poyg_algn_simple_2(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int 
   int rc = op.run(ctx, cm, POYG_MODE_LINEAR_COST, s1, l1, s2, l2, deltawh);
   if (!rc && cost) *cost = op.h_cost;
   return rc;
+}
+
+// The direction matrix of algn_fill_plane_2 as Matrix.default holds it
+// after cost_2 (the reader stubs mat_CAML_get_value, src/matrices.c:236-249,
+// and algn_CAML_create_backtrack, src/algn.c:4301-4328, index it as
+// row * l2 + column): the band that the pair's fill wrote, un-phased into
+// full rows; 0 outside the band (the reference leaves those cells as the
+// previous fill left them).
+int poyg_algn_direction_matrix(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int l1, const uint8_t *s2,
+                               int l2, int deltawh, uint16_t *dm, int64_t cap, int *cost) {
+  if (l1 < l2 || l2 < 1 || !dm || cap < (int64_t)l1 * l2) {
+    set_error("poyg_algn_direction_matrix: s1 must be at least as long as s2, dm must hold l1 * l2 entries");
+    return POYG_EINVAL;
+  }
+  OnePair op;
+  int rc = op.run(ctx, cm, POYG_MODE_LINEAR_ALIGN, s1, l1, s2, l2, deltawh);
+  if (rc) return rc;
+  poyg_plan *P = op.plan;
+  const LinGeom g = lin_geom(l1, l2, deltawh);
+  int maxw = 0;
+  int64_t cells = 0;
+  lin_band_extent(l1, l2, g, maxw, cells);
+  const int S = lin_dm_stride(maxw);
+  std::vector<uint8_t> band((size_t)l1 * S);
+  HIP_TRY(hipMemcpy(band.data(), P->dm.p, band.size(), hipMemcpyDeviceToHost));
+  // rows of the systolic fill start at phase (L + lo - lo_first) mod 4,
+  // L the row's place in its strip (64 rows; 128 in the packed fill); the
+  // row kernels (s2 longer than 2,048) at phase 0
+  const int strip = P->n_c16 ? 128 : 64;
+  memset(dm, 0, sizeof(uint16_t) * (size_t)l1 * l2);
+  for (int i = 0; i < l1; ++i) {
+    int lo, hi;
+    lin_row_host(i, l1, l2, g, lo, hi);
+    int phase = 0;
+    if (i >= 1 && l2 <= kLinStageHost) {
+      const int L = (i - 1) % strip;
+      int slo, shi;
+      lin_row_host(i - L, l1, l2, g, slo, shi);
+      phase = (L + lo - slo) & 3;
+    }
+    for (int j = lo; j <= hi; ++j) dm[(size_t)i * l2 + j] = band[(size_t)i * S + phase + (j - lo)];
+  }
+  if (cost) *cost = op.h_cost;
+  return POYG_OK;
 }
 
 int poyg_algn_align_2d(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int l1, const uint8_t *s2, int l2,

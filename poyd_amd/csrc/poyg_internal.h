@@ -43,6 +43,7 @@ struct PairMeta {
   int32_t cap;       // capacity of one output sequence
   int32_t swap;      // 1 when the rows sequence is the caller's second one
   int32_t p0, p1, p2;  // mode specific band parameters
+  int32_t p3;          // linear: 1 when the packed 16-bit fill (128-row strips) fills the pair
 };
 
 // Outputs of the backtrace that the affine fill kernels run on each pair

@@ -19,6 +19,7 @@
 
 #include <type_traits>
 
+#include "poyg_band.h"
 #include "poyg_internal.h"
 
 namespace poyg {
@@ -131,6 +132,7 @@ __device__ __forceinline__ PairMeta uniform_meta(const PairMeta *meta, int p) {
   m.p0 = uniform32(m.p0);
   m.p1 = uniform32(m.p1);
   m.p2 = uniform32(m.p2);
+  m.p3 = uniform32(m.p3);
   return m;
 }
 
@@ -2137,6 +2139,233 @@ __global__ __launch_bounds__(kWave * kSysWaves) void lin_sys_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Linear band, 16-bit (algn_fill_plane_2, src/algn.c:872-968: the cost of
+// Sequence.Align.cost_2, and with kDm the direction band that
+// backtrack_2d reads)
+// ---------------------------------------------------------------------------
+// One wave per pair, two rows per lane in the two 16-bit halves of a
+// register, so that every packed add / min / saturation handles two cells.
+// A strip of 128 rows runs as an anti-diagonal wavefront: at step t lane L
+// holds row i0 + 2L at column c = c0 + t - 2L (low half) and row i0 + 2L + 1
+// at column c - 1 (high half).  A cell's neighbours are then all of the
+// previous two steps:
+//   up    low: lane L-1's high half (DPP wave_shr:1; lane 0 reads the
+//              previous strip's last row from LDS); high: own low half
+//   left  both halves of the own previous value
+//   diag  the previous step's `up`
+// Cells outside a row's band hold 0xffff, and every candidate is a
+// saturating add, so an out-of-band neighbour never wins: that stands in
+// for the reference's explicit band-edge cases (no INSERT at a row's first
+// cell, no DELETE at its last cell of a right-open row, the first-column
+// cells).  The last column's tail rule and the first column's tail rule
+// reduce to the plain DELETE when tail[x] == cost[x][gap] for every x: the
+// plan sends a pair here only then, only when every cost of the pair stays
+// below 0xffff, and only when its shorter sequence has 2 .. kLinStage
+// positions.  The lane (and half) holding a strip's last row stores it in
+// LDS for the next strip; the pair's cost is that row's entry at column
+// n - 1 after the last strip.
+//
+// Direction codes (kDm): a candidate's bit (ALIGN 1, INSERT 2, DELETE 4) is
+// set when it equals the cell's minimum -- with the 0xffff neighbours the
+// excluded moves never do, so the codes equal the reference's, the first
+// column's forced DELETE and the last column's tail rule included (tail ==
+// deletion cost).  Row r's bytes start at phase (r - i0 + lo_r - lo_i0)
+// mod 4 of its stride (i0 the first row of r's 128-row strip), so that
+// byte (c - lo_r) lies at step t mod 4 of its dword: every lane fills two
+// dwords (its two rows) in step and stores them every fourth step.
+// PairMeta::p3 = 1 tells the backtrace this phase.
+constexpr int kC16Waves = 4;    // waves per workgroup, one pair each
+constexpr int kC16Pad = 192;    // LDS columns kept before column 0 and after the last one
+constexpr int kC16U = 8;        // steps per unrolled group
+constexpr int kC16Rows = 2 * kWave;
+
+typedef unsigned short c16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ c16x2 c16v(uint32_t x) { return __builtin_bit_cast(c16x2, x); }
+__device__ __forceinline__ uint32_t c16u(c16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ uint32_t c16_add_sat(uint32_t a, uint32_t b) {
+  return c16u(__builtin_elementwise_add_sat(c16v(a), c16v(b)));
+}
+__device__ __forceinline__ uint32_t c16_sub_sat(uint32_t a, uint32_t b) {
+  return c16u(__builtin_elementwise_sub_sat(c16v(a), c16v(b)));
+}
+__device__ __forceinline__ uint32_t c16_min(uint32_t a, uint32_t b) {
+  return c16u(__builtin_elementwise_min(c16v(a), c16v(b)));
+}
+__device__ __forceinline__ uint32_t c16_add(uint32_t a, uint32_t b) { return c16u(c16v(a) + c16v(b)); }
+// per half: b where g == 0, 0xffff where g > 0 (g * 0xffff + b, saturated)
+__device__ __forceinline__ uint32_t c16_gate(uint32_t g, uint32_t b) {
+  uint32_t r;
+  asm("v_pk_mad_u16 %0, %1, -1, %2 clamp" : "=v"(r) : "v"(g), "v"(b));
+  return r;
+}
+
+// LDS words per wave for pairs whose shorter sequence has at most n_max
+// positions: column words, the hand-off row, 64 lanes x kC16U dummy slots.
+__host__ __device__ __forceinline__ int c16_cols(int n_max) { return (n_max + 2 * kC16Pad + 63) & ~63; }
+__host__ __device__ __forceinline__ int c16_wave_words(int n_max) { return 2 * c16_cols(n_max) + kWave * kC16U; }
+
+template <bool kDm>
+__global__ __launch_bounds__(kWave * kC16Waves) void lin_cost16_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, const int *__restrict__ aux,
+    int32_t *__restrict__ cost_out, uint8_t *dm_base, int n_max) {
+  __shared__ uint16_t T[32 * 32];
+  extern __shared__ uint32_t c16_dyn[];
+  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) T[k] = (uint16_t)min(tcm->cost[k], 0xffff);
+  __syncthreads();
+  const int lane = lane_id();
+  const int cols = c16_cols(n_max);
+  uint32_t *const base = c16_dyn + (size_t)wave_in_group() * c16_wave_words(n_max);
+  uint32_t *const Wc = base + kC16Pad;          // column c: gcost[x] | 2x << 16 (x = s2[c] & 31)
+  uint32_t *const R = base + cols + kC16Pad;    // hand-off row: value of column c in the high half
+  uint32_t *const D = base + 2 * cols + lane * kC16U;   // this lane's dummy slots
+  for (;;) {
+    const int k = next_item(counter);
+    if (k >= n) break;
+    const int p = uniform32(order[k]);
+    const PairMeta m = uniform_meta(meta, p);
+    const uint8_t *s1 = seq + m.off_r;   // the longer sequence: rows
+    const uint8_t *s2 = seq + m.off_c;
+    const int M = m.lr, N = m.lc;
+    const LinGeom g{m.p0, m.p1, m.p2};
+    const int *GP = aux + m.aux_off + N;
+    int lo0, hi0;
+    lin_row_host(0, M, N, g, lo0, hi0);
+    // column words and row 0 (algn_fill_first_row, :584-607, or the first
+    // row of algn_fill_plane); 0xffff outside the band
+    for (int c = lane - kC16Pad; c < N + kC16Pad; c += kWave) {
+      const int x = (c >= 0 && c < N) ? (s2[c] & 31) : 0;
+      Wc[c] = (uint32_t)T[kGap * 32 + x] | ((uint32_t)(2 * x) << 16);
+      R[c] = (c >= 0 && c <= hi0) ? ((uint32_t)GP[c] << 16) | 0xffffu : 0xffffffffu;
+    }
+    uint8_t *dm = dm_base + m.dm_off;
+    const int S = m.stride;
+    if (kDm)
+      for (int j = lane; j <= hi0; j += kWave) dm[j] = j == 0 ? 1 : 2;   // row 0: ALIGN, then INSERT
+    const __amdgpu_buffer_rsrc_t drsrc = __builtin_amdgcn_make_buffer_rsrc(dm, 0, kDm ? M * S : 0, 0x00020000);
+    wave_sync_mem();
+    int last_half = 1;
+    const int nstrips = (M - 1 + kC16Rows - 1) / kC16Rows;
+    for (int s = 0; s < nstrips; ++s) {
+      const int i0 = 1 + kC16Rows * s;
+      int c0, hi_first, lo_l, hi_l;
+      lin_row_host(i0, M, N, g, c0, hi_first);
+      const int r_last = min(i0 + kC16Rows - 1, M - 1);
+      lin_row_host(r_last, M, N, g, lo_l, hi_l);
+      const int wl = (r_last - i0) >> 1, wh = (r_last - i0) & 1;   // lane and half of the last row
+      const int t_end = hi_l - c0 + 2 * wl + wh;
+      // one step more (the cell right of the band), rounded to two groups
+      const int steps = (t_end + 2 + 2 * kC16U - 1) / (2 * kC16U) * (2 * kC16U);
+      // this lane's rows
+      const int ra = i0 + 2 * lane, rb = ra + 1;
+      const bool va = ra < M, vb = rb < M;
+      int lo_a, hi_a, lo_b, hi_b;
+      lin_row_host(va ? ra : M - 1, M, N, g, lo_a, hi_a);
+      lin_row_host(vb ? rb : M - 1, M, N, g, lo_b, hi_b);
+      const int sa = va ? (s1[ra] & 31) : 0, sb = vb ? (s1[rb] & 31) : 0;
+      const uint32_t cdel = (uint32_t)T[sa * 32 + kGap] | ((uint32_t)T[sb * 32 + kGap] << 16);
+      const uint16_t *ta = T + sa * 32, *tb = T + sb * 32;
+      // column - lo of the step's cell (rows past the last one: never inside)
+      uint32_t kv = (va ? (uint32_t)(c0 - 2 * lane - lo_a) & 0xffffu : 0x8000u) |
+                    ((vb ? (uint32_t)(c0 - 2 * lane - 1 - lo_b) & 0xffffu : 0x8000u) << 16);
+      const uint32_t wid = (va ? (uint32_t)(hi_a - lo_a) : 0u) | ((vb ? (uint32_t)(hi_b - lo_b) : 0u) << 16);
+      const bool writer = lane == wl;
+      uint32_t *wp = writer ? R + (c0 - 2 * wl - wh) : D;
+      const int winc = writer ? kC16U : 0;
+      const uint32_t *wc = Wc + (c0 - 2 * lane);
+      const uint32_t *rr = R + c0;
+      // byte offsets of the dwords that steps 0..3 fill (rows past the last
+      // one store nothing: no byte of theirs is ever inside)
+      int oa = 0, ob = 0;
+      if (kDm) {
+        oa = ra * S + ((lo_a - c0 + 2 * lane) & 3) + (c0 - 2 * lane - lo_a);
+        ob = rb * S + ((lo_b - c0 + 2 * lane + 1) & 3) + (c0 - 2 * lane - 1 - lo_b);
+      }
+      uint32_t acc_a = 0, acc_b = 0;
+      uint32_t V = 0xffffffffu;
+      uint32_t upp = (R[c0 - 1] >> 16) | 0xffff0000u;   // lane 0's first diagonal; the rest lie outside
+      uint32_t wprev = wc[-1];
+      // A group of kC16U steps reads its substitution costs at its start;
+      // column words and lane 0's hand-off words are read one group ahead,
+      // into two register sets that alternate (no copies between groups).
+      uint32_t wA[kC16U], fA[kC16U], wB[kC16U], fB[kC16U];
+      auto load_wf = [&](uint32_t (&w)[kC16U], uint32_t (&f)[kC16U], int off) {
+#pragma unroll
+        for (int u = 0; u < kC16U; ++u) {
+          w[u] = wc[off + u];
+          f[u] = rr[off + u];
+        }
+      };
+      auto group = [&](const uint32_t (&w)[kC16U], const uint32_t (&f)[kC16U]) {
+        uint32_t cr[kC16U];
+#pragma unroll
+        for (int u = 0; u < kC16U; ++u) {
+          const uint32_t wq = u ? w[u - 1] : wprev;
+          cr[u] = (uint32_t)*(const uint16_t *)((const char *)ta + (w[u] >> 16)) |
+                  ((uint32_t)*(const uint16_t *)((const char *)tb + (wq >> 16)) << 16);
+        }
+#pragma unroll
+        for (int u = 0; u < kC16U; ++u) {
+          const uint32_t wq = u ? w[u - 1] : wprev;
+          const uint32_t sh = (uint32_t)__builtin_amdgcn_update_dpp((int)f[u], (int)V, 0x138 /* wave_shr:1 */, 0xf,
+                                                                    0xf, false);
+          const uint32_t up = __builtin_amdgcn_alignbit(V, sh, 16);
+          const uint32_t gj = __builtin_amdgcn_perm(wq, w[u], 0x05040100u);
+          const uint32_t t1 = c16_add_sat(up, cdel);    // DELETE
+          const uint32_t t2 = c16_add_sat(V, gj);       // INSERT
+          const uint32_t t3 = c16_add_sat(upp, cr[u]);  // ALIGN
+          const uint32_t best = c16_min(c16_min(t1, t2), t3);
+          const uint32_t gt = c16_sub_sat(kv, wid);
+          V = c16_gate(gt, best);
+          kv = c16_add(kv, 0x00010001u);
+          wp[u] = V;
+          upp = up;
+          if (kDm) {
+            // per half: 1 for a candidate equal to the minimum inside the band
+            const uint32_t ib = c16_sub_sat(0x00010001u, gt);
+            const uint32_t q1 = c16_sub_sat(ib, c16_sub_sat(t1, best));
+            const uint32_t q2 = c16_sub_sat(ib, c16_sub_sat(t2, best));
+            const uint32_t q3 = c16_sub_sat(ib, c16_sub_sat(t3, best));
+            const uint32_t msk = c16_add(c16_add(q3, c16_add(q2, q2)), c16u(c16v(q1) << (unsigned short)2));
+            // byte u mod 4 of the two dwords (the low half's code is byte 0
+            // of msk, the high half's byte 2); position 0 starts a dword
+            constexpr uint32_t sel_a[4] = {0x0c0c0c00u, 0x0c0c0004u, 0x0c000504u, 0x00060504u};
+            constexpr uint32_t sel_b[4] = {0x0c0c0c02u, 0x0c0c0204u, 0x0c020504u, 0x02060504u};
+            acc_a = __builtin_amdgcn_perm(acc_a, msk, sel_a[u & 3]);
+            acc_b = __builtin_amdgcn_perm(acc_b, msk, sel_b[u & 3]);
+            if ((u & 3) == 3) {
+              __builtin_amdgcn_raw_buffer_store_b32(acc_a, drsrc, acc_a ? oa : kSysDrop, 0, 0);
+              __builtin_amdgcn_raw_buffer_store_b32(acc_b, drsrc, acc_b ? ob : kSysDrop, 0, 0);
+              oa += 4;
+              ob += 4;
+            }
+          }
+        }
+        wprev = w[kC16U - 1];
+        wp += winc;
+      };
+      load_wf(wA, fA, 0);
+      dpp_exec_guard();
+      for (int t0 = 0; t0 < steps; t0 += 2 * kC16U) {
+        load_wf(wB, fB, kC16U);
+        group(wA, fA);
+        load_wf(wA, fA, 2 * kC16U);
+        group(wB, fB);
+        wc += 2 * kC16U;
+        rr += 2 * kC16U;
+      }
+      last_half = wh;
+      wave_sync_mem();
+    }
+    if (lane == 0) {
+      const uint32_t e = R[N - 1];
+      cost_out[p] = M == 1 ? GP[N - 1] : (int32_t)(last_half ? e >> 16 : e & 0xffffu);
+    }
+    wave_sync_mem();
+  }
+}
+
 template <bool kDm>
 __global__ __launch_bounds__(kWave * kLinWaves) void lin_fill_lds_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
@@ -2301,7 +2530,8 @@ __global__ __launch_bounds__(kWave * 4) void lin_trace_wave_kernel(
         // its strip and c0 the strip's first column
         int phase = 0;
         if (N <= kLinStage) {
-          const int L = (max(r, 1) - 1) & (kWave - 1);
+          // strips of 64 rows (systolic fill) or 128 (packed fill, p3 = 1)
+          const int L = (max(r, 1) - 1) & (m.p3 ? 2 * kWave - 1 : kWave - 1);
           int lk, rk, slo, shi;
           lin_row(max(r, 1) - L, M, N, cs, w, h, slo, shi, lk, rk);
           phase = r >= 1 ? (L + rlo - slo) & 3 : 0;
@@ -2666,6 +2896,49 @@ hipError_t launch_lin_sys(bool with_dm, int grid, const PairMeta *meta, const in
   else
     hipLaunchKernelGGL(lin_sys_kernel<false>, dim3(grid), dim3(kWave * kSysWaves), 0, s, meta, order, n, counter,
                        seq, tcm, dm, aux, cost, status, spin_limit);
+  return hipGetLastError();
+}
+
+// Waves per workgroup of the packed cost fill: as many as the per-workgroup
+// LDS limit allows (at most kC16Waves), so that long columns still launch.
+static int c16_waves(int n_max) {
+  static int limit = -1;
+  if (limit < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      v = 65536;
+    limit = v;
+  }
+  const int per_wave = 4 * c16_wave_words(n_max);
+  const int avail = limit - (int)(sizeof(uint16_t) * 32 * 32);
+  return std::max(1, std::min(kC16Waves, avail / per_wave));
+}
+
+int lin_cost16_blocks_per_cu(bool with_dm, int n_max) {
+  const int waves = c16_waves(n_max);
+  int b = 0;
+  const hipError_t e =
+      with_dm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_cost16_kernel<true>, kWave * waves,
+                                                             4 * waves * c16_wave_words(n_max))
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_cost16_kernel<false>, kWave * waves,
+                                                             4 * waves * c16_wave_words(n_max));
+  if (e != hipSuccess) b = 1;
+  return std::max(1, b) * waves;   // in waves
+}
+
+hipError_t launch_lin_cost16(bool with_dm, int grid_waves, const PairMeta *meta, const int32_t *order, int n,
+                             int *counter, const uint8_t *seq, const DevTcm *tcm, const int *aux, int32_t *cost,
+                             uint8_t *dm, int n_max, hipStream_t s) {
+  const int waves = c16_waves(n_max);
+  const int grid = std::max(1, (grid_waves + waves - 1) / waves);
+  const int lds = 4 * waves * c16_wave_words(n_max);
+  if (with_dm)
+    hipLaunchKernelGGL(lin_cost16_kernel<true>, dim3(grid), dim3(kWave * waves), lds, s, meta, order, n, counter, seq,
+                       tcm, aux, cost, dm, n_max);
+  else
+    hipLaunchKernelGGL(lin_cost16_kernel<false>, dim3(grid), dim3(kWave * waves), lds, s, meta, order, n, counter,
+                       seq, tcm, aux, cost, dm, n_max);
   return hipGetLastError();
 }
 

@@ -122,6 +122,7 @@ SIGNATURES = {
     "poyg_union_make": (C.c_int, [vp, u8p, u8p, u8p, C.c_int, u8p, i32p, C.c_int, u8p, i32p, C.c_int, u8p, i32p,
                                   i32p, i32p, C.c_int, i32p]),
     "poyg_union_merge": (C.c_int, [vp, vp, C.c_int32, vp, vp, vp]),
+    "poyg_algn_direction_matrix": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, C.c_int, vp, C.c_int64, i32p]),
     "poyg_mlp_param_count": (C.c_int64, []),
     "poyg_mlp_create": (C.c_int, [vp, C.c_int, C.c_uint64, C.POINTER(vp)]),
     "poyg_mlp_destroy": (None, [vp]),
@@ -430,6 +431,17 @@ class Align:
                                                C.byref(cost)), "align_affine_3")
         med, mwg, ri, rj = (_tail(bufs[k], int(lens[k])) for k in range(4))
         return med, ri, rj, cost.value, mwg
+
+    def direction_matrix(self, s1, s2, cm: CostMatrix, deltawh: int):
+        """(cost, l1 x l2 uint16 direction matrix) of algn_fill_plane_2 on
+        (s1 longer or equal, s2) with band parameter deltawh, as the readers
+        of Matrix.default see it (0 outside the band)."""
+        a, b = _u8(s1), _u8(s2)
+        dm = np.zeros(len(a) * len(b), np.uint16)
+        cost = C.c_int32(0)
+        _check(load().poyg_algn_direction_matrix(self.ctx.h, cm.h, _p8(a), len(a), _p8(b), len(b), int(deltawh),
+                                                 dm.ctypes.data, dm.size, C.byref(cost)), "algn_direction_matrix")
+        return cost.value, dm.reshape(len(a), len(b))
 
     def median_2(self, a, b, cm: CostMatrix):
         x, y = _u8(a), _u8(b)

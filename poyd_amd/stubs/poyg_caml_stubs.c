@@ -446,6 +446,62 @@ value powell_3D_align_bc(value *argv, int argn) {
     return powell_3D_align(argv[0], argv[1], argv[2], argv[3], argv[4], argv[5], argv[6], argv[7], argv[8]);
 }
 
+/* ---- sequence unions (src/union.c) ---------------------------------------
+ * Sequence.Unions.u keeps its sequence and three UNION_OFFT bigarrays
+ * (offset, union_c1, union_c2; int16 by default) right-aligned in their
+ * capacity (union_CAML_unwrap, src/union.c:189-207).  The engine reads the
+ * children left-aligned in int32 and writes the result right-aligned; the
+ * stub widens and narrows. */
+#include <caml/bigarray.h>
+#include "union.h"   /* POY's: UNION_OFFT */
+
+static int32_t *union_offsets_i32(value u, int len) {
+    const UNION_OFFT *off = (const UNION_OFFT *)Caml_ba_data_val(Field(u, 1));
+    int cap = (int)Caml_ba_array_val(Field(u, 1))->dim[0];
+    int32_t *w = malloc(sizeof(int32_t) * (len > 0 ? len : 1));
+    for (int k = 0; k < len; ++k) w[k] = off[cap - len + k];
+    return w;
+}
+
+/* union_CAML_make (src/union.c:209-226) */
+value union_CAML_make(value s1, value s2, value smedian, value a, value b, value c, value cm) {
+    CAMLparam5(s1, s2, a, b, c);
+    CAMLxparam2(smedian, cm);
+    seqt x, y, m, ua, ub, uc;
+    Seq_custom_val(x, s1);
+    Seq_custom_val(y, s2);
+    Seq_custom_val(m, smedian);
+    Seq_custom_val(ua, Field(a, 0));
+    Seq_custom_val(ub, Field(b, 0));
+    Seq_custom_val(uc, Field(c, 0));
+    int32_t *ea = union_offsets_i32(a, ua->len), *eb = union_offsets_i32(b, ub->len);
+    int cap = uc->cap, n = 0;
+    uint8_t *items = malloc(cap > 0 ? cap : 1);
+    int32_t *oe = malloc(sizeof(int32_t) * (cap > 0 ? cap : 1));
+    int32_t *o1 = malloc(sizeof(int32_t) * (cap > 0 ? cap : 1));
+    int32_t *o2 = malloc(sizeof(int32_t) * (cap > 0 ? cap : 1));
+    int rc = poyg_union_make(Poyg_cm_val(cm), x->begin, y->begin, m->begin, x->len, ua->begin, ea, ua->len,
+                             ub->begin, eb, ub->len, items, oe, o1, o2, cap, &n);
+    if (rc == POYG_OK) {
+        UNION_OFFT *de = (UNION_OFFT *)Caml_ba_data_val(Field(c, 1));
+        UNION_OFFT *d1 = (UNION_OFFT *)Caml_ba_data_val(Field(c, 2));
+        UNION_OFFT *d2 = (UNION_OFFT *)Caml_ba_data_val(Field(c, 3));
+        put_right_aligned(uc, items, cap, n);
+        for (int k = cap - n; k < cap; ++k) {
+            de[k] = (UNION_OFFT)oe[k];
+            d1[k] = (UNION_OFFT)o1[k];
+            d2[k] = (UNION_OFFT)o2[k];
+        }
+    }
+    free(ea); free(eb); free(items); free(oe); free(o1); free(o2);
+    poyg_fail(rc);
+    CAMLreturn(Val_unit);
+}
+
+value union_CAML_make_b(value *argv, int argn) {
+    return union_CAML_make(argv[0], argv[1], argv[2], argv[3], argv[4], argv[5], argv[6]);
+}
+
 /* ---- Fitch (src/nonaddCSc.c), compiled once per CHARSIZE ---------------
  * The character sets stay in POY's custom block (struct _naca_t,
  * src/nonaddCSc.c:292-300: len, heur, code, added_cost, data in vectors of

@@ -512,9 +512,15 @@ def test_linear_fill_wait_gives_up_reported(ctx, oracle, monkeypatch):
     cm = engine.CostMatrix.of_sub_indel(1, 2)
     ref = oracle.CM2.sub_indel(1, 2)
     monkeypatch.setenv("POYG_SYS_SPIN_LIMIT", "0")
-    # each failed pair counts once, however many of its waves gave up
-    with pytest.raises(engine.PoygError, match=r"\b[1-4] pairs failed"):
-        engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
+    # these pairs would go to the packed fill, which has no waits: the
+    # systolic fill takes them here
+    ctx.set_tuning("lin_cost_kernel", 1)
+    try:
+        # each failed pair counts once, however many of its waves gave up
+        with pytest.raises(engine.PoygError, match=r"\b[1-4] pairs failed"):
+            engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
+    finally:
+        ctx.set_tuning("lin_cost_kernel", -1)
     monkeypatch.delenv("POYG_SYS_SPIN_LIMIT")
     res = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
     for k, (a, b) in enumerate(zip(A, B)):
