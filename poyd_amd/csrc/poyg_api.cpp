@@ -109,20 +109,28 @@ int ctx_cus(poyg_ctx *c) { return c ? c->cus : 1; }
 // power of two and kept: a released block waits in `pending` until a
 // device-wide synchronization (poyg_ctx_sync, or the cache's own once the
 // pending bytes pass kPoolPendingMax), after which no queued work can still
-// use it, and is then handed out again.  Larger blocks are allocated and
-// freed directly.
+// use it, and is then handed out again.  Larger blocks (a batch's
+// direction band or prefix sums: gigabytes for a TBR round's median plans,
+// where a fresh hipMalloc / hipFree pair took longer than the plan's
+// kernels) are rounded up to a size class and kept the same way, up to
+// kBigCachedMax bytes; a request takes the smallest kept block that holds
+// it.
 namespace {
 constexpr int kPoolMinLog = 8, kPoolMaxLog = 26;   // 256 B .. 64 MiB
 constexpr size_t kPoolMax = (size_t)1 << kPoolMaxLog;
 constexpr size_t kPoolPendingMax = (size_t)256 << 20;
 constexpr size_t kPoolCachedMax = (size_t)1 << 30;
+constexpr size_t kBigCachedMax = (size_t)48 << 30;
 constexpr int kPoolDevices = 64;
+constexpr int kBigClass = -2;
 struct DevPool {
   std::mutex mu;
   std::vector<void *> free_[kPoolMaxLog + 1];
   std::vector<std::pair<void *, int>> pending;
-  std::unordered_map<void *, int> cls;   // every block handed out: its class (-1 = not cached)
-  size_t pending_bytes = 0, free_bytes = 0;
+  std::unordered_map<void *, int> cls;   // every block handed out: its class (kBigClass: a large block)
+  std::unordered_map<void *, size_t> big_size;          // large blocks handed out
+  std::vector<std::pair<void *, size_t>> big_pending, big_free;
+  size_t pending_bytes = 0, free_bytes = 0, big_free_bytes = 0;
 };
 DevPool g_pool[kPoolDevices];
 
@@ -151,12 +159,54 @@ void pool_promote(DevPool &P) {
     P.free_bytes += sz;
   }
   P.pending.clear();
+  for (auto &b : P.big_pending) {
+    if (P.big_free_bytes + b.second > kBigCachedMax) {
+      (void)hipFree(b.first);
+      continue;
+    }
+    P.big_free.push_back(b);
+    P.big_free_bytes += b.second;
+  }
+  P.big_pending.clear();
   P.pending_bytes = 0;
 }
 
 hipError_t pool_alloc(void **p, size_t n) {
   DevPool *P = pool_of_current();
   const int c = pool_class(n);
+  if (P && c < 0) {
+    // size classes a quarter of a power of two apart (at most 25% unused),
+    // so that the similar batches of a round share blocks
+    size_t top = kPoolMax;
+    while (top * 2 <= n) top *= 2;
+    const size_t grain = std::max(kPoolMax, top / 4);
+    const size_t want = (n + grain - 1) / grain * grain;
+    std::lock_guard<std::mutex> g(P->mu);
+    int best = -1;
+    for (int k = 0; k < (int)P->big_free.size(); ++k)
+      if (P->big_free[k].second >= want && (best < 0 || P->big_free[k].second < P->big_free[best].second)) best = k;
+    if (best >= 0) {
+      *p = P->big_free[best].first;
+      P->big_size[*p] = P->big_free[best].second;
+      P->big_free_bytes -= P->big_free[best].second;
+      P->big_free.erase(P->big_free.begin() + best);
+      P->cls[*p] = kBigClass;
+      return hipSuccess;
+    }
+    hipError_t e = hipMalloc(p, want);
+    if (e != hipSuccess && !P->big_free.empty()) {
+      // return the kept blocks to the device and try once more
+      for (auto &b : P->big_free) (void)hipFree(b.first);
+      P->big_free.clear();
+      P->big_free_bytes = 0;
+      (void)hipGetLastError();
+      e = hipMalloc(p, want);
+    }
+    if (e != hipSuccess) return e;
+    P->cls[*p] = kBigClass;
+    P->big_size[*p] = want;
+    return hipSuccess;
+  }
   if (P && c >= 0) {
     std::lock_guard<std::mutex> g(P->mu);
     if (!P->free_[c].empty()) {
@@ -185,9 +235,17 @@ hipError_t pool_release(void *p) {
   if (it == P->cls.end()) return hipErrorInvalidValue;
   const int c = it->second;
   P->cls.erase(it);
-  if (c < 0) return hipFree(p);
-  P->pending.emplace_back(p, c);
-  P->pending_bytes += (size_t)1 << c;
+  if (c == kBigClass) {
+    const size_t sz = P->big_size[p];
+    P->big_size.erase(p);
+    P->big_pending.emplace_back(p, sz);
+    P->pending_bytes += sz;
+  } else if (c < 0) {
+    return hipFree(p);
+  } else {
+    P->pending.emplace_back(p, c);
+    P->pending_bytes += (size_t)1 << c;
+  }
   if (P->pending_bytes > kPoolPendingMax) {
     const hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) return e;
