@@ -484,6 +484,17 @@ def bench_tree(args) -> int:
                       "its longest pair takes on one workgroup)",
         },
     }
+    # Iterative ApproxD on the built tree (outside the timed steps): the
+    # readjustment passes of AllDirChar.adjust_tree from its diagnosis
+    ta = time.perf_counter()
+    ev = treepass.TreeEval(ds, r.tree, be)
+    _adj, a_cost, kept = ev.approx_d(d, iterations=2)
+    ctx.sync()
+    out["config"]["approx_d"] = {
+        "passes": 2, "passes_kept": int(kept), "ms": round(1e3 * (time.perf_counter() - ta), 1),
+        "tree_cost_before": int(d.cost), "tree_cost_after": int(a_cost + d.nonadd),
+        "alignments": int(ev.n_align),
+    }
     if not args.no_cpu:
         cb = cpu_baseline_tree(ds, r.tree, args.cpu_seconds)
         out["cpu_baseline"] = cb

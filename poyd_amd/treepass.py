@@ -266,6 +266,159 @@ class TreeEval:
         return Diagnosis(total + rmed.na_cost, int(best), root, singles, self.n_align, rmed.na_cost)
 
 
+    # ---------------------------------------------------------- ApproxD
+    def post_order(self) -> list[int]:
+        """Tree.post_order_node_visit from the handle (src/tree.ml:1549-1625):
+        every vertex after the subtrees of its two other neighbours, taken
+        in the order of its neighbour record; an interior handle visits its
+        first neighbour's side, then the other two, then itself."""
+        t = self.t
+        out = []
+
+        def traverse(pred, v):
+            stack = [(pred, v, False)]
+            while stack:
+                p, x, done = stack.pop()
+                if done or x in t.leaves:
+                    out.append(x)
+                    continue
+                c1, c2 = t.others(x, p)
+                stack += [(p, x, True), (x, c2, False), (x, c1, False)]
+
+        h = t.handle
+        if h in t.leaves:
+            traverse(h, t.nbrs[h][0])
+        else:
+            for u in t.nbrs[h]:
+                traverse(h, u)
+        out.append(h)
+        return out
+
+    def _readjust(self, jobs):
+        """Sequence.readjust (src/sequence.ml:1166-1199) through
+        SeqCS.readjust's `ApproxD branch (src/seqCS.ml:680-728), per
+        character, for a batch of vertices.  jobs: (ch1, ch2, parent, mine,
+        chars) with per-character sequence lists and the characters to
+        check (None: all).  For each character: the three pairwise medians
+        ab, bc, ac (align_2 + median_2, or align_affine_3's median), each
+        aligned with the third sequence; the least total gives the centre
+        (make_center's order on ties: ab|c before the others, ac|b before
+        bc|a), and the new
+        sequence is Align.closest of the third sequence and that pairwise
+        median.  Returns per job (new sequences, set of changed
+        characters)."""
+        L = len(self.ds.chars)
+        todo = []
+        for j, (a, b, p, m, chars) in enumerate(jobs):
+            for c in (range(L) if chars is None else sorted(chars)):
+                if is_empty(a[c]) or is_empty(b[c]) or is_empty(p[c]):
+                    raise NotImplementedError("ApproxD readjustment with an empty sequence among the three")
+                todo.append((j, c))
+        out = [(list(m), set()) for (_a, _b, _p, m, _c) in jobs]
+        if not todo:
+            return out
+        r1 = self.be.median([x for (j, c) in todo
+                             for x in ((jobs[j][0][c], jobs[j][1][c]), (jobs[j][1][c], jobs[j][2][c]),
+                                       (jobs[j][0][c], jobs[j][2][c]))])
+        pw = [(r1[3 * k], r1[3 * k + 1], r1[3 * k + 2]) for k in range(len(todo))]   # ab, bc, ac
+        r2 = self.be.median([x for k, (j, c) in enumerate(todo)
+                             for x in ((pw[k][0][0], jobs[j][2][c]), (pw[k][1][0], jobs[j][0][c]),
+                                       (pw[k][2][0], jobs[j][1][c]))])
+        self.n_align += 6 * len(todo)
+        picks = []
+        for k, (j, c) in enumerate(todo):
+            a, b, p = jobs[j][0][c], jobs[j][1][c], jobs[j][2][c]
+            (ab, cab), (bc, cbc), (ac, cac) = pw[k]
+            cabc = int(r2[3 * k][1]) + int(cab)
+            cbca = int(r2[3 * k + 1][1]) + int(cbc)
+            cacb = int(r2[3 * k + 2][1]) + int(cac)
+            if cabc <= cbca:
+                picks.append((p, ab) if cabc <= cacb else (b, ac))
+            elif cbca < cacb:
+                picks.append((a, bc))
+            else:
+                picks.append((b, ac))
+        closest = self._to_single([([third], [med]) for third, med in picks])
+        for k, (j, c) in enumerate(todo):
+            s = closest[k][0]
+            seqs, changed = out[j]
+            old = jobs[j][3][c]
+            if not (len(s) == len(old) and np.array_equal(s, old)):
+                changed.add(c)
+            seqs[c] = s
+        return out
+
+    def _edge_cost(self, seqs: dict) -> int:
+        """check_cost (src/allDirChar.ml:185-213) over the given per-vertex
+        sequences: DOS.distance summed over every edge."""
+        pairs = []
+        for (p, v) in self.t.pre_order_edges():
+            for c in range(len(self.ds.chars)):
+                x, y = seqs[p][c], seqs[v][c]
+                if is_empty(x) or is_empty(y):
+                    continue
+                pairs.append((x, y))
+        if not pairs:
+            return 0
+        self.n_align += len(pairs)
+        return int(sum(int(x) for x in self.be.distance(pairs)))
+
+    def approx_d(self, diag: Diagnosis | None = None, iterations: int | None = None, sequential: bool = False):
+        """Iterative `ApproxD (AllDirChar.downpass, src/allDirChar.ml:879-883:
+        internal_downpass, pick_best_root, assign_single, then adjust_tree,
+        :488-670).  Starting from the single assignment, every interior
+        vertex in post order (vertices not touched by the previous pass'
+        changes are skipped, and only the characters that changed next to
+        them are checked) is readjusted against its three neighbours
+        (Node.readjust with ch1, ch2, parent = the first, second and third
+        neighbour of its record, Tree.Interior); a pass is kept only when it
+        changed something and lowered the tree cost, and at most
+        `iterations` passes run (None: until a pass changes nothing or does
+        not lower the cost).  A vertex reads the current sequences of its
+        neighbours, so the passes run in layers: a vertex waits only for its
+        neighbours that come earlier in the post order (sequential=True runs
+        one vertex at a time; the results are the same).  Returns
+        (adjusted sequences per vertex, their tree cost, passes kept)."""
+        d = diag if diag is not None else self.diagnose()
+        t = self.t
+        adj = {v: list(s) for v, s in d.singles.items()}
+        order = self.post_order()
+        prev = d.cost - d.nonadd
+        affected = {v: None for v in order}
+        count = (1 << 30) if iterations is None else int(iterations)
+        kept = 0
+        while count > 0:
+            new = dict(adj)
+            nxt: dict = {}
+            changed = False
+            verts = [v for v in order if v in affected and v not in t.leaves]
+            if sequential:
+                layers = [[v] for v in verts]
+            else:
+                lvl: dict = {}
+                for v in verts:
+                    lvl[v] = 1 + max([lvl[u] for u in t.nbrs[v] if u in lvl], default=-1)
+                layers = [[v for v in verts if lvl[v] == k] for k in range(max(lvl.values(), default=-1) + 1)]
+            for batch in layers:
+                jobs = [(new[t.nbrs[v][0]], new[t.nbrs[v][1]], new[t.nbrs[v][2]], new[v], affected[v]) for v in batch]
+                for v, (seqs, modified) in zip(batch, self._readjust(jobs)):
+                    if not modified:
+                        continue
+                    new[v] = seqs
+                    changed = True
+                    for x in (*t.nbrs[v], v):
+                        nxt[x] = (nxt.get(x) or set()) | modified
+            if not changed:
+                break
+            cost = self._edge_cost(new)
+            if cost >= prev:
+                break
+            adj, prev, affected = new, cost, nxt
+            kept += 1
+            count -= 1
+        return adj, prev, kept
+
+
 # ---------------------------------------------------------------- backends
 
 class EngineBackend:

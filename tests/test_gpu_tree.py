@@ -62,3 +62,27 @@ def test_gpu_wagner_build_matches_oracle(ctx, n_taxa, n_loci, lmin, lmax, n_nona
     gd = treepass.TreeEval(ds, g.tree, gb).diagnose()
     od = treepass.TreeEval(ds, o.tree, ob).diagnose()
     assert (gd.cost, gd.nonadd, gd.root) == (od.cost, od.nonadd, od.root)
+
+
+@pytest.mark.parametrize("n_taxa,seed,affine", [(12, 4, False), (24, 6, False), (12, 8, True)])
+def test_gpu_approx_d_matches_oracle(ctx, n_taxa, seed, affine):
+    """Iterative ApproxD with every alignment on the GPU equals the same
+    passes on the oracle: adjusted sequences, tree cost, passes kept."""
+    from poyd_amd import engine, synth, treebuild
+    from oracle import pyoracle as po
+    print(f"approx_d {n_taxa} taxa affine={affine}", flush=True)
+    w = synth.tree_workload(n_taxa=n_taxa, n_loci=2, lmin=60, lmax=300, n_nonadd=0, seed=seed)
+    ds = treebuild.dataset_of(w)
+    if affine:
+        gb = treepass.EngineBackend(ctx, engine.CostMatrix.of_sub_indel(1, 1, 3))
+        ob = OracleBackend(po.CM2.sub_indel(1, 1, 3), True)
+    else:
+        gb = treepass.EngineBackend(ctx, engine.CostMatrix.of_sub_indel(1, 2))
+        ob = OracleBackend(po.CM2.sub_indel(1, 2), False)
+    tree = treebuild.wagner(ds, ob).tree
+    g_adj, g_cost, g_kept = treepass.TreeEval(ds, tree, gb).approx_d()
+    o_adj, o_cost, o_kept = treepass.TreeEval(ds, tree, ob).approx_d()
+    assert (g_cost, g_kept) == (o_cost, o_kept)
+    for v in o_adj:
+        for c in range(len(ds.chars)):
+            assert np.array_equal(g_adj[v][c], o_adj[v][c]), (v, c)
