@@ -15,6 +15,10 @@
 
 namespace poyg {
 hipError_t launch_reset(int *counters, int n, hipStream_t s);
+hipError_t launch_plan_start(int *ctr, int nctr, int *status, int nstatus, TraceOut *tro, uint8_t *out,
+                             int32_t *out_len, hipStream_t s);
+hipError_t launch_collect_results(const int32_t *cost, const int32_t *status, const int32_t *lens, int slots,
+                                  const uint32_t *out, int64_t out_words, uint32_t *host, hipStream_t s);
 hipError_t launch_prep(bool affine, int grid, const PairMeta *meta, int n, const uint8_t *seq, const DevTcm *tcm,
                        int *aux, hipStream_t s);
 hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, const int32_t *order, int n,
@@ -30,7 +34,6 @@ hipError_t launch_aff_quad(bool bt, int grid, const PairMeta *meta, const int32_
 hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
                            uint32_t *sink, const TraceOut *tro, hipStream_t s);
-hipError_t launch_set_trace_out(TraceOut *dst, uint8_t *out, int32_t *out_len, int32_t *status, hipStream_t s);
 hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, const uint8_t *seq,
                             const DevTcm *tcm, const uint8_t *dm, uint8_t *out, int32_t *out_len, int32_t *status,
                             hipStream_t s);
@@ -97,7 +100,35 @@ struct poyg_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;    // second stream: fills beside the main one
   hipStream_t side2 = nullptr;   // third stream: the widest bands (lane sets, row kernels)
+  // pinned host staging of the per-call entry points (one call at a time per
+  // context: every call ends with a synchronization of `stream`): the plan's
+  // descriptor block, and the call's sequences and results
+  uint8_t *pin_plan = nullptr, *pin_io = nullptr;
+  size_t pin_plan_cap = 0, pin_io_cap = 0;
+  void *pin_io_dev = nullptr;   // pin_io's address for kernels (they write the results there)
 };
+
+// Grows a pinned staging buffer of the context (its contents are not kept);
+// dev, if given, receives the buffer's address for kernels.
+static uint8_t *ctx_pinned(uint8_t *&buf, size_t &cap, size_t bytes, void **dev = nullptr) {
+  if (bytes <= cap && buf) return buf;
+  if (buf) (void)hipHostFree(buf);
+  buf = nullptr;
+  cap = 0;
+  size_t want = 1 << 16;
+  while (want < bytes) want *= 2;
+  if (hipHostMalloc((void **)&buf, want, hipHostMallocDefault) != hipSuccess) {
+    buf = nullptr;
+    return nullptr;
+  }
+  if (dev && hipHostGetDevicePointer(dev, buf, 0) != hipSuccess) {
+    (void)hipHostFree(buf);
+    buf = nullptr;
+    return nullptr;
+  }
+  cap = want;
+  return buf;
+}
 
 namespace poyg {
 int ctx_cus(poyg_ctx *c) { return c ? c->cus : 1; }
@@ -254,6 +285,34 @@ hipError_t pool_release(void *p) {
   return hipSuccess;
 }
 
+// A block whose every use was on a stream that the caller has synchronized
+// since: no queued work can touch it, so it is handed out again at once.
+hipError_t pool_release_idle(void *p) {
+  if (!p) return hipSuccess;
+  DevPool *P = pool_of_current();
+  if (!P) return hipFree(p);
+  std::lock_guard<std::mutex> g(P->mu);
+  auto it = P->cls.find(p);
+  if (it == P->cls.end()) return hipErrorInvalidValue;
+  const int c = it->second;
+  P->cls.erase(it);
+  if (c == kBigClass) {
+    const size_t sz = P->big_size[p];
+    P->big_size.erase(p);
+    if (P->big_free_bytes + sz > kBigCachedMax) return hipFree(p);
+    P->big_free.emplace_back(p, sz);
+    P->big_free_bytes += sz;
+  } else if (c < 0) {
+    return hipFree(p);
+  } else {
+    const size_t sz = (size_t)1 << c;
+    if (P->free_bytes + sz > kPoolCachedMax) return hipFree(p);
+    P->free_[c].push_back(p);
+    P->free_bytes += sz;
+  }
+  return hipSuccess;
+}
+
 // after a device-wide synchronization of the current device
 void pool_synced() {
   DevPool *P = pool_of_current();
@@ -283,8 +342,9 @@ struct DevBuf {
     bytes = n;
     return 0;
   }
-  void release() {
-    if (p && own) (void)pool_release(p);
+  // idle: every use of the block has completed (see pool_release_idle)
+  void release(bool idle = false) {
+    if (p && own) (void)(idle ? pool_release_idle(p) : pool_release(p));
     p = nullptr;
     bytes = 0;
     own = true;
@@ -324,6 +384,7 @@ struct poyg_plan {
   bool timing = false;
   float fill_ms = 0.f, trace_ms = 0.f;
   hipStream_t last_stream = nullptr;   // the stream of the last run (its status is read there)
+  bool idle = false;   // set by an owner that synchronized every run: buffers go straight back to the cache
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;   // hand-off to and from the side streams
   ~poyg_plan() {
@@ -331,7 +392,7 @@ struct poyg_plan {
     for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_sys, &order_c16, &order_quad, &order_diag[0],
                       &order_diag[1], &order_diag[2], &order_rows, &dm, &aux, &rows, &counters, &status,
                       &sink, &pure, &trout, &blob})
-      b->release();
+      b->release(idle);
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
@@ -373,6 +434,9 @@ int poyg_ctx_create(int device, poyg_ctx **out) {
 
 void poyg_ctx_destroy(poyg_ctx *ctx) {
   if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->pin_plan) (void)hipHostFree(ctx->pin_plan);
+  if (ctx->pin_io) (void)hipHostFree(ctx->pin_io);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->side2) (void)hipStreamDestroy(ctx->side2);
@@ -531,9 +595,32 @@ static constexpr int kAffRBHost = 256;   // must match kAffRB in the kernels
 static constexpr int kLinRBHost = 512;   // must match kLinRB
 static constexpr int kLinStageHost = 2048;   // must match kLinStage
 
+// The per-call route: the pair's two sequences travel in the plan's
+// descriptor block (one upload from the context's pinned staging, with no
+// synchronization).  The caller runs the plan on ctx->stream with d_seq =
+// dev and synchronizes that stream before the next per-call plan reuses the
+// staging.
+struct PerCallSeqs {
+  const uint8_t *a, *b;
+  int la, lb;
+  int64_t off_b;                  // b's offset from a (16-byte aligned)
+  const uint8_t *dev = nullptr;   // out: a's copy on the device
+};
+
+static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, const int64_t *off_a,
+                           const int32_t *len_a, const int64_t *off_b, const int32_t *len_b,
+                           const int32_t *deltaw, poyg_plan **out, PerCallSeqs *pc);
+
 extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, const int64_t *off_a,
                                const int32_t *len_a, const int64_t *off_b, const int32_t *len_b,
                                const int32_t *deltaw, poyg_plan **out) {
+  return plan_pairs_impl(ctx, cm, mode, n, off_a, len_a, off_b, len_b, deltaw, out, nullptr);
+}
+
+static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, const int64_t *off_a,
+                           const int32_t *len_a, const int64_t *off_b, const int32_t *len_b,
+                           const int32_t *deltaw, poyg_plan **out, PerCallSeqs *pc) {
+  const bool per_call = pc != nullptr;
   if (!ctx || !cm || !out || n < 0 || (n > 0 && (!off_a || !len_a || !off_b || !len_b))) {
     set_error("poyg_plan_pairs: bad arguments");
     return POYG_EINVAL;
@@ -581,7 +668,10 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     c16_cmax = std::max<int64_t>(c16_cmax, dt.cost[x]);
   }
   // tuned launch shapes (see poyg_ctx_set_tuning)
-  const int quad_max = ctx->tune[kTuneQuadMaxLanes] > 0 ? ctx->tune[kTuneQuadMaxLanes] : 16 * kQuadH;
+  // (a per-call pair runs alone: the one-pair-per-wave kernel steps one
+  // diagonal pair per lane instead of three, a shorter latency)
+  const int quad_max =
+      per_call ? 0 : (ctx->tune[kTuneQuadMaxLanes] > 0 ? ctx->tune[kTuneQuadMaxLanes] : 16 * kQuadH);
   const int side_div = ctx->tune[kTuneSideDiv] > 0 ? ctx->tune[kTuneSideDiv] : 4;
   const int sys_bpc = ctx->tune[kTuneSysBlocks] > 0 ? std::min(ctx->tune[kTuneSysBlocks], lin_sys_blocks_per_cu(align))
                                                     : lin_sys_blocks_per_cu(align);
@@ -825,9 +915,23 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
     sec_off.push_back(blob_bytes);
     blob_bytes += (std::max<size_t>(sec.second.second, 16) + 255) & ~(size_t)255;
   }
-  std::vector<uint8_t> staging(blob_bytes, 0);
+  const size_t seq_at = blob_bytes;   // the per-call route's sequences
+  if (per_call) blob_bytes += (size_t)(pc->off_b + pc->lb + 255) & ~(size_t)255;
+  std::vector<uint8_t> staging_v;
+  uint8_t *staging = per_call ? ctx_pinned(ctx->pin_plan, ctx->pin_plan_cap, blob_bytes) : nullptr;
+  const bool sync_upload = staging == nullptr;
+  if (!staging) {
+    staging_v.assign(blob_bytes, 0);
+    staging = staging_v.data();
+  } else {
+    memset(staging, 0, seq_at);
+  }
   for (size_t k = 0; k < sections.size(); ++k)
-    if (sections[k].second.second) memcpy(staging.data() + sec_off[k], sections[k].second.first, sections[k].second.second);
+    if (sections[k].second.second) memcpy(staging + sec_off[k], sections[k].second.first, sections[k].second.second);
+  if (per_call) {
+    memcpy(staging + seq_at, pc->a, pc->la);
+    memcpy(staging + seq_at + pc->off_b, pc->b, pc->lb);
+  }
   int rc = 0;
   rc |= P->blob.alloc(blob_bytes);
   if (!rc)
@@ -852,13 +956,14 @@ extern "C" int poyg_plan_pairs(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32
   }
   hipStream_t s = ctx->stream;
   hipError_t e = hipSuccess;
-  e = hipMemcpyAsync(P->blob.p, staging.data(), blob_bytes, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  e = hipMemcpyAsync(P->blob.p, staging, blob_bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && sync_upload) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete P;
     set_error("poyg_plan_pairs: upload failed: %s", hipGetErrorString(e));
     return POYG_EDEVICE;
   }
+  if (per_call) pc->dev = (const uint8_t *)P->blob.p + seq_at;
   *out = P;
   return POYG_OK;
 }
@@ -890,11 +995,9 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   const PairMeta *meta = (const PairMeta *)P->meta.p;
   const DevTcm *tcm = (const DevTcm *)P->tcm.p;
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[0], s));
-  HIP_TRY(launch_reset(ctr, 8, s));
-  HIP_TRY(launch_reset((int *)P->status.p, kStatusFlags + P->n, s));
+  HIP_TRY(launch_plan_start(ctr, 8, (int *)P->status.p, kStatusFlags + P->n,
+                            affine && align ? (TraceOut *)P->trout.p : nullptr, d_out, d_out_len, s));
   P->last_stream = s;
-  if (affine && align)
-    HIP_TRY(launch_set_trace_out((TraceOut *)P->trout.p, d_out, d_out_len, (int32_t *)P->status.p, s));
   if (!affine || P->n_lds + P->n_glb)
     HIP_TRY(launch_prep(affine, std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * 8)), meta, P->n, d_seq, tcm,
                         (int *)P->aux.p, s));
@@ -1009,43 +1112,54 @@ extern "C" int poyg_plan_status(poyg_plan *P, int32_t *bad) {
 namespace {
 
 struct OnePair {
-  DevBuf seq, cost, out, len;
-  std::vector<uint8_t> h_out;
-  std::vector<int32_t> h_len;
+  DevBuf cost, out, len;
+  const uint8_t *h_out = nullptr;   // results in the context's pinned staging
+  const int32_t *h_len = nullptr;
   int32_t h_cost = 0;
   poyg_plan *plan = nullptr;
+  bool idle = false;   // every queued use of the buffers has completed
   ~OnePair() {
-    seq.release(); cost.release(); out.release(); len.release();
+    cost.release(idle); out.release(idle); len.release(idle);
+    if (plan) plan->idle = idle;
     poyg_plan_destroy(plan);
   }
+  // One upload (descriptors and both sequences, from pinned staging), the
+  // plan's kernels, one kernel that writes every result into pinned host
+  // memory, and one synchronization.
   int run(poyg_ctx *ctx, const poyg_cm *cm, int mode, const uint8_t *a, int la, const uint8_t *b, int lb,
           int dw) {
     if (!a || !b || la < 1 || lb < 1) {
       set_error("sequence arguments must be non-empty (leading gap included)");
       return POYG_EINVAL;
     }
-    int64_t offa = 0, offb = round16(la);
-    int rc = poyg_plan_pairs(ctx, cm, mode, 1, &offa, &la, &offb, &lb, &dw, &plan);
-    if (rc) return rc;
-    if (seq.alloc((size_t)(offb + lb)) || cost.alloc(4)) return POYG_ENOMEM;
-    const int slots = plan->slots;
-    if (slots && (out.alloc((size_t)plan->out_bytes) || len.alloc(sizeof(int32_t) * slots))) return POYG_ENOMEM;
     hipStream_t s = ctx->stream;
-    HIP_TRY(hipMemcpyAsync((uint8_t *)seq.p + offa, a, la, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync((uint8_t *)seq.p + offb, b, lb, hipMemcpyHostToDevice, s));
-    rc = poyg_plan_run(plan, (const uint8_t *)seq.p, (int32_t *)cost.p, (uint8_t *)out.p, (int32_t *)len.p, s);
+    HIP_TRY(hipStreamSynchronize(s));   // the staging of an earlier call is free
+    int64_t offa = 0, offb = round16(la);
+    PerCallSeqs in{a, b, la, lb, offb};
+    int rc = plan_pairs_impl(ctx, cm, mode, 1, &offa, &la, &offb, &lb, &dw, &plan, &in);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(&h_cost, cost.p, 4, hipMemcpyDeviceToHost, s));
-    if (slots) {
-      h_out.resize(plan->out_bytes);
-      h_len.resize(slots);
-      HIP_TRY(hipMemcpyAsync(h_out.data(), out.p, plan->out_bytes, hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipMemcpyAsync(h_len.data(), len.p, sizeof(int32_t) * slots, hipMemcpyDeviceToHost, s));
+    const int slots = plan->slots;
+    const int64_t out_bytes = slots ? plan->out_bytes : 0;   // a multiple of 16
+    const size_t res_bytes = 80 + (size_t)out_bytes;
+    uint8_t *pin = ctx_pinned(ctx->pin_io, ctx->pin_io_cap, res_bytes, &ctx->pin_io_dev);
+    if (!pin) {
+      set_error("pinned staging of %zu bytes failed", res_bytes);
+      return POYG_ENOMEM;
     }
+    if (cost.alloc(4)) return POYG_ENOMEM;
+    if (slots && (out.alloc((size_t)out_bytes) || len.alloc(sizeof(int32_t) * slots))) return POYG_ENOMEM;
+    rc = poyg_plan_run(plan, in.dev, (int32_t *)cost.p, (uint8_t *)out.p, (int32_t *)len.p, s);
+    if (rc) return rc;
+    HIP_TRY(launch_collect_results((const int32_t *)cost.p, (const int32_t *)plan->status.p,
+                                   (const int32_t *)len.p, slots, (const uint32_t *)out.p, out_bytes / 4,
+                                   (uint32_t *)ctx->pin_io_dev, s));
     HIP_TRY(hipStreamSynchronize(s));
-    int32_t bad = 0;
-    poyg_plan_status(plan, &bad);
-    if (bad) {
+    idle = true;
+    const int32_t *ri = (const int32_t *)pin;
+    h_cost = ri[0];
+    h_len = ri + 4;
+    h_out = pin + 80;
+    if (ri[1]) {
       set_error("backtrace left the filled band");
       return POYG_EBAND;
     }
@@ -1059,7 +1173,7 @@ struct OnePair {
       set_error("output capacity %d is smaller than the result (%d)", cap, l);
       return POYG_EINVAL;
     }
-    memcpy(dst + cap - l, h_out.data() + (size_t)k * c + (c - l), l);
+    memcpy(dst + cap - l, h_out + (size_t)k * c + (c - l), l);
     if (olen) *olen = l;
     return POYG_OK;
   }

@@ -16,6 +16,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include <type_traits>
 
@@ -149,6 +152,33 @@ __device__ __forceinline__ int next_item(int *counter) {
 
 __global__ void reset_counters_kernel(int *c, int n) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) c[k] = 0;
+}
+
+// The start of a pair plan's run in one launch: its work-list counters and
+// status block cleared, and (alignment modes of the affine plans) the
+// backtrace's output pointers set.
+__global__ void plan_start_kernel(int *ctr, int nctr, int *status, int nstatus, TraceOut *tro, uint8_t *out,
+                                  int32_t *out_len) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  for (int k = t; k < nctr; k += stride) ctr[k] = 0;
+  for (int k = t; k < nstatus; k += stride) status[k] = 0;
+  if (tro && t == 0) *tro = TraceOut{out, out_len, (int32_t *)status};
+}
+
+// The per-call route's results, written into pinned host memory: word 0 the
+// cost, word 1 the failed-pair count (status word 0), words 4.. the output
+// lengths, and from word 20 (byte 80) the output sequences.  The system-scope
+// fence makes the writes visible to the host once the stream is synchronized.
+__global__ void collect_results_kernel(const int32_t *cost, const int32_t *status, const int32_t *lens, int slots,
+                                       const uint32_t *out, int64_t out_words, uint32_t *host) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  if (t == 0) {
+    host[0] = (uint32_t)cost[0];
+    host[1] = (uint32_t)status[0];
+  }
+  if (t < slots) host[4 + t] = (uint32_t)lens[t];
+  for (int64_t w = t; w < out_words; w += stride) host[20 + w] = out[w];
+  __threadfence_system();
 }
 
 // Marks pair p of the run as failed: its flag (word kStatusFlags + p of the
@@ -2768,6 +2798,21 @@ hipError_t launch_reset(int *counters, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_plan_start(int *ctr, int nctr, int *status, int nstatus, TraceOut *tro, uint8_t *out,
+                             int32_t *out_len, hipStream_t s) {
+  const int grid = std::max(1, std::min((std::max(nctr, nstatus) + 255) / 256, 1024));
+  hipLaunchKernelGGL(plan_start_kernel, dim3(grid), dim3(256), 0, s, ctr, nctr, status, nstatus, tro, out, out_len);
+  return hipGetLastError();
+}
+
+hipError_t launch_collect_results(const int32_t *cost, const int32_t *status, const int32_t *lens, int slots,
+                                  const uint32_t *out, int64_t out_words, uint32_t *host, hipStream_t s) {
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((out_words + 255) / 256, 256));
+  hipLaunchKernelGGL(collect_results_kernel, dim3(grid), dim3(256), 0, s, cost, status, lens, slots, out, out_words,
+                     host);
+  return hipGetLastError();
+}
+
 hipError_t launch_prep(bool affine, int grid, const PairMeta *meta, int n, const uint8_t *seq, const DevTcm *tcm,
                        int *aux, hipStream_t s) {
   if (affine)
@@ -2798,13 +2843,35 @@ hipError_t launch_aff_fill(bool bt, bool glb, int grid, const PairMeta *meta, co
   return hipGetLastError();
 }
 
+// Occupancy answers per (device, kernel, launch shape), kept for the
+// process: they depend only on the kernel's resources and the device, and a
+// plan of one pair would otherwise spend more time asking than running.
+static std::mutex g_occ_mu;
+static std::map<std::pair<int, int64_t>, int> g_occ;
+template <typename F>
+static int occupancy_cached(int64_t key, F query) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return query();
+  {
+    std::lock_guard<std::mutex> g(g_occ_mu);
+    const auto it = g_occ.find({dev, key});
+    if (it != g_occ.end()) return it->second;
+  }
+  const int v = query();
+  std::lock_guard<std::mutex> g(g_occ_mu);
+  g_occ[{dev, key}] = v;
+  return v;
+}
+
 // Workgroups of aff_diag_kernel<bt, R> one CU holds at once (register and
 // LDS limits), for a persistent grid that is exactly resident.
 template <bool kBt, int R>
 static int diag_occupancy() {
-  int nb = 0;
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_diag_kernel<kBt, R>, kWave * 4, 0);
-  return (e == hipSuccess && nb > 0) ? nb : 2;
+  return occupancy_cached(0x100 + 16 * kBt + R, [] {
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_diag_kernel<kBt, R>, kWave * 4, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 2;
+  });
 }
 int aff_diag_blocks_per_cu(bool bt, int sets) {
   if (sets == 1) return bt ? diag_occupancy<true, 1>() : diag_occupancy<false, 1>();
@@ -2824,15 +2891,6 @@ static void launch_diag_r(bool bt, int grid, const PairMeta *meta, const int32_t
                        tcm, dm, cost, sink, tro);
 }
 
-__global__ void set_trace_out_kernel(TraceOut *dst, uint8_t *out, int32_t *out_len, int32_t *status) {
-  if (threadIdx.x == 0) *dst = TraceOut{out, out_len, status};
-}
-
-hipError_t launch_set_trace_out(TraceOut *dst, uint8_t *out, int32_t *out_len, int32_t *status, hipStream_t s) {
-  hipLaunchKernelGGL(set_trace_out_kernel, dim3(1), dim3(kWave), 0, s, dst, out, out_len, status);
-  return hipGetLastError();
-}
-
 hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
                            uint32_t *sink, const TraceOut *tro, hipStream_t s) {
@@ -2844,9 +2902,11 @@ hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, co
 
 template <bool kBt>
 static int quad_occupancy() {
-  int nb = 0;
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_quad_kernel<kBt, 3>, kWave * 4, 0);
-  return (e == hipSuccess && nb > 0) ? nb : 1;
+  return occupancy_cached(0x200 + kBt, [] {
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_quad_kernel<kBt, 3>, kWave * 4, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+  });
 }
 int aff_quad_blocks_per_cu(bool bt) { return bt ? quad_occupancy<true>() : quad_occupancy<false>(); }
 
@@ -2879,10 +2939,12 @@ hipError_t launch_aff_trace(const PairMeta *meta, const int32_t *order, int n, c
 
 template <bool kDm>
 static int sys_occupancy() {
-  int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_sys_kernel<kDm>, kWave * kSysWaves, 0) != hipSuccess)
-    b = 1;
-  return std::max(1, b);
+  return occupancy_cached(0x300 + kDm, [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_sys_kernel<kDm>, kWave * kSysWaves, 0) != hipSuccess)
+      b = 1;
+    return std::max(1, b);
+  });
 }
 int lin_sys_blocks_per_cu(bool with_dm) { return with_dm ? sys_occupancy<true>() : sys_occupancy<false>(); }
 
@@ -2917,14 +2979,16 @@ static int c16_waves(int n_max) {
 
 int lin_cost16_blocks_per_cu(bool with_dm, int n_max) {
   const int waves = c16_waves(n_max);
-  int b = 0;
-  const hipError_t e =
-      with_dm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_cost16_kernel<true>, kWave * waves,
-                                                             4 * waves * c16_wave_words(n_max))
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_cost16_kernel<false>, kWave * waves,
-                                                             4 * waves * c16_wave_words(n_max));
-  if (e != hipSuccess) b = 1;
-  return std::max(1, b) * waves;   // in waves
+  const int lds = 4 * waves * c16_wave_words(n_max);
+  // the launch shape depends on n_max only through waves and lds
+  return occupancy_cached(((int64_t)0x400 + with_dm) << 32 | (int64_t)lds << 4 | waves, [&] {
+    int b = 0;
+    const hipError_t e =
+        with_dm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_cost16_kernel<true>, kWave * waves, lds)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_cost16_kernel<false>, kWave * waves, lds);
+    if (e != hipSuccess) b = 1;
+    return std::max(1, b) * waves;   // in waves
+  });
 }
 
 hipError_t launch_lin_cost16(bool with_dm, int grid_waves, const PairMeta *meta, const int32_t *order, int n,
@@ -2967,9 +3031,11 @@ hipError_t launch_lin_fill(bool with_dm, bool glb, int grid, const PairMeta *met
 }
 
 int lin_trace_blocks_per_cu() {
-  int nb = 0;
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lin_trace_wave_kernel, kWave * 4, 0);
-  return (e == hipSuccess && nb > 0) ? nb : 2;
+  return occupancy_cached(0x500, [] {
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lin_trace_wave_kernel, kWave * 4, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 2;
+  });
 }
 
 hipError_t launch_lin_trace(int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,

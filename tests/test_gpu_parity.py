@@ -240,9 +240,11 @@ def test_per_call_entry_points(ctx, oracle):
     ref = oracle.CM2.sub_indel(1, 2)
     cma = engine.CostMatrix.of_sub_indel(1, 1, 3)
     refa = oracle.CM2.sub_indel(1, 1, 3)
-    for k in range(12):
-        a = rand_seq(rng, int(rng.integers(1, 200)))
-        b = mutate(rng, a, 0.1, 0.05) if k % 2 else rand_seq(rng, int(rng.integers(1, 200)))
+    # short pairs, then the lengths of the per-call benchmark and longer
+    sizes = [int(rng.integers(1, 200)) for _ in range(12)] + [300, 300, 1000, 1000, 2500]
+    for k, n in enumerate(sizes):
+        a = rand_seq(rng, n)
+        b = mutate(rng, a, 0.1, 0.05) if k % 2 else rand_seq(rng, n if n >= 300 else int(rng.integers(1, 200)))
         assert al.cost_2(a, b, cm) == oracle.cost_2(a, b, ref)
         x, y, c = al.align_2(a, b, cm)
         c2, ox, oy = oracle.align_2(a, b, ref)
