@@ -334,9 +334,11 @@ def main() -> int:
         matrix_cells = cells
     else:
         matrix_cells = int(((w.a.ln.astype(np.int64) - 1) * (w.b.ln.astype(np.int64) - 1)).sum())
-    kernel = ("fill stage with the backtrace fused: aff_quad_kernel (main and side-stream launches) "
-              "and the one-set and lane-set fills, each wave tracing its pairs after filling them; "
-              "one span of HIP events on the main stream") if kind == "affine" \
+    kernel = ("fill stage: aff_pk_kernel (eight pairs per wave in 16-bit halves; main and side-stream "
+              "launches), the one-set and lane-set fills (each wave tracing its pairs after filling them) "
+              "and the 32-bit fill of the pairs whose cost saturates 16 bits; one span of HIP events on "
+              "the main stream (the packed fill's backtraces, aff_trace_list_kernel, are backtrace_ms)") \
+        if kind == "affine" \
         else "cube_rows_kernel (every row class)" if kind == "cube" \
         else "lin_sys_kernel (systolic strips, four waves per pair)"
     out = {

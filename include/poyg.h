@@ -59,9 +59,13 @@ int poyg_ctx_sync(poyg_ctx *ctx);
  * "aff_side_div" (2..64, default 4: the side streams get at most
  * CUs / value compute units), "lin_sys_blocks_per_cu" and
  * "lin_trace_blocks_per_cu" (1..occupancy: resident workgroups per CU of the
- * linear fill and backtrace), "cube_waves_per_cu" (1..64, default 8).  A
- * negative value restores the default.  poyd_amd/autotune.py picks them by
- * timed trials. */
+ * linear fill and backtrace), "cube_waves_per_cu" (1..64, default 8),
+ * "lin_cost_kernel" (1: the 32-bit systolic linear fill for every pair it
+ * takes; 2, the default: the packed 16-bit fill where a pair's costs allow
+ * it), "aff_narrow_kernel" (1: the 32-bit four-pair affine fill for the
+ * narrow bands; 2, the default: the packed eight-pair fill).  A negative
+ * value restores the default.  poyd_amd/autotune.py picks them by timed
+ * trials. */
 int poyg_ctx_set_tuning(poyg_ctx *ctx, const char *knob, int value);
 int poyg_ctx_get_tuning(poyg_ctx *ctx, const char *knob);
 /* "<device name>:<gfx arch>:<CUs>cu", the key of tuned settings. */

@@ -30,7 +30,16 @@ hipError_t launch_pair_pure(int grid, const PairMeta *meta, const int32_t *order
                             uint8_t *pure, hipStream_t s);
 hipError_t launch_aff_quad(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                            const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
-                           const uint8_t *pure, const TraceOut *tro, hipStream_t s);
+                           const uint8_t *pure, const TraceOut *tro, const int *n_dev, hipStream_t s);
+hipError_t launch_aff_pk(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                         const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
+                         const uint8_t *pure, int32_t *redo, int *redo_n, int32_t *tlist, int *tlist_n,
+                         hipStream_t s);
+int aff_pk_blocks_per_cu(bool bt);
+int aff_trace_list_blocks_per_cu();
+hipError_t launch_aff_trace_list(int grid, const PairMeta *meta, const int32_t *list, const int *list_n, int *counter,
+                                 const uint8_t *seq, const DevTcm *tcm, const uint8_t *dm, const TraceOut *tro,
+                                 hipStream_t s);
 hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, const int32_t *order, int n,
                            int *counter, const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost,
                            uint32_t *sink, const TraceOut *tro, hipStream_t s);
@@ -86,17 +95,20 @@ using namespace poyg;
 // Launch-shape knobs a tuner may set per context (-1: the built-in choice).
 // lin_cost_kernel: 1 = the 32-bit systolic fill for every linear pair that
 // fits it, 2 = the packed 16-bit fill wherever a pair allows it (built-in).
+// aff_narrow_kernel: 1 = the 32-bit four-pair fill for the narrow affine
+// bands, 2 = the packed eight-pair fill (built-in).
 enum TuneKnob : int {
-  kTuneQuadMaxLanes, kTuneSideDiv, kTuneSysBlocks, kTuneTraceBlocks, kTuneCubeWaves, kTuneLinCost, kTuneKnobs
+  kTuneQuadMaxLanes, kTuneSideDiv, kTuneSysBlocks, kTuneTraceBlocks, kTuneCubeWaves, kTuneLinCost, kTuneAffNarrow,
+  kTuneKnobs
 };
 static const char *const kTuneNames[kTuneKnobs] = {"aff_quad_max_lanes", "aff_side_div", "lin_sys_blocks_per_cu",
                                                    "lin_trace_blocks_per_cu", "cube_waves_per_cu",
-                                                   "lin_cost_kernel"};
+                                                   "lin_cost_kernel", "aff_narrow_kernel"};
 
 struct poyg_ctx {
   int device = 0;
   int cus = 256;
-  int tune[kTuneKnobs] = {-1, -1, -1, -1, -1, -1};
+  int tune[kTuneKnobs] = {-1, -1, -1, -1, -1, -1, -1};
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;    // second stream: fills beside the main one
   hipStream_t side2 = nullptr;   // third stream: the widest bands (lane sets, row kernels)
@@ -370,9 +382,16 @@ struct poyg_plan {
   // (linear pairs: the systolic kernel takes every pair whose column
   // sequence fits its LDS row, the row kernels the rest)
   DevBuf tcm, meta, order_all, order_lds, order_glb, order_sys, order_c16, order_quad, order_diag[3], order_rows,
-      dm, aux, rows, counters, status, sink, pure, trout, blob;
+      dm, aux, rows, counters, status, sink, pure, trout, blob, redo, tlist;
   int n_lds = 0, n_glb = 0, n_sys = 0, n_quad = 0, n_diag[3] = {0, 0, 0}, n_rows = 0;
   int n_c16 = 0, grid_c16 = 0, c16_nmax = 0;   // cost-only linear pairs of the packed 16-bit fill
+  // the narrow affine bands (order_quad) go to the packed eight-pair fill;
+  // its pairs whose cost saturates 16 bits are listed in `redo` for the
+  // 32-bit four-pair fill (grid_redo workgroups), launched after it
+  // (alignment modes: the packed fill lists its other pairs in `tlist` for
+  // the backtrace kernel, grid_tlist workgroups)
+  bool aff_pk = false;
+  int grid_redo = 0, grid_tlist = 0;
   int grid_lds = 0, grid_glb = 0, grid_sys = 0, grid_quad = 0, grid_diag[3] = {0, 0, 0},
       row_width = 0;
   // the fill with the most cells runs on the main stream ("main": 1 the
@@ -391,7 +410,7 @@ struct poyg_plan {
     if (triple) poyg_triple_plan_destroy(triple);
     for (DevBuf *b : {&tcm, &meta, &order_all, &order_lds, &order_glb, &order_sys, &order_c16, &order_quad, &order_diag[0],
                       &order_diag[1], &order_diag[2], &order_rows, &dm, &aux, &rows, &counters, &status,
-                      &sink, &pure, &trout, &blob})
+                      &sink, &pure, &trout, &blob, &redo, &tlist})
       b->release(idle);
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -402,6 +421,7 @@ struct poyg_plan {
 };
 
 static int64_t round16(int64_t v) { return (v + 15) & ~(int64_t)15; }
+static constexpr int kPlanCounters = 16;   // work-list counters of a pair plan's run
 static constexpr int kQuadH = 3;   // diagonal pairs per lane of the four-pair kernel
 
 extern "C" {
@@ -493,7 +513,7 @@ int poyg_ctx_set_tuning(poyg_ctx *ctx, const char *knob, int value) {
     set_error("poyg_ctx_set_tuning: unknown knob %s", knob ? knob : "(null)");
     return POYG_EINVAL;
   }
-  static const int lo[kTuneKnobs] = {41, 2, 1, 1, 1, 1}, hi[kTuneKnobs] = {48, 64, 64, 64, 64, 2};
+  static const int lo[kTuneKnobs] = {41, 2, 1, 1, 1, 1, 1}, hi[kTuneKnobs] = {48, 64, 64, 64, 64, 2, 2};
   if (value >= 0 && (value < lo[k] || value > hi[k])) {
     set_error("poyg_ctx_set_tuning: %s = %d outside %d..%d", knob, value, lo[k], hi[k]);
     return POYG_EINVAL;
@@ -811,8 +831,16 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
                                              ctx->cus * aff_diag_blocks_per_cu(align, q == 0 ? 1 : 2 * q)));
   }
   P->n_quad = (int)o_quad.size();
-  if (P->n_quad)   // four pairs per wave, four waves per workgroup
-    P->grid_quad = std::max(1, std::min((P->n_quad + 15) / 16, ctx->cus * aff_quad_blocks_per_cu(align)));
+  P->aff_pk = P->n_quad > 0 && ctx->tune[kTuneAffNarrow] != 1;
+  // workgroups of the narrow-band fill per CU (eight or four pairs per wave)
+  auto narrow_bpc = [&]() { return P->aff_pk ? aff_pk_blocks_per_cu(align) : aff_quad_blocks_per_cu(align); };
+  if (P->n_quad)   // four waves per workgroup
+    P->grid_quad = std::max(1, std::min((P->n_quad + (P->aff_pk ? 31 : 15)) / (P->aff_pk ? 32 : 16),
+                                        ctx->cus * narrow_bpc()));
+  if (P->aff_pk) {
+    P->grid_redo = std::max(1, std::min((P->n_quad + 15) / 16, ctx->cus));
+    if (align) P->grid_tlist = std::max(1, std::min((P->n_quad + 3) / 4, ctx->cus * aff_trace_list_blocks_per_cu()));
+  }
   P->row_width = row_width;
   const int per_cu = affine ? 4 : 8;
   P->grid_lds = std::max(1, std::min((P->n_lds + 3) / 4, ctx->cus * per_cu));
@@ -835,12 +863,12 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
       if (P->n_diag[q]) side2 += cus_of(P->grid_diag[q], aff_diag_blocks_per_cu(align, 2 * q));
     if (P->n_lds) side2 += cus_of(P->grid_lds, per_cu);
     if (P->n_glb) side2 += cus_of(P->grid_glb, 8);
-    if (P->n_quad && P->main_fill != 1) side += cus_of(P->grid_quad, aff_quad_blocks_per_cu(align));
+    if (P->n_quad && P->main_fill != 1) side += cus_of(P->grid_quad, narrow_bpc());
     if (P->n_diag[0] && P->main_fill != 2) side += cus_of(P->grid_diag[0], aff_diag_blocks_per_cu(align, 1));
     if ((side || side2) && P->main_fill) {
       side = std::min(side, ctx->cus / side_div);
       side2 = std::min(side2, ctx->cus / side_div);
-      const int bpc = P->main_fill == 1 ? aff_quad_blocks_per_cu(align) : aff_diag_blocks_per_cu(align, 1);
+      const int bpc = P->main_fill == 1 ? narrow_bpc() : aff_diag_blocks_per_cu(align, 1);
       int &grid_main = P->main_fill == 1 ? P->grid_quad : P->grid_diag[0];
       grid_main = std::max(1, std::min(grid_main, (ctx->cus - side - side2) * bpc));
       // a second (third) launch of the main kernel follows the side work on
@@ -941,10 +969,12 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
   rc |= P->dm.alloc((size_t)dm_bytes);
   rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
   rc |= P->rows.alloc(P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16);
-  rc |= P->counters.alloc(sizeof(int) * 8);
+  rc |= P->counters.alloc(sizeof(int) * kPlanCounters);
   rc |= P->status.alloc(sizeof(int32_t) * ((size_t)kStatusFlags + n));
   rc |= P->sink.alloc(sizeof(uint32_t) * kWave);
   rc |= P->trout.alloc(sizeof(TraceOut));
+  if (P->aff_pk) rc |= P->redo.alloc(sizeof(int32_t) * (size_t)P->n_quad);
+  if (P->aff_pk && align) rc |= P->tlist.alloc(sizeof(int32_t) * (size_t)P->n_quad);
   if (!rc && affine && P->grid_help &&
       (hipEventCreateWithFlags(&P->fork, hipEventDisableTiming) != hipSuccess ||
        hipEventCreateWithFlags(&P->join, hipEventDisableTiming) != hipSuccess ||
@@ -995,7 +1025,7 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   const PairMeta *meta = (const PairMeta *)P->meta.p;
   const DevTcm *tcm = (const DevTcm *)P->tcm.p;
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[0], s));
-  HIP_TRY(launch_plan_start(ctr, 8, (int *)P->status.p, kStatusFlags + P->n,
+  HIP_TRY(launch_plan_start(ctr, kPlanCounters, (int *)P->status.p, kStatusFlags + P->n,
                             affine && align ? (TraceOut *)P->trout.p : nullptr, d_out, d_out_len, s));
   P->last_stream = s;
   if (!affine || P->n_lds + P->n_glb)
@@ -1021,9 +1051,17 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
     }
     uint8_t *dm = (uint8_t *)P->dm.p;
     uint32_t *sink = (uint32_t *)P->sink.p;
+    // the narrow bands: the packed fill queues the pairs whose cost saturates
+    // 16 bits (count in ctr[5]) for the 32-bit fill (work counter ctr[7]),
+    // and lists the others for the backtrace (count ctr[8], work counter
+    // ctr[9])
     auto quad = [&](int grid, hipStream_t st) {
+      if (P->aff_pk)
+        return launch_aff_pk(align, grid, meta, (const int32_t *)P->order_quad.p, P->n_quad, ctr + 6, d_seq, tcm, dm,
+                             d_cost, sink, (const uint8_t *)P->pure.p, (int32_t *)P->redo.p, ctr + 5,
+                             (int32_t *)P->tlist.p, ctr + 8, st);
       return launch_aff_quad(align, grid, meta, (const int32_t *)P->order_quad.p, P->n_quad, ctr + 6, d_seq, tcm, dm,
-                             d_cost, sink, (const uint8_t *)P->pure.p, (const TraceOut *)P->trout.p, st);
+                             d_cost, sink, (const uint8_t *)P->pure.p, (const TraceOut *)P->trout.p, nullptr, st);
     };
     auto diag = [&](int q, int grid, hipStream_t st) {
       return launch_aff_diag(align, q == 0 ? 1 : 2 * q, grid, meta, (const int32_t *)P->order_diag[q].p,
@@ -1056,6 +1094,10 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
       HIP_TRY(hipStreamWaitEvent(s, P->join, 0));
       HIP_TRY(hipStreamWaitEvent(s, P->join2, 0));
     }
+    if (P->aff_pk)   // every launch of the packed fill is done: its redo list is complete
+      HIP_TRY(launch_aff_quad(align, P->grid_redo, meta, (const int32_t *)P->redo.p, P->n_quad, ctr + 7, d_seq, tcm,
+                              dm, d_cost, sink, (const uint8_t *)P->pure.p, (const TraceOut *)P->trout.p, ctr + 5,
+                              s));
   } else {
     if (P->n_c16)
       HIP_TRY(launch_lin_cost16(align, P->grid_c16, meta, (const int32_t *)P->order_c16.p, P->n_c16, ctr + 4, d_seq,
@@ -1075,7 +1117,11 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[1], s));
   if (align) {
     if (affine) {
-      // the four-pair and anti-diagonal fills have traced their own pairs
+      // the four-pair and anti-diagonal fills have traced their own pairs;
+      // the packed fill listed its pairs for the backtrace kernel
+      if (P->aff_pk)
+        HIP_TRY(launch_aff_trace_list(P->grid_tlist, meta, (const int32_t *)P->tlist.p, ctr + 8, ctr + 9, d_seq, tcm,
+                                      (const uint8_t *)P->dm.p, (const TraceOut *)P->trout.p, s));
       if (P->n_rows)
         HIP_TRY(launch_aff_trace(meta, (const int32_t *)P->order_rows.p, P->n_rows, d_seq, tcm,
                                  (const uint8_t *)P->dm.p, d_out, d_out_len, (int32_t *)P->status.p, s));

@@ -45,6 +45,28 @@ __device__ __forceinline__ void dpp_exec_guard() {
   __builtin_amdgcn_sched_barrier(0);
 }
 
+// Two unsigned 16-bit halves in one register (v_pk_* operations).
+typedef unsigned short c16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ c16x2 c16v(uint32_t x) { return __builtin_bit_cast(c16x2, x); }
+__device__ __forceinline__ uint32_t c16u(c16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ uint32_t c16_add_sat(uint32_t a, uint32_t b) {
+  return c16u(__builtin_elementwise_add_sat(c16v(a), c16v(b)));
+}
+__device__ __forceinline__ uint32_t c16_sub_sat(uint32_t a, uint32_t b) {
+  return c16u(__builtin_elementwise_sub_sat(c16v(a), c16v(b)));
+}
+__device__ __forceinline__ uint32_t c16_min(uint32_t a, uint32_t b) {
+  return c16u(__builtin_elementwise_min(c16v(a), c16v(b)));
+}
+__device__ __forceinline__ uint32_t c16_add(uint32_t a, uint32_t b) { return c16u(c16v(a) + c16v(b)); }
+// per half: b where g == 0, 0xffff where g > 0 (g * 0xffff + b, saturated)
+__device__ __forceinline__ uint32_t c16_gate(uint32_t g, uint32_t b) {
+  uint32_t r;
+  asm("v_pk_mad_u16 %0, %1, -1, %2 clamp" : "=v"(r) : "v"(g), "v"(b));
+  return r;
+}
+
+
 // Cross-lane operations use DPP (VALU) and v_readlane only: lane i receives
 // lane i-1's value (lane 0 receives `fill`).
 __device__ __forceinline__ int shfl_up1(int v, int fill) {
@@ -1285,11 +1307,16 @@ template <bool kBt, int H>
 __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
     const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, int32_t *cost_out,
-    uint32_t *sink, const uint8_t *__restrict__ pure, const TraceOut *trop) {
+    uint32_t *sink, const uint8_t *__restrict__ pure, const TraceOut *trop, const int *n_dev) {
   __shared__ int s_cost[32 * 32];
   __shared__ int s_prep[32];
   __shared__ uint8_t s_med[kBt ? 32 * 32 : 4];
   __shared__ uint32_t s_tile[4][kBt ? kWave * 16 : 1];   // the backtrace's tile, per wave
+  // n_dev: the list's length is on the device (the packed kernel's redo list)
+  if (n_dev) {
+    n = uniform32(*n_dev);
+    if (n == 0) return;   // uniform over the workgroup
+  }
   load_tables(tcm, s_cost, s_prep, kBt ? s_med : nullptr);
   const int go = tcm->gap_open;
   const int lane = lane_id();
@@ -1530,6 +1557,495 @@ __global__ __launch_bounds__(kWave * 4, 3) void aff_quad_kernel(
         aff_trace_own<16>(uniform_meta(meta, pr), pr, seq, dm_base, s_med, tile, tro);
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Affine band, eight pairs per wave in 16-bit halves.
+// ---------------------------------------------------------------------------
+// aff_quad_kernel's layout (16-lane rows, H = 3 diagonal pairs per lane) with
+// two pairs per row: every register holds one quantity of two pairs, pair 2r
+// of the wave's items in the low halves of row r's lanes and pair 2r + 1 in
+// the high halves, so one packed instruction (v_pk_add_u16 with clamp,
+// v_pk_min_u16, v_pk_sub_u16, bit selects) does the work of two.
+//
+// A value v is held as min(4 v, 0xffff): the two low bits are free for the
+// rank of a ranked minimum (value << 2 | rank in aff_diag_cell), and sums and
+// minima of saturated values are the saturated sums and minima of the true
+// values, so every value below 0xfffc is exact.  The sentinel is 0xffff in
+// both halves (kHigh saturates), which makes "closed" a bitwise OR with the
+// inverted band mask.  The backtrace follows states whose values are at most
+// the pair's cost, and a comparison of two values is exact whenever the
+// smaller one is, so a pair whose cost is below 0x3fff gets exactly
+// aff_quad_kernel's directions and cost.  A pair whose final value saturated
+// is queued (redo list) for aff_quad_kernel, launched after this kernel.
+constexpr uint32_t kPkHigh = 0xffffffffu;
+constexpr uint32_t kPkRank1 = 0x00010001u, kPkRank2 = 0x00020002u, kPkRank3 = 0x00030003u;
+
+__device__ __forceinline__ uint32_t c16_max(uint32_t a, uint32_t b) {
+  return c16u(__builtin_elementwise_max(c16v(a), c16v(b)));
+}
+// per half: 0xffff where the bit `bit` of the half is set
+template <int kBit>
+__device__ __forceinline__ uint32_t c16_bitmask(uint32_t x) {
+  typedef short s16x2 __attribute__((ext_vector_type(2)));
+  const c16x2 up = c16v(x) << (c16x2){15 - kBit, 15 - kBit};
+  const s16x2 m = __builtin_bit_cast(s16x2, up) >> (s16x2){15, 15};
+  return __builtin_bit_cast(uint32_t, m);
+}
+// per half: 1 where a < b, else 0 (as two packed operations: written out,
+// the compiler turns the saturated difference's minimum with 1 into a
+// compare and a select per half)
+__device__ __forceinline__ uint32_t c16_lt01(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm("v_pk_min_u16 %0, %1, %2" : "=v"(r) : "v"(c16_sub_sat(b, a)), "s"(kPkRank1));
+  return r;
+}
+// two 16-bit table entries at the byte offsets held in the halves of a
+__device__ __forceinline__ uint32_t lds_pair(const uint16_t *t, uint32_t a) {
+  const uint32_t lo = *(const uint16_t *)((const char *)t + (a & 0xffffu));
+  const uint32_t hi = *(const uint16_t *)((const char *)t + (a >> 16));
+  return lo | (hi << 16);
+}
+// 0xffff in the low half if a, in the high half if b
+__device__ __forceinline__ uint32_t halves(bool a, bool b) { return (a ? 0xffffu : 0u) | (b ? 0xffff0000u : 0u); }
+
+// Characters are held scaled for the table reads: a rows character x as
+// 64 x (the byte offset of its row of the 32 x 32 table of 16-bit entries),
+// a columns character as 2 x; the gap bit is then bit 10 (rows) / bit 5
+// (columns).
+constexpr uint32_t kPkRowGap = 16u * 64u, kPkColGap = 16u * 2u;
+
+// One cell of two pairs: aff_diag_cell's recurrence in packed halves.  The
+// edge arguments (kSteady false) are per pair; i and j are the same for
+// both pairs of a register (same step, same virtual lane).
+template <bool kBt, bool kSteady, bool kPure, bool kGapFree>
+__device__ __forceinline__ uint32_t aff_pk_cell(AffDiag &S, uint32_t eh_l, uint32_t cb_l, uint32_t ev_u,
+                                                uint32_t cb_u, int i, int j, int laA, int laB, int lbA, int lbB,
+                                                int E0A, int E0B, uint32_t go4, const uint16_t *s_c4, uint32_t ic,
+                                                uint32_t ip, uint32_t si_ge, uint32_t si_go, uint32_t sive,
+                                                uint32_t jc, uint32_t gap_row, uint32_t sj_go, uint32_t hx,
+                                                uint32_t &fcost, uint32_t open_mask, uint32_t left_mask) {
+  if (kGapFree) {
+    si_go = go4;
+    sj_go = go4;
+    sive = si_ge;
+    hx = gap_row;
+  }
+  uint32_t ev, endv_n, ext_v;
+  if (kGapFree) {
+    // both candidates add si_ge: min(ev_u, cb_u + go) + si_ge
+    const uint32_t t_v = c16_add_sat(cb_u, go4);
+    endv_n = c16_lt01(ev_u, t_v);   // 1 where the open is the smaller
+    ev = c16_add_sat(c16_min(ev_u, t_v), si_ge);
+    ext_v = c16_add_sat(ev_u, si_ge);
+  } else {
+    ext_v = c16_add_sat(ev_u, sive);
+    const uint32_t open_v = c16_add_sat(c16_add_sat(cb_u, si_go), si_ge);
+    endv_n = c16_lt01(ext_v, open_v);
+    ev = c16_min(ext_v, open_v);
+  }
+  uint32_t ic_g = 0, jc_g = 0;
+  if (!kGapFree) {
+    ic_g = c16_bitmask<10>(ic);
+    jc_g = c16_bitmask<5>(jc);
+  }
+  const uint32_t dg = kGapFree ? lds_pair(s_c4, ic + jc) : lds_pair(s_c4, (ic & 0x03c003c0u) + (jc & 0x001e001eu));
+  const uint32_t fv = kGapFree ? c16_add_sat(S.ev2, dg) : c16_add_sat(c16_add_sat(S.ev2, dg), sj_go & ic_g);
+  const uint32_t fh = kGapFree ? c16_add_sat(S.eh2, dg) : c16_add_sat(c16_add_sat(S.eh2, dg), si_go & jc_g);
+  const uint32_t fc = c16_add_sat(S.cb2, dg);
+  uint32_t eb, endb_n, kc;
+  if (kPure) {
+    eb = kPkHigh;
+    endb_n = 0;
+    kc = c16_min(c16_min(fh, fv | kPkRank2), fc | kPkRank3);
+  } else {
+    const uint32_t both_g = ic_g & jc_g;
+    const uint32_t diag = ~both_g;   // 0 where both characters carry the gap, else the sentinel
+    const uint32_t ext_b = c16_add_sat(S.eb2, diag);
+    uint32_t open_b;
+    if (kBt) {
+      open_b = c16_add_sat(S.cb2, diag);
+    } else {
+      const uint32_t ip_g = c16_bitmask<10>(ip);
+      const uint32_t flag2 = ~ip_g & ~jc_g;
+      const uint32_t add = (both_g & ~flag2 & c16_add_sat(go4, go4)) | ~both_g;
+      open_b = c16_add_sat(S.cb2, add);
+    }
+    endb_n = c16_lt01(ext_b, open_b);
+    eb = c16_min(ext_b, open_b);
+    const uint32_t fd = c16_add_sat(c16_add_sat(S.eb2, dg), c16_max(si_go, sj_go));
+    kc = c16_min(c16_min(fh, fd | kPkRank1), c16_min(fv | kPkRank2, fc | kPkRank3));
+  }
+  uint32_t cb = kc & 0xfffcfffcu;
+  const uint32_t anext = c16_add(kc, kPkRank1) & kPkRank3;
+  uint32_t eh, endh_n;
+  if (kGapFree) {
+    const uint32_t t_h = c16_add_sat(cb_l, go4);
+    endh_n = c16_lt01(eh_l, t_h);
+    eh = c16_add_sat(c16_min(eh_l, t_h), gap_row);
+  } else {
+    const uint32_t bval = c16_add_sat(c16_add_sat(cb_l, sj_go), gap_row);
+    const uint32_t ext_h = c16_add_sat(eh_l, hx);
+    endh_n = c16_lt01(ext_h, bval);
+    eh = c16_min(ext_h, bval);
+  }
+  const uint32_t kf = kPure ? c16_min(c16_min(eh, cb | kPkRank1), ev | kPkRank2)
+                            : c16_min(c16_min(eh, cb | kPkRank1), c16_min(ev | kPkRank2, eb | kPkRank3));
+  // todo | END_V << 2 | END_H << 3 | END_B << 4 | anext << 5, from the
+  // inverted flags (one XOR turns them back)
+  uint32_t byte = kf & kPkRank3;
+  byte |= endv_n << 2;
+  byte |= endh_n << 3;
+  byte |= endb_n << 4;
+  byte |= anext << 5;
+  byte ^= 0x001c001cu;
+  if (kBt) asm volatile("" : "+v"(byte));
+  if (kSteady) {
+    const uint32_t ev_closed = ext_v | ~left_mask;
+    ev = (open_mask & ev) | (~open_mask & ev_closed);
+    eh |= ~open_mask;
+    eb |= ~open_mask;
+    cb |= ~open_mask;
+    const uint32_t byte_closed = (left_mask & (kAffLeft * 0x00010001u)) | (~left_mask & (kAffRight * 0x00010001u));
+    byte = (open_mask & byte) | (~open_mask & byte_closed);
+  } else {
+    (void)open_mask;
+    (void)left_mask;
+    const int d = j - i;
+    const bool vA = i >= 0 && i <= laA && j >= 0 && j <= lbA && d <= E0A;
+    const bool vB = i >= 0 && i <= laB && j >= 0 && j <= lbB && d <= E0B;
+    const bool lside = i >= 1 && (j == 0 || d == -40);
+    const bool lbA_ = vA && lside, lbB_ = vB && lside;
+    const bool rbA = vA && i >= 1 && d == E0A, rbB = vB && i >= 1 && d == E0B;
+    const bool r0A = vA && i == 0, r0B = vB && i == 0;
+    const bool j0 = j == 0;
+    uint32_t m_open = halves(vA && !lbA_ && !rbA && !r0A, vB && !lbB_ && !rbB && !r0B);
+    uint32_t m_lb = halves(lbA_, lbB_);
+    uint32_t m_r0 = halves(r0A, r0B);
+    uint32_t m_fin = halves(vA && i == laA && j == lbA, vB && i == laB && j == lbB);
+    asm volatile("" : "+v"(m_open), "+v"(m_lb), "+v"(m_r0), "+v"(m_fin));
+    const uint32_t r0eh = j0 ? go4 : c16_add_sat(eh_l, gap_row);
+    const uint32_t alt_ev = (m_lb & ext_v) | (~m_lb & ((m_r0 & (j0 ? go4 : kPkHigh)) | ~m_r0));
+    ev = (m_open & ev) | (~m_open & alt_ev);
+    eh = (m_open & eh) | (~m_open & ((m_r0 & r0eh) | ~m_r0));
+    eb = (m_open & eb) | (~m_open & ((m_r0 & (j0 ? 0u : kPkHigh)) | ~m_r0));
+    cb = (m_open & cb) | (~m_open & ((m_r0 & (j0 ? 0u : r0eh)) | ~m_r0));
+    const uint32_t keep = m_open | m_r0;
+    const uint32_t edge_byte = (m_lb & (kAffLeft * 0x00010001u)) | (~m_lb & (kAffRight * 0x00010001u));
+    byte = (keep & byte) | (~keep & edge_byte);
+    fcost = (m_fin & kf) | (~m_fin & fcost);
+  }
+  S.eh2 = S.eh1; S.ev2 = S.ev1; S.eb2 = S.eb1; S.cb2 = S.cb1;
+  S.eh1 = (int)eh; S.ev1 = (int)ev; S.eb1 = (int)eb; S.cb1 = (int)cb;
+  return byte;
+}
+
+template <int H>
+struct PkLanes {
+  AffDiag S[H];
+  uint32_t ic[H], ip[H], si_ge[H], si_go[H], sive[H];   // row i = s + 20 - L (scaled characters) and its gap costs
+  uint32_t jc[H], jp[H], gap_row[H], sj_go[H], hx[H];   // column and its gap costs
+  uint32_t fcost[H];
+};
+
+// A rows character (scaled, see kPkRowGap) of both pairs at position idx
+// of their rows sequences (0 outside 0..la).
+__device__ __forceinline__ uint32_t pk_row_char(const uint8_t *A0, const uint8_t *A1, int idx, int la0, int la1) {
+  return ((uint32_t)load_or0(A0, idx, la0) << 6) | ((uint32_t)load_or0(A1, idx, la1) << 22);
+}
+__device__ __forceinline__ uint32_t pk_col_char(const uint8_t *B0, const uint8_t *B1, int idx, int lb0, int lb1) {
+  return ((uint32_t)load_or0(B0, idx, lb0) << 1) | ((uint32_t)load_or0(B1, idx, lb1) << 17);
+}
+
+template <bool kBt, int H>
+__global__ __launch_bounds__(kWave * 4, 2) void aff_pk_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, uint8_t *dm_base, int32_t *cost_out,
+    uint32_t *sink, const uint8_t *__restrict__ pure, int32_t *redo, int *redo_n, int32_t *tlist, int *tlist_n) {
+  __shared__ uint16_t s_c4[32 * 32];   // min(4 cost, 0xffff)
+  __shared__ uint16_t s_p4[32];        // min(4 prepend, 0xffff)
+  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_c4[k] = (uint16_t)min(4ll * tcm->cost[k], 0xffffll);
+  for (int k = threadIdx.x; k < 32; k += blockDim.x) s_p4[k] = (uint16_t)min(4ll * tcm->prepend[k], 0xffffll);
+  __syncthreads();
+  const uint32_t go4 = (uint32_t)min(4ll * tcm->gap_open, 0xffffll) * 0x00010001u;
+  const int lane = lane_id();
+  const int q = lane & 15;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(counter, 8);
+    base = uniform32(base);
+    if (base >= n) break;
+    // row r of the wave takes items base + 2r (low halves) and base + 2r + 1
+    // (high halves); slots past the list repeat the last item and store
+    // nothing
+    const int itA = base + 2 * (lane >> 4), itB = itA + 1;
+    const bool actA = itA < n, actB = itB < n;
+    const int pA = order[min(itA, n - 1)], pB = order[min(itB, n - 1)];
+    const PairMeta mA = meta[pA], mB = meta[pB];
+    const uint8_t *AA = seq + mA.off_r, *AB = seq + mB.off_r;
+    const uint8_t *BA = seq + mA.off_c, *BB = seq + mB.off_c;
+    const int laA = mA.lr - 1, lbA = mA.lc - 1, E0A = mA.p0, WA = mA.p1;
+    const int laB = mB.lr - 1, lbB = mB.lc - 1, E0B = mB.p0, WB = mB.p1;
+    uint32_t *dmwA = (uint32_t *)(dm_base + mA.dm_off);
+    uint32_t *dmwB = (uint32_t *)(dm_base + mB.dm_off);
+    PkLanes<H> Q;
+    int Lk[H];
+    uint32_t open0[H], open1[H];
+    const uint32_t lmask = q == 0 ? kPkHigh : 0u;
+    // row constants of both pairs (scaled characters)
+    auto row_consts = [&](uint32_t ic, uint32_t ip, int i, uint32_t &ge, uint32_t &gop, uint32_t &ve) {
+      ge = lds_pair(s_c4, ic + 2u * kGap * 0x00010001u);
+      const uint32_t ic_g = c16_bitmask<10>(ic), ip_g = c16_bitmask<10>(ip);
+      gop = go4 & ~(~ip_g & ic_g);
+      const uint32_t m = i > 1 ? (ip_g & ~ic_g) : 0u;
+      ve = (m & c16_add_sat(gop, ge)) | (~m & ge);
+    };
+    auto col_consts = [&](uint32_t jc, uint32_t jp, int j, uint32_t &gr, uint32_t &gop, uint32_t &x) {
+      gr = lds_pair(s_p4, jc);
+      const uint32_t jc_g = c16_bitmask<5>(jc), jp_g = c16_bitmask<5>(jp);
+      gop = go4 & ~(~jp_g & jc_g);
+      const uint32_t m = j == 1 ? 0u : (jp_g & ~jc_g);
+      x = (m & c16_add_sat(gop, gr)) | (~m & gr);
+    };
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const int L = H * q + k;
+      Lk[k] = L;
+      // the state before step 0: row 19 - L, column L - 20
+      Q.ic[k] = pk_row_char(AA, AB, 19 - L, laA, laB);
+      Q.ip[k] = pk_row_char(AA, AB, 18 - L, laA, laB);
+      row_consts(Q.ic[k], Q.ip[k], 19 - L, Q.si_ge[k], Q.si_go[k], Q.sive[k]);
+      Q.jc[k] = pk_col_char(BA, BB, L - 20, lbA, lbB);
+      Q.jp[k] = pk_col_char(BA, BB, L - 21, lbA, lbB);
+      col_consts(Q.jc[k], Q.jp[k], L - 20, Q.gap_row[k], Q.sj_go[k], Q.hx[k]);
+      Q.S[k] = AffDiag{(int)kPkHigh, (int)kPkHigh, (int)kPkHigh, (int)kPkHigh,
+                       (int)kPkHigh, (int)kPkHigh, (int)kPkHigh, (int)kPkHigh};
+      Q.fcost[k] = kPkHigh;
+      const int d0 = 2 * L - 40;
+      open0[k] = halves(!(L == 0 || d0 >= E0A), !(L == 0 || d0 >= E0B));
+      open1[k] = halves(!(d0 + 1 >= E0A), !(d0 + 1 >= E0B));
+    }
+    const int GsA = (laA + lbA + 4) >> 2, GsB = (laB + lbB + 4) >> 2;
+    const int pfA = actA ? pure[pA] : 3, pfB = actB ? pure[pB] : 3;
+    // steady groups of each pair (all cells inside the sequences)
+    auto steady = [&](bool act, int la, int lb, int E0, int &glo, int &ghi) {
+      const int t_lo = E0 + 82, t_hi = min(2 * lb - E0, 2 * la - 40);
+      glo = act ? (t_lo + 3) >> 2 : 0;
+      ghi = act ? (t_hi >= 3 ? (t_hi - 3) / 4 + 1 : 0) : 0x3fffffff;
+    };
+    int gloA, ghiA, gloB, ghiB;
+    steady(actA, laA, lbA, E0A, gloA, ghiA);
+    steady(actB, laB, lbB, E0B, gloB, ghiB);
+    int groups = 0, g_lo = 0, g_hi = 0x3fffffff, all_pure = 1, all_free = 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int fA = __builtin_amdgcn_readlane(pfA, 16 * r), fB = __builtin_amdgcn_readlane(pfB, 16 * r);
+      all_pure &= fA != 0 && fB != 0;
+      all_free &= fA == 3 && fB == 3;
+      groups = max(groups, max(__builtin_amdgcn_readlane(actA ? GsA : 0, 16 * r),
+                               __builtin_amdgcn_readlane(actB ? GsB : 0, 16 * r)));
+      g_lo = max(g_lo, max(__builtin_amdgcn_readlane(gloA, 16 * r), __builtin_amdgcn_readlane(gloB, 16 * r)));
+      g_hi = min(g_hi, min(__builtin_amdgcn_readlane(ghiA, 16 * r), __builtin_amdgcn_readlane(ghiB, 16 * r)));
+    }
+    // characters entering the row at pair-step s: A[s + 20] at lane 0 and
+    // B[s - 20 + 16 H] at lane 15, from 16-step windows as in aff_quad_kernel
+    uint32_t awin = pk_row_char(AA, AB, 20 + q, laA, laB);
+    uint32_t bwin = pk_col_char(BA, BB, 16 * H - 20 + ((q + 1) & 15), lbA, lbB);
+    uint32_t anext = 0, bnext = 0;
+    auto group = [&](int g, auto variant_tag) {
+      constexpr int kVar = decltype(variant_tag)::value;
+      constexpr bool kSt = kVar >= 1, kPu = kVar >= 2, kGF = kVar == 3;
+      if ((g & 7) == 0) {
+        if (g) {
+          awin = anext;
+          bwin = bnext;
+        }
+        anext = pk_row_char(AA, AB, 2 * g + 16 + 20 + q, laA, laB);
+        bnext = pk_col_char(BA, BB, 2 * g + 16 + 16 * H - 20 + ((q + 1) & 15), lbA, lbB);
+      }
+      {
+        int laAg = laA, laBg = laB, lbAg = lbA, lbBg = lbB, E0Ag = E0A, E0Bg = E0B, Lg[H];
+        uint32_t e0g[H], e1g[H], lmg = lmask;
+        asm volatile("" : "+v"(laAg), "+v"(laBg), "+v"(lbAg), "+v"(lbBg), "+v"(E0Ag), "+v"(E0Bg), "+v"(lmg));
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          Lg[k] = Lk[k];
+          e0g[k] = open0[k];
+          e1g[k] = open1[k];
+          asm volatile("" : "+v"(Lg[k]), "+v"(e0g[k]), "+v"(e1g[k]));
+        }
+        uint32_t by[H][4];   // direction bytes: even/odd step of the group's two pair-steps
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int s = 2 * g + h;
+          {   // even step: next row character into every lane's virtual lane 0
+            dpp_exec_guard();
+            uint32_t nic = (uint32_t)dpp_row_from_below((int)Q.ic[H - 1], (int)awin);
+            uint32_t eh_l0 = (uint32_t)dpp_row_from_below(Q.S[H - 1].eh1, (int)kPkHigh);
+            uint32_t cb_l0 = (uint32_t)dpp_row_from_below(Q.S[H - 1].cb1, (int)kPkHigh);
+            awin = (uint32_t)dpp_row_rotate_down((int)awin);
+            asm volatile("" : "+v"(nic), "+v"(eh_l0), "+v"(cb_l0), "+v"(awin));
+            const uint32_t old0 = Q.ic[0];
+#pragma unroll
+            for (int k = H - 1; k > 0; --k) {
+              Q.ip[k] = Q.ic[k];
+              Q.ic[k] = Q.ic[k - 1];
+              Q.si_ge[k] = Q.si_ge[k - 1];
+              if (!kGF) {
+                Q.si_go[k] = Q.si_go[k - 1];
+                Q.sive[k] = Q.sive[k - 1];
+              }
+            }
+            Q.ip[0] = old0;
+            Q.ic[0] = nic;
+            if (kGF) Q.si_ge[0] = lds_pair(s_c4, nic + 2u * kGap * 0x00010001u);
+            else row_consts(nic, old0, s + 20 - Lg[0], Q.si_ge[0], Q.si_go[0], Q.sive[0]);
+#pragma unroll
+            for (int k = H - 1; k >= 0; --k) {
+              const uint32_t eh_l = k == 0 ? eh_l0 : (uint32_t)Q.S[k > 0 ? k - 1 : 0].eh1;
+              const uint32_t cb_l = k == 0 ? cb_l0 : (uint32_t)Q.S[k > 0 ? k - 1 : 0].cb1;
+              by[k][2 * h] = aff_pk_cell<kBt, kSt, kPu, kGF>(
+                  Q.S[k], eh_l, cb_l, (uint32_t)Q.S[k].ev1, (uint32_t)Q.S[k].cb1, s + 20 - Lg[k], s - 20 + Lg[k],
+                  laAg, laBg, lbAg, lbBg, E0Ag, E0Bg, go4, s_c4, Q.ic[k], Q.ip[k], Q.si_ge[k], Q.si_go[k],
+                  Q.sive[k], Q.jc[k], Q.gap_row[k], Q.sj_go[k], Q.hx[k], Q.fcost[k], e0g[k], k == 0 ? lmg : 0u);
+            }
+          }
+          {   // odd step: next column character into every lane's virtual lane H-1
+            dpp_exec_guard();
+            uint32_t njc = (uint32_t)dpp_row_from_above((int)Q.jc[0], (int)bwin);
+            uint32_t ev_u0 = (uint32_t)dpp_row_from_above(Q.S[0].ev1, (int)kPkHigh);
+            uint32_t cb_u0 = (uint32_t)dpp_row_from_above(Q.S[0].cb1, (int)kPkHigh);
+            bwin = (uint32_t)dpp_row_rotate_down((int)bwin);
+            asm volatile("" : "+v"(njc), "+v"(ev_u0), "+v"(cb_u0), "+v"(bwin));
+            const uint32_t oldt = Q.jc[H - 1];
+#pragma unroll
+            for (int k = 0; k < H - 1; ++k) {
+              Q.jp[k] = Q.jc[k];
+              Q.jc[k] = Q.jc[k + 1];
+              Q.gap_row[k] = Q.gap_row[k + 1];
+              if (!kGF) {
+                Q.sj_go[k] = Q.sj_go[k + 1];
+                Q.hx[k] = Q.hx[k + 1];
+              }
+            }
+            Q.jp[H - 1] = oldt;
+            Q.jc[H - 1] = njc;
+            if (kGF) Q.gap_row[H - 1] = lds_pair(s_p4, njc);
+            else col_consts(njc, oldt, s - 19 + Lg[H - 1], Q.gap_row[H - 1], Q.sj_go[H - 1], Q.hx[H - 1]);
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+              const uint32_t ev_u = k == H - 1 ? ev_u0 : (uint32_t)Q.S[k < H - 1 ? k + 1 : 0].ev1;
+              const uint32_t cb_u = k == H - 1 ? cb_u0 : (uint32_t)Q.S[k < H - 1 ? k + 1 : 0].cb1;
+              by[k][2 * h + 1] = aff_pk_cell<kBt, kSt, kPu, kGF>(
+                  Q.S[k], (uint32_t)Q.S[k].eh1, (uint32_t)Q.S[k].cb1, ev_u, cb_u, s + 20 - Lg[k], s - 19 + Lg[k],
+                  laAg, laBg, lbAg, lbBg, E0Ag, E0Bg, go4, s_c4, Q.ic[k], Q.ip[k], Q.si_ge[k], Q.si_go[k],
+                  Q.sive[k], Q.jc[k], Q.gap_row[k], Q.sj_go[k], Q.hx[k], Q.fcost[k], e1g[k], 0u);
+            }
+          }
+        }
+        if (kBt) {
+          const bool ginA = actA && g < GsA, ginB = actB && g < GsB;
+#pragma unroll
+          for (int k = 0; k < H; ++k) {
+            // [even h0, odd h0, even h1, odd h1] of each pair, as
+            // aff_quad_kernel stores them
+            const uint32_t t0 = __builtin_amdgcn_perm(by[k][1], by[k][0], 0x06020400u);
+            const uint32_t t1 = __builtin_amdgcn_perm(by[k][3], by[k][2], 0x06020400u);
+            const uint32_t accA = __builtin_amdgcn_perm(t1, t0, 0x05040100u);
+            const uint32_t accB = __builtin_amdgcn_perm(t1, t0, 0x07060302u);
+            uint32_t *dA = (ginA && Lk[k] < WA) ? dmwA + (size_t)g * WA + Lk[k] : sink + lane;
+            uint32_t *dB = (ginB && Lk[k] < WB) ? dmwB + (size_t)g * WB + Lk[k] : sink + lane;
+            *dA = accA;
+            *dB = accB;
+          }
+        }
+      }
+    };
+    if (groups >= 1536) __builtin_amdgcn_s_setprio(2);
+    else if (groups >= 768) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+    const int gs_lo = min(g_lo, groups), gs_hi = max(gs_lo, min(g_hi, groups));
+    int g = 0;
+    for (; g < gs_lo; ++g) group(g, std::integral_constant<int, 0>{});
+    if (all_free) {
+      for (; g < gs_hi; ++g) group(g, std::integral_constant<int, 3>{});
+      // the gap-free groups kept only si_ge and gap_row; for rows and
+      // columns without gap-bearing characters the other costs follow
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        Q.si_go[k] = go4;
+        Q.sive[k] = Q.si_ge[k];
+        Q.sj_go[k] = go4;
+        Q.hx[k] = Q.gap_row[k];
+      }
+    } else if (all_pure) {
+      for (; g < gs_hi; ++g) group(g, std::integral_constant<int, 2>{});
+    } else {
+      for (; g < gs_hi; ++g) group(g, std::integral_constant<int, 1>{});
+    }
+    for (; g < groups; ++g) group(g, std::integral_constant<int, 0>{});
+    // cell (la, lb) of each pair: diagonal pair (lb - la + 40) / 2 = H qs + ks
+    auto final_of = [&](int la, int lb, uint32_t shift, int &qs, uint32_t &fc) {
+      const int Ls = (lb - la + 40) >> 1;
+      qs = Ls / H;
+      const int ks = Ls - H * (Ls / H);
+      fc = Q.fcost[0];
+#pragma unroll
+      for (int k = 1; k < H; ++k) fc = ks == k ? Q.fcost[k] : fc;
+      fc = (fc >> shift) & 0xffffu;
+    };
+    int qsA, qsB;
+    uint32_t fcA, fcB;
+    final_of(laA, lbA, 0, qsA, fcA);
+    final_of(laB, lbB, 16, qsB, fcB);
+    // per pair: its cost, or -1 when its final value saturated (redone by
+    // the 32-bit kernel); lane 16 r holds row r's low pair, lane 16 r + 1 its
+    // high pair
+    int mine = -2;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int cA = __builtin_amdgcn_readlane((int)fcA, 16 * r + __builtin_amdgcn_readlane(qsA, 16 * r));
+      const int cB = __builtin_amdgcn_readlane((int)fcB, 16 * r + __builtin_amdgcn_readlane(qsB, 16 * r));
+      mine = lane == 16 * r ? cA : mine;
+      mine = lane == 16 * r + 1 ? cB : mine;
+    }
+    const bool slot = (q < 2) && ((q == 0) ? actA : actB);
+    const int pmine = q == 0 ? pA : pB;
+    const bool sat = (uint32_t)(mine & 0xfffc) >= 0xfffcu;
+    // the backtraces run in aff_trace_list_kernel, after every fill: a wave
+    // that traced its eight pairs here would hold its SIMD's issue slots
+    // through eight serial walks (two waves per SIMD fit this kernel)
+    if (slot) {
+      if (sat) {
+        redo[atomicAdd(redo_n, 1)] = pmine;
+      } else {
+        cost_out[pmine] = (mine & 0xfffc) >> 2;
+        if (kBt) tlist[atomicAdd(tlist_n, 1)] = pmine;
+      }
+    }
+  }
+}
+
+// The backtraces of the pairs that the packed fill listed, one wave per pair
+// (aff_trace_pair over tiles of 16 groups).  Launched after every fill of
+// the run (the bands are complete); the list's length is on the device.
+__global__ __launch_bounds__(kWave * 4, 5) void aff_trace_list_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ list, const int *list_n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, const uint8_t *dm_base, const TraceOut *trop) {
+  __shared__ uint8_t s_med[32 * 32];
+  __shared__ uint32_t s_tile[4][kWave * 16];
+  const int n = uniform32(*list_n);
+  if (n == 0) return;   // uniform over the workgroup
+  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) s_med[k] = tcm->median[k];
+  __syncthreads();
+  const TraceOut tro = *trop;
+  uint32_t *tile = s_tile[wave_in_group()];
+  for (;;) {
+    const int k = next_item(counter);
+    if (k >= n) break;
+    const int p = uniform32(list[k]);
+    aff_trace_own<16>(uniform_meta(meta, p), p, seq, dm_base, s_med, tile, tro);
   }
 }
 
@@ -2210,26 +2726,6 @@ constexpr int kC16Pad = 192;    // LDS columns kept before column 0 and after th
 constexpr int kC16U = 8;        // steps per unrolled group
 constexpr int kC16Rows = 2 * kWave;
 
-typedef unsigned short c16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ c16x2 c16v(uint32_t x) { return __builtin_bit_cast(c16x2, x); }
-__device__ __forceinline__ uint32_t c16u(c16x2 x) { return __builtin_bit_cast(uint32_t, x); }
-__device__ __forceinline__ uint32_t c16_add_sat(uint32_t a, uint32_t b) {
-  return c16u(__builtin_elementwise_add_sat(c16v(a), c16v(b)));
-}
-__device__ __forceinline__ uint32_t c16_sub_sat(uint32_t a, uint32_t b) {
-  return c16u(__builtin_elementwise_sub_sat(c16v(a), c16v(b)));
-}
-__device__ __forceinline__ uint32_t c16_min(uint32_t a, uint32_t b) {
-  return c16u(__builtin_elementwise_min(c16v(a), c16v(b)));
-}
-__device__ __forceinline__ uint32_t c16_add(uint32_t a, uint32_t b) { return c16u(c16v(a) + c16v(b)); }
-// per half: b where g == 0, 0xffff where g > 0 (g * 0xffff + b, saturated)
-__device__ __forceinline__ uint32_t c16_gate(uint32_t g, uint32_t b) {
-  uint32_t r;
-  asm("v_pk_mad_u16 %0, %1, -1, %2 clamp" : "=v"(r) : "v"(g), "v"(b));
-  return r;
-}
-
 // LDS words per wave for pairs whose shorter sequence has at most n_max
 // positions: column words, the hand-off row, 64 lanes x kC16U dummy slots.
 __host__ __device__ __forceinline__ int c16_cols(int n_max) { return (n_max + 2 * kC16Pad + 63) & ~63; }
@@ -2901,6 +3397,16 @@ hipError_t launch_aff_diag(bool bt, int sets, int grid, const PairMeta *meta, co
 }
 
 template <bool kBt>
+static int pk_occupancy() {
+  return occupancy_cached(0x600 + kBt, [] {
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_pk_kernel<kBt, 3>, kWave * 4, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+  });
+}
+int aff_pk_blocks_per_cu(bool bt) { return bt ? pk_occupancy<true>() : pk_occupancy<false>(); }
+
+template <bool kBt>
 static int quad_occupancy() {
   return occupancy_cached(0x200 + kBt, [] {
     int nb = 0;
@@ -2918,13 +3424,42 @@ hipError_t launch_pair_pure(int grid, const PairMeta *meta, const int32_t *order
 
 hipError_t launch_aff_quad(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
                            const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
-                           const uint8_t *pure, const TraceOut *tro, hipStream_t s) {
+                           const uint8_t *pure, const TraceOut *tro, const int *n_dev, hipStream_t s) {
   if (bt)
     hipLaunchKernelGGL((aff_quad_kernel<true, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
-                       tcm, dm, cost, sink, pure, tro);
+                       tcm, dm, cost, sink, pure, tro, n_dev);
   else
     hipLaunchKernelGGL((aff_quad_kernel<false, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
-                       tcm, dm, cost, sink, pure, tro);
+                       tcm, dm, cost, sink, pure, tro, n_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_aff_pk(bool bt, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                         const uint8_t *seq, const DevTcm *tcm, uint8_t *dm, int32_t *cost, uint32_t *sink,
+                         const uint8_t *pure, int32_t *redo, int *redo_n, int32_t *tlist, int *tlist_n,
+                         hipStream_t s) {
+  if (bt)
+    hipLaunchKernelGGL((aff_pk_kernel<true, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
+                       tcm, dm, cost, sink, pure, redo, redo_n, tlist, tlist_n);
+  else
+    hipLaunchKernelGGL((aff_pk_kernel<false, 3>), dim3(grid), dim3(kWave * 4), 0, s, meta, order, n, counter, seq,
+                       tcm, dm, cost, sink, pure, redo, redo_n, tlist, tlist_n);
+  return hipGetLastError();
+}
+
+int aff_trace_list_blocks_per_cu() {
+  return occupancy_cached(0x700, [] {
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aff_trace_list_kernel, kWave * 4, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 2;
+  });
+}
+
+hipError_t launch_aff_trace_list(int grid, const PairMeta *meta, const int32_t *list, const int *list_n, int *counter,
+                                 const uint8_t *seq, const DevTcm *tcm, const uint8_t *dm, const TraceOut *tro,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(aff_trace_list_kernel, dim3(grid), dim3(kWave * 4), 0, s, meta, list, list_n, counter, seq, tcm,
+                     dm, tro);
   return hipGetLastError();
 }
 

@@ -169,6 +169,46 @@ def test_affine_block_indels(ctx, oracle, sub, indel, go):
             assert np.array_equal(res.seq(k, slot), want), (k, slot, len(a), len(b))
 
 
+@pytest.mark.parametrize("alphabet", ["dna", "any"])
+def test_affine_packed_fill_saturation(ctx, oracle, alphabet):
+    """The packed 16-bit narrow-band fill holds values as 4 v saturated at
+    0xffff: pairs whose cost reaches 0x3fff are queued for the 32-bit fill.
+    Costs of 40 / 30 / 50 put short DNA pairs below that bound and longer
+    ones above it, in one batch; every output equals the oracle's and the
+    32-bit fill's."""
+    from poyd_amd import engine
+    note(f"affine packed fill saturation alphabet={alphabet}")
+    rng = np.random.default_rng(911 + (alphabet == "any"))
+    A, B = [], []
+    for k in range(96):
+        L = int(rng.integers(20, 900))
+        a = rand_seq(rng, L, alphabet)
+        b = mutate(rng, a, 0.3, 0.05) if k % 3 else rand_seq(rng, L + int(rng.integers(-15, 15)), alphabet)
+        A.append(a)
+        B.append(b)
+    cm = engine.CostMatrix.of_sub_indel(40, 30, 50)
+    ref = oracle.CM2.sub_indel(40, 30, 50)
+    res = engine.run_pairs(ctx, cm, engine.POYG_MODE_AFFINE_ALIGN, batch_of(A), batch_of(B))
+    cres = engine.run_pairs(ctx, cm, engine.POYG_MODE_AFFINE_COST, batch_of(A), batch_of(B))
+    ctx.set_tuning("aff_narrow_kernel", 1)
+    try:
+        res32 = engine.run_pairs(ctx, cm, engine.POYG_MODE_AFFINE_ALIGN, batch_of(A), batch_of(B))
+    finally:
+        ctx.set_tuning("aff_narrow_kernel", -1)
+    costs = []
+    for k, (a, b) in enumerate(zip(A, B)):
+        med, ri, rj, cost, mwg = oracle.align_affine_3(a, b, ref)
+        costs.append(cost)
+        assert res.cost[k] == cost and res32.cost[k] == cost, (k, len(a), len(b))
+        assert cres.cost[k] == oracle.cost_affine_3(a, b, ref), (k, len(a), len(b))
+        for slot, want in enumerate((med, mwg, ri, rj)):
+            assert np.array_equal(res.seq(k, slot), want), (k, slot, len(a), len(b))
+            assert np.array_equal(res32.seq(k, slot), want), (k, slot, len(a), len(b))
+    costs = np.array(costs)
+    if alphabet == "dna":   # gap-bearing characters make the other alphabet's alignments cheap
+        assert (costs < 0x3fff).sum() >= 10 and (costs >= 0x3fff).sum() >= 5, costs
+
+
 @pytest.mark.parametrize("sub,indel", [(1, 2), (1, 1), (2, 1)])
 @pytest.mark.parametrize("alphabet", ["dna", "ambig"])
 def test_linear_batch_matches_oracle(ctx, oracle, sub, indel, alphabet):
