@@ -51,6 +51,10 @@ hipError_t launch_lin_sys(bool with_dm, int grid, const PairMeta *meta, const in
                           int32_t *status, int spin_limit, hipStream_t s);
 int lin_sys_blocks_per_cu(bool with_dm);
 int lin_cost16_blocks_per_cu(bool with_dm, int n_max);   // resident waves per CU
+int lin_c16mw_blocks_per_cu(bool with_dm, int n_max);
+hipError_t launch_lin_c16mw(bool with_dm, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                            const uint8_t *seq, const DevTcm *tcm, const int *aux, int32_t *cost, uint8_t *dm,
+                            int n_max, hipStream_t s);
 hipError_t launch_lin_cost16(bool with_dm, int grid_waves, const PairMeta *meta, const int32_t *order, int n,
                              int *counter, const uint8_t *seq, const DevTcm *tcm, const int *aux, int32_t *cost,
                              uint8_t *dm, int n_max, hipStream_t s);
@@ -94,7 +98,9 @@ using namespace poyg;
 
 // Launch-shape knobs a tuner may set per context (-1: the built-in choice).
 // lin_cost_kernel: 1 = the 32-bit systolic fill for every linear pair that
-// fits it, 2 = the packed 16-bit fill wherever a pair allows it (built-in).
+// fits it; the packed 16-bit fill wherever a pair allows it with 2 = one
+// pair per wave for large batches and four waves per pair for small ones
+// (built-in), 3 = one pair per wave, 4 = four waves per pair.
 // aff_narrow_kernel: 1 = the 32-bit four-pair fill for the narrow affine
 // bands, 2 = the packed eight-pair fill (built-in).
 enum TuneKnob : int {
@@ -384,7 +390,8 @@ struct poyg_plan {
   DevBuf tcm, meta, order_all, order_lds, order_glb, order_sys, order_c16, order_quad, order_diag[3], order_rows,
       dm, aux, rows, counters, status, sink, pure, trout, blob, redo, tlist;
   int n_lds = 0, n_glb = 0, n_sys = 0, n_quad = 0, n_diag[3] = {0, 0, 0}, n_rows = 0;
-  int n_c16 = 0, grid_c16 = 0, c16_nmax = 0;   // cost-only linear pairs of the packed 16-bit fill
+  int n_c16 = 0, grid_c16 = 0, c16_nmax = 0;   // linear pairs of the packed 16-bit fill
+  bool c16_mw = false;                          // ... four waves per pair (grid_c16 counts workgroups)
   // the narrow affine bands (order_quad) go to the packed eight-pair fill;
   // its pairs whose cost saturates 16 bits are listed in `redo` for the
   // 32-bit four-pair fill (grid_redo workgroups), launched after it
@@ -513,7 +520,7 @@ int poyg_ctx_set_tuning(poyg_ctx *ctx, const char *knob, int value) {
     set_error("poyg_ctx_set_tuning: unknown knob %s", knob ? knob : "(null)");
     return POYG_EINVAL;
   }
-  static const int lo[kTuneKnobs] = {41, 2, 1, 1, 1, 1, 1}, hi[kTuneKnobs] = {48, 64, 64, 64, 64, 2, 2};
+  static const int lo[kTuneKnobs] = {41, 2, 1, 1, 1, 1, 1}, hi[kTuneKnobs] = {48, 64, 64, 64, 64, 4, 2};
   if (value >= 0 && (value < lo[k] || value > hi[k])) {
     set_error("poyg_ctx_set_tuning: %s = %d outside %d..%d", knob, value, lo[k], hi[k]);
     return POYG_EINVAL;
@@ -847,8 +854,17 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
   P->grid_glb = std::max(1, std::min(P->n_glb, 256));
   if (P->n_sys) P->grid_sys = std::max(1, std::min(P->n_sys, ctx->cus * sys_bpc));
   P->n_c16 = (int)o_c16.size();
-  if (P->n_c16)   // one pair per wave; grid_c16 counts waves
-    P->grid_c16 = std::max(1, std::min(P->n_c16, ctx->cus * lin_cost16_blocks_per_cu(align, P->c16_nmax)));
+  if (P->n_c16) {
+    // a batch that one pair per wave leaves far from filling the chip (tree
+    // levels, per-call entries) gets four waves per pair: each pair's
+    // latency is about a third
+    const int lk = ctx->tune[kTuneLinCost];
+    P->c16_mw = lk == 4 || (lk != 3 && P->n_c16 <= 2 * ctx->cus);
+    if (P->c16_mw)   // grid_c16 counts workgroups
+      P->grid_c16 = std::max(1, std::min(P->n_c16, ctx->cus * lin_c16mw_blocks_per_cu(align, P->c16_nmax)));
+    else   // one pair per wave; grid_c16 counts waves
+      P->grid_c16 = std::max(1, std::min(P->n_c16, ctx->cus * lin_cost16_blocks_per_cu(align, P->c16_nmax)));
+  }
   if (affine) {
     // The fill with the most cells is the main kernel.  The side stream's
     // kernels are persistent too; the main kernel leaves them whole CUs, or
@@ -1099,7 +1115,10 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
                               dm, d_cost, sink, (const uint8_t *)P->pure.p, (const TraceOut *)P->trout.p, ctr + 5,
                               s));
   } else {
-    if (P->n_c16)
+    if (P->n_c16 && P->c16_mw)
+      HIP_TRY(launch_lin_c16mw(align, P->grid_c16, meta, (const int32_t *)P->order_c16.p, P->n_c16, ctr + 4, d_seq,
+                               tcm, (const int *)P->aux.p, d_cost, (uint8_t *)P->dm.p, P->c16_nmax, s));
+    else if (P->n_c16)
       HIP_TRY(launch_lin_cost16(align, P->grid_c16, meta, (const int32_t *)P->order_c16.p, P->n_c16, ctr + 4, d_seq,
                                 tcm, (const int *)P->aux.p, d_cost, (uint8_t *)P->dm.p, P->c16_nmax, s));
     if (P->n_sys)

@@ -2731,6 +2731,152 @@ constexpr int kC16Rows = 2 * kWave;
 __host__ __device__ __forceinline__ int c16_cols(int n_max) { return (n_max + 2 * kC16Pad + 63) & ~63; }
 __host__ __device__ __forceinline__ int c16_wave_words(int n_max) { return 2 * c16_cols(n_max) + kWave * kC16U; }
 
+// One 128-row strip of the packed 16-bit fill on one wave: the state a
+// strip carries from step to step.  begin() sets up strip i0's rows, reading
+// the hand-off row of the strip above from Rin and writing this strip's last
+// row into Rout (the same buffer when one wave runs the strips in turn: the
+// writes trail the reads); run16() advances it 16 steps.
+template <bool kDm>
+struct C16Strip {
+  const uint16_t *ta, *tb;
+  uint32_t cdel, kv, wid, V, upp, wprev, acc_a, acc_b;
+  uint32_t *wp;
+  const uint32_t *wc, *rr;
+  int winc, oa, ob, steps, t, wh;
+  __amdgpu_buffer_rsrc_t drsrc;
+  uint32_t wA[kC16U], fA[kC16U], wB[kC16U], fB[kC16U];
+
+  __device__ __forceinline__ void load_wf(uint32_t (&w)[kC16U], uint32_t (&f)[kC16U], int off) {
+#pragma unroll
+    for (int u = 0; u < kC16U; ++u) {
+      w[u] = wc[off + u];
+      f[u] = rr[off + u];
+    }
+  }
+
+  __device__ __forceinline__ void begin(int i0, int M, int N, const LinGeom &g, const uint8_t *s1, const uint16_t *T,
+                                        const uint32_t *Wc, const uint32_t *Rin, uint32_t *Rout, uint32_t *D, int S,
+                                        __amdgpu_buffer_rsrc_t rsrc) {
+    const int lane = lane_id();
+    drsrc = rsrc;
+    int c0, hi_first, lo_l, hi_l;
+    lin_row_host(i0, M, N, g, c0, hi_first);
+    const int r_last = min(i0 + kC16Rows - 1, M - 1);
+    lin_row_host(r_last, M, N, g, lo_l, hi_l);
+    const int wl = (r_last - i0) >> 1;   // lane and half of the last row
+    wh = (r_last - i0) & 1;
+    const int t_end = hi_l - c0 + 2 * wl + wh;
+    // one step more (the cell right of the band), rounded to two groups
+    steps = (t_end + 2 + 2 * kC16U - 1) / (2 * kC16U) * (2 * kC16U);
+    t = 0;
+    // this lane's rows
+    const int ra = i0 + 2 * lane, rb = ra + 1;
+    const bool va = ra < M, vb = rb < M;
+    int lo_a, hi_a, lo_b, hi_b;
+    lin_row_host(va ? ra : M - 1, M, N, g, lo_a, hi_a);
+    lin_row_host(vb ? rb : M - 1, M, N, g, lo_b, hi_b);
+    const int sa = va ? (s1[ra] & 31) : 0, sb = vb ? (s1[rb] & 31) : 0;
+    cdel = (uint32_t)T[sa * 32 + kGap] | ((uint32_t)T[sb * 32 + kGap] << 16);
+    ta = T + sa * 32;
+    tb = T + sb * 32;
+    // column - lo of the step's cell (rows past the last one: never inside)
+    kv = (va ? (uint32_t)(c0 - 2 * lane - lo_a) & 0xffffu : 0x8000u) |
+         ((vb ? (uint32_t)(c0 - 2 * lane - 1 - lo_b) & 0xffffu : 0x8000u) << 16);
+    wid = (va ? (uint32_t)(hi_a - lo_a) : 0u) | ((vb ? (uint32_t)(hi_b - lo_b) : 0u) << 16);
+    const bool writer = lane == wl;
+    wp = writer ? Rout + (c0 - 2 * wl - wh) : D;
+    winc = writer ? kC16U : 0;
+    wc = Wc + (c0 - 2 * lane);
+    rr = Rin + c0;
+    // byte offsets of the dwords that steps 0..3 fill (rows past the last
+    // one store nothing: no byte of theirs is ever inside)
+    oa = ob = 0;
+    if (kDm) {
+      oa = ra * S + ((lo_a - c0 + 2 * lane) & 3) + (c0 - 2 * lane - lo_a);
+      ob = rb * S + ((lo_b - c0 + 2 * lane + 1) & 3) + (c0 - 2 * lane - 1 - lo_b);
+    }
+    acc_a = acc_b = 0;
+    V = 0xffffffffu;
+    upp = (Rin[c0 - 1] >> 16) | 0xffff0000u;   // lane 0's first diagonal; the rest lie outside
+    wprev = wc[-1];
+    // A group of kC16U steps reads its substitution costs at its start;
+    // column words and lane 0's hand-off words are read one group ahead,
+    // into two register sets that alternate (no copies between groups).
+    load_wf(wA, fA, 0);
+  }
+
+  __device__ __forceinline__ void group(const uint32_t (&w)[kC16U], const uint32_t (&f)[kC16U]) {
+    uint32_t cr[kC16U];
+#pragma unroll
+    for (int u = 0; u < kC16U; ++u) {
+      const uint32_t wq = u ? w[u - 1] : wprev;
+      cr[u] = (uint32_t)*(const uint16_t *)((const char *)ta + (w[u] >> 16)) |
+              ((uint32_t)*(const uint16_t *)((const char *)tb + (wq >> 16)) << 16);
+    }
+#pragma unroll
+    for (int u = 0; u < kC16U; ++u) {
+      const uint32_t wq = u ? w[u - 1] : wprev;
+      const uint32_t sh =
+          (uint32_t)__builtin_amdgcn_update_dpp((int)f[u], (int)V, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+      const uint32_t up = __builtin_amdgcn_alignbit(V, sh, 16);
+      const uint32_t gj = __builtin_amdgcn_perm(wq, w[u], 0x05040100u);
+      const uint32_t t1 = c16_add_sat(up, cdel);    // DELETE
+      const uint32_t t2 = c16_add_sat(V, gj);       // INSERT
+      const uint32_t t3 = c16_add_sat(upp, cr[u]);  // ALIGN
+      const uint32_t best = c16_min(c16_min(t1, t2), t3);
+      const uint32_t gt = c16_sub_sat(kv, wid);
+      V = c16_gate(gt, best);
+      kv = c16_add(kv, 0x00010001u);
+      wp[u] = V;
+      upp = up;
+      if (kDm) {
+        // per half: 1 for a candidate equal to the minimum inside the band
+        const uint32_t ib = c16_sub_sat(0x00010001u, gt);
+        const uint32_t q1 = c16_sub_sat(ib, c16_sub_sat(t1, best));
+        const uint32_t q2 = c16_sub_sat(ib, c16_sub_sat(t2, best));
+        const uint32_t q3 = c16_sub_sat(ib, c16_sub_sat(t3, best));
+        const uint32_t msk = c16_add(c16_add(q3, c16_add(q2, q2)), c16u(c16v(q1) << (unsigned short)2));
+        // byte u mod 4 of the two dwords (the low half's code is byte 0
+        // of msk, the high half's byte 2); position 0 starts a dword
+        constexpr uint32_t sel_a[4] = {0x0c0c0c00u, 0x0c0c0004u, 0x0c000504u, 0x00060504u};
+        constexpr uint32_t sel_b[4] = {0x0c0c0c02u, 0x0c0c0204u, 0x0c020504u, 0x02060504u};
+        acc_a = __builtin_amdgcn_perm(acc_a, msk, sel_a[u & 3]);
+        acc_b = __builtin_amdgcn_perm(acc_b, msk, sel_b[u & 3]);
+        if ((u & 3) == 3) {
+          __builtin_amdgcn_raw_buffer_store_b32(acc_a, drsrc, acc_a ? oa : kSysDrop, 0, 0);
+          __builtin_amdgcn_raw_buffer_store_b32(acc_b, drsrc, acc_b ? ob : kSysDrop, 0, 0);
+          oa += 4;
+          ob += 4;
+        }
+      }
+    }
+    wprev = w[kC16U - 1];
+    wp += winc;
+  }
+
+  __device__ __forceinline__ void run16() {
+    load_wf(wB, fB, kC16U);
+    group(wA, fA);
+    load_wf(wA, fA, 2 * kC16U);
+    group(wB, fB);
+    wc += 2 * kC16U;
+    rr += 2 * kC16U;
+    t += 2 * kC16U;
+  }
+};
+
+// Column words of the pair (gap cost of s2[c] | 2 s2[c] << 16, 0xffff past
+// the ends) into Wc and row 0 into the hand-off row R0 (0xffff outside the
+// band), by `count` threads from `first`.
+__device__ __forceinline__ void c16_pair_rows(const uint8_t *s2, int N, const int *GP, int hi0, const uint16_t *T,
+                                              uint32_t *Wc, uint32_t *R0, int first, int count) {
+  for (int c = first - kC16Pad; c < N + kC16Pad; c += count) {
+    const int x = (c >= 0 && c < N) ? (s2[c] & 31) : 0;
+    Wc[c] = (uint32_t)T[kGap * 32 + x] | ((uint32_t)(2 * x) << 16);
+    R0[c] = (c >= 0 && c <= hi0) ? ((uint32_t)GP[c] << 16) | 0xffffu : 0xffffffffu;
+  }
+}
+
 template <bool kDm>
 __global__ __launch_bounds__(kWave * kC16Waves) void lin_cost16_kernel(
     const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
@@ -2759,12 +2905,8 @@ __global__ __launch_bounds__(kWave * kC16Waves) void lin_cost16_kernel(
     int lo0, hi0;
     lin_row_host(0, M, N, g, lo0, hi0);
     // column words and row 0 (algn_fill_first_row, :584-607, or the first
-    // row of algn_fill_plane); 0xffff outside the band
-    for (int c = lane - kC16Pad; c < N + kC16Pad; c += kWave) {
-      const int x = (c >= 0 && c < N) ? (s2[c] & 31) : 0;
-      Wc[c] = (uint32_t)T[kGap * 32 + x] | ((uint32_t)(2 * x) << 16);
-      R[c] = (c >= 0 && c <= hi0) ? ((uint32_t)GP[c] << 16) | 0xffffu : 0xffffffffu;
-    }
+    // row of algn_fill_plane)
+    c16_pair_rows(s2, N, GP, hi0, T, Wc, R, lane, kWave);
     uint8_t *dm = dm_base + m.dm_off;
     const int S = m.stride;
     if (kDm)
@@ -2774,114 +2916,11 @@ __global__ __launch_bounds__(kWave * kC16Waves) void lin_cost16_kernel(
     int last_half = 1;
     const int nstrips = (M - 1 + kC16Rows - 1) / kC16Rows;
     for (int s = 0; s < nstrips; ++s) {
-      const int i0 = 1 + kC16Rows * s;
-      int c0, hi_first, lo_l, hi_l;
-      lin_row_host(i0, M, N, g, c0, hi_first);
-      const int r_last = min(i0 + kC16Rows - 1, M - 1);
-      lin_row_host(r_last, M, N, g, lo_l, hi_l);
-      const int wl = (r_last - i0) >> 1, wh = (r_last - i0) & 1;   // lane and half of the last row
-      const int t_end = hi_l - c0 + 2 * wl + wh;
-      // one step more (the cell right of the band), rounded to two groups
-      const int steps = (t_end + 2 + 2 * kC16U - 1) / (2 * kC16U) * (2 * kC16U);
-      // this lane's rows
-      const int ra = i0 + 2 * lane, rb = ra + 1;
-      const bool va = ra < M, vb = rb < M;
-      int lo_a, hi_a, lo_b, hi_b;
-      lin_row_host(va ? ra : M - 1, M, N, g, lo_a, hi_a);
-      lin_row_host(vb ? rb : M - 1, M, N, g, lo_b, hi_b);
-      const int sa = va ? (s1[ra] & 31) : 0, sb = vb ? (s1[rb] & 31) : 0;
-      const uint32_t cdel = (uint32_t)T[sa * 32 + kGap] | ((uint32_t)T[sb * 32 + kGap] << 16);
-      const uint16_t *ta = T + sa * 32, *tb = T + sb * 32;
-      // column - lo of the step's cell (rows past the last one: never inside)
-      uint32_t kv = (va ? (uint32_t)(c0 - 2 * lane - lo_a) & 0xffffu : 0x8000u) |
-                    ((vb ? (uint32_t)(c0 - 2 * lane - 1 - lo_b) & 0xffffu : 0x8000u) << 16);
-      const uint32_t wid = (va ? (uint32_t)(hi_a - lo_a) : 0u) | ((vb ? (uint32_t)(hi_b - lo_b) : 0u) << 16);
-      const bool writer = lane == wl;
-      uint32_t *wp = writer ? R + (c0 - 2 * wl - wh) : D;
-      const int winc = writer ? kC16U : 0;
-      const uint32_t *wc = Wc + (c0 - 2 * lane);
-      const uint32_t *rr = R + c0;
-      // byte offsets of the dwords that steps 0..3 fill (rows past the last
-      // one store nothing: no byte of theirs is ever inside)
-      int oa = 0, ob = 0;
-      if (kDm) {
-        oa = ra * S + ((lo_a - c0 + 2 * lane) & 3) + (c0 - 2 * lane - lo_a);
-        ob = rb * S + ((lo_b - c0 + 2 * lane + 1) & 3) + (c0 - 2 * lane - 1 - lo_b);
-      }
-      uint32_t acc_a = 0, acc_b = 0;
-      uint32_t V = 0xffffffffu;
-      uint32_t upp = (R[c0 - 1] >> 16) | 0xffff0000u;   // lane 0's first diagonal; the rest lie outside
-      uint32_t wprev = wc[-1];
-      // A group of kC16U steps reads its substitution costs at its start;
-      // column words and lane 0's hand-off words are read one group ahead,
-      // into two register sets that alternate (no copies between groups).
-      uint32_t wA[kC16U], fA[kC16U], wB[kC16U], fB[kC16U];
-      auto load_wf = [&](uint32_t (&w)[kC16U], uint32_t (&f)[kC16U], int off) {
-#pragma unroll
-        for (int u = 0; u < kC16U; ++u) {
-          w[u] = wc[off + u];
-          f[u] = rr[off + u];
-        }
-      };
-      auto group = [&](const uint32_t (&w)[kC16U], const uint32_t (&f)[kC16U]) {
-        uint32_t cr[kC16U];
-#pragma unroll
-        for (int u = 0; u < kC16U; ++u) {
-          const uint32_t wq = u ? w[u - 1] : wprev;
-          cr[u] = (uint32_t)*(const uint16_t *)((const char *)ta + (w[u] >> 16)) |
-                  ((uint32_t)*(const uint16_t *)((const char *)tb + (wq >> 16)) << 16);
-        }
-#pragma unroll
-        for (int u = 0; u < kC16U; ++u) {
-          const uint32_t wq = u ? w[u - 1] : wprev;
-          const uint32_t sh = (uint32_t)__builtin_amdgcn_update_dpp((int)f[u], (int)V, 0x138 /* wave_shr:1 */, 0xf,
-                                                                    0xf, false);
-          const uint32_t up = __builtin_amdgcn_alignbit(V, sh, 16);
-          const uint32_t gj = __builtin_amdgcn_perm(wq, w[u], 0x05040100u);
-          const uint32_t t1 = c16_add_sat(up, cdel);    // DELETE
-          const uint32_t t2 = c16_add_sat(V, gj);       // INSERT
-          const uint32_t t3 = c16_add_sat(upp, cr[u]);  // ALIGN
-          const uint32_t best = c16_min(c16_min(t1, t2), t3);
-          const uint32_t gt = c16_sub_sat(kv, wid);
-          V = c16_gate(gt, best);
-          kv = c16_add(kv, 0x00010001u);
-          wp[u] = V;
-          upp = up;
-          if (kDm) {
-            // per half: 1 for a candidate equal to the minimum inside the band
-            const uint32_t ib = c16_sub_sat(0x00010001u, gt);
-            const uint32_t q1 = c16_sub_sat(ib, c16_sub_sat(t1, best));
-            const uint32_t q2 = c16_sub_sat(ib, c16_sub_sat(t2, best));
-            const uint32_t q3 = c16_sub_sat(ib, c16_sub_sat(t3, best));
-            const uint32_t msk = c16_add(c16_add(q3, c16_add(q2, q2)), c16u(c16v(q1) << (unsigned short)2));
-            // byte u mod 4 of the two dwords (the low half's code is byte 0
-            // of msk, the high half's byte 2); position 0 starts a dword
-            constexpr uint32_t sel_a[4] = {0x0c0c0c00u, 0x0c0c0004u, 0x0c000504u, 0x00060504u};
-            constexpr uint32_t sel_b[4] = {0x0c0c0c02u, 0x0c0c0204u, 0x0c020504u, 0x02060504u};
-            acc_a = __builtin_amdgcn_perm(acc_a, msk, sel_a[u & 3]);
-            acc_b = __builtin_amdgcn_perm(acc_b, msk, sel_b[u & 3]);
-            if ((u & 3) == 3) {
-              __builtin_amdgcn_raw_buffer_store_b32(acc_a, drsrc, acc_a ? oa : kSysDrop, 0, 0);
-              __builtin_amdgcn_raw_buffer_store_b32(acc_b, drsrc, acc_b ? ob : kSysDrop, 0, 0);
-              oa += 4;
-              ob += 4;
-            }
-          }
-        }
-        wprev = w[kC16U - 1];
-        wp += winc;
-      };
-      load_wf(wA, fA, 0);
+      C16Strip<kDm> st;
+      st.begin(1 + kC16Rows * s, M, N, g, s1, T, Wc, R, R, D, S, drsrc);
       dpp_exec_guard();
-      for (int t0 = 0; t0 < steps; t0 += 2 * kC16U) {
-        load_wf(wB, fB, kC16U);
-        group(wA, fA);
-        load_wf(wA, fA, 2 * kC16U);
-        group(wB, fB);
-        wc += 2 * kC16U;
-        rr += 2 * kC16U;
-      }
-      last_half = wh;
+      while (st.t < st.steps) st.run16();
+      last_half = st.wh;
       wave_sync_mem();
     }
     if (lane == 0) {
@@ -2889,6 +2928,112 @@ __global__ __launch_bounds__(kWave * kC16Waves) void lin_cost16_kernel(
       cost_out[p] = M == 1 ? GP[N - 1] : (int32_t)(last_half ? e >> 16 : e & 0xffffu);
     }
     wave_sync_mem();
+  }
+}
+
+// The packed 16-bit fill with the four waves of a workgroup on one pair, for
+// batches too small to fill the chip with one pair per wave (tree levels,
+// per-call entries): wave w runs strips w, w + 4, ..., the workgroup
+// advancing in chunks of 64 steps with a barrier between chunks.  Strip s
+// reads the hand-off row that strip s - 1 writes into buffer (s - 1) mod 4;
+// it starts enough chunks after strip s - 1 that every column it reads
+// (lane 0's row, read up to 23 steps ahead) was written by strip s - 1's
+// last row in an earlier chunk: strip s - 1 writes column c at its step
+// c - c0(s - 1) + 127 at the latest, strip s reads it at its step
+// c - c0(s) - 23 at the earliest, so the lag is c0(s) - c0(s - 1) + 150 steps
+// plus one chunk, rounded up to chunks.  A wave takes its next strip when
+// the previous one is done.  The schedule is the same arithmetic in every
+// wave (no waits: only the barrier orders the buffers).
+constexpr int kC16Chunk = 64;   // steps between the workgroup's barriers
+constexpr int kC16MwStrips = 512;   // strips the multi-wave fill schedules (rows up to 65,537)
+__host__ __device__ __forceinline__ int c16mw_words(int n_max) { return 5 * c16_cols(n_max) + kC16Waves * kWave * kC16U; }
+
+template <bool kDm>
+__global__ __launch_bounds__(kWave * kC16Waves) void lin_c16mw_kernel(
+    const PairMeta *__restrict__ meta, const int32_t *__restrict__ order, int n, int *counter,
+    const uint8_t *__restrict__ seq, const DevTcm *__restrict__ tcm, const int *__restrict__ aux,
+    int32_t *__restrict__ cost_out, uint8_t *dm_base, int n_max) {
+  __shared__ uint16_t T[32 * 32];
+  __shared__ int s_start[kC16MwStrips];   // first chunk of each strip
+  __shared__ int s_item, s_chunks;
+  extern __shared__ uint32_t c16_dyn[];
+  for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) T[k] = (uint16_t)min(tcm->cost[k], 0xffff);
+  const int lane = lane_id(), wv = wave_in_group();
+  const int cols = c16_cols(n_max);
+  uint32_t *const Wc = c16_dyn + kC16Pad;
+  uint32_t *const Rb = c16_dyn + cols + kC16Pad;   // hand-off buffer r at Rb + r * cols
+  uint32_t *const D = c16_dyn + 5 * cols + (wv * kWave + lane) * kC16U;
+  for (;;) {
+    __syncthreads();   // the previous pair is done in every wave
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int k = uniform32(s_item);
+    if (k >= n) break;
+    const int p = uniform32(order[k]);
+    const PairMeta m = uniform_meta(meta, p);
+    const uint8_t *s1 = seq + m.off_r;
+    const uint8_t *s2 = seq + m.off_c;
+    const int M = m.lr, N = m.lc;
+    const LinGeom g{m.p0, m.p1, m.p2};
+    const int *GP = aux + m.aux_off + N;
+    int lo0, hi0;
+    lin_row_host(0, M, N, g, lo0, hi0);
+    const int nstrips = (M - 1 + kC16Rows - 1) / kC16Rows;
+    // row 0 is the hand-off row of strip -1 (buffer 3)
+    c16_pair_rows(s2, N, GP, hi0, T, Wc, Rb + 3 * cols, threadIdx.x, blockDim.x);
+    uint8_t *dm = dm_base + m.dm_off;
+    const int S = m.stride;
+    if (kDm)
+      for (int j = threadIdx.x; j <= hi0; j += blockDim.x) dm[j] = j == 0 ? 1 : 2;   // row 0: ALIGN, then INSERT
+    if (threadIdx.x == 0) {
+      int prev_c0 = 0, end = 0, ends[kC16Waves] = {0, 0, 0, 0};
+      for (int s = 0; s < nstrips; ++s) {
+        const int i0 = 1 + kC16Rows * s;
+        int c0, hi_first, lo_l, hi_l;
+        lin_row_host(i0, M, N, g, c0, hi_first);
+        const int r_last = min(i0 + kC16Rows - 1, M - 1);
+        lin_row_host(r_last, M, N, g, lo_l, hi_l);
+        const int t_end = hi_l - c0 + 2 * ((r_last - i0) >> 1) + ((r_last - i0) & 1);
+        const int steps = (t_end + 2 + 2 * kC16U - 1) / (2 * kC16U) * (2 * kC16U);
+        int st = s == 0 ? 0 : s_start[s - 1] + (c0 - prev_c0 + 150 + 2 * kC16Chunk - 1) / kC16Chunk;
+        st = max(st, ends[s % kC16Waves]);   // this wave's previous strip is done
+        s_start[s] = st;
+        ends[s % kC16Waves] = st + (steps + kC16Chunk - 1) / kC16Chunk;
+        end = max(end, ends[s % kC16Waves]);
+        prev_c0 = c0;
+      }
+      s_chunks = end;
+    }
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t drsrc = __builtin_amdgcn_make_buffer_rsrc(dm, 0, kDm ? M * S : 0, 0x00020000);
+    const int chunks = uniform32(s_chunks);
+    C16Strip<kDm> st;
+    int cur = wv;   // this wave's strip
+    bool on = false;
+    for (int ch = 0; ch < chunks; ++ch) {
+      if (ch) __syncthreads();
+      if (cur < nstrips && !on && ch == s_start[cur]) {
+        st.begin(1 + kC16Rows * cur, M, N, g, s1, T, Wc, Rb + ((cur + 3) & 3) * cols, Rb + (cur & 3) * cols, D, S,
+                 drsrc);
+        on = true;
+      }
+      if (on) {
+        dpp_exec_guard();
+#pragma unroll 1
+        for (int q = 0; q < kC16Chunk / (2 * kC16U) && st.t < st.steps; ++q) st.run16();
+        if (st.t >= st.steps) {
+          on = false;
+          cur += kC16Waves;
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int last_half = 1;
+      if (nstrips) last_half = (M - 1 - (1 + kC16Rows * (nstrips - 1))) & 1;
+      const uint32_t e = Rb[((nstrips - 1) & 3) * cols + N - 1];
+      cost_out[p] = M == 1 ? GP[N - 1] : (int32_t)(last_half ? e >> 16 : e & 0xffffu);
+    }
   }
 }
 
@@ -3537,6 +3682,30 @@ hipError_t launch_lin_cost16(bool with_dm, int grid_waves, const PairMeta *meta,
                        tcm, aux, cost, dm, n_max);
   else
     hipLaunchKernelGGL(lin_cost16_kernel<false>, dim3(grid), dim3(kWave * waves), lds, s, meta, order, n, counter,
+                       seq, tcm, aux, cost, dm, n_max);
+  return hipGetLastError();
+}
+
+int lin_c16mw_blocks_per_cu(bool with_dm, int n_max) {
+  const int lds = 4 * c16mw_words(n_max);
+  return occupancy_cached(((int64_t)0x800 + with_dm) << 32 | (int64_t)lds, [&] {
+    int b = 0;
+    const hipError_t e =
+        with_dm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_c16mw_kernel<true>, kWave * kC16Waves, lds)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, lin_c16mw_kernel<false>, kWave * kC16Waves, lds);
+    return (e == hipSuccess && b > 0) ? b : 1;
+  });
+}
+
+hipError_t launch_lin_c16mw(bool with_dm, int grid, const PairMeta *meta, const int32_t *order, int n, int *counter,
+                            const uint8_t *seq, const DevTcm *tcm, const int *aux, int32_t *cost, uint8_t *dm,
+                            int n_max, hipStream_t s) {
+  const int lds = 4 * c16mw_words(n_max);
+  if (with_dm)
+    hipLaunchKernelGGL(lin_c16mw_kernel<true>, dim3(grid), dim3(kWave * kC16Waves), lds, s, meta, order, n, counter,
+                       seq, tcm, aux, cost, dm, n_max);
+  else
+    hipLaunchKernelGGL(lin_c16mw_kernel<false>, dim3(grid), dim3(kWave * kC16Waves), lds, s, meta, order, n, counter,
                        seq, tcm, aux, cost, dm, n_max);
   return hipGetLastError();
 }

@@ -34,17 +34,29 @@ def ctx():
         pytest.skip(f"no GPU: {e}")
 
 
+def with_kernels(ctx, run):
+    """run() under the packed fill's choice (default), one pair per wave (3),
+    four waves per pair (4) and the systolic fill only (1)."""
+    out = {}
+    for knob in (-1, 3, 4, 1):
+        ctx.set_tuning("lin_cost_kernel", knob)
+        try:
+            out[knob] = run()
+        finally:
+            ctx.set_tuning("lin_cost_kernel", -1)
+    return out
+
+
 def both_kernels(ctx, cm, A, B, dws=None):
-    """Costs of a LINEAR_COST batch with the packed fill (default) and with
+    """Costs of a LINEAR_COST batch with the packed fill (default; the one
+    pair per wave and four waves per pair forms give the same costs) and with
     the systolic fill only."""
     from poyd_amd import engine
-    packed = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_COST, batch_of(A), batch_of(B), dws).cost.copy()
-    ctx.set_tuning("lin_cost_kernel", 1)
-    try:
-        systolic = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_COST, batch_of(A), batch_of(B), dws).cost.copy()
-    finally:
-        ctx.set_tuning("lin_cost_kernel", -1)
-    return packed, systolic
+    r = with_kernels(ctx, lambda: engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_COST, batch_of(A), batch_of(B),
+                                                   dws).cost.copy())
+    assert np.array_equal(r[3], r[-1]) and np.array_equal(r[4], r[-1]), \
+        np.nonzero((r[3] != r[-1]) | (r[4] != r[-1]))[0][:10]
+    return r[-1], r[1]
 
 
 def shape_pairs(rng, alphabet):
@@ -160,14 +172,16 @@ def test_costs_beyond_16_bits_use_the_systolic_fill(ctx, oracle):
 
 
 def both_align(ctx, cm, A, B):
+    """LINEAR_ALIGN results of the packed fill (default; its two forms give
+    the same bytes) and of the systolic fill only."""
     from poyd_amd import engine
-    packed = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
-    ctx.set_tuning("lin_cost_kernel", 1)
-    try:
-        systolic = engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B))
-    finally:
-        ctx.set_tuning("lin_cost_kernel", -1)
-    return packed, systolic
+    r = with_kernels(ctx, lambda: engine.run_pairs(ctx, cm, engine.POYG_MODE_LINEAR_ALIGN, batch_of(A), batch_of(B)))
+    for knob in (3, 4):
+        assert np.array_equal(r[knob].cost, r[-1].cost), knob
+        for k in range(len(A)):
+            for slot in range(3):
+                assert np.array_equal(r[knob].seq(k, slot), r[-1].seq(k, slot)), (knob, k, slot)
+    return r[-1], r[1]
 
 
 @pytest.mark.parametrize("sub,indel,alphabet", [(1, 2, "dna"), (1, 1, "ambig"), (3, 2, "any")])
@@ -214,7 +228,7 @@ def test_packed_align_large_batch(ctx, oracle):
         assert packed.cost[k] == cost and np.array_equal(packed.seq(k, 0), oa), k
 
 
-@pytest.mark.parametrize("kernel", [-1, 1])
+@pytest.mark.parametrize("kernel", [-1, 3, 1])
 def test_direction_matrix_equals_oracle(ctx, oracle, kernel):
     """poyg_algn_direction_matrix (the Matrix.default readers' view) equals
     the oracle's algn_nw direction matrix cell for cell, for the packed and
