@@ -239,6 +239,21 @@ int poyg_plan_set_timing(poyg_plan *plan, int enable);
  * correct run).  0 when every pair completed normally. */
 int poyg_plan_status(poyg_plan *plan, int32_t *bad);
 
+/* A batch of pairs whose sequences are in HOST memory, planned, run on the
+ * context's stream and read back in one call (no reference counterpart: the
+ * batch route of a host-side tree driver, with one upload of descriptors and
+ * sequences and one synchronization).  seq / seq_bytes: the sequences; the
+ * off / len arrays index into seq (host arrays, as poyg_plan_pairs).
+ * cost[n] receives the costs; in the alignment modes out (out_cap bytes, at
+ * least poyg_plan_out_bytes of the same plan: slots of cap_k bytes per pair
+ * in pair order, laid out as above) and out_len[n * slots] the outputs.
+ * *cells: the band cells evaluated; *failed: as poyg_plan_status (the other
+ * pairs' results are valid).  Not for concurrent use of one context. */
+int poyg_run_pairs_host(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, const uint8_t *seq,
+                        int64_t seq_bytes, const int64_t *off_a, const int32_t *len_a, const int64_t *off_b,
+                        const int32_t *len_b, const int32_t *deltaw, int32_t *cost, uint8_t *out, int64_t out_cap,
+                        int32_t *out_len, int64_t *cells, int32_t *failed);
+
 /* ---- batched three-sequence alignment --------------------------------- */
 /* Plans n triples for algn_CAML_align_3d + algn_CAML_median_3
  * (src/algn.c:3457-3478, 4004-4017, 4223-4235).  Runs through

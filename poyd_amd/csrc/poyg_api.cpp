@@ -17,7 +17,7 @@ namespace poyg {
 hipError_t launch_reset(int *counters, int n, hipStream_t s);
 hipError_t launch_plan_start(int *ctr, int nctr, int *status, int nstatus, TraceOut *tro, uint8_t *out,
                              int32_t *out_len, hipStream_t s);
-hipError_t launch_collect_results(const int32_t *cost, const int32_t *status, const int32_t *lens, int slots,
+hipError_t launch_collect_results(const uint32_t *small, int64_t small_words, const int32_t *status,
                                   const uint32_t *out, int64_t out_words, uint32_t *host, hipStream_t s);
 hipError_t launch_prep(bool affine, int grid, const PairMeta *meta, int n, const uint8_t *seq, const DevTcm *tcm,
                        int *aux, hipStream_t s);
@@ -622,14 +622,14 @@ static constexpr int kAffRBHost = 256;   // must match kAffRB in the kernels
 static constexpr int kLinRBHost = 512;   // must match kLinRB
 static constexpr int kLinStageHost = 2048;   // must match kLinStage
 
-// The per-call route: the pair's two sequences travel in the plan's
-// descriptor block (one upload from the context's pinned staging, with no
-// synchronization).  The caller runs the plan on ctx->stream with d_seq =
-// dev and synchronizes that stream before the next per-call plan reuses the
-// staging.
+// The host-sequence route: the sequences travel in the plan's descriptor
+// block (one upload from the context's pinned staging, with no
+// synchronization): `a` at offset 0 and, if given, `b` at off_b.  The caller
+// runs the plan on ctx->stream with d_seq = dev and synchronizes that stream
+// before the next such plan reuses the staging.
 struct PerCallSeqs {
   const uint8_t *a, *b;
-  int la, lb;
+  int64_t la, lb;
   int64_t off_b;                  // b's offset from a (16-byte aligned)
   const uint8_t *dev = nullptr;   // out: a's copy on the device
 };
@@ -960,7 +960,8 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
     blob_bytes += (std::max<size_t>(sec.second.second, 16) + 255) & ~(size_t)255;
   }
   const size_t seq_at = blob_bytes;   // the per-call route's sequences
-  if (per_call) blob_bytes += (size_t)(pc->off_b + pc->lb + 255) & ~(size_t)255;
+  if (per_call)   // a at 0, b (if any) at off_b
+    blob_bytes += (size_t)(std::max(pc->la, pc->b ? pc->off_b + pc->lb : 0) + 255) & ~(size_t)255;
   std::vector<uint8_t> staging_v;
   uint8_t *staging = per_call ? ctx_pinned(ctx->pin_plan, ctx->pin_plan_cap, blob_bytes) : nullptr;
   const bool sync_upload = staging == nullptr;
@@ -973,8 +974,8 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
   for (size_t k = 0; k < sections.size(); ++k)
     if (sections[k].second.second) memcpy(staging + sec_off[k], sections[k].second.first, sections[k].second.second);
   if (per_call) {
-    memcpy(staging + seq_at, pc->a, pc->la);
-    memcpy(staging + seq_at + pc->off_b, pc->b, pc->lb);
+    if (pc->la) memcpy(staging + seq_at, pc->a, (size_t)pc->la);
+    if (pc->b && pc->lb) memcpy(staging + seq_at + pc->off_b, pc->b, (size_t)pc->lb);
   }
   int rc = 0;
   rc |= P->blob.alloc(blob_bytes);
@@ -1176,55 +1177,76 @@ extern "C" int poyg_plan_status(poyg_plan *P, int32_t *bad) {
 // ---------------------------------------------------------------------------
 namespace {
 
-struct OnePair {
-  DevBuf cost, out, len;
-  const uint8_t *h_out = nullptr;   // results in the context's pinned staging
-  const int32_t *h_len = nullptr;
-  int32_t h_cost = 0;
+// The host-sequence route: one upload (descriptor block and sequences from
+// pinned staging), the plan's kernels, one kernel that copies every result
+// into the context's pinned result buffer, one synchronization.  The results
+// (and the plan, for readers of its direction band) stay valid until the next
+// call on the context.
+struct HostRun {
+  DevBuf small, out;
   poyg_plan *plan = nullptr;
   bool idle = false;   // every queued use of the buffers has completed
-  ~OnePair() {
-    cost.release(idle); out.release(idle); len.release(idle);
+  const int32_t *h_cost = nullptr, *h_len = nullptr;
+  const uint8_t *h_out = nullptr;
+  int32_t failed = 0;
+  ~HostRun() {
+    small.release(idle);
+    out.release(idle);
     if (plan) plan->idle = idle;
     poyg_plan_destroy(plan);
   }
-  // One upload (descriptors and both sequences, from pinned staging), the
-  // plan's kernels, one kernel that writes every result into pinned host
-  // memory, and one synchronization.
+  int run(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, PerCallSeqs &in, const int64_t *off_a,
+          const int32_t *len_a, const int64_t *off_b, const int32_t *len_b, const int32_t *dw) {
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipStreamSynchronize(s));   // the staging of an earlier call is free
+    int rc = plan_pairs_impl(ctx, cm, mode, n, off_a, len_a, off_b, len_b, dw, &plan, &in);
+    if (rc) return rc;
+    const int slots = plan->slots;
+    // device results: costs (n, rounded to 4 words), then output lengths
+    const int64_t n_cost = ((int64_t)n + 3) & ~(int64_t)3;
+    const int64_t small_words = n_cost + (((int64_t)n * slots + 3) & ~(int64_t)3);
+    const int64_t out_bytes = slots ? plan->out_bytes : 0;   // a multiple of 16
+    const size_t res_bytes = 64 + 4 * (size_t)small_words + (size_t)out_bytes;
+    uint8_t *pin = ctx_pinned(ctx->pin_io, ctx->pin_io_cap, res_bytes, &ctx->pin_io_dev);
+    if (!pin) {
+      set_error("pinned staging of %zu bytes failed", res_bytes);
+      return POYG_ENOMEM;
+    }
+    if (small.alloc(4 * (size_t)small_words)) return POYG_ENOMEM;
+    if (slots && out.alloc((size_t)out_bytes)) return POYG_ENOMEM;
+    int32_t *d_small = (int32_t *)small.p;
+    rc = poyg_plan_run(plan, in.dev, d_small, (uint8_t *)out.p, slots ? d_small + n_cost : nullptr, s);
+    if (rc) return rc;
+    // pinned: word 0 the failed-pair count, from word 16 the device results,
+    // then the output sequences
+    HIP_TRY(launch_collect_results((const uint32_t *)small.p, small_words, (const int32_t *)plan->status.p,
+                                   (const uint32_t *)out.p, out_bytes / 4, (uint32_t *)ctx->pin_io_dev, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    idle = true;
+    const int32_t *h = (const int32_t *)pin;
+    failed = h[0];
+    h_cost = h + 16;
+    h_len = h + 16 + n_cost;
+    h_out = pin + 64 + 4 * small_words;
+    return POYG_OK;
+  }
+};
+
+// A one-pair run of the host-sequence route (the per-call entry points).
+struct OnePair : HostRun {
+  int32_t h_cost1 = 0;
   int run(poyg_ctx *ctx, const poyg_cm *cm, int mode, const uint8_t *a, int la, const uint8_t *b, int lb,
           int dw) {
     if (!a || !b || la < 1 || lb < 1) {
       set_error("sequence arguments must be non-empty (leading gap included)");
       return POYG_EINVAL;
     }
-    hipStream_t s = ctx->stream;
-    HIP_TRY(hipStreamSynchronize(s));   // the staging of an earlier call is free
     int64_t offa = 0, offb = round16(la);
     PerCallSeqs in{a, b, la, lb, offb};
-    int rc = plan_pairs_impl(ctx, cm, mode, 1, &offa, &la, &offb, &lb, &dw, &plan, &in);
+    int rc = HostRun::run(ctx, cm, mode, 1, in, &offa, &la, &offb, &lb, &dw);
     if (rc) return rc;
-    const int slots = plan->slots;
-    const int64_t out_bytes = slots ? plan->out_bytes : 0;   // a multiple of 16
-    const size_t res_bytes = 80 + (size_t)out_bytes;
-    uint8_t *pin = ctx_pinned(ctx->pin_io, ctx->pin_io_cap, res_bytes, &ctx->pin_io_dev);
-    if (!pin) {
-      set_error("pinned staging of %zu bytes failed", res_bytes);
-      return POYG_ENOMEM;
-    }
-    if (cost.alloc(4)) return POYG_ENOMEM;
-    if (slots && (out.alloc((size_t)out_bytes) || len.alloc(sizeof(int32_t) * slots))) return POYG_ENOMEM;
-    rc = poyg_plan_run(plan, in.dev, (int32_t *)cost.p, (uint8_t *)out.p, (int32_t *)len.p, s);
-    if (rc) return rc;
-    HIP_TRY(launch_collect_results((const int32_t *)cost.p, (const int32_t *)plan->status.p,
-                                   (const int32_t *)len.p, slots, (const uint32_t *)out.p, out_bytes / 4,
-                                   (uint32_t *)ctx->pin_io_dev, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    idle = true;
-    const int32_t *ri = (const int32_t *)pin;
-    h_cost = ri[0];
-    h_len = ri + 4;
-    h_out = pin + 80;
-    if (ri[1]) {
+    h_cost1 = h_cost[0];
+    if (failed) {
       set_error("backtrace left the filled band");
       return POYG_EBAND;
     }
@@ -1248,6 +1270,43 @@ struct OnePair {
 
 extern "C" {
 
+int poyg_run_pairs_host(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, const uint8_t *seq,
+                        int64_t seq_bytes, const int64_t *off_a, const int32_t *len_a, const int64_t *off_b,
+                        const int32_t *len_b, const int32_t *deltaw, int32_t *cost, uint8_t *out, int64_t out_cap,
+                        int32_t *out_len, int64_t *cells, int32_t *failed) {
+  if (!ctx || !cm || n < 0 || seq_bytes < 0 || (n > 0 && (!seq || !off_a || !len_a || !off_b || !len_b || !cost)) ||
+      mode < POYG_MODE_LINEAR_COST || mode > POYG_MODE_AFFINE_ALIGN) {
+    set_error("poyg_run_pairs_host: bad arguments");
+    return POYG_EINVAL;
+  }
+  if (cells) *cells = 0;
+  if (failed) *failed = 0;
+  if (n == 0) return POYG_OK;
+  for (int32_t k = 0; k < n; ++k)
+    if (off_a[k] < 0 || off_b[k] < 0 || len_a[k] < 1 || len_b[k] < 1 || off_a[k] + len_a[k] > seq_bytes ||
+        off_b[k] + len_b[k] > seq_bytes) {
+      set_error("poyg_run_pairs_host: pair %d lies outside the %lld sequence bytes", k, (long long)seq_bytes);
+      return POYG_EINVAL;
+    }
+  HostRun hr;
+  PerCallSeqs in{seq, nullptr, seq_bytes, 0, 0};
+  int rc = hr.run(ctx, cm, mode, n, in, off_a, len_a, off_b, len_b, deltaw);
+  if (rc) return rc;
+  const int slots = hr.plan->slots;
+  if (slots) {
+    if (!out || !out_len || out_cap < hr.plan->out_bytes) {
+      set_error("poyg_run_pairs_host: the outputs need %lld bytes", (long long)hr.plan->out_bytes);
+      return POYG_EINVAL;
+    }
+    memcpy(out, hr.h_out, (size_t)hr.plan->out_bytes);
+    memcpy(out_len, hr.h_len, sizeof(int32_t) * (size_t)n * slots);
+  }
+  memcpy(cost, hr.h_cost, sizeof(int32_t) * (size_t)n);
+  if (cells) *cells = hr.plan->cells;
+  if (failed) *failed = hr.failed;
+  return POYG_OK;
+}
+
 int poyg_algn_simple_2(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int l1, const uint8_t *s2, int l2,
                        int deltawh, int *cost) {
   if (l1 < l2) {
@@ -1256,7 +1315,7 @@ int poyg_algn_simple_2(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int 
   }
   OnePair op;
   int rc = op.run(ctx, cm, POYG_MODE_LINEAR_COST, s1, l1, s2, l2, deltawh);
-  if (!rc && cost) *cost = op.h_cost;
+  if (!rc && cost) *cost = op.h_cost1;
   return rc;
 }
 
@@ -1300,7 +1359,7 @@ int poyg_algn_direction_matrix(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *
     }
     for (int j = lo; j <= hi; ++j) dm[(size_t)i * l2 + j] = band[(size_t)i * S + phase + (j - lo)];
   }
-  if (cost) *cost = op.h_cost;
+  if (cost) *cost = op.h_cost1;
   return POYG_OK;
 }
 
@@ -1322,7 +1381,7 @@ int poyg_algn_align_2d(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *s1, int 
   int rc = swaped ? op.run(ctx, cm, POYG_MODE_LINEAR_ALIGN, s1, l1, s2, l2, deltawh)
                   : op.run(ctx, cm, POYG_MODE_LINEAR_ALIGN, s2, l2, s1, l1, deltawh);
   if (rc) return rc;
-  if (cost) *cost = op.h_cost;
+  if (cost) *cost = op.h_cost1;
   rc = op.take(swaped ? 0 : 1, r1, cap, rlen);
   if (!rc) rc = op.take(swaped ? 1 : 0, r2, cap, nullptr);
   return rc;
@@ -1334,7 +1393,7 @@ int poyg_algn_align_affine_3(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *si
   OnePair op;
   int rc = op.run(ctx, cm, POYG_MODE_AFFINE_ALIGN, si, li, sj, lj, 0);
   if (rc) return rc;
-  if (cost) *cost = op.h_cost;
+  if (cost) *cost = op.h_cost1;
   uint8_t *dst[4] = {median, medianwg, resi, resj};
   for (int k = 0; k < 4 && !rc; ++k) rc = op.take(k, dst[k], cap, lens ? lens + k : nullptr);
   return rc;
@@ -1344,7 +1403,7 @@ int poyg_algn_cost_affine_3(poyg_ctx *ctx, const poyg_cm *cm, const uint8_t *si,
                             int lj, int *cost) {
   OnePair op;
   int rc = op.run(ctx, cm, POYG_MODE_AFFINE_COST, si, li, sj, lj, 0);
-  if (!rc && cost) *cost = op.h_cost;
+  if (!rc && cost) *cost = op.h_cost1;
   return rc;
 }
 

@@ -187,19 +187,17 @@ __global__ void plan_start_kernel(int *ctr, int nctr, int *status, int nstatus, 
   if (tro && t == 0) *tro = TraceOut{out, out_len, (int32_t *)status};
 }
 
-// The per-call route's results, written into pinned host memory: word 0 the
-// cost, word 1 the failed-pair count (status word 0), words 4.. the output
-// lengths, and from word 20 (byte 80) the output sequences.  The system-scope
-// fence makes the writes visible to the host once the stream is synchronized.
-__global__ void collect_results_kernel(const int32_t *cost, const int32_t *status, const int32_t *lens, int slots,
+// The host-sequence route's results, written into pinned host memory: word
+// 0 the failed-pair count (status word 0), from word 16 the run's small
+// results (costs, output lengths), then the output sequences.  The
+// system-scope fence makes the writes visible to the host once the stream is
+// synchronized.
+__global__ void collect_results_kernel(const uint32_t *small, int64_t small_words, const int32_t *status,
                                        const uint32_t *out, int64_t out_words, uint32_t *host) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
-  if (t == 0) {
-    host[0] = (uint32_t)cost[0];
-    host[1] = (uint32_t)status[0];
-  }
-  if (t < slots) host[4 + t] = (uint32_t)lens[t];
-  for (int64_t w = t; w < out_words; w += stride) host[20 + w] = out[w];
+  if (t == 0) host[0] = (uint32_t)status[0];
+  for (int64_t w = t; w < small_words; w += stride) host[16 + w] = small[w];
+  for (int64_t w = t; w < out_words; w += stride) host[16 + small_words + w] = out[w];
   __threadfence_system();
 }
 
@@ -3446,10 +3444,11 @@ hipError_t launch_plan_start(int *ctr, int nctr, int *status, int nstatus, Trace
   return hipGetLastError();
 }
 
-hipError_t launch_collect_results(const int32_t *cost, const int32_t *status, const int32_t *lens, int slots,
+hipError_t launch_collect_results(const uint32_t *small, int64_t small_words, const int32_t *status,
                                   const uint32_t *out, int64_t out_words, uint32_t *host, hipStream_t s) {
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((out_words + 255) / 256, 256));
-  hipLaunchKernelGGL(collect_results_kernel, dim3(grid), dim3(256), 0, s, cost, status, lens, slots, out, out_words,
+  const int64_t words = std::max(small_words, out_words);
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((words + 255) / 256, 1024));
+  hipLaunchKernelGGL(collect_results_kernel, dim3(grid), dim3(256), 0, s, small, small_words, status, out, out_words,
                      host);
   return hipGetLastError();
 }

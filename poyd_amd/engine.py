@@ -47,6 +47,8 @@ SIGNATURES = {
     "poyg_dev_upload": (C.c_int, [vp, vp, vp, C.c_int64]),
     "poyg_dev_download": (C.c_int, [vp, vp, vp, C.c_int64]),
     "poyg_plan_status": (C.c_int, [vp, i32p]),
+    "poyg_run_pairs_host": (C.c_int, [vp, vp, C.c_int, C.c_int32, u8p, C.c_int64, i64p, i32p, i64p, i32p, i32p, i32p,
+                                      u8p, C.c_int64, i32p, i64p, i32p]),
     "poyg_cm_create": (C.c_int, [C.c_int] * 5 + [C.POINTER(vp)]),
     "poyg_cm_destroy": (None, [vp]),
     "poyg_cm_set_cost": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
@@ -685,33 +687,38 @@ def _gaps_per_item(batch) -> np.ndarray:
 
 def run_pairs(ctx: Context, cm: CostMatrix, mode: int, a_batch, b_batch, dws=None,
               check_band: bool = True) -> PairResult:
-    """Convenience: upload two SeqBatch-like objects (seq/off/ln), run a
-    plan and download every result."""
-    seq = np.concatenate([a_batch.seq, b_batch.seq])
-    off_a = a_batch.off
-    off_b = b_batch.off + len(a_batch.seq)
+    """Convenience: run two SeqBatch-like objects (seq/off/ln, host memory)
+    as one batch through poyg_run_pairs_host (one upload, one
+    synchronization) and return every result."""
+    seq = np.ascontiguousarray(np.concatenate([a_batch.seq, b_batch.seq]), dtype=np.uint8)
+    off_a = np.ascontiguousarray(a_batch.off, dtype=np.int64)
+    off_b = np.ascontiguousarray(np.asarray(b_batch.off, np.int64) + len(a_batch.seq))
+    la = np.ascontiguousarray(a_batch.ln, dtype=np.int32)
+    lb = np.ascontiguousarray(b_batch.ln, dtype=np.int32)
+    n = len(la)
     if dws is None and mode in (POYG_MODE_LINEAR_COST, POYG_MODE_LINEAR_ALIGN):
         dws = batch_deltaw(a_batch, b_batch)
-    plan = PairPlan(ctx, cm, mode, off_a, a_batch.ln, off_b, b_batch.ln, dws)
-    d_seq = DeviceBuffer.from_array(ctx, seq)
-    # costs and output lengths share one block (one download for both)
-    n_cost = (max(plan.n, 1) + 63) // 64 * 64
-    n_len = plan.n * plan.slots
-    d_small = DeviceBuffer(ctx, 4 * (n_cost + max(n_len, 1)))
-    d_out = DeviceBuffer(ctx, max(plan.out_bytes, 16)) if plan.slots else None
-    plan.run(d_seq.ptr, d_small.ptr, d_out.ptr if d_out else None,
-             d_small.ptr + 4 * n_cost if plan.slots else None)
-    ctx.sync()
-    if check_band:
-        bad = C.c_int32(0)
-        _check(load().poyg_plan_status(plan.h, C.byref(bad)), "poyg_plan_status")
-        if bad.value:
-            raise PoygError(f"{bad.value} pairs failed (a backtrace left the filled band, "
-                            "or a fill's wait for the previous strip gave up)")
-    small = d_small.download(np.int32, n_cost + n_len)
-    res = PairResult(small[:plan.n].copy(),
-                     d_out.download(np.uint8, plan.out_bytes) if d_out else None,
-                     small[n_cost:n_cost + n_len].copy() if plan.slots else None,
-                     plan.out_off, plan.cap, plan.slots, int(plan.cells))
-    plan.close()
-    return res
+    dw = None if dws is None else np.ascontiguousarray(dws, dtype=np.int32)
+    slots = {POYG_MODE_LINEAR_ALIGN: 3, POYG_MODE_AFFINE_ALIGN: 4}.get(mode, 0)
+    # the plan's output layout: slots of round16(la + lb + 2) bytes per pair
+    cap = ((la.astype(np.int64) + lb + 2 + 15) // 16) * 16
+    out_off = np.zeros(n, np.int64)
+    if n > 1:
+        out_off[1:] = np.cumsum(slots * cap[:-1])
+    out_bytes = int(slots * cap.sum()) if slots else 0
+    cost = np.zeros(n, np.int32)
+    out = np.empty(max(out_bytes, 16), np.uint8) if slots else None
+    out_len = np.zeros(max(n * slots, 1), np.int32) if slots else None
+    cells = C.c_int64(0)
+    failed = C.c_int32(0)
+    _check(load().poyg_run_pairs_host(ctx.h, cm.h, mode, n, _p8(seq), len(seq), off_a.ctypes.data_as(i64p),
+                                      la.ctypes.data_as(i32p), off_b.ctypes.data_as(i64p), lb.ctypes.data_as(i32p),
+                                      None if dw is None else dw.ctypes.data_as(i32p), cost.ctypes.data_as(i32p),
+                                      None if out is None else _p8(out), out_bytes,
+                                      None if out_len is None else out_len.ctypes.data_as(i32p), C.byref(cells),
+                                      C.byref(failed)), "poyg_run_pairs_host")
+    if check_band and failed.value:
+        raise PoygError(f"{failed.value} pairs failed (a backtrace left the filled band, "
+                        "or a fill's wait for the previous strip gave up)")
+    return PairResult(cost, out, out_len, out_off if slots else None, cap if slots else None, slots,
+                      int(cells.value))
