@@ -243,7 +243,9 @@ int poyg_plan_status(poyg_plan *plan, int32_t *bad);
  * context's stream and read back in one call (no reference counterpart: the
  * batch route of a host-side tree driver, with one upload of descriptors and
  * sequences and one synchronization).  seq / seq_bytes: the sequences; the
- * off / len arrays index into seq (host arrays, as poyg_plan_pairs).
+ * off / len arrays index into seq (host arrays, as poyg_plan_pairs); deltaw
+ * as poyg_plan_pairs, or NULL in the linear modes for the default band of
+ * Sequence.Align.cost_2 (?deltaw absent) of every pair.
  * cost[n] receives the costs; in the alignment modes out (out_cap bytes, at
  * least poyg_plan_out_bytes of the same plan: slots of cap_k bytes per pair
  * in pair order, laid out as above) and out_len[n * slots] the outputs.

@@ -1288,6 +1288,27 @@ int poyg_run_pairs_host(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, c
       set_error("poyg_run_pairs_host: pair %d lies outside the %lld sequence bytes", k, (long long)seq_bytes);
       return POYG_EINVAL;
     }
+  // linear modes without deltaw: the default band of Sequence.Align.cost_2
+  // (src/sequence.ml:691-713, ?deltaw absent): the larger count of
+  // gap-bearing elements, plus half of 10% of the longer length when the
+  // lengths differ by less than that, else 2
+  std::vector<int32_t> dw_default;
+  const bool linear = mode == POYG_MODE_LINEAR_COST || mode == POYG_MODE_LINEAR_ALIGN;
+  if (linear && !deltaw) {
+    dw_default.resize(n);
+    auto gaps = [&](int64_t off, int32_t len) {
+      int64_t g = 0;
+      for (int32_t x = 0; x < len; ++x) g += (seq[off + x] & cm->gap) != 0;
+      return g;
+    };
+    for (int32_t k = 0; k < n; ++k) {
+      const int64_t longer = std::max(len_a[k], len_b[k]), dif = std::abs((int64_t)len_a[k] - len_b[k]);
+      const int64_t lower = (int64_t)((double)longer * 0.10);
+      dw_default[k] = (int32_t)(std::max(gaps(off_a[k], len_a[k]), gaps(off_b[k], len_b[k])) +
+                                (dif < lower ? lower / 2 : 2));
+    }
+    deltaw = dw_default.data();
+  }
   HostRun hr;
   PerCallSeqs in{seq, nullptr, seq_bytes, 0, 0};
   int rc = hr.run(ctx, cm, mode, n, in, off_a, len_a, off_b, len_b, deltaw);

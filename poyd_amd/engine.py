@@ -696,8 +696,7 @@ def run_pairs(ctx: Context, cm: CostMatrix, mode: int, a_batch, b_batch, dws=Non
     la = np.ascontiguousarray(a_batch.ln, dtype=np.int32)
     lb = np.ascontiguousarray(b_batch.ln, dtype=np.int32)
     n = len(la)
-    if dws is None and mode in (POYG_MODE_LINEAR_COST, POYG_MODE_LINEAR_ALIGN):
-        dws = batch_deltaw(a_batch, b_batch)
+    # (linear modes without dws: the library computes the default band)
     dw = None if dws is None else np.ascontiguousarray(dws, dtype=np.int32)
     slots = {POYG_MODE_LINEAR_ALIGN: 3, POYG_MODE_AFFINE_ALIGN: 4}.get(mode, 0)
     # the plan's output layout: slots of round16(la + lb + 2) bytes per pair

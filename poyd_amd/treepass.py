@@ -452,12 +452,13 @@ class EngineBackend:
         return r
 
     def median(self, pairs):
+        # (the medians are read-only views into the run's output block)
         e = self.engine
-        if self.affine:
-            r = self._run(e.POYG_MODE_AFFINE_ALIGN, pairs)
-            return [(r.seq(k, 0).copy(), int(r.cost[k])) for k in range(len(pairs))]
-        r = self._run(e.POYG_MODE_LINEAR_ALIGN, pairs)
-        return [(r.seq(k, 2).copy(), int(r.cost[k])) for k in range(len(pairs))]
+        slot = 0 if self.affine else 2
+        r = self._run(e.POYG_MODE_AFFINE_ALIGN if self.affine else e.POYG_MODE_LINEAR_ALIGN, pairs)
+        r.out.flags.writeable = False
+        costs = r.cost.tolist()
+        return [(r.seq(k, slot), costs[k]) for k in range(len(pairs))]
 
     def align(self, pairs):
         e = self.engine
