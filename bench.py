@@ -340,7 +340,7 @@ def main() -> int:
               "the main stream (the packed fill's backtraces, aff_trace_list_kernel, are backtrace_ms)") \
         if kind == "affine" \
         else "cube_rows_kernel (every row class)" if kind == "cube" \
-        else "lin_sys_kernel (systolic strips, four waves per pair)"
+        else "lin_cost16_kernel (packed 16-bit halves, one pair per wave, 128-row strips)"
     out = {
         "metric": METRIC,
         "value": round(gcups, 3),
@@ -841,7 +841,8 @@ def bench_tbr(args) -> int:
     valu = ex_cells * OPS_PER_CELL["linear"] / ex_s / 1e12
     out["roofline"] = {"bound": "valu", "achieved": round(valu, 3), "peak": round(VALU_PEAK_TOPS, 2),
                        "unit": "TOP/s (int32)", "frac": round(valu / VALU_PEAK_TOPS, 4), "traffic": None,
-                       "kernel": "exhaustive round: every linear DP of the round (lin_sys_kernel fills, "
+                       "kernel": "exhaustive round: every linear DP of the round (lin_cost16_kernel / "
+                                 "lin_c16mw_kernel fills, "
                                  "lin_trace_wave_kernel), 8 ops per band cell over the round's wall time"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_tbr(n_taxa, args.cpu_seconds)

@@ -162,14 +162,16 @@ int ctx_cus(poyg_ctx *c) { return c ? c->cus : 1; }
 // direction band or prefix sums: gigabytes for a TBR round's median plans,
 // where a fresh hipMalloc / hipFree pair took longer than the plan's
 // kernels) are rounded up to a size class and kept the same way, up to
-// kBigCachedMax bytes; a request takes the smallest kept block that holds
-// it.
+// kBigCachedShare of the device's memory (a round's plans of tens of
+// gigabytes each fit; with a smaller cap every plan paid a hipMalloc and a
+// hipFree of its band again); a request takes the smallest kept block that
+// holds it, and a failed hipMalloc first returns the kept blocks.
 namespace {
 constexpr int kPoolMinLog = 8, kPoolMaxLog = 26;   // 256 B .. 64 MiB
 constexpr size_t kPoolMax = (size_t)1 << kPoolMaxLog;
 constexpr size_t kPoolPendingMax = (size_t)256 << 20;
 constexpr size_t kPoolCachedMax = (size_t)1 << 30;
-constexpr size_t kBigCachedMax = (size_t)48 << 30;
+constexpr double kBigCachedShare = 0.6;
 constexpr int kPoolDevices = 64;
 constexpr int kBigClass = -2;
 struct DevPool {
@@ -180,6 +182,7 @@ struct DevPool {
   std::unordered_map<void *, size_t> big_size;          // large blocks handed out
   std::vector<std::pair<void *, size_t>> big_pending, big_free;
   size_t pending_bytes = 0, free_bytes = 0, big_free_bytes = 0;
+  size_t big_cap = 0;   // kBigCachedShare of the device's memory, set at first use
 };
 DevPool g_pool[kPoolDevices];
 
@@ -196,6 +199,16 @@ DevPool *pool_of_current() {
   return &g_pool[dev];
 }
 
+// the large-block cap of the current device; caller holds the lock
+size_t pool_big_cap(DevPool &P) {
+  if (!P.big_cap) {
+    size_t fr = 0, total = 0;
+    P.big_cap = hipMemGetInfo(&fr, &total) == hipSuccess && total ? (size_t)(kBigCachedShare * (double)total)
+                                                                   : (size_t)48 << 30;
+  }
+  return P.big_cap;
+}
+
 // pending -> free once the device is idle; caller holds the lock
 void pool_promote(DevPool &P) {
   for (auto &b : P.pending) {
@@ -209,7 +222,7 @@ void pool_promote(DevPool &P) {
   }
   P.pending.clear();
   for (auto &b : P.big_pending) {
-    if (P.big_free_bytes + b.second > kBigCachedMax) {
+    if (P.big_free_bytes + b.second > pool_big_cap(P)) {
       (void)hipFree(b.first);
       continue;
     }
@@ -317,7 +330,7 @@ hipError_t pool_release_idle(void *p) {
   if (c == kBigClass) {
     const size_t sz = P->big_size[p];
     P->big_size.erase(p);
-    if (P->big_free_bytes + sz > kBigCachedMax) return hipFree(p);
+    if (P->big_free_bytes + sz > pool_big_cap(*P)) return hipFree(p);
     P->big_free.emplace_back(p, sz);
     P->big_free_bytes += sz;
   } else if (c < 0) {
