@@ -280,7 +280,7 @@ def sub_result(kind: str, steps: int, warmup: int, n_check: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 10; tree: 1)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 50 for affine, 10 for the other pair workloads; tree and tbr: 1)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 2; tree: 0)")
     ap.add_argument("--workload", default="affine",
                     choices=["affine", "linear", "cube", "tree", "fitch", "tbr", "percall"])
@@ -304,7 +304,9 @@ def main() -> int:
         return 2
     tree = args.workload == "tree"
     if args.steps is None:
-        args.steps = 1 if tree or args.workload == "tbr" else 10
+        # the default affine run times 50 steps (about 0.5 s of GPU work, so
+        # that a utilisation sampler beside the run sees it)
+        args.steps = 1 if tree or args.workload == "tbr" else (50 if args.workload == "affine" else 10)
     if args.warmup is None:
         args.warmup = 0 if tree or args.workload == "tbr" else 2
     if tree:
