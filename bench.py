@@ -558,16 +558,16 @@ def bench_fitch(args) -> int:
     d_added = engine.DeviceBuffer(ctx, 8 * nn)
     stream = load_stream(ctx)
     root = 0
-    zeros = np.zeros(nn, np.int64)
 
-    def step():
-        d_added.upload(zeros)
+    def step(st=None):
+        st = stream if st is None else st
+        engine.dev_zero(ctx, d_added.ptr, 8 * nn, st)
         for a, (ia, ib, ir) in zip(down, dev_lv):
             engine._check(lib.poyg_fitch_down(ctx.h, d_chars.ptr, L, len(a), ia.ptr, ib.ptr, ir.ptr,
-                                              d_added.ptr + 8 * 0, stream), "poyg_fitch_down")
+                                              d_added.ptr + 8 * 0, st), "poyg_fitch_down")
         for a, (iA, iN, i1, i2, iF) in zip(up, dev_up):
             engine._check(lib.poyg_fitch_up(ctx.h, d_chars.ptr, L, len(a), iA.ptr, iN.ptr, i1.ptr, i2.ptr,
-                                            iF.ptr, stream), "poyg_fitch_up")
+                                            iF.ptr, st), "poyg_fitch_up")
 
     # the root's final sets are its preliminary ones: copied once after a
     # first down pass (the leaves never change)
@@ -582,7 +582,20 @@ def bench_fitch(args) -> int:
     for _ in range(args.steps):
         step()
     ctx.sync()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the same step recorded once as a graph (one launch per step)
+    g = engine.Graph(ctx)
+    step(g.stream)
+    g.end()
+    for _ in range(max(1, args.warmup)):
+        g.launch(stream)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.launch(stream)
+    ctx.sync()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    g.close()
     n_down = sum(len(a) for a in down)
     n_up = sum(len(a) for a in up)
     medians = (n_down + n_up) * L
@@ -606,6 +619,9 @@ def bench_fitch(args) -> int:
             "down_pass_nodes": n_down,
             "up_pass_nodes": n_up,
             "levels": len(down) + len(up),
+            "launch": "the step's level launches recorded once as a HIP graph (poyg_graph_*), one graph "
+                      "launch per step",
+            "eager_ms_per_step": round(eager_ms, 3),
         },
         "roofline": {
             "bound": "hbm",
@@ -614,7 +630,7 @@ def bench_fitch(args) -> int:
             "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "fitch_down_kernel + fitch_up_kernel, one launch per tree level",
+            "kernel": "fitch_down_kernel + fitch_up_kernel, one kernel per tree level, replayed as one graph",
         },
     }
     if not args.no_cpu:

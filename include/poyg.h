@@ -74,15 +74,38 @@ int poyg_ctx_device_name(poyg_ctx *ctx, char *buf, int cap);
 /* ---- device buffers (plumbing for callers without their own allocator;
  * no reference counterpart) ------------------------------------------- */
 int poyg_dev_alloc(poyg_ctx *ctx, int64_t bytes, void **out);
-/* Returns a block from poyg_dev_alloc.  Blocks of up to 64 MiB go to a
- * per-device cache and are handed out again only after the next device-wide
- * synchronization (poyg_ctx_sync); larger ones are freed at once. */
+/* Returns a block from poyg_dev_alloc.  Blocks go to a per-device cache
+ * (up to 64 MiB in power-of-two classes, at most 1 GiB of them; larger ones
+ * in classes a quarter power of two apart, up to 60% of the device's
+ * memory) and are handed out again only after the next device-wide
+ * synchronization (poyg_ctx_sync). */
 int poyg_dev_free(poyg_ctx *ctx, void *p);
 int poyg_dev_upload(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes);
 int poyg_dev_download(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes);
+/* Zeroes `bytes` (a multiple of 4) at the 4-byte aligned device address p
+ * with a kernel on `stream` (NULL: the context's stream); usable inside a
+ * graph capture, where a memset node is not. */
+int poyg_dev_zero(poyg_ctx *ctx, void *p, int64_t bytes, void *stream);
 /* The context's stream as a hipStream_t, and its device. */
 void *poyg_ctx_stream(poyg_ctx *ctx);
 int poyg_ctx_device(poyg_ctx *ctx);
+
+/* ---- recorded passes (no reference counterpart) ----------------------
+ * A pass of fixed shape over fixed device buffers -- a tree's Fitch down
+ * and up passes, or the run of one plan -- recorded once as a HIP graph and
+ * replayed.  poyg_graph_begin starts a capture on a stream of the context
+ * and returns it; every call made with that stream until poyg_graph_end
+ * (poyg_fitch_down / _up, poyg_plan_run, poyg_dev_zero) is recorded, not
+ * run.  poyg_graph_launch replays the whole pass on `stream` (NULL: the
+ * context's stream); replays of one graph on one stream run in order, and
+ * a caller that reads a plan's status after a replay synchronizes that
+ * stream first.  The buffers the recorded calls named must stay allocated
+ * while the graph is used.  One capture at a time per context. */
+typedef struct poyg_graph poyg_graph;
+int poyg_graph_begin(poyg_ctx *ctx, void **capture_stream);
+int poyg_graph_end(poyg_ctx *ctx, poyg_graph **out);
+int poyg_graph_launch(poyg_graph *g, void *stream);
+void poyg_graph_destroy(poyg_graph *g);
 
 /* ---- two-dimensional cost matrix (src/cm.c, src/cost_matrix.ml) ------- */
 /* cm_CAML_create (src/cm.c:1389-1398); only 5-bit combination alphabets. */

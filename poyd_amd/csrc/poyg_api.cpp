@@ -118,6 +118,7 @@ struct poyg_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;    // second stream: fills beside the main one
   hipStream_t side2 = nullptr;   // third stream: the widest bands (lane sets, row kernels)
+  hipStream_t capture = nullptr;   // the stream of a graph capture (created at the first one)
   // pinned host staging of the per-call entry points (one call at a time per
   // context: every call ends with a synchronization of `stream`): the plan's
   // descriptor block, and the call's sequences and results
@@ -480,6 +481,7 @@ void poyg_ctx_destroy(poyg_ctx *ctx) {
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->side2) (void)hipStreamDestroy(ctx->side2);
+  if (ctx->capture) (void)hipStreamDestroy(ctx->capture);
   delete ctx;
 }
 
@@ -517,6 +519,63 @@ int poyg_dev_download(poyg_ctx *ctx, void *dst, const void *src, int64_t bytes) 
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return POYG_OK;
 }
+int poyg_dev_zero(poyg_ctx *ctx, void *p, int64_t bytes, void *stream) {
+  if (!ctx || bytes < 0 || (bytes & 3) || ((uintptr_t)p & 3) || (bytes && !p) || bytes / 4 > INT32_MAX) {
+    set_error("poyg_dev_zero: bad arguments");
+    return POYG_EINVAL;
+  }
+  if (bytes == 0) return POYG_OK;
+  HIP_TRY(launch_reset((int *)p, (int)(bytes / 4), stream ? (hipStream_t)stream : ctx->stream));
+  return POYG_OK;
+}
+
+struct poyg_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t stream = nullptr;   // the context's stream (launches with a NULL stream)
+};
+
+int poyg_graph_begin(poyg_ctx *ctx, void **capture_stream) {
+  if (!ctx || !capture_stream) return POYG_EINVAL;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (!ctx->capture) HIP_TRY(hipStreamCreateWithFlags(&ctx->capture, hipStreamNonBlocking));
+  HIP_TRY(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeThreadLocal));
+  *capture_stream = ctx->capture;
+  return POYG_OK;
+}
+
+int poyg_graph_end(poyg_ctx *ctx, poyg_graph **out) {
+  if (!ctx || !out || !ctx->capture) return POYG_EINVAL;
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamEndCapture(ctx->capture, &g));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    set_error("poyg_graph_end: %s", hipGetErrorString(e));
+    return POYG_EDEVICE;
+  }
+  poyg_graph *r = new poyg_graph;
+  r->graph = g;
+  r->exec = ex;
+  r->stream = ctx->stream;
+  *out = r;
+  return POYG_OK;
+}
+
+int poyg_graph_launch(poyg_graph *g, void *stream) {
+  if (!g) return POYG_EINVAL;
+  HIP_TRY(hipGraphLaunch(g->exec, stream ? (hipStream_t)stream : g->stream));
+  return POYG_OK;
+}
+
+void poyg_graph_destroy(poyg_graph *g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
+
 void *poyg_ctx_stream(poyg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 int poyg_ctx_device(poyg_ctx *ctx) { return ctx ? ctx->device : -1; }
 

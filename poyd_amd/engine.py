@@ -47,6 +47,11 @@ SIGNATURES = {
     "poyg_dev_upload": (C.c_int, [vp, vp, vp, C.c_int64]),
     "poyg_dev_download": (C.c_int, [vp, vp, vp, C.c_int64]),
     "poyg_plan_status": (C.c_int, [vp, i32p]),
+    "poyg_dev_zero": (C.c_int, [vp, vp, C.c_int64, vp]),
+    "poyg_graph_begin": (C.c_int, [vp, C.POINTER(vp)]),
+    "poyg_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
+    "poyg_graph_launch": (C.c_int, [vp, vp]),
+    "poyg_graph_destroy": (None, [vp]),
     "poyg_run_pairs_host": (C.c_int, [vp, vp, C.c_int, C.c_int32, u8p, C.c_int64, i64p, i32p, i64p, i32p, i32p, i32p,
                                       u8p, C.c_int64, i32p, i64p, i32p]),
     "poyg_cm_create": (C.c_int, [C.c_int] * 5 + [C.POINTER(vp)]),
@@ -683,6 +688,43 @@ def _gaps_per_item(batch) -> np.ndarray:
     pre = np.concatenate([[0], np.cumsum(gap, dtype=np.int64)])
     off = batch.off.astype(np.int64)
     return pre[off + batch.ln] - pre[off]
+
+
+class Graph:
+    """A recorded pass (poyg_graph_*): the launches made with the capture
+    stream between record() and end(), replayed by launch()."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = None
+        s = vp()
+        _check(load().poyg_graph_begin(ctx.h, C.byref(s)), "poyg_graph_begin")
+        self.stream = s.value   # pass this stream to the calls to record
+
+    def end(self) -> "Graph":
+        g = vp()
+        _check(load().poyg_graph_end(self.ctx.h, C.byref(g)), "poyg_graph_end")
+        self.h = g.value
+        return self
+
+    def launch(self, stream=None) -> None:
+        _check(load().poyg_graph_launch(self.h, stream), "poyg_graph_launch")
+
+    def close(self) -> None:
+        if self.h:
+            load().poyg_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dev_zero(ctx: Context, ptr: int, nbytes: int, stream=None) -> None:
+    """Zero nbytes (a multiple of 4) at a device address with a kernel."""
+    _check(load().poyg_dev_zero(ctx.h, ptr, int(nbytes), stream), "poyg_dev_zero")
 
 
 def run_pairs(ctx: Context, cm: CostMatrix, mode: int, a_batch, b_batch, dws=None,
