@@ -62,7 +62,9 @@ int poyg_ctx_sync(poyg_ctx *ctx);
  * linear fill and backtrace), "cube_waves_per_cu" (1..64, default 8),
  * "lin_cost_kernel" (1: the 32-bit systolic linear fill for every pair it
  * takes; 2, the default: the packed 16-bit fill where a pair's costs allow
- * it), "aff_narrow_kernel" (1: the 32-bit four-pair affine fill for the
+ * it, one wave per pair, or four waves per pair when the batch has at most
+ * two such pairs per CU; 3 and 4: the packed fill with one, four waves per
+ * pair whatever the batch size), "aff_narrow_kernel" (1: the 32-bit four-pair affine fill for the
  * narrow bands; 2, the default: the packed eight-pair fill).  A negative
  * value restores the default.  poyd_amd/autotune.py picks them by timed
  * trials. */

@@ -2,8 +2,9 @@
 
 The plans' results never depend on these knobs (poyg_ctx_set_tuning: which
 affine bands go to the four-pairs-per-wave fill, how many CUs the side
-streams get, resident workgroups per CU of the linear fill and backtrace,
-waves per CU of the three-way cube); their speed does, per device and per
+streams get, the packed or 32-bit fill of the narrow affine bands, one or
+four waves per pair of the packed linear fill, resident workgroups per CU of
+the linear fill and backtrace, waves per CU of the three-way cube); their speed does, per device and per
 batch shape.  `tune` times trials of a caller's batch (coordinate descent
 over the knobs of its kind, each trial the median of a few plan runs) and
 stores the best settings per device, kind and length bucket in a JSON table;
@@ -23,8 +24,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SHIPPED = os.path.join(HERE, "tuned.json")
 
 KNOBS = {
-    "affine": [("aff_quad_max_lanes", [44, 46, 48]), ("aff_side_div", [3, 4, 6, 8, 16])],
-    "linear": [("lin_sys_blocks_per_cu", [1, 2, 3, 4]), ("lin_trace_blocks_per_cu", [2, 4, 7])],
+    "affine": [("aff_quad_max_lanes", [44, 46, 48]), ("aff_side_div", [3, 4, 6, 8, 16]),
+               ("aff_narrow_kernel", [1])],
+    "linear": [("lin_cost_kernel", [3, 4]), ("lin_sys_blocks_per_cu", [1, 2, 3, 4]),
+               ("lin_trace_blocks_per_cu", [2, 4, 7])],
     "cube": [("cube_waves_per_cu", [4, 6, 8, 12, 16])],
 }
 
