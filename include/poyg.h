@@ -187,12 +187,15 @@ int poyg_algn_align_3d(poyg_ctx *ctx, const poyg_cm3 *cm3, const uint8_t *s1, in
                        const uint8_t *s2, int l2, const uint8_t *s3, int l3,
                        uint8_t *r1, uint8_t *r2, uint8_t *r3, int cap, int *rlen, int *cost);
 /* powell_3D_align (src/ukkCommon.c:110-151; the Iterative ThreeD
- * readjustment, src/sequence.ml:1078-1139): the least cost of Powell's
- * three-way affine model (mismatch mm, gap opening go, extension ge) and an
- * alignment of that cost, right-aligned in ra/rb/rc (cap >= la+lb+lc) with
- * the leading gap.  The cost is the reference's; the reference's search may
- * pick another alignment of the same cost.  POYG_EINVAL, as the reference's
- * failure, when a character past the leading gap has no base bit. */
+ * readjustment, src/sequence.ml:1078-1139): the reference's check-point
+ * Ukkonen search (src/ukk.checkp.c:113-781) over Powell's three-way affine
+ * model (mismatch mm > 0, gap opening go >= 0, extension ge > 0): its cost
+ * (the model's least cost; 1 for three equal strings, as the reference's
+ * loop starts at cost 1) and the alignment it picks, right-aligned in
+ * ra/rb/rc (cap >= la+lb+lc) with the leading gap.  POYG_EINVAL, as the
+ * reference's failure, when a character past the leading gap has no base
+ * bit; POYG_EUNSUPPORTED past 65,534 characters a sequence; POYG_ENOMEM when
+ * the search's cost levels would exceed 24 GiB. */
 int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, int lb, const uint8_t *c, int lc,
                    int mm, int go, int ge, uint8_t *ra, uint8_t *rb, uint8_t *rc, int cap, int *rlen, int *cost);
 /* The direction matrix that the readers of Matrix.default see after

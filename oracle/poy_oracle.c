@@ -1469,3 +1469,451 @@ int po_powell_3d(const uint8_t *A, int la, const uint8_t *B, int lb, const uint8
   for (int q = 0; q < 3; ++q) free(x[q]);
   return cost;
 }
+
+/* ---- R15: the check-point Ukkonen search itself (src/ukk.checkp.c) -------
+ * Restates the search powell_3D_align runs, so that the alignment it picks
+ * among those of least cost is the reference's:
+ *   store: a cell per (ab = a - b, ac = a - c, cost d, state s) holding the
+ *     furthest a reached (dist), a tag (d + the pass's offset: a cell is
+ *     valid when its tag matches) and its "from" record; the reference keeps
+ *     2 maxSingleStep cost planes and reuses plane d mod that count
+ *     (src/ukkCommon.h:101-260 with FIXED_NUM_PLANES, src/ukk.checkp.c:133);
+ *     `planes_mode` 1 keeps every plane instead (no reuse);
+ *   check-point cells: (dist, d + 1) for d in [CPcost, CPcost + CPwidth),
+ *     CPwidth = maxSingleStep planes;
+ *   ukk (Ukk, :578-606): outside withinMatrix (:545-576) -INFINITY; a valid
+ *     cell's dist; else calc, store the check point, raise furthestReached;
+ *   calc (calcUkk, :608-781): the state's one neighbour diagonal, every
+ *     previous state in order (first the cost without the second mismatch
+ *     when whichCharCost says 1, else, for states with a second cost, the
+ *     cost one mismatch lower), then the same state one cost lower, then
+ *     for the all-match state the furthest state of this cost extended
+ *     along equal characters; a candidate replaces the best only when
+ *     strictly further; the from record is the candidate's coordinates in a
+ *     base case, the candidate's own record above the check-point band,
+ *     this cell inside the band, cost -1 below it;
+ *   the driver (doUkk, :113-214): the common prefix, costs 1, 2, ... at the
+ *     end diagonal until the best state reaches |A|, the check-point cost
+ *     set to d + 1 after the first cost whose furthest reach is |A| / 2;
+ *     then the end cell's from record splits the alignment at the check
+ *     point (getSplitRecurse, :324-368; the later half first), or, when it
+ *     has none, the whole range is searched again (doUkkInLimits,
+ *     :216-322: a fresh offset, the start cell preset, base case when the
+ *     range is at most CPwidth, else CPcost = (fCost + sCost - CPwidth + 1)
+ *     / 2);
+ *   traceBack (:370-454): from the end cell back along from records, a run
+ *     of match columns down to the previous cell's dist, then one column of
+ *     the cell's state; printTraceBack (:456-500): the common prefix, the
+ *     columns reversed, bases as masks (char_to_base), the leading gap.
+ * Model tables as setup (src/ukkCommon.c:247-330); okIndex and
+ * whichCharCost as src/ukkCommon.c:155-195. */
+#define UK_INF 5000   /* INFINITY = MAXINT / 2 */
+#define UK_BLK 8
+
+typedef struct {
+  int dist;
+  int64_t tag;
+  int fab, fac, fcost, fstate;
+} uk_cell;
+
+typedef struct {
+  int dist, cost;
+} uk_cp;
+
+typedef struct {
+  void ***plane;   /* plane -> block table -> block */
+  int nplanes, cap, aliasing;
+  size_t elem;
+} uk_store;
+
+typedef struct {
+  /* model */
+  int ns, nb[PW_NSTATES], cont[PW_NSTATES], second[PW_NSTATES], trans[PW_NSTATES][PW_NSTATES];
+  int mis, go, ge, max_step;
+  /* strings, each followed by a 0 as copySequence leaves them */
+  char *A, *B, *C;
+  int la, lb, lc;
+  /* stores */
+  uk_store U, CP;
+  int ab_off, ac_off, ab_blocks, ac_blocks;
+  /* search state */
+  int64_t offset, final_cost;
+  int furthest, cp_on_dist, cp_cost, cp_width, complete_from;
+  int sab, sac, scost, sstate;
+  int endA, endB, endC;
+  /* output columns, in reverse */
+  char *ra, *rb, *rc;
+  int n, out_cap, oom;
+} uk_ctx;
+
+static void *uk_slot(uk_ctx *k, uk_store *st, int ab, int ac, int d, int s) {
+  int p = st->aliasing ? d % st->nplanes : d;
+  if (p >= st->cap) {
+    int ncap = st->cap ? st->cap : 16;
+    while (ncap <= p) ncap *= 2;
+    void ***np = (void ***)realloc(st->plane, sizeof(void **) * (size_t)ncap);
+    if (!np) { k->oom = 1; return NULL; }
+    memset(np + st->cap, 0, sizeof(void **) * (size_t)(ncap - st->cap));
+    st->plane = np;
+    st->cap = ncap;
+  }
+  if (!st->plane[p]) {
+    st->plane[p] = (void **)calloc((size_t)k->ab_blocks * k->ac_blocks, sizeof(void *));
+    if (!st->plane[p]) { k->oom = 1; return NULL; }
+  }
+  const int i = (ab + k->ab_off) / UK_BLK, j = (ac + k->ac_off) / UK_BLK;
+  void **blk = &st->plane[p][(size_t)i * k->ac_blocks + j];
+  if (!*blk) {
+    *blk = calloc((size_t)UK_BLK * UK_BLK * k->ns, st->elem);
+    if (!*blk) { k->oom = 1; return NULL; }
+  }
+  const int ai = (ab + k->ab_off) - i * UK_BLK, ci = (ac + k->ac_off) - j * UK_BLK;
+  return (char *)*blk + (((size_t)ai * UK_BLK + ci) * k->ns + s) * st->elem;
+}
+
+static uk_cell uk_dummy;
+static uk_cell *uk_U(uk_ctx *k, int ab, int ac, int d, int s) {
+  uk_cell *c = (uk_cell *)uk_slot(k, &k->U, ab, ac, d, s);
+  return c ? c : &uk_dummy;
+}
+static uk_cp uk_cp_dummy;
+static uk_cp *uk_CP(uk_ctx *k, int ab, int ac, int d, int s) {
+  uk_cp *c = (uk_cp *)uk_slot(k, &k->CP, ab, ac, d, s);
+  return c ? c : &uk_cp_dummy;
+}
+
+static void uk_free_store(uk_ctx *k, uk_store *st) {
+  for (int p = 0; p < st->cap; ++p) {
+    if (!st->plane[p]) continue;
+    for (size_t b = 0; b < (size_t)k->ab_blocks * k->ac_blocks; ++b) free(st->plane[p][b]);
+    free(st->plane[p]);
+  }
+  free(st->plane);
+}
+
+static int uk_ok(int a, int da, int end) {
+  if (a < 0) return 0;
+  return da ? a < end : a <= end;
+}
+
+static int uk_chars(char a, char b, char c) {
+  if (a == b && a == c) return 0;
+  return (a == b || a == c || b == c) ? 1 : 2;
+}
+
+static int uk_within(uk_ctx *k, int ab, int ac, int d) {
+  if (d < 0) return 0;
+  int v[3] = {abs(k->sab - ab), abs(k->sac - ac), abs((k->sac - k->sab) - (ac - ab))};
+  /* the two smallest */
+  for (int x = 0; x < 2; ++x)
+    for (int y = x + 1; y < 3; ++y)
+      if (v[y] < v[x]) { int t = v[x]; v[x] = v[y]; v[y] = t; }
+  const int g = v[0], h = v[1];
+  int cheapest;
+  if (k->sstate == 0)
+    cheapest = (g ? k->go + g * k->ge : 0) + (h ? k->go + h * k->ge : 0);
+  else
+    cheapest = g * k->ge + h * k->ge;
+  return cheapest + k->scost <= d;
+}
+
+static int uk_calc(uk_ctx *k, int ab, int ac, int d, int to);
+
+static int uk_ukk(uk_ctx *k, int ab, int ac, int d, int s) {
+  if (!uk_within(k, ab, ac, d)) return -UK_INF;
+  uk_cell *c = uk_U(k, ab, ac, d, s);
+  if (c->tag == d + k->offset) return c->dist;
+  uk_calc(k, ab, ac, d, s);
+  c = uk_U(k, ab, ac, d, s);
+  if (d >= k->cp_cost && d < k->cp_cost + k->cp_width) {
+    uk_cp *p = uk_CP(k, ab, ac, d, s);
+    p->dist = c->dist;
+    p->cost = d + 1;
+  }
+  if (c->dist > k->furthest) k->furthest = c->dist;
+  return c->dist;
+}
+
+static int uk_calc(uk_ctx *k, int ab, int ac, int d, int to) {
+  int fab = 0, fac = 0, fcost = -1, fstate = 0;
+  int best = -UK_INF;
+  if (d >= k->cp_cost && d < k->cp_cost + k->cp_width) {
+    fab = ab; fac = ac; fcost = d; fstate = to;
+  }
+  const int da = k->nb[to] & 1, db = (k->nb[to] >> 1) & 1, dc = (k->nb[to] >> 2) & 1;
+  const int ab1 = ab - da + db, ac1 = ac - da + dc;
+#define UK_TAKE(cab, cac, ccost, cstate)                                   \
+  do {                                                                     \
+    if (k->complete_from) {                                                \
+      fab = (cab); fac = (cac); fcost = (ccost); fstate = (cstate);        \
+    } else if (d >= k->cp_cost + k->cp_width) {                            \
+      const uk_cell *q = uk_U(k, (cab), (cac), (ccost), (cstate));         \
+      fab = q->fab; fac = q->fac; fcost = q->fcost; fstate = q->fstate;    \
+    }                                                                      \
+  } while (0)
+  if (ab1 >= -k->endB && ab1 <= k->endA && ac1 >= -k->endC && ac1 <= k->endA) {
+    for (int fs = 0; fs < k->ns; ++fs) {
+      const int cost = d - k->trans[fs][to] - k->cont[to];
+      int dist = -UK_INF, from_cost = -UK_INF;
+      const int a1 = uk_ukk(k, ab1, ac1, cost, fs);
+      if (uk_ok(a1, da, k->endA) && uk_ok(a1 - ab1, db, k->endB) && uk_ok(a1 - ac1, dc, k->endC) &&
+          uk_chars(da ? k->A[a1] : '-', db ? k->B[a1 - ab1] : '-', dc ? k->C[a1 - ac1] : '-') == 1) {
+        from_cost = cost;
+        dist = a1 + da;
+      } else {
+        if (!k->second[to]) continue;
+        const int a2 = uk_ukk(k, ab1, ac1, cost - k->mis, fs);
+        if (uk_ok(a2, da, k->endA) && uk_ok(a2 - ab1, db, k->endB) && uk_ok(a2 - ac1, dc, k->endC)) {
+          from_cost = cost - k->mis;
+          dist = a2 + da;
+        }
+      }
+      if (best < dist) {
+        best = dist;
+        UK_TAKE(ab1, ac1, from_cost, fs);
+      }
+    }
+  }
+  {
+    const int dist = uk_ukk(k, ab, ac, d - 1, to);
+    if (uk_ok(dist, 0, k->endA) && uk_ok(dist - ab, 0, k->endB) && uk_ok(dist - ac, 0, k->endC) && best < dist) {
+      best = dist;
+      UK_TAKE(ab, ac, d - 1, to);
+    }
+  }
+  if (to == 0) {
+    int dist = -UK_INF, from_state = -1;
+    for (int s = 0; s < k->ns; ++s) {
+      const int x = s == 0 ? best : uk_ukk(k, ab, ac, d, s);
+      if (x > dist) { dist = x; from_state = s; }
+    }
+    while (uk_ok(dist, 1, k->endA) && uk_ok(dist - ab, 1, k->endB) && uk_ok(dist - ac, 1, k->endC) &&
+           k->A[dist] == k->B[dist - ab] && k->A[dist] == k->C[dist - ac])
+      ++dist;
+    if (dist > best) {
+      best = dist;
+      if (from_state != 0) UK_TAKE(ab, ac, d, from_state);
+    }
+  }
+#undef UK_TAKE
+  uk_cell *c = uk_U(k, ab, ac, d, to);
+  c->dist = best;
+  c->tag = d + k->offset;
+  c->fab = fab; c->fac = fac; c->fcost = fcost; c->fstate = fstate;
+  return best;
+}
+
+static int uk_best(uk_ctx *k, int ab, int ac, int d, int want_state) {
+  int best = -UK_INF, st = -1;
+  for (int s = 0; s < k->ns; ++s) {
+    const uk_cell *c = uk_U(k, ab, ac, d, s);
+    if (c->tag == d + k->offset && c->dist > best) { best = c->dist; st = s; }
+  }
+  return want_state ? st : best;
+}
+
+static void uk_emit(uk_ctx *k, char a, char b, char c) {
+  if (k->n >= k->out_cap) { k->oom = 1; return; }
+  k->ra[k->n] = a; k->rb[k->n] = b; k->rc[k->n] = c;
+  ++k->n;
+}
+
+static void uk_trace(uk_ctx *k, int sab, int sac, int scost, int sstate, int fab, int fac, int fcost, int fstate) {
+  int ab = fab, ac = fac, d = fcost, s = fstate;
+  while ((ab != sab || ac != sac || d != scost || s != sstate) && !k->oom) {
+    const uk_cell *c = uk_U(k, ab, ac, d, s);
+    int a = c->dist;
+    const int nab = c->fab, nac = c->fac, nd = c->fcost, nst = c->fstate;
+    int b = a - ab, cc = a - ac;
+    const int a1 = uk_U(k, nab, nac, nd, nst)->dist;
+    const int b1 = a1 - nab, c1 = a1 - nac;
+    while (a > a1 && b > b1 && cc > c1) {
+      --a; --b; --cc;
+      uk_emit(k, k->A[a], k->B[b], k->C[cc]);
+    }
+    if (a != a1 || b != b1 || cc != c1) {
+      const char x = a > a1 ? k->A[--a] : '-';
+      const char y = b > b1 ? k->B[--b] : '-';
+      const char z = cc > c1 ? k->C[--cc] : '-';
+      uk_emit(k, x, y, z);
+    }
+    if (nd < 0 || nd > d) { k->oom = 1; return; }   /* a broken chain: give up rather than loop */
+    ab = nab; ac = nac; d = nd; s = nst;
+  }
+}
+
+static int uk_limits(uk_ctx *k, int sab, int sac, int scost, int sstate, int sdist, int fab, int fac, int fcost,
+                     int fstate, int fdist);
+
+static int uk_split(uk_ctx *k, int sab, int sac, int scost, int sstate, int sdist, int fab, int fac, int fcost,
+                    int fstate, int fdist) {
+  const uk_cell *c = uk_U(k, fab, fac, fcost, fstate);
+  const int f_ab = c->fab, f_ac = c->fac, f_cost = c->fcost, f_state = c->fstate;
+  if (f_cost < 0) { k->oom = 1; return -1; }
+  uk_cp *p = uk_CP(k, f_ab, f_ac, f_cost, f_state);
+  if (p->cost == 0) p->cost = 1;
+  const int cpdist = p->dist;
+  const int len = uk_limits(k, f_ab, f_ac, f_cost, f_state, cpdist, fab, fac, fcost, fstate, fdist);
+  if (k->oom) return -1;
+  uk_limits(k, sab, sac, scost, sstate, sdist, f_ab, f_ac, f_cost, f_state, cpdist);
+  return len;
+}
+
+static int uk_limits(uk_ctx *k, int sab, int sac, int scost, int sstate, int sdist, int fab, int fac, int fcost,
+                     int fstate, int fdist) {
+  if (k->oom) return -1;
+  k->sab = sab; k->sac = sac; k->scost = scost; k->sstate = sstate;
+  k->endA = fdist; k->endB = fdist - fab; k->endC = fdist - fac;
+  k->complete_from = 0;
+  k->offset += k->final_cost + 1;
+  uk_cell *s0 = uk_U(k, sab, sac, scost, sstate);
+  s0->dist = sdist;
+  s0->tag = scost + k->offset;
+  const int guard = fcost + k->ns * k->max_step + 64;   /* the search reaches fdist at fcost */
+  if (fcost - scost <= k->cp_width) {
+    k->complete_from = 1;
+    int i = scost - 1, dist;
+    do {
+      ++i;
+      dist = uk_ukk(k, fab, fac, i, fstate);
+    } while (dist < fdist && i < guard && !k->oom);
+    if (dist < fdist) { k->oom = 1; return -1; }
+    fcost = i;
+    uk_trace(k, sab, sac, scost, sstate, fab, fac, fcost, fstate);
+    k->complete_from = 0;
+    return uk_best(k, fab, fac, fcost, 0);
+  }
+  k->cp_cost = (fcost + scost - k->cp_width + 1) / 2;
+  int i = scost - 1, dist;
+  do {
+    ++i;
+    uk_ukk(k, fab, fac, i, 0);
+    dist = uk_ukk(k, fab, fac, i, fstate);
+  } while (dist < fdist && i < guard && !k->oom);
+  if (dist < fdist) { k->oom = 1; return -1; }
+  return uk_split(k, sab, sac, scost, sstate, sdist, fab, fac, i, fstate, fdist);
+}
+
+static void uk_model(uk_ctx *k, int mm, int go, int ge) {
+  pw_state S[PW_NSTATES];
+  int trans[PW_NSTATES][PW_NSTATES];
+  k->ns = pw_setup(S, trans, go);
+  k->mis = mm; k->go = go; k->ge = ge;
+  int max_cost = 0;
+  for (int t = 0; t < k->ns; ++t) {
+    k->nb[t] = S[t].adv[0] | (S[t].adv[1] << 1) | (S[t].adv[2] << 2);
+    const int nm = 3 - S[t].n_del - S[t].n_ins;
+    if (S[t].n_ins) { k->cont[t] = ge; k->second[t] = 0; }
+    else if (nm == 3) { k->cont[t] = mm; k->second[t] = 1; }
+    else if (S[t].n_del == 1) { k->cont[t] = ge; k->second[t] = 1; }
+    else { k->cont[t] = 2 * ge; k->second[t] = 0; }
+  }
+  for (int a = 0; a < k->ns; ++a)
+    for (int b = 0; b < k->ns; ++b) {
+      k->trans[a][b] = trans[a][b];
+      const int nm = 3 - S[b].n_del - S[b].n_ins;
+      const int c = trans[a][b] + k->cont[b] + mm * (nm - 1);
+      if (c > max_cost) max_cost = c;
+    }
+  k->max_step = max_cost;
+}
+
+/* The reference's own search: returns the cost (or -1: a character without
+ * a base bit, memory, or a search that did not close) and, when oa is given,
+ * the alignment it prints (masks, gap 16, leading gap first; cap >=
+ * la + lb + lc).  planes_mode 0 reuses cost planes as the reference does, 1
+ * keeps them all. */
+int po_powell_ukk(const uint8_t *A, int la, const uint8_t *B, int lb, const uint8_t *Cs, int lc, int mm, int go,
+                  int ge, int planes_mode, uint8_t *oa, uint8_t *ob, uint8_t *oc, int cap, int *olen) {
+  if (la < 1 || lb < 1 || lc < 1) return -1;
+  uk_ctx k;
+  memset(&k, 0, sizeof k);
+  uk_model(&k, mm, go, ge);
+  const uint8_t *src[3] = {A, B, Cs};
+  const int n[3] = {la - 1, lb - 1, lc - 1};
+  char *str[3];
+  for (int q = 0; q < 3; ++q) {
+    str[q] = (char *)calloc((size_t)n[q] + 2, 1);
+    if (!str[q]) { for (int r = 0; r < q; ++r) free(str[r]); return -1; }
+    for (int i = 0; i < n[q]; ++i) {
+      str[q][i] = (char)pw_base(src[q][i + 1]);
+      if (!str[q][i]) { for (int r = 0; r <= q; ++r) free(str[r]); return -1; }
+    }
+  }
+  k.A = str[0]; k.B = str[1]; k.C = str[2];
+  k.la = n[0]; k.lb = n[1]; k.lc = n[2];
+  k.ab_off = k.lb; k.ac_off = k.lc;
+  k.ab_blocks = (k.la + k.lb + 1) / UK_BLK + 1;
+  k.ac_blocks = (k.la + k.lc + 1) / UK_BLK + 1;
+  k.U.elem = sizeof(uk_cell);
+  k.U.aliasing = planes_mode == 0;
+  k.U.nplanes = 2 * k.max_step;
+  k.cp_width = k.max_step;
+  k.CP.elem = sizeof(uk_cp);
+  k.CP.aliasing = 1;
+  k.CP.nplanes = k.cp_width;
+  k.out_cap = k.la + k.lb + k.lc + 2;
+  k.ra = (char *)malloc((size_t)k.out_cap);
+  k.rb = (char *)malloc((size_t)k.out_cap);
+  k.rc = (char *)malloc((size_t)k.out_cap);
+  k.offset = 1;
+  k.furthest = -1;
+  int cost = -1;
+  if (k.ra && k.rb && k.rc) {
+    int start = 0;
+    while (start < k.la && k.A[start] == k.B[start] && k.A[start] == k.C[start]) ++start;
+    uk_cell *c0 = uk_U(&k, 0, 0, 0, 0);
+    c0->dist = start;
+    c0->tag = 0 + k.offset;
+    const int fab = k.la - k.lb, fac = k.la - k.lc;
+    k.endA = k.la; k.endB = k.lb; k.endC = k.lc;
+    k.cp_on_dist = 1;
+    k.cp_cost = UK_INF;
+    int d = 0;
+    const int guard = (k.la + k.lb + k.lc + 1) * (k.max_step + 2 * (mm > 0 ? mm : 1)) + 8;
+    do {
+      ++d;
+      uk_ukk(&k, fab, fac, d, 0);
+      if (k.cp_on_dist && k.furthest >= k.la / 2) {
+        k.cp_cost = d + 1;
+        k.cp_on_dist = 0;
+      }
+    } while (uk_best(&k, fab, fac, d, 0) < k.la && d < guard && !k.oom);
+    if (!k.oom && uk_best(&k, fab, fac, d, 0) == k.la) {
+      k.cp_on_dist = 0;
+      k.final_cost = d;
+      const int fstate = uk_best(&k, fab, fac, d, 1);
+      if (uk_U(&k, fab, fac, d, fstate)->fcost <= 0)
+        uk_limits(&k, 0, 0, 0, 0, start, fab, fac, d, fstate, k.la);
+      else
+        uk_split(&k, 0, 0, 0, 0, start, fab, fac, d, fstate, k.la);
+      if (!k.oom) cost = d;
+    }
+  }
+  if (cost >= 0 && oa) {
+    /* the common prefix, then every column in order */
+    int pre = 0;
+    while (pre < k.la && k.A[pre] == k.B[pre] && k.A[pre] == k.C[pre]) ++pre;
+    const int total = pre + k.n;
+    if (total + 1 > cap) {
+      cost = -1;
+    } else {
+      uint8_t *o[3] = {oa, ob, oc};
+      const char *col[3] = {k.ra, k.rb, k.rc};
+      for (int q = 0; q < 3; ++q) {
+        o[q][0] = 16;
+        for (int i = 0; i < pre; ++i) o[q][1 + i] = (uint8_t)(str[0][i] == 'A' ? 1 : str[0][i] == 'C' ? 2 : str[0][i] == 'G' ? 4 : 8);
+        for (int i = 0; i < k.n; ++i) {
+          const char ch = col[q][k.n - 1 - i];
+          o[q][1 + pre + i] = (uint8_t)(ch == 'A' ? 1 : ch == 'C' ? 2 : ch == 'G' ? 4 : ch == 'T' ? 8 : 16);
+        }
+      }
+      *olen = total + 1;
+    }
+  }
+  uk_free_store(&k, &k.U);
+  uk_free_store(&k, &k.CP);
+  free(k.ra); free(k.rb); free(k.rc);
+  for (int q = 0; q < 3; ++q) free(str[q]);
+  return cost;
+}

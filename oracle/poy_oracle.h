@@ -155,6 +155,11 @@ void po_nonadd_reroot_median(const uint8_t *a, const uint8_t *b, uint8_t *res, i
  * gap.  -1 when a character has no base bit. */
 int po_powell_3d(const uint8_t *a, int la, const uint8_t *b, int lb, const uint8_t *c, int lc, int mm,
                  int go, int ge, uint8_t *oa, uint8_t *ob, uint8_t *oc, int cap, int *olen);
+/* The reference's check-point Ukkonen search itself (src/ukk.checkp.c):
+ * its cost and the alignment it picks.  planes_mode 0 reuses cost planes as
+ * the reference does, 1 keeps every plane. */
+int po_powell_ukk(const uint8_t *a, int la, const uint8_t *b, int lb, const uint8_t *c, int lc, int mm, int go,
+                  int ge, int planes_mode, uint8_t *oa, uint8_t *ob, uint8_t *oc, int cap, int *olen);
 
 #ifdef __cplusplus
 }

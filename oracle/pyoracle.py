@@ -83,6 +83,8 @@ def lib():
             "po_nonadd_reroot_median": (None, [u8p, u8p, u8p, C.c_int]),
             "po_powell_3d": (C.c_int, [u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int,
                                        C.c_int, u8p, u8p, u8p, C.c_int, i32p]),
+            "po_powell_ukk": (C.c_int, [u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, u8p, u8p, u8p, C.c_int, i32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -392,6 +394,22 @@ def nonadd_distance(a, b) -> int:
 def nonadd_dist_2(a, b, c) -> int:
     x, y, z = _u8(a), _u8(b), _u8(c)
     return int(lib().po_nonadd_dist_2(_p8(x), _p8(y), _p8(z), len(x)))
+
+
+def powell_ukk(a, b, c, mm: int, go: int, ge: int, planes_mode: int = 0):
+    """The reference's own check-point Ukkonen search (src/ukk.checkp.c):
+    its cost and the alignment it picks (three aligned sequences with the
+    leading gap); None when a character has no base.  planes_mode 0 reuses
+    cost planes as the reference does, 1 keeps every plane."""
+    x, y, z = _u8(a), _u8(b), _u8(c)
+    cap = len(x) + len(y) + len(z) + 2
+    o = [np.zeros(cap, np.uint8) for _ in range(3)]
+    n = C.c_int(0)
+    cost = lib().po_powell_ukk(_p8(x), len(x), _p8(y), len(y), _p8(z), len(z), mm, go, ge, planes_mode,
+                               _p8(o[0]), _p8(o[1]), _p8(o[2]), cap, C.byref(n))
+    if cost < 0:
+        return None
+    return cost, o[0][:n.value].copy(), o[1][:n.value].copy(), o[2][:n.value].copy()
 
 
 def powell_3d(a, b, c, mm: int, go: int, ge: int):

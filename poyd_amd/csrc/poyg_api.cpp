@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "poyg_band.h"
+#include "poyg_ukk.h"
 #include "poyg_internal.h"
 
 namespace poyg {
@@ -1553,21 +1554,100 @@ int poyg_algn_align_3d(poyg_ctx *ctx, const poyg_cm3 *cm3, const uint8_t *s1, in
 // ---- powell_3D_align (src/ukkCommon.c:110-151) ---------------------------
 extern "C++" {
 namespace poyg {
-struct PwTables;
-void pw_tables(int go, PwTables *T);
-size_t pw_tables_size();
-hipError_t launch_powell(int n1, int n2, int n3, const uint8_t *x1, const uint8_t *x2, const uint8_t *x3, int *V,
-                         uint8_t *from, const PwTables *tab, int mm, int ge, uint8_t *out, int cap, int *res,
-                         hipStream_t s);
+void uk_model(int mm, int go, int ge, UkModel *M);
+hipError_t launch_ukk_reset(UkWalk *w, int first, hipStream_t s);
+hipError_t launch_ukk_geom(const UkRange &r, int ge, int c0, int c1, UkLevel *lev, hipStream_t s);
+hipError_t launch_ukk_level(const UkModel *M, const UkRange &r, int c, int64_t diags, const UkLevel *lev, uint32_t *U,
+                            const char *A, const char *B, const char *C, int target_state, int target_dist,
+                            UkWalk *w, hipStream_t s);
+hipError_t launch_ukk_esweep(const UkModel *M, const UkRange &r, int c, int final_cost, int64_t diags,
+                             const UkLevel *lev, const uint32_t *U, int *E, int64_t ring_words, int ring_n, int half,
+                             UkWalk *w, hipStream_t s);
+hipError_t launch_ukk_walk(const UkModel *M, const UkRange &r, int cpcost, int cpw, const UkLevel *lev,
+                           const uint32_t *U, int first_pass, UkWalk *w, hipStream_t s);
+hipError_t launch_ukk_trace(const UkModel *M, const UkRange &r, const UkLevel *lev, const uint32_t *U, const char *A,
+                            const char *B, const char *C, char *cols, int cap, UkWalk *w, hipStream_t s);
 }  // namespace poyg
 }
 
+namespace {
+
+// The level store of one search range on the device: the level boxes
+// (computed on the device by ukk_geom_kernel, mirrored here to size the
+// store) and 16 words per diagonal and level.
+struct UkStore {
+  DevBuf lev, U;
+  size_t lev_cap = 0, u_words = 0;
+  std::vector<UkLevel> box;   // host mirror of the current range's boxes
+  void release() {
+    lev.release();
+    U.release();
+    lev_cap = u_words = 0;
+  }
+};
+
+constexpr int64_t kUkStoreLimit = int64_t(24) << 30;   // bytes of cost levels one call may hold
+
+// Boxes of levels [c0, c1] of range r appended to the host mirror; returns
+// the store's words after level c1.
+int64_t uk_boxes(std::vector<UkLevel> &box, const UkRange &r, int ge, int c0, int c1) {
+  int64_t base = 0;
+  if (c0 > r.scost) base = box[c0 - 1 - r.scost].base + (int64_t)box[c0 - 1 - r.scost].nab * box[c0 - 1 - r.scost].nac * kUkStates;
+  box.resize((size_t)(c1 - r.scost + 1));
+  for (int c = c0; c <= c1; ++c) {
+    box[c - r.scost] = uk_level_box(r, ge, c, base);
+    base += (int64_t)box[c - r.scost].nab * box[c - r.scost].nac * kUkStates;
+  }
+  return base;
+}
+
+// Room for `words` level words (keep: copy what the store holds).
+int uk_reserve(UkStore &st, size_t levels, int64_t words, bool keep, hipStream_t s) {
+  if (words * 4 > kUkStoreLimit) {
+    set_error("powell_3d: the search needs %lld bytes of cost levels, above the %lld-byte limit", (long long)words * 4,
+              (long long)kUkStoreLimit);
+    return POYG_ENOMEM;
+  }
+  if (levels > st.lev_cap) {
+    DevBuf nl;
+    const size_t cap = std::max(levels, 2 * st.lev_cap);
+    if (nl.alloc(cap * sizeof(UkLevel))) return POYG_ENOMEM;
+    if (keep && st.lev_cap && hipMemcpyAsync(nl.p, st.lev.p, st.lev_cap * sizeof(UkLevel), hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return POYG_EDEVICE;
+    st.lev.release();
+    st.lev = nl;
+    st.lev_cap = cap;
+  }
+  if ((size_t)words > st.u_words) {
+    DevBuf nu;
+    const size_t cap = std::max((size_t)words, 2 * st.u_words);
+    if (nu.alloc(cap * 4)) return POYG_ENOMEM;
+    if (keep && st.u_words && hipMemcpyAsync(nu.p, st.U.p, st.u_words * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return POYG_EDEVICE;
+    st.U.release();
+    st.U = nu;
+    st.u_words = cap;
+  }
+  return POYG_OK;
+}
+
+}  // namespace
+
+// The check-point Ukkonen search of powell_3D_align (doUkk,
+// src/ukk.checkp.c:113-214): the first pass over costs 1, 2, ... on the
+// device in chunks of levels until the end diagonal's best state reaches
+// |A|; the check-point cost from the evaluated cells (ukk_esweep_kernel);
+// then doUkkInLimits ranges (src/ukk.checkp.c:216-322) from a stack in the
+// reference's order (a split range's later half first), each its levels,
+// then its traceback (base case, at most CPwidth costs) or its split; the
+// columns come back in reverse and go out after the common prefix
+// (printTraceBack, src/ukk.checkp.c:456-500).
 int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, int lb, const uint8_t *c, int lc,
                    int mm, int go, int ge, uint8_t *ra, uint8_t *rb, uint8_t *rc, int cap, int *rlen, int *cost) {
   if (!ctx || !a || !b || !c || !ra || !rb || !rc || la < 1 || lb < 1 || lc < 1) return POYG_EINVAL;
   // the reference's preconditions (src/ukkCommon.c:132)
-  if (mm == 0 || go < 0 || ge <= 0) {
-    set_error("powell_3d: needs mm != 0, go >= 0, ge > 0");
+  if (mm <= 0 || go < 0 || ge <= 0) {
+    set_error("powell_3d: needs mm > 0, go >= 0, ge > 0");
     return POYG_EINVAL;
   }
   if (cap < la + lb + lc) {
@@ -1575,12 +1655,16 @@ int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, in
     return POYG_EINVAL;
   }
   // the first set base bit of every mask after the leading gap
-  // (copySequence, src/ukkCommon.c:88-108)
+  // (copySequence, src/ukkCommon.c:88-108), each string followed by a 0
   const int n[3] = {la - 1, lb - 1, lc - 1};
   const uint8_t *src[3] = {a, b, c};
-  std::vector<uint8_t> x[3];
+  std::vector<char> x[3];
   for (int q = 0; q < 3; ++q) {
-    x[q].resize(std::max(n[q], 1));
+    if (n[q] > kUkMaxLen) {
+      set_error("powell_3d: sequence %d has %d positions, above %d", q + 1, n[q], kUkMaxLen);
+      return POYG_EUNSUPPORTED;
+    }
+    x[q].assign((size_t)n[q] + 8, 0);
     for (int i = 0; i < n[q]; ++i) {
       const uint8_t m = src[q][i + 1];
       const char ch = (m & 1) ? 'A' : (m & 2) ? 'C' : (m & 4) ? 'G' : (m & 8) ? 'T' : 0;
@@ -1589,63 +1673,160 @@ int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, in
                   i + 1, q + 1);
         return POYG_EINVAL;
       }
-      x[q][i] = (uint8_t)ch;
+      x[q][i] = ch;
     }
   }
-  const int64_t plane = (int64_t)(n[0] + 1) * (n[1] + 1);
-  const int64_t cells = plane * (n[2] + 1);
-  const int64_t from_bytes = cells * 16, v_bytes = 4 * plane * 16 * 4;
-  if (from_bytes > (int64_t(16) << 30)) {
-    set_error("powell_3d: %lld lattice cells exceed the 1 Gi-cell limit", (long long)cells);
-    return POYG_ENOMEM;
-  }
-  const int dcap = n[0] + n[1] + n[2] + 2;
-  std::vector<uint8_t> tab(pw_tables_size());
-  pw_tables(go, (PwTables *)tab.data());
-  DevBuf dx[3], dv, dfrom, dtab, dout, dres;
-  for (int q = 0; q < 3; ++q)
-    if (dx[q].alloc(x[q].size())) return POYG_ENOMEM;
-  if (dv.alloc((size_t)v_bytes) || dfrom.alloc((size_t)from_bytes) || dtab.alloc(tab.size()) ||
-      dout.alloc((size_t)3 * dcap) || dres.alloc(8)) {
-    for (auto &d : dx) d.release();
-    dv.release(); dfrom.release(); dtab.release(); dout.release(); dres.release();
-    set_error("powell_3d: device allocation failed");
-    return POYG_ENOMEM;
-  }
+  const int LA = n[0], LB = n[1], LC = n[2];
+  UkModel M;
+  uk_model(mm, go, ge, &M);
+  const int cpw = M.max_step;
+  int start = 0;   // the common prefix (src/ukk.checkp.c:139-147)
+  while (start < LA && x[0][start] == x[1][start] && x[0][start] == x[2][start]) ++start;
+  const int col_cap = LA + LB + LC + 2;
+  // an upper bound of the least cost: every column costs at most
+  // maxSingleStep + 2 mismatches
+  const int cost_bound = (LA + LB + LC + 1) * (M.max_step + 2 * mm) + 8;
+
   hipStream_t s = ctx->stream;
+  DevBuf dx[3], dM, dW, dCols, dE;
+  UkStore st;
+  auto cleanup = [&]() {
+    for (auto &d : dx) d.release();
+    dM.release(); dW.release(); dCols.release(); dE.release();
+    st.release();
+  };
+  for (int q = 0; q < 3; ++q)
+    if (dx[q].alloc(x[q].size())) { cleanup(); return POYG_ENOMEM; }
+  if (dM.alloc(sizeof(UkModel)) || dW.alloc(sizeof(UkWalk)) || dCols.alloc((size_t)3 * col_cap)) {
+    cleanup();
+    return POYG_ENOMEM;
+  }
   hipError_t e = hipSuccess;
   for (int q = 0; q < 3 && e == hipSuccess; ++q)
     e = hipMemcpyAsync(dx[q].p, x[q].data(), x[q].size(), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(dtab.p, tab.data(), tab.size(), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess)
-    e = launch_powell(n[0], n[1], n[2], (const uint8_t *)dx[0].p, (const uint8_t *)dx[1].p,
-                      (const uint8_t *)dx[2].p, (int *)dv.p, (uint8_t *)dfrom.p, (const PwTables *)dtab.p, mm, ge,
-                      (uint8_t *)dout.p, dcap, (int *)dres.p, s);
-  std::vector<uint8_t> h_out((size_t)3 * dcap);
-  int h_res[2] = {0, 0};
-  if (e == hipSuccess) e = hipMemcpyAsync(h_out.data(), dout.p, h_out.size(), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(h_res, dres.p, 8, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  for (auto &d : dx) d.release();
-  dv.release(); dfrom.release(); dtab.release(); dout.release(); dres.release();
-  if (e != hipSuccess) {
-    set_error("powell_3d: %s", hipGetErrorString(e));
-    return POYG_EDEVICE;
+  if (e == hipSuccess) e = hipMemcpyAsync(dM.p, &M, sizeof M, hipMemcpyHostToDevice, s);
+  const char *A = (const char *)dx[0].p, *B = (const char *)dx[1].p, *C = (const char *)dx[2].p;
+  const UkModel *dm = (const UkModel *)dM.p;
+  UkWalk *dw = (UkWalk *)dW.p;
+  UkWalk hw;
+  int rc_ = POYG_OK;
+  auto fail = [&](int code, const char *what) {
+    if (code == POYG_EDEVICE) set_error("powell_3d: %s: %s", what, hipGetErrorString(e));
+    else if (code == POYG_EBAND) set_error("powell_3d: %s", what);
+    rc_ = code;
+    return code;
+  };
+  auto read_walk = [&]() -> bool {
+    if (e == hipSuccess) e = hipMemcpyAsync(&hw, dw, sizeof hw, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess;
+  };
+  // ---- first pass: levels 0, 1, ... until the end's best state reaches |A|
+  UkRange r0{0, 0, 0, 0, start, LA - LB, LA - LC, 0, 0, LA, LA, LB, LC};
+  if (e == hipSuccess) e = launch_ukk_reset(dw, 1, s);
+  int final_cost = -1;
+  {
+    const int chunk = 32;
+    st.box.clear();
+    for (int c0 = 0; e == hipSuccess && final_cost < 0; c0 += chunk) {
+      if (c0 > cost_bound) { cleanup(); return fail(POYG_EBAND, "the first pass did not reach the end"); }
+      const int c1 = c0 + chunk - 1;
+      const int64_t words = uk_boxes(st.box, r0, ge, c0, c1);
+      const int rr = uk_reserve(st, (size_t)(c1 + 1), words, true, s);
+      if (rr) { cleanup(); return rr; }
+      if (e == hipSuccess) e = launch_ukk_geom(r0, ge, c0, c1, (UkLevel *)st.lev.p, s);
+      for (int cc = c0; cc <= c1 && e == hipSuccess; ++cc)
+        e = launch_ukk_level(dm, r0, cc, (int64_t)st.box[cc].nab * st.box[cc].nac, (const UkLevel *)st.lev.p,
+                             (uint32_t *)st.U.p, A, B, C, -1, LA, dw, s);
+      if (!read_walk()) break;
+      if (hw.reached <= c1) final_cost = hw.reached;
+    }
   }
-  const int len = h_res[0];
-  if (h_res[1] < 0) {
-    set_error("powell_3d: the traceback left the lattice");
-    return POYG_EBAND;
+  if (e != hipSuccess) { cleanup(); return fail(POYG_EDEVICE, "first pass"); }
+  // ---- the check-point cost: cells evaluated by the top-level calls
+  r0.fcost = final_cost;
+  {
+    const int ring_n = M.max_step + 1;
+    const int64_t ring_words = (int64_t)st.box[final_cost].nab * st.box[final_cost].nac * kUkStates;
+    if (dE.alloc((size_t)(ring_words * ring_n * 4))) { cleanup(); return POYG_ENOMEM; }
+    for (int cc = final_cost; cc >= 0 && e == hipSuccess; --cc)
+      e = launch_ukk_esweep(dm, r0, cc, final_cost, (int64_t)st.box[cc].nab * st.box[cc].nac,
+                            (const UkLevel *)st.lev.p, (const uint32_t *)st.U.p, (int *)dE.p, ring_words, ring_n,
+                            LA / 2, dw, s);
+    if (e == hipSuccess)
+      e = launch_ukk_walk(dm, r0, 0, cpw, (const UkLevel *)st.lev.p, (const uint32_t *)st.U.p, 1, dw, s);
+    if (!read_walk()) { cleanup(); return fail(POYG_EDEVICE, "check-point walk"); }
+    dE.release();
+    if (hw.error) { cleanup(); return fail(POYG_EBAND, "the first pass's walk found no end state"); }
   }
-  if (len < 1 || len > dcap || len > cap) {
+  // ---- doUkkInLimits ranges, later halves first
+  std::vector<UkRange> stack;
+  r0.fstate = hw.fstate;
+  if (hw.f_cost < 0) {
+    stack.push_back(r0);   // checked too late: the whole range again
+  } else {
+    const UkRange first{0, 0, 0, 0, start, hw.f_ab, hw.f_ac, hw.f_cost, hw.f_state, hw.f_dist, hw.f_dist,
+                        hw.f_dist - hw.f_ab, hw.f_dist - hw.f_ac};
+    const UkRange later{hw.f_ab, hw.f_ac, hw.f_cost, hw.f_state, hw.f_dist, r0.fab, r0.fac, final_cost, r0.fstate,
+                        LA, LA, LB, LC};
+    stack.push_back(first);
+    stack.push_back(later);
+  }
+  size_t ranges = 0;
+  while (!stack.empty() && e == hipSuccess) {
+    const UkRange r = stack.back();
+    stack.pop_back();
+    if (++ranges > (size_t)8 * col_cap + 64) { cleanup(); return fail(POYG_EBAND, "the check-point recursion did not close"); }
+    st.box.clear();
+    const int64_t words = uk_boxes(st.box, r, ge, r.scost, r.fcost);
+    const int rr = uk_reserve(st, (size_t)(r.fcost - r.scost + 1), words, false, s);
+    if (rr) { cleanup(); return rr; }
+    e = launch_ukk_reset(dw, 0, s);
+    if (e == hipSuccess) e = launch_ukk_geom(r, ge, r.scost, r.fcost, (UkLevel *)st.lev.p, s);
+    for (int cc = r.scost; cc <= r.fcost && e == hipSuccess; ++cc)
+      e = launch_ukk_level(dm, r, cc, (int64_t)st.box[cc - r.scost].nab * st.box[cc - r.scost].nac,
+                           (const UkLevel *)st.lev.p, (uint32_t *)st.U.p, A, B, C, r.fstate, r.fdist, dw, s);
+    if (e != hipSuccess) break;
+    if (r.fcost - r.scost <= cpw) {
+      e = launch_ukk_trace(dm, r, (const UkLevel *)st.lev.p, (const uint32_t *)st.U.p, A, B, C, (char *)dCols.p,
+                           col_cap, dw, s);
+      continue;
+    }
+    const int cpcost = (r.fcost + r.scost - cpw + 1) / 2;
+    e = launch_ukk_walk(dm, r, cpcost, cpw, (const UkLevel *)st.lev.p, (const uint32_t *)st.U.p, 0, dw, s);
+    if (!read_walk()) break;
+    if (hw.error || hw.f_cost < 0) { cleanup(); return fail(POYG_EBAND, "a range's end has no check point"); }
+    const UkRange first{r.sab, r.sac, r.scost, r.sstate, r.sdist, hw.f_ab, hw.f_ac, hw.f_cost, hw.f_state, hw.f_dist,
+                        hw.f_dist, hw.f_dist - hw.f_ab, hw.f_dist - hw.f_ac};
+    const UkRange later{hw.f_ab, hw.f_ac, hw.f_cost, hw.f_state, hw.f_dist, r.fab, r.fac, hw.reached, r.fstate,
+                        r.fdist, r.endA, r.endB, r.endC};
+    stack.push_back(first);
+    stack.push_back(later);
+  }
+  std::vector<char> cols((size_t)3 * col_cap);
+  if (read_walk() && hw.ncols > 0)
+    e = hipMemcpy(cols.data(), dCols.p, (size_t)3 * hw.ncols, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) { cleanup(); return fail(POYG_EDEVICE, "search"); }
+  cleanup();
+  if (hw.error) return fail(POYG_EBAND, "a traceback left its range");
+  const int len = 1 + start + hw.ncols;
+  if (len > cap) {
     set_error("powell_3d: alignment of %d columns does not fit", len);
     return POYG_EINVAL;
   }
+  auto mask = [](char ch) -> uint8_t {
+    return (uint8_t)(ch == 'A' ? 1 : ch == 'C' ? 2 : ch == 'G' ? 4 : ch == 'T' ? 8 : 16);
+  };
   uint8_t *dst[3] = {ra, rb, rc};
-  for (int q = 0; q < 3; ++q) memcpy(dst[q] + cap - len, h_out.data() + (size_t)q * dcap + (dcap - len), len);
+  for (int q = 0; q < 3; ++q) {
+    uint8_t *o = dst[q] + cap - len;
+    o[0] = 16;
+    for (int i = 0; i < start; ++i) o[1 + i] = mask(x[0][i]);
+    for (int i = 0; i < hw.ncols; ++i) o[1 + start + i] = mask(cols[(size_t)3 * (hw.ncols - 1 - i) + q]);
+  }
   if (rlen) *rlen = len;
-  if (cost) *cost = h_res[1];
-  return POYG_OK;
+  if (cost) *cost = final_cost;
+  return rc_;
 }
 
 // ---- host-side column rules (O(L) per call) ------------------------------

@@ -499,10 +499,33 @@ def test_powell_matches_oracle(ctx, oracle, mm, go, ge):
                 x = np.concatenate([x, rand_seq(rng, n + 1 - len(x))[1:]]).astype(np.uint8)
             t.append(x)
         cost, al = engine.powell_3d(ctx, *t, mm, go, ge)
-        want = oracle.powell_3d(*t, mm, go, ge)
-        assert cost == want[0], (n1, n2, n3)
+        # the reference's own search picks the alignment (its cost is the
+        # model's least cost, except that three equal strings cost 1: the
+        # search's loop starts at cost 1)
+        want = oracle.powell_ukk(*t, mm, go, ge)
+        least = oracle.powell_3d(*t, mm, go, ge)[0]
+        same = all(len(x) == len(t[0]) and np.array_equal(x, t[0]) for x in t)
+        assert cost == want[0] == (1 if same else least), (n1, n2, n3)
         for q in range(3):
             assert np.array_equal(al[q], want[1 + q]), (n1, n2, n3, q)
+
+
+def test_powell_search_choices(ctx, oracle):
+    # divergent triples, where least-cost alignments are many and the
+    # check-point recursion runs several levels deep: the engine's alignment
+    # is the reference search's, column for column
+    from poyd_amd import engine
+    note("powell_3d search choices")
+    rng = np.random.default_rng(58)
+    for k in range(8):
+        mm, go, ge = [(1, 2, 1), (2, 0, 1), (1, 3, 2), (3, 1, 1), (1, 3, 1)][k % 5]
+        anc = rand_seq(rng, int(rng.integers(40, 100)))
+        t = [mutate(rng, anc, 0.3, 0.1) for _ in range(3)]
+        cost, al = engine.powell_3d(ctx, *t, mm, go, ge)
+        want = oracle.powell_ukk(*t, mm, go, ge)
+        assert cost == want[0] == oracle.powell_3d(*t, mm, go, ge)[0], k
+        for q in range(3):
+            assert np.array_equal(al[q], want[1 + q]), (k, q)
 
 
 def test_powell_rejects_gap_only_characters(ctx):

@@ -564,6 +564,84 @@ def test_powell_alignment_recosts_to_its_cost(oracle):
         assert min(best.values()) == cost
 
 
+def test_powell_search_cost_is_the_least_cost(oracle):
+    # the restated check-point search (po_powell_ukk) ends at the model's
+    # least cost (brute force on tiny inputs, the exhaustive DP beyond), but
+    # at cost 1 for three equal strings: its loop starts at cost 1
+    rng = np.random.default_rng(45)
+    bases = "ACGT"
+    code = {"A": 1, "C": 2, "G": 4, "T": 8}
+    for mm, go, ge in [(1, 2, 1), (2, 0, 1), (1, 3, 2), (3, 1, 1)]:
+        for _ in range(8):
+            lens = [int(v) for v in rng.integers(0, 3, size=3)]
+            x = ["".join(bases[int(v)] for v in rng.integers(0, 3, size=n)) for n in lens]
+            enc = [np.array([16] + [code[ch] for ch in s], np.uint8) for s in x]
+            got = oracle.powell_ukk(*enc, mm, go, ge)
+            want = 1 if x[0] == x[1] == x[2] else powell_brute_force(x, mm, go, ge)
+            assert got[0] == want, (x, mm, go, ge)
+        for _ in range(6):
+            anc = rng.choice([1, 2, 4, 8], size=int(rng.integers(5, 40)))
+            enc = [np.concatenate([[16], anc[rng.random(len(anc)) > 0.25]]).astype(np.uint8) for _ in range(3)]
+            if any(len(e) == 1 for e in enc) or all(np.array_equal(e, enc[0]) for e in enc):
+                continue
+            assert oracle.powell_ukk(*enc, mm, go, ge)[0] == oracle.powell_3d(*enc, mm, go, ge)[0]
+
+
+def test_powell_search_alignment_recosts_and_strips(oracle):
+    # the search's alignment strips back to its inputs and recosts
+    # (alignmentCost over the least labelling of its columns) to its cost
+    rng = np.random.default_rng(46)
+    states = powell_states()
+    inv = {1: "A", 2: "C", 4: "G", 8: "T"}
+    for k in range(16):
+        mm, go, ge = [(1, 2, 1), (2, 0, 1), (1, 3, 2), (3, 1, 1)][k % 4]
+        anc = rng.choice([1, 2, 4, 8], size=int(rng.integers(4, 30)))
+        x = []
+        for _ in range(3):
+            body = anc[rng.random(len(anc)) > 0.2].copy()
+            flip = rng.random(len(body)) < 0.2
+            body[flip] = rng.choice([1, 2, 4, 8], size=int(flip.sum()))
+            x.append(np.concatenate([[16], body, rng.choice([1, 2, 4, 8], size=int(rng.integers(0, 3)))]).astype(np.uint8))
+        cost, a, b, c = oracle.powell_ukk(*x, mm, go, ge)
+        for out, src in zip((a, b, c), x):
+            assert out[0] == 16 and np.array_equal(out[1:][out[1:] != 16], src[1:])
+        best = {(0, 0, 0): 0}
+        for col in range(1, len(a)):
+            chars = [a[col], b[col], c[col]]
+            has = tuple(int(v != 16) for v in chars)
+            nxt = {}
+            for prev, cst in best.items():
+                for st, adv in states:
+                    if adv != has:
+                        continue
+                    v = cst + powell_column_cost(prev, st, [inv.get(int(ch)) for ch in chars], mm, go, ge)
+                    nxt[st] = min(nxt.get(st, v), v)
+            best = nxt
+        same = all(len(v) == len(x[0]) and np.array_equal(v, x[0]) for v in x)
+        assert min(best.values()) == (0 if same else cost), k
+
+
+def test_powell_search_plane_reuse_changes_nothing(oracle):
+    # the reference keeps 2 maxSingleStep cost planes and reuses plane
+    # d mod that count; keeping every plane gives the same costs and the same
+    # alignments (no reused plane is read stale on these inputs)
+    rng = np.random.default_rng(47)
+    for k in range(24):
+        mm, go, ge = [(1, 2, 1), (2, 0, 1), (1, 3, 2), (3, 1, 1)][k % 4]
+        anc = rng.choice([1, 2, 4, 8], size=int(rng.integers(10, 60)))
+        x = []
+        for _ in range(3):
+            body = anc[rng.random(len(anc)) > 0.2].copy()
+            flip = rng.random(len(body)) < 0.25
+            body[flip] = rng.choice([1, 2, 4, 8], size=int(flip.sum()))
+            x.append(np.concatenate([[16], body]).astype(np.uint8))
+        reuse = oracle.powell_ukk(*x, mm, go, ge, 0)
+        keep = oracle.powell_ukk(*x, mm, go, ge, 1)
+        assert reuse[0] == keep[0]
+        for p, q in zip(reuse[1:], keep[1:]):
+            assert np.array_equal(p, q), k
+
+
 def test_powell_rejects_gap_only_characters(oracle):
     x = np.array([16, 1, 16, 2], np.uint8)
     assert oracle.powell_3d(x, x, x, 1, 2, 1) is None
