@@ -283,7 +283,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 50 for affine, 10 for the other pair workloads; tree and tbr: 1)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 2; tree: 0)")
     ap.add_argument("--workload", default="affine",
-                    choices=["affine", "linear", "cube", "tree", "fitch", "tbr", "percall"])
+                    choices=["affine", "linear", "cube", "tree", "fitch", "tbr", "percall", "powell"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sub", action="store_true", help="skip the linear and three-way sub-results")
@@ -317,6 +317,8 @@ def main() -> int:
         return bench_tbr(args)
     if args.workload == "percall":
         return bench_percall(args)
+    if args.workload == "powell":
+        return bench_powell(args)
 
     r = run_workload(args.workload, args.steps, args.warmup, args.tune)
     w, kind, ms, cells = r["w"], r["kind"], r["ms"], r["cells"]
@@ -980,6 +982,51 @@ def bench_percall(args) -> int:
             log(json.dumps(rows[-1]))
     print(json.dumps({"metric": "per-call latency of the stub entry points", "unit": "us per call",
                       "rows": rows, "cpu": cpu_model()}), flush=True)
+    return 0
+
+
+def bench_powell(args) -> int:
+    """Powell's three-way search (powell_3D_align -> poyg_powell_3d), one
+    triple per call: three descendants of a random ancestor (substitutions
+    and indels at the given rates), mismatch 1, gap opening 2, extension 1;
+    against the oracle's restatement of the same search on one core where
+    that finishes in seconds (the shorter, closer triples), with the
+    alignments compared column for column."""
+    from oracle import pyoracle as po
+    from poyd_amd import engine
+    ctx = engine.Context(0)
+    rng = np.random.default_rng(5)
+
+    def descendant(anc, ps, pi):
+        out = [16]
+        for x in anc:
+            r = rng.random()
+            if r < pi / 2:
+                continue
+            if r < pi:
+                out.append(int(rng.choice([1, 2, 4, 8])))
+            out.append(int(rng.choice([1, 2, 4, 8])) if rng.random() < ps else int(x))
+        return np.array(out, np.uint8)
+
+    rows = []
+    for n, ps, pi in [(100, 0.1, 0.04), (300, 0.1, 0.04), (300, 0.2, 0.08), (1000, 0.05, 0.02), (1000, 0.1, 0.04)]:
+        anc = rng.choice([1, 2, 4, 8], size=n)
+        t = [descendant(anc, ps, pi) for _ in range(3)]
+        engine.powell_3d(ctx, *t, 1, 2, 1)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            cost, al = engine.powell_3d(ctx, *t, 1, 2, 1)
+        g_ms = (time.perf_counter() - t0) / 3 * 1e3
+        row = {"length": n, "substitutions": ps, "indels": pi, "cost": cost, "gpu_ms": round(g_ms, 1)}
+        if n <= 300 and ps <= 0.1 and not args.no_cpu:
+            t0 = time.perf_counter()
+            want = po.powell_ukk(*t, 1, 2, 1)
+            row["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+            row["same_alignment"] = bool(want[0] == cost and all(np.array_equal(p, q) for p, q in zip(al, want[1:])))
+        rows.append(row)
+        log(json.dumps(row))
+    print(json.dumps({"metric": "Powell three-way search latency", "unit": "ms per triple", "rows": rows,
+                      "cpu": cpu_model()}), flush=True)
     return 0
 
 
