@@ -255,6 +255,8 @@ po_cm2 *po_cm2_of_sub_indel(int sub, int indel) {
 }
 
 /* set_affine, src/cost_matrix.ml:405-416. */
+int po_cm2_gap_open(const po_cm2 *c) { return c->gap_open; }
+
 void po_cm2_set_affine(po_cm2 *m, int model, int gap_open) {
     m->cost_model = model;
     m->gap_open = model == 1 ? gap_open : 0;

@@ -60,6 +60,8 @@ po_cm2 *po_cm2_of_sub_indel(int sub, int indel);
 /* Cost_matrix.Two_D.set_affine (src/cost_matrix.ml:405-416); model 0 =
  * Linnear, 1 = Affine gap_open, 2 = No_Alignment. */
 void po_cm2_set_affine(po_cm2 *c, int model, int gap_open);
+/* the matrix's gap opening (Cost_matrix.Two_D.gap_opening). */
+int po_cm2_gap_open(const po_cm2 *c);
 /* Cost_matrix.Three_D.of_two_dim (src/cost_matrix.ml:803-861). */
 po_cm3 *po_cm3_of_two_dim(const po_cm2 *c);
 void po_cm3_free(po_cm3 *c);

@@ -83,6 +83,7 @@ def lib():
             "po_nonadd_reroot_median": (None, [u8p, u8p, u8p, C.c_int]),
             "po_powell_3d": (C.c_int, [u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int,
                                        C.c_int, u8p, u8p, u8p, C.c_int, i32p]),
+            "po_cm2_gap_open": (C.c_int, [vp]),
             "po_powell_ukk": (C.c_int, [u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_int, u8p, u8p, u8p, C.c_int, i32p]),
         }
@@ -135,6 +136,10 @@ class CM2:
 
     def cost(self, a: int, b: int) -> int:
         return lib().po_cm2_cost(self.h, a, b)
+
+    @property
+    def gap_open(self) -> int:
+        return lib().po_cm2_gap_open(self.h)
 
     def worst(self, a: int, b: int) -> int:
         return lib().po_cm2_worst(self.h, a, b)

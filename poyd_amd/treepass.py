@@ -348,6 +348,38 @@ class TreeEval:
             seqs[c] = s
         return out
 
+    def _readjust_3d(self, jobs):
+        """Sequence.Align.readjust_3d (src/sequence.ml:1117-1139) through
+        SeqCS.readjust's `ThreeD branch (src/seqCS.ml:680-728), per
+        character, for a batch of vertices (jobs as _readjust): when the two
+        children, the vertex and the parent all have the same length the
+        vertex keeps its sequence; otherwise align_3_powell_inter
+        (:1089-1115): Powell's three-way search of (ch1, ch2, parent) with
+        mismatch cost(A, C), gap opening (affine matrices) and extension
+        cost(A, gap), and the new sequence is the Three_D median of every
+        column, gaps dropped, behind one leading gap."""
+        L = len(self.ds.chars)
+        out = [(list(m), set()) for (_a, _b, _p, m, _c) in jobs]
+        todo = []
+        for j, (a, b, p, m, chars) in enumerate(jobs):
+            for c in (range(L) if chars is None else sorted(chars)):
+                if is_empty(a[c]) or is_empty(b[c]) or is_empty(p[c]):
+                    raise NotImplementedError("ThreeD readjustment with an empty sequence among the three")
+                if len(a[c]) == len(b[c]) and len(m[c]) == len(p[c]) and len(a[c]) == len(m[c]):
+                    continue
+                todo.append((j, c))
+        if not todo:
+            return out
+        res = self.be.powell_median([(jobs[j][0][c], jobs[j][1][c], jobs[j][2][c]) for (j, c) in todo])
+        self.n_align += len(todo)
+        for (j, c), (med, _cost) in zip(todo, res):
+            seqs, changed = out[j]
+            old = jobs[j][3][c]
+            if not (len(med) == len(old) and np.array_equal(med, old)):
+                changed.add(c)
+            seqs[c] = med
+        return out
+
     def _edge_cost(self, seqs: dict) -> int:
         """check_cost (src/allDirChar.ml:185-213) over the given per-vertex
         sequences: DOS.distance summed over every edge."""
@@ -363,7 +395,8 @@ class TreeEval:
         self.n_align += len(pairs)
         return int(sum(int(x) for x in self.be.distance(pairs)))
 
-    def approx_d(self, diag: Diagnosis | None = None, iterations: int | None = None, sequential: bool = False):
+    def approx_d(self, diag: Diagnosis | None = None, iterations: int | None = None, sequential: bool = False,
+                 three_d: bool = False):
         """Iterative `ApproxD (AllDirChar.downpass, src/allDirChar.ml:879-883:
         internal_downpass, pick_best_root, assign_single, then adjust_tree,
         :488-670).  Starting from the single assignment, every interior
@@ -377,8 +410,10 @@ class TreeEval:
         not lower the cost).  A vertex reads the current sequences of its
         neighbours, so the passes run in layers: a vertex waits only for its
         neighbours that come earlier in the post order (sequential=True runs
-        one vertex at a time; the results are the same).  Returns
-        (adjusted sequences per vertex, their tree cost, passes kept)."""
+        one vertex at a time; the results are the same).  three_d: Iterative
+        `ThreeD, the same passes with Sequence.Align.readjust_3d (Powell's
+        three-way search) as the readjustment.  Returns (adjusted sequences
+        per vertex, their tree cost, passes kept)."""
         d = diag if diag is not None else self.diagnose()
         t = self.t
         adj = {v: list(s) for v, s in d.singles.items()}
@@ -401,7 +436,8 @@ class TreeEval:
                 layers = [[v for v in verts if lvl[v] == k] for k in range(max(lvl.values(), default=-1) + 1)]
             for batch in layers:
                 jobs = [(new[t.nbrs[v][0]], new[t.nbrs[v][1]], new[t.nbrs[v][2]], new[v], affected[v]) for v in batch]
-                for v, (seqs, modified) in zip(batch, self._readjust(jobs)):
+                readjust = self._readjust_3d if three_d else self._readjust
+                for v, (seqs, modified) in zip(batch, readjust(jobs)):
                     if not modified:
                         continue
                     new[v] = seqs
@@ -471,6 +507,24 @@ class EngineBackend:
     def fitch(self, a, b):
         self.fitch_chars += int(np.size(a))
         return self.engine.fitch_pairs(self.ctx, a, b)
+
+    def powell_median(self, triples):
+        """align_3_powell_inter (src/sequence.ml:1089-1115) per triple on
+        the GPU: poyg_powell_3d's alignment, then the Three_D median of
+        every column (host lookups), gaps dropped, one leading gap."""
+        e = self.engine
+        if getattr(self, "_med3", None) is None:
+            cm3 = e.CostMatrix3(self.cm)
+            self._med3 = np.array([[[cm3.median(a, b, c) for c in range(32)] for b in range(32)] for a in range(32)],
+                                  np.uint8)
+        mm, ge = self.cm.cost(1, 2), self.cm.cost(1, 16)
+        go = self.cm.gap_opening if self.affine else 0
+        out = []
+        for a, b, p in triples:
+            cost, (x, y, z) = e.powell_3d(self.ctx, a, b, p, mm, go, ge)
+            med = self._med3[x, y, z]
+            out.append((np.concatenate([[GAP], med[med != GAP]]).astype(np.uint8), cost))
+        return out
 
     def distance(self, pairs):
         e = self.engine

@@ -44,6 +44,20 @@ class OracleBackend:
             added.append(c)
         return np.array(out, np.uint8).reshape(len(a), -1), np.array(added, np.int64)
 
+    def powell_median(self, triples):
+        """align_3_powell_inter (src/sequence.ml:1089-1115) on the oracle:
+        the restated search's alignment, the Three_D median of every
+        column, gaps dropped, one leading gap."""
+        cm3 = po.CM3(self.cm)
+        mm, ge = self.cm.cost(1, 2), self.cm.cost(1, 16)
+        go = self.cm.gap_open if self.affine else 0
+        out = []
+        for a, b, p in triples:
+            cost, x, y, z = po.powell_ukk(a, b, p, mm, go, ge)
+            med = [cm3.median(int(u), int(v), int(w)) for u, v, w in zip(x, y, z)]
+            out.append((np.array([16] + [m for m in med if m != 16], np.uint8), cost))
+        return out
+
     def distance(self, pairs):
         if self.affine:
             return [po.cost_affine_3(a, b, self.cm) for a, b in pairs]

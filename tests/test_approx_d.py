@@ -62,3 +62,40 @@ def test_readjust_centre_rule():
     (seqs, changed), = ev._readjust([([s, s], [s, s], [s, s], [other, s], None)])
     assert np.array_equal(seqs[0], s) and np.array_equal(seqs[1], s)
     assert changed == {0}
+
+
+@pytest.mark.parametrize("n_taxa,seed", [(8, 3), (10, 4)])
+def test_three_d_layered_passes_equal_sequential_passes(n_taxa, seed):
+    """Iterative ThreeD (readjust_3d: Powell's search of the two children
+    and the parent, the Three_D median of its columns): the layered schedule
+    gives the one-vertex-at-a-time result, and passes never raise the cost."""
+    ds, tree, be = case(n_taxa, seed, lmax=100)
+    ev = treepass.TreeEval(ds, tree, be)
+    d = ev.diagnose()
+    adj_l, cost_l, kept_l = ev.approx_d(d, three_d=True)
+    adj_s, cost_s, kept_s = treepass.TreeEval(ds, tree, be).approx_d(d, sequential=True, three_d=True)
+    assert cost_l == cost_s and kept_l == kept_s
+    for v in adj_l:
+        for c in range(len(ds.chars)):
+            assert np.array_equal(adj_l[v][c], adj_s[v][c]), (v, c)
+    assert cost_l <= d.cost - d.nonadd
+    assert cost_l == treepass.TreeEval(ds, tree, be)._edge_cost(adj_l)
+
+
+def test_readjust_3d_rules():
+    """readjust_3d keeps the vertex's sequence when the two children, the
+    vertex and the parent have one length; otherwise the new sequence is the
+    Three_D median of the search's columns (three copies of a sequence: the
+    sequence itself, gaps dropped behind the leading gap)."""
+    ds, tree, be = case(8, 3)
+    ev = treepass.TreeEval(ds, tree, be)
+    s = np.array([16, 1, 2, 4, 8, 1, 1, 2], np.uint8)
+    t = np.array([16, 8, 8, 8, 2, 2, 2, 1], np.uint8)
+    (seqs, changed), = ev._readjust_3d([([s, s], [s, t], [s, s], [t, s], None)])
+    # character 0: s, s, s around t of the same length: unchanged
+    assert np.array_equal(seqs[0], t) and 0 not in changed
+    u = np.array([16, 1, 2, 4, 8], np.uint8)
+    (seqs, changed), = ev._readjust_3d([([s], [s], [s], [u], {0})])
+    assert np.array_equal(seqs[0], s) and changed == {0}
+    med, cost = be.powell_median([(s, s, s)])[0]
+    assert np.array_equal(med, s) and cost == 1   # the search's loop starts at cost 1

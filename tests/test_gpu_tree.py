@@ -86,3 +86,28 @@ def test_gpu_approx_d_matches_oracle(ctx, n_taxa, seed, affine):
     for v in o_adj:
         for c in range(len(ds.chars)):
             assert np.array_equal(g_adj[v][c], o_adj[v][c]), (v, c)
+
+
+@pytest.mark.parametrize("n_taxa,seed,affine", [(12, 4, False), (10, 8, True)])
+def test_gpu_three_d_matches_oracle(ctx, n_taxa, seed, affine):
+    """Iterative ThreeD (readjust_3d: Powell's search on the GPU, the
+    Three_D median of its columns) equals the same passes on the oracle's
+    restatement of the search: adjusted sequences, tree cost, passes kept."""
+    from poyd_amd import engine, synth, treebuild
+    from oracle import pyoracle as po
+    print(f"three_d {n_taxa} taxa affine={affine}", flush=True)
+    w = synth.tree_workload(n_taxa=n_taxa, n_loci=2, lmin=60, lmax=140, n_nonadd=0, seed=seed)
+    ds = treebuild.dataset_of(w)
+    if affine:
+        gb = treepass.EngineBackend(ctx, engine.CostMatrix.of_sub_indel(1, 1, 3))
+        ob = OracleBackend(po.CM2.sub_indel(1, 1, 3), True)
+    else:
+        gb = treepass.EngineBackend(ctx, engine.CostMatrix.of_sub_indel(1, 2))
+        ob = OracleBackend(po.CM2.sub_indel(1, 2), False)
+    tree = treebuild.wagner(ds, ob).tree
+    g_adj, g_cost, g_kept = treepass.TreeEval(ds, tree, gb).approx_d(three_d=True)
+    o_adj, o_cost, o_kept = treepass.TreeEval(ds, tree, ob).approx_d(three_d=True)
+    assert (g_cost, g_kept) == (o_cost, o_kept)
+    for v in o_adj:
+        for c in range(len(ds.chars)):
+            assert np.array_equal(g_adj[v][c], o_adj[v][c]), (v, c)
