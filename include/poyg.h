@@ -301,6 +301,13 @@ int poyg_plan_triples(poyg_ctx *ctx, const poyg_cm3 *cm3, int32_t n, const int64
 int poyg_fitch_down(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n,
                     const int32_t *d_ia, const int32_t *d_ib, const int32_t *d_ires,
                     int64_t *d_added, void *stream);
+/* The down pass on host rows (NonaddCS8.median, src/nonaddCS.ml:119-122,
+ * through char_nonadd_CAML_basic_median / nonadd_median,
+ * src/nonaddCSc.c:422-462, for a batch of row pairs): res
+ * (n x len) and added (n) receive the medians and change counts; one upload,
+ * one download, one synchronization. */
+int poyg_fitch_pairs_host(poyg_ctx *ctx, const uint8_t *a, const uint8_t *b, int32_t n, int32_t len, uint8_t *res,
+                          int64_t *added);
 /* nonadd_median_3 (src/nonaddCSc.c:472-544): F = uppass(A, N, D1, D2). */
 int poyg_fitch_up(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n, const int32_t *d_iA,
                   const int32_t *d_iN, const int32_t *d_iD1, const int32_t *d_iD2,

@@ -1998,6 +1998,52 @@ int poyg_fitch_up(poyg_ctx *ctx, uint8_t *d_chars, int32_t len, int32_t n, const
   return poyg_fitch_up_w(ctx, 1, d_chars, len, n, d_iA, d_iN, d_iD1, d_iD2, d_iF, stream);
 }
 
+// Fitch down-pass medians of n row pairs on host arrays (one byte per
+// character): one upload of both row sets from the context's pinned staging
+// (with the zeroed change counts and the row index lists), the kernel, one
+// download of the medians and counts into pinned memory, one
+// synchronization.
+int poyg_fitch_pairs_host(poyg_ctx *ctx, const uint8_t *a, const uint8_t *b, int32_t n, int32_t len, uint8_t *res,
+                          int64_t *added) {
+  if (!ctx || n < 0 || len < 0 || (n && (!a || !b || !res || !added))) {
+    set_error("poyg_fitch_pairs_host: bad arguments");
+    return POYG_EINVAL;
+  }
+  if (n == 0) return POYG_OK;
+  const size_t nm = (size_t)n * len;
+  const size_t add_off = (3 * nm + 255) & ~(size_t)255;   // the counts right after the medians
+  const size_t idx_off = (add_off + 8 * (size_t)n + 255) & ~(size_t)255;
+  const size_t bytes = idx_off + 12 * (size_t)n;
+  uint8_t *stage = ctx_pinned(ctx->pin_plan, ctx->pin_plan_cap, bytes);
+  uint8_t *back = ctx_pinned(ctx->pin_io, ctx->pin_io_cap, add_off + 8 * (size_t)n - 2 * nm, &ctx->pin_io_dev);
+  if (!stage || !back) return POYG_ENOMEM;
+  hipStream_t s = ctx->stream;
+  // the staging buffers are free once the context's stream is idle
+  HIP_TRY(hipStreamSynchronize(s));
+  memcpy(stage, a, nm);
+  memcpy(stage + nm, b, nm);
+  memset(stage + 2 * nm, 0, idx_off - 2 * nm);
+  int32_t *idx = (int32_t *)(stage + idx_off);
+  for (int32_t k = 0; k < 3 * n; ++k) idx[k] = k;
+  DevBuf d;
+  if (d.alloc(bytes)) return POYG_ENOMEM;
+  uint8_t *dp = (uint8_t *)d.p;
+  hipError_t e = hipMemcpyAsync(dp, stage, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = launch_fitch_down(1, dp, len, n, (const int32_t *)(dp + idx_off), (const int32_t *)(dp + idx_off) + n,
+                          (const int32_t *)(dp + idx_off) + 2 * n, (int64_t *)(dp + add_off), s);
+  if (e == hipSuccess) e = hipMemcpyAsync(back, dp + 2 * nm, add_off + 8 * (size_t)n - 2 * nm, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  d.release(e == hipSuccess);
+  if (e != hipSuccess) {
+    set_error("poyg_fitch_pairs_host: %s", hipGetErrorString(e));
+    return POYG_EDEVICE;
+  }
+  memcpy(res, back, nm);
+  memcpy(added, back + (add_off - 2 * nm), 8 * (size_t)n);
+  return POYG_OK;
+}
+
 // One node on host arrays of `len` characters of `width` bytes: two inputs
 // (down pass) or four (up pass).
 static int fitch_host(poyg_ctx *ctx, int width, const void *const *in, int nin, void *res, int len, int64_t *added) {

@@ -92,6 +92,7 @@ SIGNATURES = {
     "poyg_algn_cost_affine_3": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, i32p]),
     "poyg_algn_align_3d": (C.c_int, [vp, vp, u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, u8p, u8p, u8p,
                                      C.c_int, i32p, i32p]),
+    "poyg_fitch_pairs_host": (C.c_int, [vp, u8p, u8p, C.c_int32, C.c_int32, u8p, i64p]),
     "poyg_powell_3d": (C.c_int, [vp, u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int, C.c_int,
                                  u8p, u8p, u8p, C.c_int, i32p, i32p]),
     "poyg_algn_median_2_no_gaps": (C.c_int, [vp, u8p, u8p, C.c_int, u8p, C.c_int, i32p]),
@@ -641,23 +642,12 @@ def fitch_pairs(ctx: Context, a: np.ndarray, b: np.ndarray):
     n, m = a.shape
     if n == 0:
         return np.zeros((0, m), np.uint8), np.zeros(0, np.int64)
-    # one block, one upload: rows a, b, the result rows, the change counts
-    # (right after the results, so that one download returns both), then
-    # the row index lists
-    nm = n * m
-    add_off = (3 * nm + 255) // 256 * 256
-    idx_off = (add_off + 8 * n + 255) // 256 * 256
-    blob = np.zeros(idx_off + 12 * n, np.uint8)
-    blob[:nm] = a.reshape(-1)
-    blob[nm:2 * nm] = b.reshape(-1)
-    blob[idx_off:].view(np.int32)[:] = np.arange(3 * n, dtype=np.int32)
-    d = DeviceBuffer.from_array(ctx, blob)
-    _check(load().poyg_fitch_down(ctx.h, d.ptr, m, n, d.ptr + idx_off, d.ptr + idx_off + 4 * n,
-                                  d.ptr + idx_off + 8 * n, d.ptr + add_off, None), "poyg_fitch_down")
-    back = np.empty(add_off + 8 * n - 2 * nm, np.uint8)
-    _check(load().poyg_dev_download(ctx.h, back.ctypes.data, d.ptr + 2 * nm, back.nbytes), "poyg_dev_download")
-    res = back[:nm].reshape(n, m).copy()
-    return res, back[add_off - 2 * nm:].view(np.int64).copy()
+    # one call: one upload, the kernel, one download, one synchronization
+    res = np.empty((n, m), np.uint8)
+    added = np.empty(n, np.int64)
+    _check(load().poyg_fitch_pairs_host(ctx.h, _p8(a), _p8(b), n, m, _p8(res), added.ctypes.data_as(i64p)),
+           "poyg_fitch_pairs_host")
+    return res, added
 
 
 def batch_deltaw(a_batch, b_batch, dw=None) -> np.ndarray:

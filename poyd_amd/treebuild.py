@@ -66,12 +66,13 @@ def join_costs(ev: TreeEval, x: int, edges, emed) -> np.ndarray:
     ds, be = ev.ds, ev.be
     total = np.zeros(len(edges), np.int64)
     pairs, where = [], []
+    leaf = [ds.chars[c][x - 1] for c in range(len(ds.chars))]
+    live = [c for c in range(len(ds.chars)) if not is_empty(leaf[c])]
     for k, e in enumerate(edges):
-        for c in range(len(ds.chars)):
-            a, b = ds.chars[c][x - 1], emed[k].seqs[c]
-            if is_empty(a) or is_empty(b):
+        for c in live:
+            if emed[k].is_empty(c):
                 continue
-            pairs.append((a, b))
+            pairs.append((leaf[c], emed[k].seqs[c]))
             where.append(k)
     if pairs:
         d = be.distance(pairs)

@@ -95,6 +95,12 @@ class DirData:
     mcc: int
     na: np.ndarray | None = None   # nonadditive state sets (Fitch median)
     na_cost: int = 0
+    empty: list | None = None      # is_empty of each sequence, once known
+
+    def is_empty(self, c: int) -> bool:
+        if self.empty is None:
+            self.empty = [is_empty(x) for x in self.seqs]
+        return self.empty[c]
 
 
 @dataclass
@@ -124,25 +130,25 @@ class TreeEval:
         """jobs: list of (x: DirData, y: DirData).  Returns DirData list."""
         pairs, where = [], []
         out = []
+        L = len(self.ds.chars)
         for j, (x, y) in enumerate(jobs):
             first, second = (x, y) if x.mcc < y.mcc else (y, x)
-            seqs, cost = [], 0
-            for c in range(len(self.ds.chars)):
-                a, b = first.seqs[c], second.seqs[c]
-                if is_empty(a):
-                    seqs.append(b)
-                elif is_empty(b):
-                    seqs.append(a)
+            seqs, empty = [None] * L, [False] * L
+            for c in range(L):
+                if first.is_empty(c):
+                    seqs[c], empty[c] = second.seqs[c], second.is_empty(c)
+                elif second.is_empty(c):
+                    seqs[c] = first.seqs[c]
                 else:
-                    seqs.append(None)
-                    pairs.append((a, b))
+                    pairs.append((first.seqs[c], second.seqs[c]))
                     where.append((j, c))
-            out.append(DirData(seqs, x.cost + y.cost, min(x.mcc, y.mcc)))
+            out.append(DirData(seqs, x.cost + y.cost, min(x.mcc, y.mcc), empty=empty))
         if pairs:
             res = self.be.median(pairs)
             self.n_align += len(pairs)
             for (j, c), (m, cost) in zip(where, res):
                 out[j].seqs[c] = m
+                out[j].empty[c] = is_empty(m)
                 out[j].cost += int(cost)
         if self.ds.nonadd is not None and jobs:
             med, added = self.be.fitch(np.stack([x.na for x, _ in jobs]), np.stack([y.na for _, y in jobs]))
