@@ -476,6 +476,13 @@ class PairResult:
         n = int(self.out_len[k * self.slots + slot])
         return self.out[base + cap - n: base + cap]
 
+    def seqs(self, slot: int) -> list:
+        """seq(k, slot) of every pair."""
+        end = (self.out_off + (slot + 1) * self.cap).tolist()
+        n = self.out_len[slot::self.slots].tolist()
+        out = self.out
+        return [out[e - m: e] for e, m in zip(end, n)]
+
 
 class PairPlan:
     """A batched pairwise plan over device-resident sequences (torch tensors
@@ -727,6 +734,23 @@ def run_pairs(ctx: Context, cm: CostMatrix, mode: int, a_batch, b_batch, dws=Non
     off_b = np.ascontiguousarray(np.asarray(b_batch.off, np.int64) + len(a_batch.seq))
     la = np.ascontiguousarray(a_batch.ln, dtype=np.int32)
     lb = np.ascontiguousarray(b_batch.ln, dtype=np.int32)
+    return _run_host(ctx, cm, mode, seq, off_a, la, off_b, lb, dws, check_band)
+
+
+def run_pair_list(ctx: Context, cm: CostMatrix, mode: int, pairs, dws=None, check_band: bool = True) -> PairResult:
+    """run_pairs over a list of (a, b) host sequences: both sides of every
+    pair in one concatenation, in pair order."""
+    flat = [x for p in pairs for x in p]
+    ln = np.fromiter((len(x) for x in flat), np.int64, len(flat))
+    off = np.zeros(len(flat), np.int64)
+    if len(flat) > 1:
+        np.cumsum(ln[:-1], out=off[1:])
+    seq = np.concatenate(flat).astype(np.uint8, copy=False) if flat else np.zeros(0, np.uint8)
+    return _run_host(ctx, cm, mode, seq, off[0::2].copy(), ln[0::2].astype(np.int32), off[1::2].copy(),
+                     ln[1::2].astype(np.int32), dws, check_band)
+
+
+def _run_host(ctx, cm, mode, seq, off_a, la, off_b, lb, dws, check_band) -> PairResult:
     n = len(la)
     # (linear modes without dws: the library computes the default band)
     dw = None if dws is None else np.ascontiguousarray(dws, dtype=np.int32)

@@ -497,10 +497,11 @@ class EngineBackend:
         # (the medians are read-only views into the run's output block)
         e = self.engine
         slot = 0 if self.affine else 2
-        r = self._run(e.POYG_MODE_AFFINE_ALIGN if self.affine else e.POYG_MODE_LINEAR_ALIGN, pairs)
+        r = e.run_pair_list(self.ctx, self.cm, e.POYG_MODE_AFFINE_ALIGN if self.affine else e.POYG_MODE_LINEAR_ALIGN,
+                            pairs)
+        self.cells += r.cells
         r.out.flags.writeable = False
-        costs = r.cost.tolist()
-        return [(r.seq(k, slot), costs[k]) for k in range(len(pairs))]
+        return list(zip(r.seqs(slot), r.cost.tolist()))
 
     def align(self, pairs):
         e = self.engine
