@@ -2932,7 +2932,7 @@ __global__ __launch_bounds__(kWave * kC16Waves) void lin_cost16_kernel(
 // The packed 16-bit fill with the four waves of a workgroup on one pair, for
 // batches too small to fill the chip with one pair per wave (tree levels,
 // per-call entries): wave w runs strips w, w + 4, ..., the workgroup
-// advancing in chunks of 64 steps with a barrier between chunks.  Strip s
+// advancing in chunks of 32 steps with a barrier between chunks.  Strip s
 // reads the hand-off row that strip s - 1 writes into buffer (s - 1) mod 4;
 // it starts enough chunks after strip s - 1 that every column it reads
 // (lane 0's row, read up to 23 steps ahead) was written by strip s - 1's
@@ -2942,7 +2942,7 @@ __global__ __launch_bounds__(kWave * kC16Waves) void lin_cost16_kernel(
 // plus one chunk, rounded up to chunks.  A wave takes its next strip when
 // the previous one is done.  The schedule is the same arithmetic in every
 // wave (no waits: only the barrier orders the buffers).
-constexpr int kC16Chunk = 64;   // steps between the workgroup's barriers
+constexpr int kC16Chunk = 32;   // steps between the workgroup's barriers
 constexpr int kC16MwStrips = 512;   // strips the multi-wave fill schedules (rows up to 65,537)
 __host__ __device__ __forceinline__ int c16mw_words(int n_max) { return 5 * c16_cols(n_max) + kC16Waves * kWave * kC16U; }
 
