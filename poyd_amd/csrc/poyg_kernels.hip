@@ -189,16 +189,17 @@ __global__ void plan_start_kernel(int *ctr, int nctr, int *status, int nstatus, 
 
 // The host-sequence route's results, written into pinned host memory: word
 // 0 the failed-pair count (status word 0), from word 16 the run's small
-// results (costs, output lengths), then the output sequences.  The
-// system-scope fence makes the writes visible to the host once the stream is
-// synchronized.
+// results (costs, output lengths), then the output sequences; the host
+// reads them once the stream is synchronized.
 __global__ void collect_results_kernel(const uint32_t *small, int64_t small_words, const int32_t *status,
                                        const uint32_t *out, int64_t out_words, uint32_t *host) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
   if (t == 0) host[0] = (uint32_t)status[0];
   for (int64_t w = t; w < small_words; w += stride) host[16 + w] = small[w];
   for (int64_t w = t; w < out_words; w += stride) host[16 + small_words + w] = out[w];
-  __threadfence_system();
+  // (no fence: the kernel's completion, which the stream synchronization
+  // waits for, makes its writes visible to the host; a system-scope fence
+  // in every wave cost an L2 write-back per wave)
 }
 
 // Marks pair p of the run as failed: its flag (word kStatusFlags + p of the
