@@ -426,6 +426,7 @@ struct poyg_plan {
   float fill_ms = 0.f, trace_ms = 0.f;
   hipStream_t last_stream = nullptr;   // the stream of the last run (its status is read there)
   bool idle = false;   // set by an owner that synchronized every run: buffers go straight back to the cache
+  bool preset = false;   // the upload initialised the first run's counters, status and backtrace pointers
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;   // hand-off to and from the side streams
   ~poyg_plan() {
@@ -705,6 +706,11 @@ struct PerCallSeqs {
   int64_t la, lb;
   int64_t off_b;                  // b's offset from a (16-byte aligned)
   const uint8_t *dev = nullptr;   // out: a's copy on the device
+  // out: the run's results in the plan's block (costs, then output
+  // lengths: small_words words; the output sequences)
+  int32_t *small_dev = nullptr;
+  uint8_t *out_dev = nullptr;
+  int64_t small_words = 0;
 };
 
 static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n, const int64_t *off_a,
@@ -1035,14 +1041,40 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
   const size_t seq_at = blob_bytes;   // the per-call route's sequences
   if (per_call)   // a at 0, b (if any) at off_b
     blob_bytes += (size_t)(std::max(pc->la, pc->b ? pc->off_b + pc->lb : 0) + 255) & ~(size_t)255;
+  // the host-sequence route runs its plan once: the run's work-list
+  // counters, status block and backtrace pointers travel initialised with
+  // the descriptors (no start kernel), and its results (costs, output
+  // lengths, output sequences) sit behind them in the same block, not
+  // uploaded
+  size_t ctr_at = 0, st_at = 0, tro_at = 0, small_at = 0, res_at = 0;
+  int64_t small_words = 0;
+  auto r256 = [](size_t x) { return (std::max<size_t>(x, 16) + 255) & ~(size_t)255; };
+  if (per_call) {
+    ctr_at = blob_bytes;
+    blob_bytes += r256(sizeof(int) * kPlanCounters);
+    st_at = blob_bytes;
+    blob_bytes += r256(sizeof(int32_t) * ((size_t)kStatusFlags + n));
+    tro_at = blob_bytes;
+    blob_bytes += r256(sizeof(TraceOut));
+  }
+  const size_t upload_bytes = blob_bytes;
+  if (per_call) {
+    const int64_t n_cost = ((int64_t)n + 3) & ~(int64_t)3;
+    small_words = n_cost + (((int64_t)n * P->slots + 3) & ~(int64_t)3);
+    small_at = blob_bytes;
+    blob_bytes += r256(4 * (size_t)small_words);
+    res_at = blob_bytes;
+    blob_bytes += r256((size_t)out_bytes);
+  }
   std::vector<uint8_t> staging_v;
-  uint8_t *staging = per_call ? ctx_pinned(ctx->pin_plan, ctx->pin_plan_cap, blob_bytes) : nullptr;
+  uint8_t *staging = per_call ? ctx_pinned(ctx->pin_plan, ctx->pin_plan_cap, upload_bytes) : nullptr;
   const bool sync_upload = staging == nullptr;
   if (!staging) {
-    staging_v.assign(blob_bytes, 0);
+    staging_v.assign(upload_bytes, 0);
     staging = staging_v.data();
   } else {
     memset(staging, 0, seq_at);
+    memset(staging + ctr_at, 0, upload_bytes - ctr_at);
   }
   for (size_t k = 0; k < sections.size(); ++k)
     if (sections[k].second.second) memcpy(staging + sec_off[k], sections[k].second.first, sections[k].second.second);
@@ -1059,10 +1091,24 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
   rc |= P->dm.alloc((size_t)dm_bytes);
   rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
   rc |= P->rows.alloc(P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16);
-  rc |= P->counters.alloc(sizeof(int) * kPlanCounters);
-  rc |= P->status.alloc(sizeof(int32_t) * ((size_t)kStatusFlags + n));
+  if (per_call && !rc) {
+    P->counters.view(P->blob, ctr_at, sizeof(int) * kPlanCounters);
+    P->status.view(P->blob, st_at, sizeof(int32_t) * ((size_t)kStatusFlags + n));
+    P->trout.view(P->blob, tro_at, sizeof(TraceOut));
+    uint8_t *base = (uint8_t *)P->blob.p;
+    pc->small_dev = (int32_t *)(base + small_at);
+    pc->out_dev = base + res_at;
+    pc->small_words = small_words;
+    const int64_t n_cost = ((int64_t)n + 3) & ~(int64_t)3;
+    const TraceOut t{pc->out_dev, P->slots ? pc->small_dev + n_cost : nullptr, (int32_t *)(base + st_at)};
+    memcpy(staging + tro_at, &t, sizeof t);
+    P->preset = true;
+  } else if (!per_call) {
+    rc |= P->counters.alloc(sizeof(int) * kPlanCounters);
+    rc |= P->status.alloc(sizeof(int32_t) * ((size_t)kStatusFlags + n));
+    rc |= P->trout.alloc(sizeof(TraceOut));
+  }
   rc |= P->sink.alloc(sizeof(uint32_t) * kWave);
-  rc |= P->trout.alloc(sizeof(TraceOut));
   if (P->aff_pk) rc |= P->redo.alloc(sizeof(int32_t) * (size_t)P->n_quad);
   if (P->aff_pk && align) rc |= P->tlist.alloc(sizeof(int32_t) * (size_t)P->n_quad);
   if (!rc && affine && P->grid_help &&
@@ -1076,7 +1122,7 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
   }
   hipStream_t s = ctx->stream;
   hipError_t e = hipSuccess;
-  e = hipMemcpyAsync(P->blob.p, staging, blob_bytes, hipMemcpyHostToDevice, s);
+  e = hipMemcpyAsync(P->blob.p, staging, upload_bytes, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && sync_upload) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete P;
@@ -1115,8 +1161,11 @@ extern "C" int poyg_plan_run(poyg_plan *P, const uint8_t *d_seq, int32_t *d_cost
   const PairMeta *meta = (const PairMeta *)P->meta.p;
   const DevTcm *tcm = (const DevTcm *)P->tcm.p;
   if (P->timing) HIP_TRY(hipEventRecord(P->ev[0], s));
-  HIP_TRY(launch_plan_start(ctr, kPlanCounters, (int *)P->status.p, kStatusFlags + P->n,
-                            affine && align ? (TraceOut *)P->trout.p : nullptr, d_out, d_out_len, s));
+  if (P->preset)   // the first run of a host-route plan: the upload set the counters, status and pointers
+    P->preset = false;
+  else
+    HIP_TRY(launch_plan_start(ctr, kPlanCounters, (int *)P->status.p, kStatusFlags + P->n,
+                              affine && align ? (TraceOut *)P->trout.p : nullptr, d_out, d_out_len, s));
   P->last_stream = s;
   if (!affine || P->n_lds + P->n_glb)
     HIP_TRY(launch_prep(affine, std::max(1, std::min((P->n + 3) / 4, P->ctx->cus * 8)), meta, P->n, d_seq, tcm,
@@ -1250,21 +1299,19 @@ extern "C" int poyg_plan_status(poyg_plan *P, int32_t *bad) {
 // ---------------------------------------------------------------------------
 namespace {
 
-// The host-sequence route: one upload (descriptor block and sequences from
-// pinned staging), the plan's kernels, one kernel that copies every result
-// into the context's pinned result buffer, one synchronization.  The results
+// The host-sequence route: one upload (descriptor block, sequences and the
+// run's initialised counters from pinned staging), the plan's kernels (no
+// start kernel), one kernel that copies every result into the context's
+// pinned result buffer, one synchronization.  The results
 // (and the plan, for readers of its direction band) stay valid until the next
 // call on the context.
 struct HostRun {
-  DevBuf small, out;
   poyg_plan *plan = nullptr;
-  bool idle = false;   // every queued use of the buffers has completed
+  bool idle = false;   // every queued use of the plan's buffers has completed
   const int32_t *h_cost = nullptr, *h_len = nullptr;
   const uint8_t *h_out = nullptr;
   int32_t failed = 0;
   ~HostRun() {
-    small.release(idle);
-    out.release(idle);
     if (plan) plan->idle = idle;
     poyg_plan_destroy(plan);
   }
@@ -1275,9 +1322,10 @@ struct HostRun {
     int rc = plan_pairs_impl(ctx, cm, mode, n, off_a, len_a, off_b, len_b, dw, &plan, &in);
     if (rc) return rc;
     const int slots = plan->slots;
-    // device results: costs (n, rounded to 4 words), then output lengths
+    // device results (in the plan's block): costs (n, rounded to 4 words),
+    // then output lengths, then the output sequences
     const int64_t n_cost = ((int64_t)n + 3) & ~(int64_t)3;
-    const int64_t small_words = n_cost + (((int64_t)n * slots + 3) & ~(int64_t)3);
+    const int64_t small_words = in.small_words;
     const int64_t out_bytes = slots ? plan->out_bytes : 0;   // a multiple of 16
     const size_t res_bytes = 64 + 4 * (size_t)small_words + (size_t)out_bytes;
     uint8_t *pin = ctx_pinned(ctx->pin_io, ctx->pin_io_cap, res_bytes, &ctx->pin_io_dev);
@@ -1285,15 +1333,13 @@ struct HostRun {
       set_error("pinned staging of %zu bytes failed", res_bytes);
       return POYG_ENOMEM;
     }
-    if (small.alloc(4 * (size_t)small_words)) return POYG_ENOMEM;
-    if (slots && out.alloc((size_t)out_bytes)) return POYG_ENOMEM;
-    int32_t *d_small = (int32_t *)small.p;
-    rc = poyg_plan_run(plan, in.dev, d_small, (uint8_t *)out.p, slots ? d_small + n_cost : nullptr, s);
+    int32_t *d_small = in.small_dev;
+    rc = poyg_plan_run(plan, in.dev, d_small, in.out_dev, slots ? d_small + n_cost : nullptr, s);
     if (rc) return rc;
     // pinned: word 0 the failed-pair count, from word 16 the device results,
     // then the output sequences
-    HIP_TRY(launch_collect_results((const uint32_t *)small.p, small_words, (const int32_t *)plan->status.p,
-                                   (const uint32_t *)out.p, out_bytes / 4, (uint32_t *)ctx->pin_io_dev, s));
+    HIP_TRY(launch_collect_results((const uint32_t *)d_small, small_words, (const int32_t *)plan->status.p,
+                                   (const uint32_t *)in.out_dev, out_bytes / 4, (uint32_t *)ctx->pin_io_dev, s));
     HIP_TRY(hipStreamSynchronize(s));
     idle = true;
     const int32_t *h = (const int32_t *)pin;

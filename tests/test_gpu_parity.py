@@ -437,6 +437,25 @@ def test_fitch_matches_oracle(ctx, oracle):
         assert np.array_equal(res, oracle.nonadd_median_3(a, b, c, d))
 
 
+def test_fitch_host_batches_match_oracle(ctx, oracle):
+    """poyg_fitch_pairs_host (engine.fitch_pairs): batches of row pairs over
+    host arrays, every median row and change count equal to the oracle's
+    nonadd_median, for odd lengths, one row and many, and an empty batch."""
+    from poyd_amd import engine
+    note("fitch host batches")
+    rng = np.random.default_rng(14)
+    for n, length in ((1, 1), (3, 17), (40, 1001), (700, 64), (5, 10000)):
+        a = rng.integers(1, 256, (n, length)).astype(np.uint8)
+        b = rng.integers(1, 256, (n, length)).astype(np.uint8)
+        b[:, ::3] = a[:, ::3]   # some characters agree
+        res, added = engine.fitch_pairs(ctx, a, b)
+        for k in range(n):
+            want, wadd = oracle.nonadd_median(a[k], b[k])
+            assert np.array_equal(res[k], want) and int(added[k]) == wadd, (n, length, k)
+    res, added = engine.fitch_pairs(ctx, np.zeros((0, 8), np.uint8), np.zeros((0, 8), np.uint8))
+    assert res.shape == (0, 8) and added.shape == (0,)
+
+
 def test_workload_pairwise_linear_1000(ctx, oracle):
     """The 1,000-pair linear workload (L 200-400, 10% subs, 2% indels),
     every pair bit-exact: cost, both aligned sequences and the median."""
