@@ -1723,6 +1723,7 @@ int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, in
     }
   }
   const int LA = n[0], LB = n[1], LC = n[2];
+  HIP_TRY(hipSetDevice(ctx->device));   // (a caller's worker thread may start on another device)
   UkModel M;
   uk_model(mm, go, ge, &M);
   const int cpw = M.max_step;

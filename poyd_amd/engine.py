@@ -186,6 +186,7 @@ class Context:
         h = vp()
         _check(load().poyg_ctx_create(device, C.byref(h)), "poyg_ctx_create")
         self.h = h
+        self.device = device
 
     def sync(self):
         _check(load().poyg_ctx_sync(self.h), "poyg_ctx_sync")

@@ -515,10 +515,15 @@ class EngineBackend:
         self.fitch_chars += int(np.size(a))
         return self.engine.fitch_pairs(self.ctx, a, b)
 
+    POWELL_WORKERS = 8   # searches in flight at once (each on a context of its own)
+
     def powell_median(self, triples):
         """align_3_powell_inter (src/sequence.ml:1089-1115) per triple on
         the GPU: poyg_powell_3d's alignment, then the Three_D median of
-        every column (host lookups), gaps dropped, one leading gap."""
+        every column (host lookups), gaps dropped, one leading gap.  A
+        search is a chain of small launches, so several run at once: worker
+        threads, each with a context (stream) of its own, share the batch
+        (results in the batch's order)."""
         e = self.engine
         if getattr(self, "_med3", None) is None:
             cm3 = e.CostMatrix3(self.cm)
@@ -526,11 +531,36 @@ class EngineBackend:
                                   np.uint8)
         mm, ge = self.cm.cost(1, 2), self.cm.cost(1, 16)
         go = self.cm.gap_opening if self.affine else 0
-        out = []
-        for a, b, p in triples:
-            cost, (x, y, z) = e.powell_3d(self.ctx, a, b, p, mm, go, ge)
+
+        def one(ctx, t):
+            cost, (x, y, z) = e.powell_3d(ctx, t[0], t[1], t[2], mm, go, ge)
             med = self._med3[x, y, z]
-            out.append((np.concatenate([[GAP], med[med != GAP]]).astype(np.uint8), cost))
+            return np.concatenate([[GAP], med[med != GAP]]).astype(np.uint8), cost
+
+        k = min(self.POWELL_WORKERS, len(triples))
+        if k <= 1:
+            return [one(self.ctx, t) for t in triples]
+        import threading
+        if len(getattr(self, "_powell_ctx", [])) < k - 1:
+            self._powell_ctx = [e.Context(getattr(self.ctx, "device", 0)) for _ in range(k - 1)]
+        ctxs = [self.ctx] + self._powell_ctx[:k - 1]
+        out = [None] * len(triples)
+        errors = []
+
+        def work(w):
+            try:
+                for i in range(w, len(triples), k):
+                    out[i] = one(ctxs[w], triples[i])
+            except Exception as exc:   # re-raised by the caller's thread
+                errors.append(exc)
+
+        threads = [threading.Thread(target=work, args=(w,)) for w in range(k)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        if errors:
+            raise errors[0]
         return out
 
     def distance(self, pairs):
