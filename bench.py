@@ -501,6 +501,15 @@ def bench_tree(args) -> int:
         "tree_cost_before": int(d.cost), "tree_cost_after": int(a_cost + d.nonadd),
         "alignments": int(ev.n_align),
     }
+    # one Iterative ThreeD pass (Powell's search per vertex and locus)
+    ta = time.perf_counter()
+    ev = treepass.TreeEval(ds, r.tree, be)
+    _adj, t_cost, kept = ev.approx_d(d, iterations=1, three_d=True)
+    ctx.sync()
+    out["config"]["three_d"] = {
+        "passes": 1, "passes_kept": int(kept), "ms": round(1e3 * (time.perf_counter() - ta), 1),
+        "tree_cost_after": int(t_cost + d.nonadd), "powell_searches": int(ev.n_align),
+    }
     if not args.no_cpu:
         cb = cpu_baseline_tree(ds, r.tree, args.cpu_seconds)
         out["cpu_baseline"] = cb
