@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <mutex>
 
 #include "poyg_internal.h"
 #include "poyg_ukk.h"
@@ -469,12 +470,19 @@ __global__ void ukk_reset_kernel(UkWalk *w, int first) {
   }
 }
 
+// Searches may run on several host threads at once (a context each); their
+// launches go through one lock, so that the launch path sees one thread at a
+// time (the waits, where the time goes, stay concurrent).
+static std::mutex g_ukk_launch;
+
 hipError_t launch_ukk_reset(UkWalk *w, int first, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_ukk_launch);
   hipLaunchKernelGGL(ukk_reset_kernel, dim3(1), dim3(64), 0, s, w, first);
   return hipGetLastError();
 }
 
 hipError_t launch_ukk_geom(const UkRange &r, int ge, int c0, int c1, UkLevel *lev, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_ukk_launch);
   hipLaunchKernelGGL(ukk_geom_kernel, dim3(1), dim3(64), 0, s, r, ge, c0, c1, lev);
   return hipGetLastError();
 }
@@ -482,6 +490,7 @@ hipError_t launch_ukk_geom(const UkRange &r, int ge, int c0, int c1, UkLevel *le
 hipError_t launch_ukk_level(const UkModel *M, const UkRange &r, int c, int64_t diags, const UkLevel *lev, uint32_t *U,
                             const char *A, const char *B, const char *C, int target_state, int target_dist,
                             UkWalk *w, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_ukk_launch);
   if (diags <= 0) return hipSuccess;
   const int64_t blocks = (diags * kUkStates + 255) / 256;
   hipLaunchKernelGGL(ukk_level_kernel, dim3((unsigned)blocks), dim3(256), 0, s, M, r, c, lev, U, A, B, C,
@@ -492,6 +501,7 @@ hipError_t launch_ukk_level(const UkModel *M, const UkRange &r, int c, int64_t d
 hipError_t launch_ukk_esweep(const UkModel *M, const UkRange &r, int c, int final_cost, int64_t diags,
                              const UkLevel *lev, const uint32_t *U, int *E, int64_t ring_words, int ring_n, int half,
                              UkWalk *w, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_ukk_launch);
   if (diags <= 0) return hipSuccess;
   const int64_t blocks = (diags * kUkStates + 255) / 256;
   hipLaunchKernelGGL(ukk_esweep_kernel, dim3((unsigned)blocks), dim3(256), 0, s, M, r, c, final_cost, lev, U, E,
@@ -501,12 +511,14 @@ hipError_t launch_ukk_esweep(const UkModel *M, const UkRange &r, int c, int fina
 
 hipError_t launch_ukk_walk(const UkModel *M, const UkRange &r, int cpcost, int cpw, const UkLevel *lev,
                            const uint32_t *U, int first_pass, UkWalk *w, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_ukk_launch);
   hipLaunchKernelGGL(ukk_walk_kernel, dim3(1), dim3(64), 0, s, M, r, cpcost, cpw, lev, U, first_pass, w);
   return hipGetLastError();
 }
 
 hipError_t launch_ukk_trace(const UkModel *M, const UkRange &r, const UkLevel *lev, const uint32_t *U, const char *A,
                             const char *B, const char *C, char *cols, int cap, UkWalk *w, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_ukk_launch);
   hipLaunchKernelGGL(ukk_trace_kernel, dim3(1), dim3(64), 0, s, M, r, lev, U, A, B, C, cols, cap, w);
   return hipGetLastError();
 }
