@@ -1601,8 +1601,8 @@ int poyg_algn_align_3d(poyg_ctx *ctx, const poyg_cm3 *cm3, const uint8_t *s1, in
 extern "C++" {
 namespace poyg {
 void uk_model(int mm, int go, int ge, UkModel *M);
-hipError_t launch_ukk_reset(UkWalk *w, int first, hipStream_t s);
-hipError_t launch_ukk_geom(const UkRange &r, int ge, int c0, int c1, UkLevel *lev, hipStream_t s);
+hipError_t launch_ukk_geom(const UkRange &r, int ge, int c0, int c1, UkLevel *lev, UkWalk *w, int reset,
+                           hipStream_t s);
 hipError_t launch_ukk_level(const UkModel *M, const UkRange &r, int c, int64_t diags, const UkLevel *lev, uint32_t *U,
                             const char *A, const char *B, const char *C, int target_state, int target_dist,
                             UkWalk *w, hipStream_t s);
@@ -1770,7 +1770,6 @@ int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, in
   };
   // ---- first pass: levels 0, 1, ... until the end's best state reaches |A|
   UkRange r0{0, 0, 0, 0, start, LA - LB, LA - LC, 0, 0, LA, LA, LB, LC};
-  if (e == hipSuccess) e = launch_ukk_reset(dw, 1, s);
   int final_cost = -1;
   {
     const int chunk = 32;
@@ -1781,7 +1780,7 @@ int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, in
       const int64_t words = uk_boxes(st.box, r0, ge, c0, c1);
       const int rr = uk_reserve(st, (size_t)(c1 + 1), words, true, s);
       if (rr) { cleanup(); return rr; }
-      if (e == hipSuccess) e = launch_ukk_geom(r0, ge, c0, c1, (UkLevel *)st.lev.p, s);
+      if (e == hipSuccess) e = launch_ukk_geom(r0, ge, c0, c1, (UkLevel *)st.lev.p, dw, c0 == 0 ? 1 : -1, s);
       for (int cc = c0; cc <= c1 && e == hipSuccess; ++cc)
         e = launch_ukk_level(dm, r0, cc, (int64_t)st.box[cc].nab * st.box[cc].nac, (const UkLevel *)st.lev.p,
                              (uint32_t *)st.U.p, A, B, C, -1, LA, dw, s);
@@ -1828,8 +1827,7 @@ int poyg_powell_3d(poyg_ctx *ctx, const uint8_t *a, int la, const uint8_t *b, in
     const int64_t words = uk_boxes(st.box, r, ge, r.scost, r.fcost);
     const int rr = uk_reserve(st, (size_t)(r.fcost - r.scost + 1), words, false, s);
     if (rr) { cleanup(); return rr; }
-    e = launch_ukk_reset(dw, 0, s);
-    if (e == hipSuccess) e = launch_ukk_geom(r, ge, r.scost, r.fcost, (UkLevel *)st.lev.p, s);
+    e = launch_ukk_geom(r, ge, r.scost, r.fcost, (UkLevel *)st.lev.p, dw, 0, s);
     for (int cc = r.scost; cc <= r.fcost && e == hipSuccess; ++cc)
       e = launch_ukk_level(dm, r, cc, (int64_t)st.box[cc - r.scost].nab * st.box[cc - r.scost].nac,
                            (const UkLevel *)st.lev.p, (uint32_t *)st.U.p, A, B, C, r.fstate, r.fdist, dw, s);

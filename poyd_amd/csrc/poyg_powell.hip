@@ -141,9 +141,13 @@ __device__ __forceinline__ void uk_load_model(UkModel &sm, const UkModel *M) {
 }  // namespace
 
 // Level boxes of levels [c0, c1] of a range (one thread; bases continue the
-// previous level's).
-__global__ void ukk_geom_kernel(UkRange r, int ge, int c0, int c1, UkLevel *lev) {
+// previous level's); reset >= 0 first resets the walk record for the range
+// (1: for a whole search).
+__device__ void ukk_reset(UkWalk *w, int first);
+
+__global__ void ukk_geom_kernel(UkRange r, int ge, int c0, int c1, UkLevel *lev, UkWalk *w, int reset) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (reset >= 0) ukk_reset(w, reset);
   int64_t base = 0;
   if (c0 > r.scost) {
     const UkLevel &p = lev[c0 - 1 - r.scost];
@@ -458,8 +462,7 @@ __global__ void ukk_trace_kernel(const UkModel *__restrict__ M, UkRange r, const
 }
 
 // A range's walk record before its levels run.
-__global__ void ukk_reset_kernel(UkWalk *w, int first) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void ukk_reset(UkWalk *w, int first) {
   w->reached = INT_MAX;
   w->f_cost = -1;
   if (first) {
@@ -470,20 +473,17 @@ __global__ void ukk_reset_kernel(UkWalk *w, int first) {
   }
 }
 
+
 // Searches may run on several host threads at once (a context each); their
 // launches go through one lock, so that the launch path sees one thread at a
 // time (the waits, where the time goes, stay concurrent).
 static std::mutex g_ukk_launch;
 
-hipError_t launch_ukk_reset(UkWalk *w, int first, hipStream_t s) {
-  std::lock_guard<std::mutex> lock(g_ukk_launch);
-  hipLaunchKernelGGL(ukk_reset_kernel, dim3(1), dim3(64), 0, s, w, first);
-  return hipGetLastError();
-}
 
-hipError_t launch_ukk_geom(const UkRange &r, int ge, int c0, int c1, UkLevel *lev, hipStream_t s) {
+hipError_t launch_ukk_geom(const UkRange &r, int ge, int c0, int c1, UkLevel *lev, UkWalk *w, int reset,
+                           hipStream_t s) {
   std::lock_guard<std::mutex> lock(g_ukk_launch);
-  hipLaunchKernelGGL(ukk_geom_kernel, dim3(1), dim3(64), 0, s, r, ge, c0, c1, lev);
+  hipLaunchKernelGGL(ukk_geom_kernel, dim3(1), dim3(64), 0, s, r, ge, c0, c1, lev, w, reset);
   return hipGetLastError();
 }
 
