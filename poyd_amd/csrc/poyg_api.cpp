@@ -1066,6 +1066,22 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
     res_at = blob_bytes;
     blob_bytes += r256((size_t)out_bytes);
   }
+  // per-call plans also carve their scratch (purity flags, direction band,
+  // prefix sums, rows, sink) from the block: one pool block per call
+  size_t pure_at = 0, dm_at = 0, aux_at = 0, rows_at = 0, sink_at = 0;
+  const size_t rows_bytes = P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16;
+  if (per_call) {
+    pure_at = blob_bytes;
+    blob_bytes += r256((size_t)n);
+    dm_at = blob_bytes;
+    blob_bytes += r256((size_t)dm_bytes);
+    aux_at = blob_bytes;
+    blob_bytes += r256(sizeof(int) * (size_t)aux_ints);
+    rows_at = blob_bytes;
+    blob_bytes += r256(rows_bytes);
+    sink_at = blob_bytes;
+    blob_bytes += r256(sizeof(uint32_t) * kWave);
+  }
   std::vector<uint8_t> staging_v;
   uint8_t *staging = per_call ? ctx_pinned(ctx->pin_plan, ctx->pin_plan_cap, upload_bytes) : nullptr;
   const bool sync_upload = staging == nullptr;
@@ -1087,10 +1103,18 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
   if (!rc)
     for (size_t k = 0; k < sections.size(); ++k)
       sections[k].first->view(P->blob, sec_off[k], std::max<size_t>(sections[k].second.second, 16));
-  rc |= P->pure.alloc((size_t)n);
-  rc |= P->dm.alloc((size_t)dm_bytes);
-  rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
-  rc |= P->rows.alloc(P->n_glb ? sizeof(int) * (size_t)P->grid_glb * (affine ? 8 : 2) * row_width : 16);
+  if (per_call && !rc) {
+    P->pure.view(P->blob, pure_at, (size_t)n);
+    P->dm.view(P->blob, dm_at, (size_t)dm_bytes);
+    P->aux.view(P->blob, aux_at, sizeof(int) * (size_t)aux_ints);
+    P->rows.view(P->blob, rows_at, rows_bytes);
+    P->sink.view(P->blob, sink_at, sizeof(uint32_t) * kWave);
+  } else if (!per_call) {
+    rc |= P->pure.alloc((size_t)n);
+    rc |= P->dm.alloc((size_t)dm_bytes);
+    rc |= P->aux.alloc(sizeof(int) * (size_t)aux_ints);
+    rc |= P->rows.alloc(rows_bytes);
+  }
   if (per_call && !rc) {
     P->counters.view(P->blob, ctr_at, sizeof(int) * kPlanCounters);
     P->status.view(P->blob, st_at, sizeof(int32_t) * ((size_t)kStatusFlags + n));
@@ -1108,7 +1132,7 @@ static int plan_pairs_impl(poyg_ctx *ctx, const poyg_cm *cm, int mode, int32_t n
     rc |= P->status.alloc(sizeof(int32_t) * ((size_t)kStatusFlags + n));
     rc |= P->trout.alloc(sizeof(TraceOut));
   }
-  rc |= P->sink.alloc(sizeof(uint32_t) * kWave);
+  if (!per_call) rc |= P->sink.alloc(sizeof(uint32_t) * kWave);
   if (P->aff_pk) rc |= P->redo.alloc(sizeof(int32_t) * (size_t)P->n_quad);
   if (P->aff_pk && align) rc |= P->tlist.alloc(sizeof(int32_t) * (size_t)P->n_quad);
   if (!rc && affine && P->grid_help &&
